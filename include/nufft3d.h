@@ -1,0 +1,123 @@
+/* nufft3d.h -- C ABI of the MI355X-native (gfx950 HIP) 3-D NUFFT.
+ *
+ * Drop-in boundary for the reference class @nufft_3d (MATLAB). A MEX gateway
+ * binds these entry points one to one; each declaration names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *   - om is [3, M] double, column-major (x,y,z of one sample adjacent), in
+ *     cycles/FOV, exactly the reference trajectory argument (nufft_3d.m:11-12).
+ *   - Images are [N1 N2 N3 nc] and k-space is [M nc], column-major,
+ *     interleaved complex: complex64 for a precision-1 plan, complex128 for a
+ *     precision-2 plan (MATLAB R2018a interleaved complex API).
+ *   - Array pointers may be host or device (HIP) memory; the library detects
+ *     which. Device pointers avoid the PCIe copies.
+ *   - Every call returns NUFFT3D_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is nufft3d_last_error().
+ *   - A plan is used by one host thread at a time (calls on one plan are
+ *     serialised by an internal mutex).
+ */
+#ifndef NUFFT3D_H
+#define NUFFT3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUFFT3D_OK 0
+#define NUFFT3D_ERR_INVALID (-1)     /* bad argument: reference error() */
+#define NUFFT3D_ERR_NOMEM (-2)       /* device or host allocation failed */
+#define NUFFT3D_ERR_DEVICE (-3)      /* HIP runtime error */
+#define NUFFT3D_ERR_UNSUPPORTED (-4) /* valid in the reference, not here */
+
+typedef struct nufft3d_plan nufft3d_plan;
+
+/* Constructor name/value pairs of nufft_3d.m:71-76 ('J','u','alpha',
+ * 'radial','gpu','d'). precision: 1 = FP32 (reference gpu=1, gpuSparse
+ * single), 2 = FP64 (reference gpu=2, gpuArray double). */
+typedef struct nufft3d_opts {
+  double J;          /* kernel width, default 4 (nufft_3d.m:27) */
+  double u;          /* oversampling, default 2 (nufft_3d.m:28) */
+  double alpha;      /* Kaiser-Bessel alpha, 0 = Beatty (nufft_3d.m:30,85-92) */
+  int32_t radial;    /* 1 = radial kernel (nufft_3d.m:31) */
+  int32_t precision; /* 1 = fp32, 2 = fp64 */
+  const double* d;   /* optional density [M] (host), nufft_3d.m:227-238 */
+  int64_t nd;        /* number of elements of d (0 = compute Pipe density) */
+  int32_t tile;      /* spread tile edge in grid points (0 = auto) */
+  int32_t chunk;     /* max samples per spread work chunk (0 = auto) */
+  int32_t device;    /* HIP device ordinal */
+  int32_t verbose;   /* print the reference's plan report to stdout */
+} nufft3d_opts;
+
+/* Per-right-hand-side result of nufft3d_inverse (MATLAB minres flag/relres). */
+typedef struct nufft3d_info {
+  int32_t nrhs;
+  int32_t iters[64];   /* iterations taken, per coil (first 64 coils) */
+  double relres[64];   /* recurrence ||b-Ax||/||b||, per coil */
+  double seconds;      /* wall time of the solve */
+} nufft3d_info;
+
+/* Fill opts with the reference defaults. */
+int nufft3d_opts_default(nufft3d_opts* opts);
+
+/* obj = nufft_3d(om,N,varargin): validation, alpha, K, scaling, bounds
+ * (nufft_3d.m:54-127), interpolation set-up (nufft_3d.m:129-212: on-the-fly
+ * weights from bin-sorted samples instead of the stored H), deap
+ * (nufft_3d.m:224, deap.m) and density (nufft_3d.m:227-238, density.m). */
+int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
+                        const nufft3d_opts* opts, nufft3d_plan** plan);
+
+/* Frees the plan and its device memory (MATLAB clearing the object). */
+int nufft3d_plan_destroy(nufft3d_plan* plan);
+
+/* k = obj.fNUFT(x) (fNUFT.m:2-9): x [N1 N2 N3 nc] -> k [M nc]. */
+int nufft3d_forward(nufft3d_plan* plan, const void* x, int32_t nc, void* k);
+
+/* x = obj.aNUFT(k) (aNUFT.m:2-9): k [M nc] -> x [N1 N2 N3 nc]. */
+int nufft3d_adjoint(nufft3d_plan* plan, const void* k, int32_t nc, void* x);
+
+/* im = obj.iNUFT(raw,maxit,damp,W) (iNUFT.m:2-138,225): raw [M nc];
+ * W NULL (density only) or nW = M or nW = ny real weights (multiplied by
+ * the density, iNUFT.m:55-74); tol 1e-4 as iNUFT.m:114 (pass <=0 for it).
+ * maxit = 0 is regridding (iNUFT.m:119-122). info may be NULL. */
+int nufft3d_inverse(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t maxit,
+                    double damp, const double* W, int64_t nW, double tol, void* im,
+                    nufft3d_info* info);
+
+/* y = iprojection(x,damp,W) with W = density (nufft_3d.m:305-313):
+ * A'WA x + damp^2 x on [N1 N2 N3 nc]. */
+int nufft3d_normal(nufft3d_plan* plan, const void* x, int32_t nc, double damp, void* y);
+
+/* obj.d (density.m:1-42): d [M] real (float or double per precision). */
+int nufft3d_density(nufft3d_plan* plan, void* d);
+
+/* obj.U (deap.m): U [N1 N2 N3] real, always double, host pointer. */
+int nufft3d_deap(nufft3d_plan* plan, double* U);
+
+/* obj.nnz (nufft_3d.m:126); obj.K (nufft_3d.m:109); obj.alpha. */
+int64_t nufft3d_nnz(const nufft3d_plan* plan);
+int nufft3d_get_params(const nufft3d_plan* plan, int32_t K[3], int32_t N[3], double* J,
+                       double* u, double* alpha, int32_t* radial, int32_t* precision);
+
+/* Bind subsequent work of this plan to a HIP stream (hipStream_t as void*). */
+int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
+
+/* Measurement hook, no counterpart in the reference (which only prints
+ * tic/toc wall times, nufft_3d.m:130,213): while enabled, every kernel launch
+ * of the plan is bracketed by HIP events on the plan's stream. */
+int nufft3d_profile_enable(nufft3d_plan* plan, int32_t enable);
+
+/* Synchronises the plan's stream and writes a JSON object
+ * {"kernel": [launches, total_ms], ...} of the launches since the last read. */
+int nufft3d_profile_read(nufft3d_plan* plan, char* buf, int64_t len);
+
+/* Text of the last failure on the calling thread ("" if none). */
+const char* nufft3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUFFT3D_H */

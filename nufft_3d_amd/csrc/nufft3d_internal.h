@@ -1,0 +1,168 @@
+// Internal plan state and helpers shared by the HIP sources.
+//
+// Device layouts (all device-resident, owned by the plan):
+//   image   : [nc][N3][N2][N1] complex, x fastest (MATLAB column-major
+//             [N1 N2 N3 nc], the reference's layout, nufft_3d.m:186)
+//   grid    : [K3][K2][K1][nc] complex, coil fastest, so one grid point of
+//             every coil is one contiguous run (nc*8 B) for the gather/spread
+//   samples : sorted by spread tile; ks[s*nc + c] internally, user arrays are
+//             [nc][M] (sample fastest) and reached through orig[s]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nufft3d.h"
+
+namespace nufft3d {
+
+void set_error(const std::string& msg);
+
+#define N3D_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::nufft3d::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return NUFFT3D_ERR_DEVICE;                                               \
+    }                                                                          \
+  } while (0)
+
+#define N3D_TRY(call)          \
+  do {                         \
+    int s_ = (call);           \
+    if (s_ != NUFFT3D_OK) return s_; \
+  } while (0)
+
+template <typename R> struct cplx_t;
+template <> struct cplx_t<float> { using type = float2; };
+template <> struct cplx_t<double> { using type = double2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+// 1-D FFT description for one axis (length L = product of radices)
+struct FftAxis {
+  int L = 1;
+  int nstage = 0;
+  int radix[24] = {0};
+  void* d_tw = nullptr;   // L roots of unity exp(-2 pi i t / L), precision of the plan
+};
+
+// Work chunk of the deterministic spread: one tile, a slice of its list
+struct SpreadChunk {
+  int tile;    // linear tile id
+  int start;   // first entry in the tile-list array
+  int count;   // entries in this chunk (0 = empty tile, writes zeros)
+  int slot;    // -1: sole chunk of its tile (writes the grid); else partial slot
+};
+
+struct SplitTile {
+  int tile;
+  int first_slot;
+  int nslots;
+  int pad;
+};
+
+struct Workspace {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// Optional per-kernel timing: HIP events recorded on the plan's stream
+// around each launch while profiling is enabled.
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+struct Profiler {
+  bool on = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+};
+
+}  // namespace nufft3d
+
+struct nufft3d_plan {
+  int prec = 1;             // 1 fp32, 2 fp64
+  double J = 4, u = 2, alpha = 0;
+  int radial = 1;
+  int N[3] = {0, 0, 0};
+  int K[3] = {0, 0, 0};
+  int W[3] = {0, 0, 0};     // candidate half range per axis (ceil(J/2); 0 on a 2-D z axis)
+  int64_t M = 0;            // samples passed (rows of H)
+  int64_t nnz = 0;          // in-bounds samples
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int verbose = 0;
+
+  // sorted, in-bounds samples
+  int* d_orig = nullptr;    // [nnz] original row of sorted sample s
+  int* d_oob = nullptr;     // [M-nnz] rows that are out of bounds
+  int4* d_base = nullptr;   // [nnz] wrapped base (round(u*om) mod K) + tile id
+  void* d_off = nullptr;    // double [3*nnz]: u*om - round(u*om) (exact)
+
+  // deapodization and density
+  void* d_U = nullptr;      // [N1*N2*N3] real
+  void* d_dens = nullptr;   // [nnz] real, sorted order
+  std::vector<double> h_U;  // fp64 copy of U (host)
+
+  // spread tiles
+  int T[3] = {8, 8, 8};
+  int ntile[3] = {1, 1, 1};
+  int ntiles = 0;
+  int* d_list = nullptr;    // tile lists of sorted sample ids
+  int64_t nlist = 0;
+  nufft3d::SpreadChunk* d_chunks = nullptr;
+  int nchunks = 0;
+  nufft3d::SplitTile* d_split = nullptr;
+  int nsplit = 0;
+  int nslots = 0;
+  int chunk_len = 2048;
+  int max_list = 0;
+
+  nufft3d::FftAxis ax[3];
+  nufft3d::Profiler prof;
+
+  // scratch, grown on demand
+  nufft3d::Workspace ws_grid, ws_t1, ws_t2, ws_samp, ws_part, ws_img, ws_host, ws_solve, ws_red;
+  std::mutex mu;
+};
+
+namespace nufft3d {
+
+int ensure(Workspace& w, size_t bytes);
+void prof_begin(nufft3d_plan* p, const char* name);
+void prof_end(nufft3d_plan* p);
+
+// RAII bracket of one kernel launch for the profiler
+struct ProfScope {
+  nufft3d_plan* p;
+  ProfScope(nufft3d_plan* q, const char* name) : p(q) { prof_begin(p, name); }
+  ~ProfScope() { prof_end(p); }
+};
+int ensure_host(Workspace& w, size_t bytes);
+
+// grid.hip
+int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
+           const void* wsamp, double scale);
+int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
+           const void* wsamp);
+int zero_oob(nufft3d_plan* p, void* k_user, int nc);
+int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp);
+
+// fft.hip
+int fft_setup_axis(nufft3d_plan* p, int axis);
+int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid);   // deap, pad, FFT
+int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
+                const void* img_add, double add_coef);                    // IFFT, crop, deap
+
+// solve.hip
+int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                 const void* wsamp, double tol, void* x, nufft3d_info* info);
+int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void* wsamp,
+                 void* y);
+int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out);
+
+}  // namespace nufft3d

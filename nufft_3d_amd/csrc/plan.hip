@@ -1,0 +1,1028 @@
+// Plan construction and the C ABI (include/nufft3d.h).
+//
+// Reference constructor: @nufft_3d/nufft_3d.m:54-251.
+//   - argument checks, alpha (Beatty), K = 2*ceil(N*u/2)   nufft_3d.m:60-111
+//   - om_s = u*double(om), inclusive bounds, nnz           nufft_3d.m:120-127
+//   - H (nufft_3d.m:129-212) is NOT stored: samples are sorted into spread
+//     tiles and the Kaiser-Bessel weights are recomputed on the fly
+//   - U = deap() (deap.m), d = density() (density.m)      nufft_3d.m:224-238
+// Entry points: fNUFT.m, aNUFT.m, iNUFT.m.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "nufft3d_internal.h"
+
+namespace nufft3d {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int ensure(Workspace& w, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (w.bytes >= bytes) return NUFFT3D_OK;
+  if (w.p) {
+    if (hipFree(w.p) != hipSuccess) { (void)hipGetLastError(); }
+    w.p = nullptr;
+    w.bytes = 0;
+  }
+  if (hipMalloc(&w.p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    w.p = nullptr;
+    set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+    return NUFFT3D_ERR_NOMEM;
+  }
+  w.bytes = bytes;
+  return NUFFT3D_OK;
+}
+
+int ensure_host(Workspace& w, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (w.bytes >= bytes) return NUFFT3D_OK;
+  if (w.p) { (void)hipHostFree(w.p); w.p = nullptr; w.bytes = 0; }
+  if (hipHostMalloc(&w.p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    w.p = nullptr;
+    set_error("pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+    return NUFFT3D_ERR_NOMEM;
+  }
+  w.bytes = bytes;
+  return NUFFT3D_OK;
+}
+
+void prof_begin(nufft3d_plan* p, const char* name) {
+  if (!p->prof.on) return;
+  hipEvent_t e[2];
+  for (int i = 0; i < 2; ++i) {
+    if (!p->prof.pool.empty()) { e[i] = p->prof.pool.back(); p->prof.pool.pop_back(); }
+    else if (hipEventCreate(&e[i]) != hipSuccess) { (void)hipGetLastError(); return; }
+  }
+  (void)hipEventRecord(e[0], p->stream);
+  p->prof.recs.push_back({name, e[0], e[1]});
+}
+
+void prof_end(nufft3d_plan* p) {
+  if (!p->prof.on || p->prof.recs.empty()) return;
+  (void)hipEventRecord(p->prof.recs.back().b, p->stream);
+}
+
+static void release(Workspace& w, bool host = false) {
+  if (!w.p) return;
+  if (host) (void)hipHostFree(w.p); else (void)hipFree(w.p);
+  w.p = nullptr;
+  w.bytes = 0;
+}
+
+static bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// ---------------------------------------------------------------- deap (host, fp64)
+// kernel.m:2-34 with complex arguments (deap evaluates beyond the support)
+static std::complex<double> bessi0_c(std::complex<double> ax) {
+  if (std::abs(ax) < 3.75) {
+    std::complex<double> y = ax / 3.75;
+    y = y * y;
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+                 y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  std::complex<double> y = 3.75 / ax;
+  std::complex<double> r = (std::exp(ax) / std::sqrt(ax)) *
+      (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
+       y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
+       y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+  if (ax.real() == 0) r *= 2.0;                          // kernel.m:33-34
+  return r;
+}
+
+static double bessi0_r(double ax) {
+  if (ax < 3.75) {
+    double y = ax / 3.75;
+    y = y * y;
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+                 y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  double y = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) *
+         (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
+          y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
+          y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+static double kernel_host(double dist2, double J, double alpha) {
+  const double t = 1.0 - dist2 / ((J / 2) * (J / 2));
+  const double b0 = bessi0_r(alpha);
+  if (t >= 0) return bessi0_r(alpha * sqrt(t)) / b0;
+  const std::complex<double> arg = alpha * std::sqrt(std::complex<double>(t, 0.0));
+  return bessi0_c(arg).real() / b0;
+}
+
+// deap.m:5-36: table at half-integer offsets, symmetric inverse FFT of the
+// table padded to 2K per axis (closed form 0.5*(T0 + 2 sum T_m cos(pi m x/K))),
+// crop to x in [-N/2, N/2), fftshift; U = 1/that (deap.m:98).
+static void deap_host(nufft3d_plan* p) {
+  const int nJ = (int)floor(p->J);
+  const bool is2d = p->N[2] == 1;
+  const int t1 = nJ + 1, t2 = nJ + 1, t3 = is2d ? 1 : nJ + 1;
+  std::vector<double> T((size_t)t1 * t2 * t3);
+  for (int dx = 0; dx < t1; ++dx)
+    for (int dy = 0; dy < t2; ++dy)
+      for (int dz = 0; dz < t3; ++dz) {
+        const double dx2 = (dx / 2.0) * (dx / 2.0);
+        const double dy2 = (dy / 2.0) * (dy / 2.0);
+        const double dz2 = (dz / 2.0) * (dz / 2.0);
+        double v;
+        if (p->radial) v = kernel_host(dx2 + dy2 + dz2, p->J, p->alpha);
+        else v = kernel_host(dx2, p->J, p->alpha) * kernel_host(dy2, p->J, p->alpha) *
+                 kernel_host(dz2, p->J, p->alpha);
+        T[dx + (size_t)t1 * (dy + (size_t)t2 * dz)] = v;
+      }
+  auto axis = [&](int n, int K, int nt) {
+    std::vector<double> A((size_t)n * nt);
+    for (int i = 0; i < n; ++i) {
+      const double x = i - n / 2;
+      for (int m = 0; m < nt; ++m)
+        A[(size_t)i * nt + m] = 0.5 * (m == 0 ? 1.0 : 2.0) * cos(M_PI * m * x / K);
+    }
+    return A;
+  };
+  const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
+  std::vector<double> A = axis(N1, p->K[0], t1), B = axis(N2, p->K[1], t2);
+  std::vector<double> Cz = is2d ? std::vector<double>(1, 1.0) : axis(N3, p->K[2], t3);
+  // stage 1: x
+  std::vector<double> U1((size_t)N1 * t2 * t3, 0.0);
+  for (int c = 0; c < t3; ++c)
+    for (int b = 0; b < t2; ++b)
+      for (int i = 0; i < N1; ++i) {
+        double s = 0;
+        for (int a = 0; a < t1; ++a) s += A[(size_t)i * t1 + a] * T[a + (size_t)t1 * (b + (size_t)t2 * c)];
+        U1[i + (size_t)N1 * (b + (size_t)t2 * c)] = s;
+      }
+  std::vector<double> U2((size_t)N1 * N2 * t3, 0.0);
+  for (int c = 0; c < t3; ++c)
+    for (int j = 0; j < N2; ++j)
+      for (int i = 0; i < N1; ++i) {
+        double s = 0;
+        for (int b = 0; b < t2; ++b) s += B[(size_t)j * t2 + b] * U1[i + (size_t)N1 * (b + (size_t)t2 * c)];
+        U2[i + (size_t)N1 * (j + (size_t)N2 * c)] = s;
+      }
+  p->h_U.assign((size_t)N1 * N2 * N3, 0.0);
+  for (int k = 0; k < N3; ++k)
+    for (int j = 0; j < N2; ++j)
+      for (int i = 0; i < N1; ++i) {
+        double s = 0;
+        if (is2d) s = U2[i + (size_t)N1 * j];
+        else
+          for (int c = 0; c < t3; ++c) s += Cz[(size_t)k * t3 + c] * U2[i + (size_t)N1 * (j + (size_t)N2 * c)];
+        p->h_U[i + (size_t)N1 * (j + (size_t)N2 * k)] = 1.0 / s;
+      }
+}
+
+// ---------------------------------------------------------------- binning kernels
+struct BinGeom {
+  double u;
+  int K1, K2, K3;
+  int T1, T2, T3;
+  int nt1, nt2, nt3;
+  int ntiles;
+  int W1, W2, W3;
+  int is2d;
+};
+
+__device__ __forceinline__ bool sample_ok(const double* om, long long m, const BinGeom& g,
+                                          double& x, double& y, double& z) {
+  x = g.u * om[3 * m + 0];
+  y = g.u * om[3 * m + 1];
+  z = g.u * om[3 * m + 2];
+  // nufft_3d.m:123-125 (the 2-D bug `a & b | c` makes every 2-D sample in bounds)
+  bool ok = x >= -g.K1 / 2.0 && x <= g.K1 / 2.0 - 1;
+  ok = y >= -g.K2 / 2.0 && y <= g.K2 / 2.0 - 1 && ok;
+  ok = (z >= -g.K3 / 2.0 && z <= g.K3 / 2.0 - 1 && ok) || g.is2d;
+  return ok;
+}
+
+__device__ __forceinline__ int wrapb(double r, int K) {
+  int b = (int)r;
+  b %= K;
+  return b < 0 ? b + K : b;
+}
+
+__global__ void k_keys(const double* __restrict__ om, long long M, BinGeom g,
+                       unsigned* __restrict__ keys, int* __restrict__ vals, int* __restrict__ counts) {
+  const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  double x, y, z;
+  unsigned key = (unsigned)g.ntiles;
+  if (sample_ok(om, m, g, x, y, z)) {
+    const int bx = wrapb(round(x), g.K1), by = wrapb(round(y), g.K2), bz = wrapb(round(z), g.K3);
+    key = (unsigned)((bz / g.T3) * g.nt2 * g.nt1 + (by / g.T2) * g.nt1 + bx / g.T1);
+  }
+  keys[m] = key;
+  vals[m] = (int)m;
+  atomicAdd(&counts[key], 1);
+}
+
+__global__ void k_gather(const double* __restrict__ om, const unsigned* __restrict__ skeys,
+                         const int* __restrict__ svals, long long M, long long nnz, BinGeom g,
+                         int* __restrict__ orig, int* __restrict__ oob, int4* __restrict__ base,
+                         double* __restrict__ off) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const int m = svals[s];
+  if (s >= nnz) { oob[s - nnz] = m; return; }
+  double x, y, z;
+  sample_ok(om, m, g, x, y, z);
+  // MATLAB round is half away from zero, as C round (nufft_3d.m:168-170)
+  const double rx = round(x), ry = round(y), rz = round(z);
+  int4 b;
+  b.x = wrapb(rx, g.K1);
+  b.y = wrapb(ry, g.K2);
+  b.z = wrapb(rz, g.K3);
+  b.w = (int)skeys[s];
+  base[s] = b;
+  orig[s] = m;
+  // om_s - round(om_s) is exact in double; dx = d - f reproduces (x - om) of
+  // nufft_3d.m:173-175 bit for bit
+  double* o = off + 3 * s;
+  o[0] = x - rx; o[1] = y - ry; o[2] = z - rz;
+}
+
+// distinct tiles touched along one axis by grid points b-W..b+W (wrapped)
+__device__ __forceinline__ int axis_tiles(int b, int W, int K, int T, int* t) {
+  int n = 0;
+  for (int d = -W; d <= W; ++d) {
+    int g = b + d;
+    g = g < 0 ? g + K : (g >= K ? g - K : g);
+    const int tt = g / T;
+    bool seen = false;
+    for (int i = 0; i < n; ++i) seen = seen || t[i] == tt;
+    if (!seen) t[n++] = tt;
+  }
+  return n;
+}
+
+__global__ void k_touch_count(const int4* __restrict__ base, long long nnz, BinGeom g,
+                              int* __restrict__ ntouch, int* __restrict__ tile_count) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int4 b = base[s];
+  int tx[16], ty[16], tz[16];
+  const int nx = axis_tiles(b.x, g.W1, g.K1, g.T1, tx);
+  const int ny = axis_tiles(b.y, g.W2, g.K2, g.T2, ty);
+  const int nz = axis_tiles(b.z, g.W3, g.K3, g.T3, tz);
+  ntouch[s] = nx * ny * nz;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) atomicAdd(&tile_count[(tz[k] * g.nt2 + ty[j]) * g.nt1 + tx[i]], 1);
+}
+
+__global__ void k_touch_fill(const int4* __restrict__ base, long long nnz, BinGeom g,
+                             const int* __restrict__ toff, unsigned* __restrict__ keys,
+                             int* __restrict__ vals) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int4 b = base[s];
+  int tx[16], ty[16], tz[16];
+  const int nx = axis_tiles(b.x, g.W1, g.K1, g.T1, tx);
+  const int ny = axis_tiles(b.y, g.W2, g.K2, g.T2, ty);
+  const int nz = axis_tiles(b.z, g.W3, g.K3, g.T3, tz);
+  int o = toff[s];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        keys[o] = (unsigned)((tz[k] * g.nt2 + ty[j]) * g.nt1 + tx[i]);
+        vals[o] = (int)s;
+        ++o;
+      }
+}
+
+// ---------------------------------------------------------------- density kernels
+template <typename R>
+__global__ void k_fill_dens(cplx<R>* __restrict__ ks, R* __restrict__ d, long long n) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  d[s] = (R)1;
+  cplx<R> v; v.x = 1; v.y = 0;
+  ks[s] = v;
+}
+
+// d = d ./ max(H*H'*d, eps) (density.m:151)
+template <typename R>
+__global__ void k_dens_update(cplx<R>* __restrict__ ks, const cplx<R>* __restrict__ hhd,
+                              R* __restrict__ d, long long n) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const R e = (R)2.220446049250313e-16;
+  const R h = hhd[s].x;
+  const R v = d[s] / (h > e ? h : e);
+  d[s] = v;
+  cplx<R> o; o.x = v; o.y = 0;
+  ks[s] = o;
+}
+
+template <typename R>
+__global__ void k_div_scalar(R* __restrict__ d, long long n, R q) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) d[s] = d[s] / q;
+}
+
+template <typename R>
+__global__ void k_gather_real(const double* __restrict__ src, const int* __restrict__ orig,
+                              const R* __restrict__ mul, long long n, R* __restrict__ dst) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  R v = (R)src[orig[s]];
+  if (mul) v *= mul[s];
+  dst[s] = v;
+}
+
+template <typename R>
+__global__ void k_scatter_dens(const R* __restrict__ d, const int* __restrict__ orig,
+                               const int* __restrict__ oob, long long nnz, long long M,
+                               R* __restrict__ out) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnz) out[orig[s]] = d[s];
+  else if (s < M) out[oob[s - nnz]] = (R)0;
+}
+
+static inline unsigned nblocks(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+template <typename R>
+static int density_t(nufft3d_plan* p) {
+  const long long n = p->nnz;
+  using C = cplx<R>;
+  N3D_HIP(hipMalloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
+  if (n == 0) return NUFFT3D_OK;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2];
+  N3D_TRY(ensure(p->ws_grid, ngrid * sizeof(C)));
+  N3D_TRY(ensure(p->ws_samp, 2 * (size_t)n * sizeof(C)));
+  C* ks = (C*)p->ws_samp.p;
+  C* hhd = ks + n;
+  R* d = (R*)p->d_dens;
+  hipLaunchKernelGGL((k_fill_dens<R>), dim3(nblocks(n)), dim3(256), 0, p->stream, ks, d, n);
+  for (int it = 0; it < 20; ++it) {                       // density.m:5,11-14
+    N3D_TRY(spread(p, ks, 1, p->ws_grid.p, 0, nullptr));  // H'd
+    N3D_TRY(interp(p, p->ws_grid.p, 1, hhd, 0, nullptr, 1.0));  // H(H'd)
+    hipLaunchKernelGGL((k_dens_update<R>), dim3(nblocks(n)), dim3(256), 0, p->stream, ks,
+                       (const C*)hhd, d, n);
+  }
+  double sum = 0;
+  N3D_TRY(sum_real(p, d, n, &sum));
+  const R q = (R)(sum / (double)p->nnz);                   // density.m:42
+  hipLaunchKernelGGL((k_div_scalar<R>), dim3(nblocks(n)), dim3(256), 0, p->stream, d, n, q);
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+static int user_density_t(nufft3d_plan* p, const double* dh, int64_t nd) {
+  const long long n = p->nnz;
+  N3D_HIP(hipMalloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
+  double* dd = nullptr;
+  N3D_HIP(hipMalloc(&dd, sizeof(double) * nd));
+  N3D_HIP(hipMemcpy(dd, dh, sizeof(double) * nd, hipMemcpyHostToDevice));
+  if (n > 0)
+    hipLaunchKernelGGL((k_gather_real<R>), dim3(nblocks(n)), dim3(256), 0, p->stream, dd,
+                       p->d_orig, (const R*)nullptr, n, (R*)p->d_dens);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  N3D_HIP(hipFree(dd));
+  // warning of nufft_3d.m:235-237: mean over all rows after zeroing out of bounds
+  std::vector<double> tmp(dh, dh + nd);
+  double s = 0;
+  std::vector<char> okm(nd, 0);
+  std::vector<int> orig(n);
+  if (n > 0) N3D_HIP(hipMemcpy(orig.data(), p->d_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < n; ++i) okm[orig[i]] = 1;
+  for (int64_t i = 0; i < nd; ++i) s += okm[i] ? tmp[i] : 0.0;
+  if (s / (double)nd != 1.0 && p->verbose)
+    printf("Warning: mean(density) should be 1 (%f)\n", s / (double)nd);
+  return NUFFT3D_OK;
+}
+
+// ---------------------------------------------------------------- plan build
+static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
+  const bool dev = is_device_ptr(om_in);
+  double* om = nullptr;
+  N3D_HIP(hipMalloc(&om, sizeof(double) * 3 * (M > 0 ? M : 1)));
+  N3D_HIP(hipMemcpy(om, om_in, sizeof(double) * 3 * M,
+                    dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  // all z == 0 -> 2-D (nufft_3d.m:104-106); decided on the host copy
+  {
+    std::vector<double> h;
+    const double* hz = om_in;
+    if (dev) {
+      h.resize(3 * M);
+      N3D_HIP(hipMemcpy(h.data(), om_in, sizeof(double) * 3 * M, hipMemcpyDeviceToHost));
+      hz = h.data();
+    }
+    bool allz = true;
+    for (int64_t m = 0; m < M && allz; ++m) allz = hz[3 * m + 2] == 0.0;
+    if (allz) p->N[2] = 1;
+    if (p->verbose) {
+      double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+      for (int64_t m = 0; m < M; ++m)
+        for (int a = 0; a < 3; ++a) { mn[a] = std::min(mn[a], hz[3 * m + a]); mx[a] = std::max(mx[a], hz[3 * m + a]); }
+      printf(" Trajectory:     min       max       matrix\n");
+      for (int a = 0; a < 3; ++a) printf("  om(%d): %12.3f %9.3f %9d\n", a + 1, mn[a], mx[a], p->N[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) p->K[a] = 2 * (int)ceil(p->N[a] * p->u / 2);   // nufft_3d.m:109
+  if (p->N[2] == 1) p->K[2] = 1;
+  if ((double)p->K[0] * p->K[1] * p->K[2] >= 2147483647.0) {
+    set_error("N or u too large");                                          // nufft_3d.m:111
+    (void)hipFree(om);
+    return NUFFT3D_ERR_INVALID;
+  }
+  const int Wj = (int)ceil(p->J / 2);                                          // nufft_3d.m:154
+  p->W[0] = Wj; p->W[1] = Wj; p->W[2] = p->N[2] == 1 ? 0 : Wj;
+  for (int a = 0; a < 3; ++a)
+    if (p->K[a] > 1 && p->K[a] < 2 * p->W[a] + 1) {
+      set_error("oversampled size K=" + std::to_string(p->K[a]) +
+                " is smaller than the kernel footprint 2*ceil(J/2)+1");
+      (void)hipFree(om);
+      return NUFFT3D_ERR_UNSUPPORTED;
+    }
+  p->M = M;
+
+  // tiles
+  if (p->N[2] == 1) { p->T[0] = p->T[1] = 16; p->T[2] = 1; }
+  for (int a = 0; a < 3; ++a) {
+    if (p->T[a] > p->K[a]) p->T[a] = p->K[a];
+    p->ntile[a] = (p->K[a] + p->T[a] - 1) / p->T[a];
+  }
+  p->ntiles = p->ntile[0] * p->ntile[1] * p->ntile[2];
+  BinGeom g;
+  g.u = p->u;
+  g.K1 = p->K[0]; g.K2 = p->K[1]; g.K3 = p->K[2];
+  g.T1 = p->T[0]; g.T2 = p->T[1]; g.T3 = p->T[2];
+  g.nt1 = p->ntile[0]; g.nt2 = p->ntile[1]; g.nt3 = p->ntile[2];
+  g.ntiles = p->ntiles;
+  g.W1 = p->W[0]; g.W2 = p->W[1]; g.W3 = p->W[2];
+  g.is2d = p->N[2] == 1;
+
+  unsigned *keys = nullptr, *skeys = nullptr;
+  int *vals = nullptr, *svals = nullptr, *counts = nullptr;
+  const long long Mm = M > 0 ? M : 1;
+  N3D_HIP(hipMalloc(&keys, sizeof(unsigned) * Mm));
+  N3D_HIP(hipMalloc(&skeys, sizeof(unsigned) * Mm));
+  N3D_HIP(hipMalloc(&vals, sizeof(int) * Mm));
+  N3D_HIP(hipMalloc(&svals, sizeof(int) * Mm));
+  N3D_HIP(hipMalloc(&counts, sizeof(int) * (p->ntiles + 1)));
+  N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (p->ntiles + 1), p->stream));
+  if (M > 0)
+    hipLaunchKernelGGL(k_keys, dim3(nblocks(M)), dim3(256), 0, p->stream, om, (long long)M, g,
+                       keys, vals, counts);
+  int end_bit = 1;
+  while ((1LL << end_bit) <= p->ntiles) ++end_bit;
+  size_t tb = 0;
+  N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, vals, svals, (int)M, 0,
+                                             end_bit, p->stream));
+  void* tmp = nullptr;
+  N3D_HIP(hipMalloc(&tmp, tb + 16));
+  if (M > 0)
+    N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, vals, svals, (int)M, 0,
+                                               end_bit, p->stream));
+  int noob = 0;
+  N3D_HIP(hipMemcpyAsync(&noob, counts + p->ntiles, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(tmp);
+  p->nnz = M - noob;
+  if (p->verbose)
+    printf("  %lld points (out of %lld) are within bounds.\n", (long long)p->nnz, (long long)M);
+  const long long nnz = p->nnz;
+  N3D_HIP(hipMalloc(&p->d_orig, sizeof(int) * (nnz > 0 ? nnz : 1)));
+  N3D_HIP(hipMalloc(&p->d_oob, sizeof(int) * (noob > 0 ? noob : 1)));
+  N3D_HIP(hipMalloc(&p->d_base, sizeof(int4) * (nnz > 0 ? nnz : 1)));
+  N3D_HIP(hipMalloc(&p->d_off, 3 * sizeof(double) * (nnz > 0 ? nnz : 1)));
+  if (M > 0)
+    hipLaunchKernelGGL(k_gather, dim3(nblocks(M)), dim3(256), 0, p->stream, om, skeys, svals,
+                       (long long)M, nnz, g, p->d_orig, p->d_oob, p->d_base, (double*)p->d_off);
+  N3D_HIP(hipGetLastError());
+  (void)hipFree(keys); (void)hipFree(skeys); (void)hipFree(vals); (void)hipFree(svals);
+  (void)hipFree(om);
+
+  // tile lists: every sample is listed in each tile its footprint touches
+  int *ntouch = nullptr, *toff = nullptr;
+  N3D_HIP(hipMalloc(&ntouch, sizeof(int) * (nnz + 1)));
+  N3D_HIP(hipMalloc(&toff, sizeof(int) * (nnz + 1)));
+  N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (p->ntiles + 1), p->stream));
+  N3D_HIP(hipMemsetAsync(ntouch, 0, sizeof(int) * (nnz + 1), p->stream));
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_touch_count, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
+                       nnz, g, ntouch, counts);
+  tb = 0;
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ntouch, toff, (int)(nnz + 1), p->stream));
+  N3D_HIP(hipMalloc(&tmp, tb + 16));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, ntouch, toff, (int)(nnz + 1), p->stream));
+  int nlist = 0;
+  N3D_HIP(hipMemcpyAsync(&nlist, toff + nnz, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  std::vector<int> hcount(p->ntiles);
+  N3D_HIP(hipMemcpyAsync(hcount.data(), counts, sizeof(int) * p->ntiles, hipMemcpyDeviceToHost,
+                         p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(tmp);
+  p->nlist = nlist;
+  unsigned *lk = nullptr, *lks = nullptr;
+  int* lv = nullptr;
+  const long long nl = nlist > 0 ? nlist : 1;
+  N3D_HIP(hipMalloc(&lk, sizeof(unsigned) * nl));
+  N3D_HIP(hipMalloc(&lks, sizeof(unsigned) * nl));
+  N3D_HIP(hipMalloc(&lv, sizeof(int) * nl));
+  N3D_HIP(hipMalloc(&p->d_list, sizeof(int) * nl));
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_touch_fill, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base, nnz,
+                       g, toff, lk, lv);
+  tb = 0;
+  N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lk, lks, lv, p->d_list, nlist, 0,
+                                             end_bit, p->stream));
+  N3D_HIP(hipMalloc(&tmp, tb + 16));
+  if (nlist > 0)
+    N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lk, lks, lv, p->d_list, nlist, 0, end_bit,
+                                               p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(tmp); (void)hipFree(lk); (void)hipFree(lks); (void)hipFree(lv);
+  (void)hipFree(ntouch); (void)hipFree(toff); (void)hipFree(counts);
+
+  // chunk table (host): split long lists so no work-group owns a hot tile alone
+  std::vector<SpreadChunk> chunks;
+  std::vector<SplitTile> split;
+  int start = 0, slots = 0, maxl = 0;
+  const int CH = p->chunk_len;
+  for (int t = 0; t < p->ntiles; ++t) {
+    const int c = hcount[t];
+    maxl = std::max(maxl, c);
+    if (c <= CH) {
+      chunks.push_back({t, start, c, -1});
+    } else {
+      const int nch = (c + CH - 1) / CH;
+      split.push_back({t, slots, nch, 0});
+      for (int j = 0; j < nch; ++j)
+        chunks.push_back({t, start + j * CH, std::min(CH, c - j * CH), slots + j});
+      slots += nch;
+    }
+    start += c;
+  }
+  p->max_list = maxl;
+  p->nchunks = (int)chunks.size();
+  p->nsplit = (int)split.size();
+  p->nslots = slots;
+  N3D_HIP(hipMalloc(&p->d_chunks, sizeof(SpreadChunk) * std::max<size_t>(1, chunks.size())));
+  N3D_HIP(hipMemcpy(p->d_chunks, chunks.data(), sizeof(SpreadChunk) * chunks.size(),
+                    hipMemcpyHostToDevice));
+  N3D_HIP(hipMalloc(&p->d_split, sizeof(SplitTile) * std::max<size_t>(1, split.size())));
+  if (!split.empty())
+    N3D_HIP(hipMemcpy(p->d_split, split.data(), sizeof(SplitTile) * split.size(),
+                      hipMemcpyHostToDevice));
+
+  // deapodization (deap.m) and FFT twiddles
+  deap_host(p);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2];
+  if (p->prec == 2) {
+    N3D_HIP(hipMalloc(&p->d_U, sizeof(double) * nimg));
+    N3D_HIP(hipMemcpy(p->d_U, p->h_U.data(), sizeof(double) * nimg, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> u32(nimg);
+    for (size_t i = 0; i < nimg; ++i) u32[i] = (float)p->h_U[i];
+    N3D_HIP(hipMalloc(&p->d_U, sizeof(float) * nimg));
+    N3D_HIP(hipMemcpy(p->d_U, u32.data(), sizeof(float) * nimg, hipMemcpyHostToDevice));
+  }
+  for (int a = 0; a < 3; ++a) N3D_TRY(fft_setup_axis(p, a));
+  return NUFFT3D_OK;
+}
+
+static void destroy_plan(nufft3d_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
+                  p->d_list, p->d_chunks, p->d_split};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  for (int a = 0; a < 3; ++a)
+    if (p->ax[a].d_tw) (void)hipFree(p->ax[a].d_tw);
+  for (auto& r : p->prof.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
+  release(p->ws_grid); release(p->ws_t1); release(p->ws_t2); release(p->ws_samp);
+  release(p->ws_part); release(p->ws_img); release(p->ws_solve); release(p->ws_red);
+  release(p->ws_host, true);
+  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+}  // namespace nufft3d
+
+using namespace nufft3d;
+
+// ================================================================ C ABI
+extern "C" {
+
+const char* nufft3d_last_error(void) { return g_last_error.c_str(); }
+
+int nufft3d_opts_default(nufft3d_opts* o) {
+  if (!o) { set_error("opts is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::memset(o, 0, sizeof(*o));
+  o->J = 4;
+  o->u = 2;
+  o->alpha = 0;
+  o->radial = 1;
+  o->precision = 1;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
+                        const nufft3d_opts* opts, nufft3d_plan** out) {
+  g_last_error.clear();
+  if (!out) { set_error("plan output pointer is NULL"); return NUFFT3D_ERR_INVALID; }
+  *out = nullptr;
+  nufft3d_opts o;
+  if (opts) o = *opts; else nufft3d_opts_default(&o);
+  if (!om || M <= 0) {
+    set_error("om must be a real array with leading dimension of 3");   // nufft_3d.m:61
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (M >= 2147483647LL) { set_error("more than 2^31-1 samples"); return NUFFT3D_ERR_UNSUPPORTED; }
+  if (!N) { set_error("N must be scalar or 3-vector of even integers"); return NUFFT3D_ERR_INVALID; }
+  if (o.u < 1 || o.u > 6) {                                               // nufft_3d.m:82-84
+    char b[96];
+    snprintf(b, sizeof b, "value of u=%f is not recommended", o.u);
+    set_error(b);
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (!(o.J > 0)) { set_error("J must be positive"); return NUFFT3D_ERR_INVALID; }
+  if ((N[0] == 0 && N[1] == 0 && N[2] == 0) || N[0] % 2 || N[1] % 2 || N[0] <= 0 || N[1] <= 0 ||
+      N[2] <= 0) {
+    set_error("N must be an even integer");                                // nufft_3d.m:101-103
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (o.precision != 1 && o.precision != 2) {
+    set_error("precision must be 1 (fp32) or 2 (fp64)");
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (hipSetDevice(o.device) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipSetDevice failed (no HIP device?)");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  nufft3d_plan* p = new nufft3d_plan();
+  p->prec = o.precision;
+  p->J = o.J;
+  p->u = o.u;
+  p->radial = o.radial ? 1 : 0;
+  p->alpha = o.alpha;
+  if (p->alpha == 0) {                                                     // nufft_3d.m:85-92
+    p->alpha = M_PI * sqrt(p->J * p->J * (1 - 0.5 / p->u) * (1 - 0.5 / p->u) - 0.8);
+    if (p->radial)
+      p->alpha = M_PI * sqrt(p->J * p->J * (1 - 0.5 / p->u) * (1 - 0.5 / p->u) - 0.6 * p->u);
+  }
+  if (o.verbose) {
+    if (p->J < 1 || p->J > 7) printf("Warning: value of J=%f is not recommended\n", p->J);
+    if (p->radial && (p->J < 3 || p->u < 1.5))
+      printf("Warning: radial kernel not recommended with J<3 or u<1.5\n");
+  }
+  for (int a = 0; a < 3; ++a) p->N[a] = N[a];
+  p->device = o.device;
+  p->verbose = o.verbose;
+  if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
+  if (o.chunk > 0) p->chunk_len = o.chunk;
+  if (p->T[0] * p->T[1] * p->T[2] > 512) {
+    set_error("tile too large (at most 512 grid points per tile)");
+    delete p;
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipStreamCreate failed");
+    delete p;
+    return NUFFT3D_ERR_DEVICE;
+  }
+  p->own_stream = true;
+  auto t0 = std::chrono::steady_clock::now();
+  int st = build_plan(p, om, M);
+  if (st == NUFFT3D_OK) {
+    if (o.d && o.nd > 0) {
+      if (o.nd != M) {
+        set_error("supplied density has wrong number of elements (should be " +
+                  std::to_string(M) + ")");                             // nufft_3d.m:230-232
+        st = NUFFT3D_ERR_INVALID;
+      } else {
+        st = p->prec == 2 ? user_density_t<double>(p, o.d, o.nd) : user_density_t<float>(p, o.d, o.nd);
+      }
+    } else {
+      st = p->prec == 2 ? density_t<double>(p) : density_t<float>(p);
+    }
+  }
+  if (st == NUFFT3D_OK && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error during plan construction");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  if (st != NUFFT3D_OK) {
+    destroy_plan(p);
+    return st;
+  }
+  if (p->verbose) {
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    printf(" Created plan in %.3f s: N=[%d %d %d] K=[%d %d %d] J=%g u=%g alpha=%.4f radial=%d\n",
+           sec, p->N[0], p->N[1], p->N[2], p->K[0], p->K[1], p->K[2], p->J, p->u, p->alpha,
+           p->radial);
+    printf("\ttiles %d (%dx%dx%d) list %lld chunks %d split %d max list %d\n", p->ntiles, p->T[0],
+           p->T[1], p->T[2], (long long)p->nlist, p->nchunks, p->nsplit, p->max_list);
+  }
+  *out = p;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_plan_destroy(nufft3d_plan* p) {
+  destroy_plan(p);
+  return NUFFT3D_OK;
+}
+
+int64_t nufft3d_nnz(const nufft3d_plan* p) { return p ? p->nnz : -1; }
+
+int nufft3d_get_params(const nufft3d_plan* p, int32_t K[3], int32_t N[3], double* J, double* u,
+                       double* alpha, int32_t* radial, int32_t* precision) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  for (int a = 0; a < 3; ++a) {
+    if (K) K[a] = p->K[a];
+    if (N) N[a] = p->N[a];
+  }
+  if (J) *J = p->J;
+  if (u) *u = p->u;
+  if (alpha) *alpha = p->alpha;
+  if (radial) *radial = p->radial;
+  if (precision) *precision = p->prec;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (p->own_stream && p->stream) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipStreamDestroy(p->stream);
+  }
+  p->own_stream = false;
+  p->stream = (hipStream_t)stream;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_profile_enable(nufft3d_plan* p, int32_t enable) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->prof.on = enable != 0;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_profile_read(nufft3d_plan* p, char* buf, int64_t len) {
+  if (!p || !buf || len <= 0) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  std::vector<std::string> names;
+  std::vector<long long> cnt;
+  std::vector<double> tot;
+  for (auto& r : p->prof.recs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) { (void)hipGetLastError(); ms = 0; }
+    size_t i = 0;
+    while (i < names.size() && names[i] != r.name) ++i;
+    if (i == names.size()) { names.push_back(r.name); cnt.push_back(0); tot.push_back(0); }
+    cnt[i] += 1;
+    tot[i] += ms;
+    p->prof.pool.push_back(r.a);
+    p->prof.pool.push_back(r.b);
+  }
+  p->prof.recs.clear();
+  std::string js = "{";
+  for (size_t i = 0; i < names.size(); ++i) {
+    char b[256];
+    snprintf(b, sizeof b, "%s\"%s\": [%lld, %.6f]", i ? ", " : "", names[i].c_str(), cnt[i], tot[i]);
+    js += b;
+  }
+  js += "}";
+  if ((int64_t)js.size() + 1 > len) { set_error("profile buffer too small"); return NUFFT3D_ERR_INVALID; }
+  std::memcpy(buf, js.c_str(), js.size() + 1);
+  return NUFFT3D_OK;
+}
+
+int nufft3d_deap(nufft3d_plan* p, double* U) {
+  if (!p || !U) { set_error("NULL argument"); return NUFFT3D_ERR_INVALID; }
+  std::memcpy(U, p->h_U.data(), sizeof(double) * p->h_U.size());
+  return NUFFT3D_OK;
+}
+
+// Stage a caller array on the device when it is host memory.
+struct Staged {
+  void* dev = nullptr;
+  bool host = false;
+};
+
+static int stage_in(nufft3d_plan* p, const void* src, size_t bytes, Workspace& ws, Staged& s) {
+  if (is_device_ptr(src)) { s.dev = const_cast<void*>(src); s.host = false; return NUFFT3D_OK; }
+  N3D_TRY(ensure(ws, bytes));
+  N3D_HIP(hipMemcpyAsync(ws.p, src, bytes, hipMemcpyHostToDevice, p->stream));
+  s.dev = ws.p;
+  s.host = true;
+  return NUFFT3D_OK;
+}
+
+static int stage_out(nufft3d_plan* p, void* dst, size_t bytes, Workspace& ws, Staged& s) {
+  if (is_device_ptr(dst)) { s.dev = dst; s.host = false; return NUFFT3D_OK; }
+  N3D_TRY(ensure(ws, bytes));
+  s.dev = ws.p;
+  s.host = true;
+  return NUFFT3D_OK;
+}
+
+static int finish_out(nufft3d_plan* p, void* dst, size_t bytes, const Staged& s) {
+  if (s.host) {
+    N3D_HIP(hipMemcpyAsync(dst, s.dev, bytes, hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+  }
+  return NUFFT3D_OK;
+}
+
+#define N3D_PLAN_GUARD(p)                                          \
+  if (!(p)) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; } \
+  std::lock_guard<std::mutex> lk_((p)->mu);                        \
+  g_last_error.clear();                                            \
+  if (hipSetDevice((p)->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; }
+
+int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
+  N3D_PLAN_GUARD(p);
+  if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  Workspace tmp_out;
+  Staged xi, ko;
+  N3D_TRY(stage_in(p, x, nimg * esz, p->ws_img, xi));
+  int st = stage_out(p, k, nk * esz, tmp_out, ko);
+  if (st) return st;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  N3D_TRY(ensure(p->ws_grid, ngrid * esz));
+  N3D_TRY(fft_forward(p, xi.dev, nc, p->ws_grid.p));                                // fNUFT.m:5-6
+  N3D_TRY(interp(p, p->ws_grid.p, nc, ko.dev, 1, nullptr, 1.0 / sqrt((double)p->nnz)));  // :8-9
+  N3D_TRY(zero_oob(p, ko.dev, nc));
+  st = finish_out(p, k, nk * esz, ko);
+  release(tmp_out);
+  if (xi.host) N3D_HIP(hipStreamSynchronize(p->stream));
+  return st;
+}
+
+int nufft3d_adjoint(nufft3d_plan* p, const void* k, int32_t nc, void* x) {
+  N3D_PLAN_GUARD(p);
+  if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  Workspace tmp_out;
+  Staged ki, xo;
+  N3D_TRY(stage_in(p, k, nk * esz, p->ws_img, ki));
+  int st = stage_out(p, x, nimg * esz, tmp_out, xo);
+  if (st) return st;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  N3D_TRY(ensure(p->ws_grid, ngrid * esz));
+  N3D_TRY(spread(p, ki.dev, nc, p->ws_grid.p, 1, nullptr));                          // aNUFT.m:5
+  N3D_TRY(fft_adjoint(p, p->ws_grid.p, nc, xo.dev, 1.0 / sqrt((double)p->nnz), nullptr, 0.0));
+  st = finish_out(p, x, nimg * esz, xo);
+  release(tmp_out);
+  if (ki.host) N3D_HIP(hipStreamSynchronize(p->stream));
+  return st;
+}
+
+// W handling of iNUFT.m:55-74 -> per sorted sample weight W(row)*d
+static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, void** out) {
+  *out = p->d_dens;
+  if (!W || nW <= 1) return NUFFT3D_OK;   // absent, scalar or empty: W = d
+  const int64_t M = p->M;
+  std::vector<double> full;
+  if (nW == M) {
+    full.assign(W, W + M);
+  } else {
+    const int64_t ny = nW;
+    if (M % ny) { set_error("W must be a vector of length ny or ny*nr"); return NUFFT3D_ERR_INVALID; }
+    const int64_t nr = M / ny;
+    full.resize(M);
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nr; ++i) full[i + nr * j] = W[j];
+  }
+  for (double v : full)
+    if (!std::isfinite(v) || v < 0) { set_error("W must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
+  double* dW = nullptr;
+  N3D_HIP(hipMalloc(&dW, sizeof(double) * M));
+  N3D_HIP(hipMemcpyAsync(dW, full.data(), sizeof(double) * M, hipMemcpyHostToDevice, p->stream));
+  const size_t esz = p->prec == 2 ? sizeof(double) : sizeof(float);
+  void* wd = nullptr;
+  N3D_HIP(hipMalloc(&wd, esz * (p->nnz > 0 ? p->nnz : 1)));
+  if (p->nnz > 0) {
+    if (p->prec == 2)
+      hipLaunchKernelGGL((k_gather_real<double>), dim3(nblocks(p->nnz)), dim3(256), 0, p->stream,
+                         dW, p->d_orig, (const double*)p->d_dens, (long long)p->nnz, (double*)wd);
+    else
+      hipLaunchKernelGGL((k_gather_real<float>), dim3(nblocks(p->nnz)), dim3(256), 0, p->stream,
+                         dW, p->d_orig, (const float*)p->d_dens, (long long)p->nnz, (float*)wd);
+  }
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(dW);
+  *out = wd;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit, double damp,
+                    const double* W, int64_t nW, double tol, void* im, nufft3d_info* info) {
+  N3D_PLAN_GUARD(p);
+  if (!raw || !im || nc <= 0) { set_error("invalid raw/im/nc"); return NUFFT3D_ERR_INVALID; }
+  if (maxit < 0) { set_error("maxit must be a nonnegative integer"); return NUFFT3D_ERR_INVALID; }
+  if (!(damp >= 0) || !std::isfinite(damp)) { set_error("damp must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
+  if (!(tol > 0)) tol = 1e-4;                                                  // iNUFT.m:114
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  void* wsamp = nullptr;
+  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  Workspace tmp_out, tmp_b;
+  Staged ri, xo;
+  int st = stage_in(p, raw, nk * esz, p->ws_img, ri);
+  if (!st) st = stage_out(p, im, nimg * esz, tmp_out, xo);
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  if (!st) st = ensure(p->ws_grid, ngrid * esz);
+  if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
+  // b = aNUFT(W.*raw) (iNUFT.m:122,130)
+  if (!st) st = permute_in(p, ri.dev, nc, p->ws_samp.p, wsamp);
+  if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr);
+  void* bdst = xo.dev;
+  if (!st && maxit > 0) {
+    st = ensure(tmp_b, nimg * esz);
+    bdst = tmp_b.p;
+  }
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, bdst, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
+  if (!st && maxit > 0)                                                        // iNUFT.m:127-136
+    st = minres_solve(p, bdst, nc, maxit, damp, wsamp, tol, xo.dev, info);
+  else if (!st && info) {
+    info->nrhs = nc;
+    for (int c = 0; c < nc && c < 64; ++c) { info->iters[c] = 0; info->relres[c] = NAN; }
+  }
+  if (!st) st = finish_out(p, im, nimg * esz, xo);
+  if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error in iNUFT");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  release(tmp_out);
+  release(tmp_b);
+  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+int nufft3d_normal(nufft3d_plan* p, const void* x, int32_t nc, double damp, void* y) {
+  N3D_PLAN_GUARD(p);
+  if (!x || !y || nc <= 0) { set_error("invalid x/y/nc"); return NUFFT3D_ERR_INVALID; }
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  Workspace tmp_out;
+  Staged xi, yo;
+  N3D_TRY(stage_in(p, x, nimg * esz, p->ws_img, xi));
+  int st = stage_out(p, y, nimg * esz, tmp_out, yo);
+  if (!st) st = apply_normal(p, xi.dev, nc, damp, p->d_dens, yo.dev);
+  if (!st) st = finish_out(p, y, nimg * esz, yo);
+  release(tmp_out);
+  if (xi.host) N3D_HIP(hipStreamSynchronize(p->stream));
+  return st;
+}
+
+int nufft3d_density(nufft3d_plan* p, void* d) {
+  N3D_PLAN_GUARD(p);
+  if (!d) { set_error("d is NULL"); return NUFFT3D_ERR_INVALID; }
+  const size_t esz = p->prec == 2 ? sizeof(double) : sizeof(float);
+  Workspace tmp_out;
+  Staged o;
+  N3D_TRY(stage_out(p, d, esz * p->M, tmp_out, o));
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_scatter_dens<double>), dim3(nblocks(p->M)), dim3(256), 0, p->stream,
+                       (const double*)p->d_dens, p->d_orig, p->d_oob, (long long)p->nnz,
+                       (long long)p->M, (double*)o.dev);
+  else
+    hipLaunchKernelGGL((k_scatter_dens<float>), dim3(nblocks(p->M)), dim3(256), 0, p->stream,
+                       (const float*)p->d_dens, p->d_orig, p->d_oob, (long long)p->nnz,
+                       (long long)p->M, (float*)o.dev);
+  N3D_HIP(hipGetLastError());
+  int st = finish_out(p, d, esz * p->M, o);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  release(tmp_out);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,348 @@
+// Normal operator and MINRES (replaces iprojection, nufft_3d.m:305-313, and
+// the MATLAB minres call of iNUFT.m:136).
+//
+// MINRES is Paige-Saunders with every scalar of the recurrence kept on the
+// device in double, one independent solve per coil (right-hand side), so an
+// iteration is a fixed launch sequence with no host round trip:
+//   y = A v                       (fwd FFT, interp*W, spread, adj FFT + damp^2 v)
+//   phase A : y -= (beta/oldb) r1, partial <v,y>        (fp64 block partials)
+//   scalar A: alfa = sum partials (fixed order)
+//   phase B : y -= (alfa/beta) r2, partial ||y||^2
+//   scalar B: beta, plane rotation, phi, relres, convergence
+//   phase C : w = (v - oldeps w1 - delta w2)/gamma; x += phi w; v = y/beta
+// Inner products and norms accumulate in double in a fixed order, so the
+// solve is bitwise reproducible.
+#include <cmath>
+#include <vector>
+
+#include "nufft3d_internal.h"
+
+namespace nufft3d {
+
+struct MrState {
+  double beta1, beta, oldb, alfa, dbar, epsln, oldeps, phibar, cs, sn, gamma, delta, phi, relres;
+  int active, upd, iters, pad;
+};
+
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// partial[c*nblk + b] = sum over block b of Re(conj(a) b) or |a|^2
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_dot_partial(const cplx<R>* __restrict__ a,
+                                                             const cplx<R>* __restrict__ b,
+                                                             long long n, double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  const int c = blockIdx.y;
+  const int nblk = gridDim.x;
+  const cplx<R>* pa = a + (long long)c * n;
+  const cplx<R>* pb = b + (long long)c * n;
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)nblk * blockDim.x) {
+    const cplx<R> x = pa[i], y = pb[i];
+    acc += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(long long)c * nblk + blockIdx.x] = s;
+}
+
+// out[c] = sum_b partial[c*nblk + b] in a fixed tree order
+__global__ void __launch_bounds__(kRedThreads) k_sum_partials(const double* __restrict__ partial,
+                                                              int nblk, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  const int c = blockIdx.x;
+  double acc = 0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc += partial[(long long)c * nblk + i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[c] = s;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_sum_real_partial(const R* __restrict__ v, long long n,
+                                                                  double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    acc += (double)v[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------- MINRES
+template <typename R>
+__global__ void k_mr_init(const cplx<R>* __restrict__ b, long long n, const double* __restrict__ bn2,
+                          MrState* __restrict__ st, cplx<R>* __restrict__ x, cplx<R>* __restrict__ r2,
+                          cplx<R>* __restrict__ v) {
+  const int c = blockIdx.y;
+  const double beta1 = sqrt(bn2[c]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    MrState s = {};
+    s.beta1 = beta1; s.beta = beta1; s.oldb = 0; s.dbar = 0; s.epsln = 0;
+    s.phibar = beta1; s.cs = -1; s.sn = 0; s.relres = beta1 > 0 ? 1.0 : 0.0;
+    s.active = beta1 > 0; s.upd = 0; s.iters = 0;
+    st[c] = s;
+  }
+  const R inv = beta1 > 0 ? (R)(1.0 / beta1) : (R)0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    const cplx<R> bv = b[o];
+    cplx<R> z; z.x = 0; z.y = 0;
+    x[o] = z;
+    r2[o] = bv;
+    cplx<R> vv; vv.x = bv.x * inv; vv.y = bv.y * inv;
+    v[o] = vv;
+  }
+}
+
+// y -= (beta/oldb) r1 (itn>=2); partial <v, y>
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_mr_phaseA(cplx<R>* __restrict__ y,
+                                                           const cplx<R>* __restrict__ r1,
+                                                           const cplx<R>* __restrict__ v, long long n,
+                                                           const MrState* __restrict__ st, int itn,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  const int c = blockIdx.y;
+  const int nblk = gridDim.x;
+  const MrState s = st[c];
+  const R coef = (itn >= 2 && s.oldb > 0) ? (R)(s.beta / s.oldb) : (R)0;
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)nblk * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    cplx<R> yv = y[o];
+    if (coef != (R)0) {
+      const cplx<R> r = r1[o];
+      yv.x -= coef * r.x;
+      yv.y -= coef * r.y;
+      y[o] = yv;
+    }
+    const cplx<R> vv = v[o];
+    acc += (double)vv.x * (double)yv.x + (double)vv.y * (double)yv.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(long long)c * nblk + blockIdx.x] = t;
+}
+
+__global__ void k_mr_scalarA(MrState* __restrict__ st, const double* __restrict__ sums) {
+  const int c = blockIdx.x;
+  if (threadIdx.x == 0) st[c].alfa = sums[c];
+}
+
+// y -= (alfa/beta) r2; partial ||y||^2
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_mr_phaseB(cplx<R>* __restrict__ y,
+                                                           const cplx<R>* __restrict__ r2, long long n,
+                                                           const MrState* __restrict__ st,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  const int c = blockIdx.y;
+  const int nblk = gridDim.x;
+  const MrState s = st[c];
+  const R coef = s.beta > 0 ? (R)(s.alfa / s.beta) : (R)0;
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)nblk * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    cplx<R> yv = y[o];
+    const cplx<R> r = r2[o];
+    yv.x -= coef * r.x;
+    yv.y -= coef * r.y;
+    y[o] = yv;
+    acc += (double)yv.x * (double)yv.x + (double)yv.y * (double)yv.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(long long)c * nblk + blockIdx.x] = t;
+}
+
+__global__ void k_mr_scalarB(MrState* __restrict__ st, const double* __restrict__ sums, double tol) {
+  const int c = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  MrState s = st[c];
+  s.upd = s.active;
+  if (!s.active) { st[c] = s; return; }
+  s.oldb = s.beta;
+  s.beta = sqrt(sums[c]);
+  s.oldeps = s.epsln;
+  s.delta = s.cs * s.dbar + s.sn * s.alfa;
+  const double gbar = s.sn * s.dbar - s.cs * s.alfa;
+  s.epsln = s.sn * s.beta;
+  s.dbar = -s.cs * s.beta;
+  double gamma = hypot(gbar, s.beta);
+  if (gamma < 2.220446049250313e-16) gamma = 2.220446049250313e-16;
+  s.gamma = gamma;
+  s.cs = gbar / gamma;
+  s.sn = s.beta / gamma;
+  s.phi = s.cs * s.phibar;
+  s.phibar = s.sn * s.phibar;
+  s.iters += 1;
+  s.relres = s.beta1 > 0 ? s.phibar / s.beta1 : 0.0;
+  s.active = s.relres > tol;
+  st[c] = s;
+}
+
+// w_new = (v - oldeps w1 - delta w2)/gamma, stored over the w1 buffer;
+// x += phi w_new; v <- y / beta
+template <typename R>
+__global__ void k_mr_phaseC(cplx<R>* __restrict__ v, cplx<R>* __restrict__ w1,
+                            const cplx<R>* __restrict__ w2, cplx<R>* __restrict__ x,
+                            const cplx<R>* __restrict__ y, long long n,
+                            const MrState* __restrict__ st) {
+  const int c = blockIdx.y;
+  const MrState s = st[c];
+  if (!s.upd) return;
+  const R oe = (R)s.oldeps, de = (R)s.delta, ig = (R)(1.0 / s.gamma), ph = (R)s.phi;
+  const R ib = s.beta > 0 ? (R)(1.0 / s.beta) : (R)0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    const cplx<R> vv = v[o], a = w1[o], b = w2[o];
+    cplx<R> w;
+    w.x = (vv.x - oe * a.x - de * b.x) * ig;
+    w.y = (vv.y - oe * a.y - de * b.y) * ig;
+    w1[o] = w;
+    cplx<R> xv = x[o];
+    xv.x += ph * w.x;
+    xv.y += ph * w.y;
+    x[o] = xv;
+    const cplx<R> yv = y[o];
+    cplx<R> nv; nv.x = yv.x * ib; nv.y = yv.y * ib;
+    v[o] = nv;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static int red_blocks(long long n) {
+  long long nb = (n + kRedThreads * 4 - 1) / (kRedThreads * 4);
+  if (nb > 512) nb = 512;
+  if (nb < 1) nb = 1;
+  return (int)nb;
+}
+
+int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out) {
+  const int nb = red_blocks(n);
+  N3D_TRY(ensure(p->ws_red, (size_t)(nb + 8) * sizeof(double)));
+  double* part = (double*)p->ws_red.p;
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_sum_real_partial<double>), dim3(nb), dim3(kRedThreads), 0, p->stream,
+                       (const double*)v, (long long)n, part);
+  else
+    hipLaunchKernelGGL((k_sum_real_partial<float>), dim3(nb), dim3(kRedThreads), 0, p->stream,
+                       (const float*)v, (long long)n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, p->stream, part, nb, part + nb);
+  N3D_HIP(hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  return NUFFT3D_OK;
+}
+
+// y = A'WA x + damp^2 x (iprojection with coils independent)
+int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void* wsamp, void* y) {
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  N3D_TRY(ensure(p->ws_grid, ngrid * esz));
+  N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz));
+  const double sc = 1.0 / sqrt((double)p->nnz);
+  N3D_TRY(fft_forward(p, x, nc, p->ws_grid.p));
+  N3D_TRY(interp(p, p->ws_grid.p, nc, p->ws_samp.p, 0, wsamp, sc));
+  N3D_TRY(spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr));
+  N3D_TRY(fft_adjoint(p, p->ws_grid.p, nc, y, sc, x, damp * damp));
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                    const void* wsamp, double tol, void* xout, nufft3d_info* info) {
+  using C = cplx<R>;
+  const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
+  const size_t vec = (size_t)n * nc * sizeof(C);
+  // vectors: v, r1, r2, y, w, w2 ; x is the caller's output
+  N3D_TRY(ensure(p->ws_solve, 6 * vec));
+  unsigned char* base = (unsigned char*)p->ws_solve.p;
+  C* v = (C*)(base + 0 * vec);
+  C* r1 = (C*)(base + 1 * vec);
+  C* r2 = (C*)(base + 2 * vec);
+  C* y = (C*)(base + 3 * vec);
+  C* w = (C*)(base + 4 * vec);
+  C* w2 = (C*)(base + 5 * vec);
+  C* x = (C*)xout;
+  const int nb = red_blocks(n);
+  N3D_TRY(ensure(p->ws_red, (size_t)nc * (nb + 1) * sizeof(double) + nc * sizeof(MrState) + 64));
+  double* part = (double*)p->ws_red.p;
+  double* sums = part + (size_t)nc * nb;
+  MrState* st = (MrState*)(sums + nc);
+  const dim3 grid(nb, nc);
+
+  hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b,
+                     (const C*)b, n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+  hipLaunchKernelGGL((k_mr_init<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b, n, sums,
+                     st, x, r2, v);
+  N3D_HIP(hipMemsetAsync(w, 0, vec, p->stream));
+  N3D_HIP(hipMemsetAsync(w2, 0, vec, p->stream));
+  for (int itn = 1; itn <= maxit; ++itn) {
+    N3D_TRY(apply_normal(p, v, nc, damp, wsamp, y));
+    hipLaunchKernelGGL((k_mr_phaseA<R>), grid, dim3(kRedThreads), 0, p->stream, y, (const C*)r1,
+                       (const C*)v, n, (const MrState*)st, itn, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    hipLaunchKernelGGL(k_mr_scalarA, dim3(nc), dim3(1), 0, p->stream, st, (const double*)sums);
+    hipLaunchKernelGGL((k_mr_phaseB<R>), grid, dim3(kRedThreads), 0, p->stream, y, (const C*)r2, n,
+                       (const MrState*)st, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    hipLaunchKernelGGL(k_mr_scalarB, dim3(nc), dim3(1), 0, p->stream, st, (const double*)sums, tol);
+    // w_new = (v - oldeps*w2 - delta*w)/gamma is written over w2, whose old
+    // value (the next iteration's w1) is dead after this update
+    hipLaunchKernelGGL((k_mr_phaseC<R>), grid, dim3(kRedThreads), 0, p->stream, v, w2,
+                       (const C*)w, x, (const C*)y, n, (const MrState*)st);
+    {
+      C* t = w2;
+      w2 = w;
+      w = t;
+    }
+    C* t = r1;
+    r1 = r2;
+    r2 = y;
+    y = t;
+    N3D_HIP(hipGetLastError());
+    if (itn % 10 == 0 && itn < maxit) {   // early exit once every column stopped
+      std::vector<MrState> hs(nc);
+      N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(MrState), hipMemcpyDeviceToHost, p->stream));
+      N3D_HIP(hipStreamSynchronize(p->stream));
+      bool any = false;
+      for (int c = 0; c < nc; ++c) any = any || hs[c].active;
+      if (!any) break;
+    }
+  }
+  std::vector<MrState> hs(nc);
+  N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(MrState), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  if (info) {
+    info->nrhs = nc;
+    for (int c = 0; c < nc && c < 64; ++c) {
+      info->iters[c] = hs[c].iters;
+      info->relres[c] = hs[c].relres;
+    }
+  }
+  return NUFFT3D_OK;
+}
+
+int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, const void* wsamp,
+                 double tol, void* x, nufft3d_info* info) {
+  return p->prec == 2 ? minres_t<double>(p, b, nc, maxit, damp, wsamp, tol, x, info)
+                      : minres_t<float>(p, b, nc, maxit, damp, wsamp, tol, x, info);
+}
+
+}  // namespace nufft3d

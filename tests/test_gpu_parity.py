@@ -1,0 +1,186 @@
+"""HIP path vs the FP64 oracle (GPU): plan, deap, density, fNUFT, aNUFT, iNUFT.
+
+Tolerances (relative L2): FP32 plans 1e-4 end to end (the stated accuracy
+gate) and 2e-5 for single operators; FP64 plans 1e-12 (oracle mode).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nufft_3d_amd import nufft_3d, synth
+from oracle import nufft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL32_OP = 2e-5
+TOL32 = 1e-4
+TOL64 = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _rand(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.normal(size=shape) + 1j * r.normal(size=shape)
+
+
+CASES = {
+    # name: (om builder, N, kwargs)
+    "radial_J4": (lambda: synth.koosh_ball(16, 60, 32), 16, dict(J=4, u=2, radial=1)),
+    "separable_J4": (lambda: synth.koosh_ball(16, 60, 32), 16, dict(J=4, u=2, radial=0)),
+    "radial_J6_u125": (lambda: synth.golden_means(16, 80, 32), 16, dict(J=6, u=1.25, radial=1)),
+    "random_oob": (lambda: np.random.default_rng(9).uniform(-9.5, 9, size=(3, 3000)), (16, 12, 20),
+                   dict(J=4, u=2, radial=1)),
+    "J5_u15_sep": (lambda: np.random.default_rng(3).uniform(-6, 6, size=(3, 1500)), 12,
+                   dict(J=5, u=1.5, radial=0)),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request):
+    build, N, kw = CASES[request.param]
+    om = build()
+    orc = O.NufftOracle(om, N, **kw)
+    g32 = nufft_3d(om, N, gpu=1, **kw)
+    g64 = nufft_3d(om, N, gpu=2, **kw)
+    print("case", request.param, "M", om.shape[1], "nnz", orc.nnz, "K", orc.K, flush=True)
+    return request.param, om, orc, g32, g64
+
+
+def test_plan_parameters(case):
+    name, om, orc, g32, g64 = case
+    for g in (g32, g64):
+        assert g.K == orc.K
+        assert g.N == orc.N
+        assert g.nnz == orc.nnz
+        assert g.alpha == pytest.approx(orc.alpha, rel=1e-15)
+
+
+def test_deap(case):
+    name, om, orc, g32, g64 = case
+    assert rel(g64.U, orc.U) < 1e-13
+    assert rel(g32.U, orc.U) < 1e-13
+
+
+def test_density(case):
+    name, om, orc, g32, g64 = case
+    assert rel(g64.d, orc.d) < TOL64
+    assert rel(g32.d, orc.d) < TOL32
+    assert np.all(g32.d[~orc.ok] == 0)
+    assert float(np.sum(g64.d)) == pytest.approx(orc.nnz, rel=1e-12)
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+def test_forward(case, nc):
+    name, om, orc, g32, g64 = case
+    x = _rand(orc.N + (nc,), 10 + nc)
+    want = orc.fNUFT(x)
+    assert rel(g64.fNUFT(x), want) < TOL64
+    assert rel(g32.fNUFT(x), want) < TOL32_OP
+    k = g32.fNUFT(x)
+    assert np.all(k[~orc.ok] == 0)
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+def test_adjoint(case, nc):
+    name, om, orc, g32, g64 = case
+    k = _rand((orc.M, nc), 20 + nc)
+    want = orc.aNUFT(k)
+    assert rel(g64.aNUFT(k), want) < TOL64
+    assert rel(g32.aNUFT(k), want) < TOL32_OP
+
+
+def test_inverse(case):
+    name, om, orc, g32, g64 = case
+    x = _rand(orc.N + (2,), 31)
+    raw = orc.fNUFT(x) + 0.01 * _rand((orc.M, 2), 32)
+    for maxit, damp in ((0, 0.0), (1, 0.0), (8, 0.05)):
+        want, winfo = orc.iNUFT(raw, maxit, damp, return_info=True)
+        got64, info64 = g64.iNUFT(raw, maxit, damp, return_info=True)
+        got32 = g32.iNUFT(raw, maxit, damp)
+        assert rel(got64, want) < 1e-10, (maxit, damp)
+        assert rel(got32, want) < TOL32, (maxit, damp)
+        if maxit > 0:
+            assert info64["iters"] == winfo["iters"]
+
+
+def test_operator_adjointness_fp64(case):
+    name, om, orc, g32, g64 = case
+    x = _rand(orc.N + (1,), 41)
+    k = _rand((orc.M, 1), 42)
+    lhs = np.vdot(g64.fNUFT(x), k)
+    rhs = np.vdot(x, g64.aNUFT(k))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_spread_is_deterministic_and_batch_independent(case):
+    name, om, orc, g32, g64 = case
+    k = _rand((orc.M, 5), 51).astype(np.complex64)
+    a = g32.aNUFT(k)
+    b = g32.aNUFT(k)
+    assert np.array_equal(a, b)
+    for c in range(5):
+        alone = g32.aNUFT(np.ascontiguousarray(k[:, c]))
+        assert np.array_equal(alone[..., 0], a[..., c])
+    f = g32.fNUFT(a)
+    f1 = g32.fNUFT(np.asfortranarray(a[..., 2]))
+    assert np.array_equal(f[:, 2], f1[:, 0])
+
+
+def test_many_coils_beyond_one_group():
+    # 20 coils: two coil groups in interp/spread launches
+    om = synth.koosh_ball(12, 40, 24)
+    orc = O.NufftOracle(om, 12)
+    g = nufft_3d(om, 12, gpu=2)
+    x = _rand((12, 12, 12, 20), 61)
+    assert rel(g.fNUFT(x), orc.fNUFT(x)) < TOL64
+    k = _rand((orc.M, 20), 62)
+    assert rel(g.aNUFT(k), orc.aNUFT(k)) < TOL64
+
+
+def test_user_density_and_weights():
+    om = synth.koosh_ball(12, 40, 24)
+    d = np.random.default_rng(4).uniform(0.5, 1.5, size=om.shape[1])
+    orc = O.NufftOracle(om, 12, d=d)
+    g = nufft_3d(om, 12, gpu=2, d=d)
+    assert rel(g.d, orc.d) < 1e-15
+    raw = _rand((orc.M, 1), 71)
+    W = np.random.default_rng(5).uniform(0.1, 1, size=40)   # one weight per spoke (ny)
+    want = orc.iNUFT(raw, 5, 0.0, W=W)
+    got = g.iNUFT(raw, 5, 0.0, W=W)
+    assert rel(got, want) < 1e-10
+
+
+def test_torch_device_arrays_match_host_path():
+    import torch
+    om = synth.koosh_ball(16, 50, 32)
+    g = nufft_3d(om, 16, gpu=1)
+    x = _rand((16, 16, 16, 2), 81).astype(np.complex64)
+    k_host = g.fNUFT(x)
+    xt = torch.from_numpy(np.ascontiguousarray(np.transpose(x, (3, 2, 1, 0)))).cuda()
+    kt = g.fNUFT(xt)
+    torch.cuda.synchronize()
+    assert np.array_equal(kt.cpu().numpy().T, k_host)
+    at = g.aNUFT(kt)
+    torch.cuda.synchronize()
+    a_host = g.aNUFT(k_host)
+    assert np.array_equal(np.transpose(at.cpu().numpy(), (3, 2, 1, 0)), a_host)
+
+
+def test_2d_trajectory():
+    # all om(3,:)==0 selects the 2-D path (nufft_3d.m:104-106), K3=1
+    r = np.random.default_rng(6)
+    om = np.vstack([r.uniform(-8, 8, size=(2, 800)), np.zeros((1, 800))])
+    orc = O.NufftOracle(om, 16)
+    g = nufft_3d(om, 16, gpu=2)
+    assert g.N == (16, 16, 1) and g.K == (32, 32, 1)
+    x = _rand((16, 16, 1, 2), 91)
+    assert rel(g.fNUFT(x), orc.fNUFT(x)) < TOL64
+    k = _rand((orc.M, 2), 92)
+    assert rel(g.aNUFT(k), orc.aNUFT(k)) < TOL64
+    assert rel(g.d, orc.d) < TOL64

@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--chunk", type=int, default=0, help="spread slice length (0 = library default)")
+    ap.add_argument("--tile", type=int, default=0, help="spread tile edge (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the NUFFT on a single MI355X "
@@ -118,7 +120,7 @@ def main():
     M = om.shape[1]
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    plan = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1)
+    plan = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1, chunk=args.chunk, tile=args.tile)
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
     log("plan %.3f s  K=%s nnz=%d" % (plan_s, plan.K, plan.nnz))

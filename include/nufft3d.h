@@ -49,6 +49,8 @@ typedef struct nufft3d_opts {
   int32_t chunk;     /* max samples per spread work chunk (0 = auto) */
   int32_t device;    /* HIP device ordinal */
   int32_t verbose;   /* print the reference's plan report to stdout */
+  int32_t compensated; /* fp32 plans: 1 = compensated (TwoSum/TwoProduct) gridding
+                          accumulation (default), 0 = plain fp32 sums */
 } nufft3d_opts;
 
 /* Per-right-hand-side result of nufft3d_inverse (MATLAB minres flag/relres). */
@@ -103,6 +105,43 @@ int nufft3d_get_params(const nufft3d_plan* plan, int32_t K[3], int32_t N[3], dou
 
 /* Bind subsequent work of this plan to a HIP stream (hipStream_t as void*). */
 int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
+
+/* Options of pruno(obj,data,varargin) (pruno.m:26-41). */
+typedef struct nufft3d_pruno_opts {
+  int32_t width;           /* kernel width, default 5 (pruno.m:27) */
+  int32_t radial;          /* 1 = spherical kernel support (pruno.m:28) */
+  double noise;            /* noise std; < 0 = estimate (pruno.m:29,97-107) */
+  double tol;              /* singular value filter threshold, 0.01 (pruno.m:30) */
+  int32_t literal_first_nv;/* 1 = the reference's selection of the first nv
+                              (largest) vectors (pruno.m:128-130); 0 = the nv
+                              null-space (smallest) vectors */
+} nufft3d_pruno_opts;
+
+int nufft3d_pruno_opts_default(nufft3d_pruno_opts* opts);
+
+/* AA = make_data_matrix(data,opts) (pruno.m:146-179, sake.m:176-208):
+ * Hankel Gram of k-space data [dims0 dims1 dims2 dims3=nc] (complex64 for a
+ * precision-1 plan, complex128 for precision 2) with a width/radial kernel;
+ * AA is complex128 [n n] column-major, n = nc*nk. FP32 plans use split-BF16
+ * MFMA with FP32 accumulation; FP64 plans compute in double. */
+int nufft3d_gram(nufft3d_plan* plan, const void* data, const int32_t dims[4], int32_t width,
+                 int32_t radial, void* AA, int32_t* n);
+
+/* K = pruno(obj,data) (pruno.m:2-143) on k-space data [N1 N2 N3 nc] of the
+ * plan's image size: returns nv, the singular values sv [n] (descending,
+ * may be NULL) and the noise std used. */
+int nufft3d_pruno(nufft3d_plan* plan, const void* data, int32_t nc,
+                  const nufft3d_pruno_opts* opts, int32_t* nv, double* sv, double* noise);
+
+/* iNUFT(raw,maxit,damp,W,'parallel-imaging-pruno',lambda) (iNUFT.m:200-218):
+ * MINRES (maxit) or regridding (maxit=0) for calibration, k-space by an
+ * unshifted FFT, pruno null kernels scaled by lambda, then `iters` coupled
+ * MINRES iterations of A'WA + damp^2 I + sum_v conj(fnull) fnull (the
+ * reference passes an undefined preconditioner M, iNUFT.m:218; none is used). */
+int nufft3d_inverse_pruno(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t maxit,
+                          double damp, const double* W, int64_t nW, double lambda,
+                          const nufft3d_pruno_opts* opts, int32_t iters, void* im,
+                          nufft3d_info* info);
 
 /* Measurement hook, no counterpart in the reference (which only prints
  * tic/toc wall times, nufft_3d.m:130,213): while enabled, every kernel launch

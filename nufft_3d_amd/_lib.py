@@ -23,7 +23,12 @@ class Opts(C.Structure):
                 ("radial", C.c_int32), ("precision", C.c_int32),
                 ("d", C.POINTER(C.c_double)), ("nd", C.c_int64),
                 ("tile", C.c_int32), ("chunk", C.c_int32), ("device", C.c_int32),
-                ("verbose", C.c_int32)]
+                ("verbose", C.c_int32), ("compensated", C.c_int32)]
+
+
+class PrunoOpts(C.Structure):
+    _fields_ = [("width", C.c_int32), ("radial", C.c_int32), ("noise", C.c_double),
+                ("tol", C.c_double), ("literal_first_nv", C.c_int32)]
 
 
 class Info(C.Structure):
@@ -53,6 +58,14 @@ SIGNATURES = {
     "nufft3d_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nufft3d_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nufft3d_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "nufft3d_pruno_opts_default": (C.c_int, [C.POINTER(PrunoOpts)]),
+    "nufft3d_gram": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
+                               C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "nufft3d_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(PrunoOpts),
+                                C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_double)]),
+    "nufft3d_inverse_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                        C.c_void_p, C.c_int64, C.c_double, C.POINTER(PrunoOpts),
+                                        C.c_int32, C.c_void_p, C.POINTER(Info)]),
     "nufft3d_last_error": (C.c_char_p, []),
 }
 
@@ -60,10 +73,19 @@ _lib = None
 
 
 def load():
-    """Load the HIP library (once). Raises if it has not been built."""
+    """Load the HIP library (once). Raises if it has not been built.
+
+    torch (when installed) is imported first so that one HIP runtime serves
+    the process: the library's libamdhip64 dependency then binds to the copy
+    torch already loaded, and torch tensors and streams can be passed in.
+    """
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIBPATH):
         raise RuntimeError(
             "libnufft3d.so is not built (expected at %s); run "

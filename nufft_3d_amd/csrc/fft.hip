@@ -264,7 +264,8 @@ __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ i
 // (outer, bundle of B consecutive inner columns).
 struct SGeom {
   int Lin, Lout, n_img;   // n_img: image size on this axis (pad/crop map)
-  int mode;               // 0: pad (Lin = n_img, Lout = L), 1: crop (Lin = L, Lout = n_img)
+  int mode;               // 0: pad (Lin = n_img, Lout = L), 1: crop (Lin = L, Lout = n_img),
+                          // 2: plain (Lin = Lout = L, no index map)
   int B;
   long long inner;
   long long nbundle;      // ceil(inner / B)
@@ -482,6 +483,62 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   }
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
+}
+
+// Unshifted 3-D DFT of an image-size array [nc][N3][N2][N1] (out of place,
+// sign -1), as fft(fft(fft(x,[],1),[],2),[],3) of iNUFT.m:204.
+template <typename R>
+static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
+  const size_t esz = sizeof(cplx<R>);
+  const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
+  for (int a = 0; a < 3; ++a)
+    if (p->N[a] > 1 && !p->axN[a].d_tw) {
+      const int keep = p->K[a];
+      p->K[a] = p->N[a];
+      FftAxis save = p->ax[a];
+      p->ax[a] = FftAxis();
+      const int st = fft_setup_axis(p, a);
+      p->axN[a] = p->ax[a];
+      p->ax[a] = save;
+      p->K[a] = keep;
+      if (st) return st;
+    }
+  const size_t total = (size_t)N1 * N2 * N3 * nc;
+  N3D_TRY(ensure(p->ws_t1, total * esz));
+  const void* src = in;
+  void* bufs[2] = {p->ws_t1.p, out};
+  int which = 0;
+  // after three passes the result must land in out: order the ping-pong so
+  const int npass = (N1 > 1) + (N2 > 1) + (N3 > 1);
+  which = npass % 2 == 1 ? 1 : 0;
+  struct AxisPass { int n; long long outer, inner; int a; } passes[3] = {
+      {N1, (long long)nc * N3 * N2, 1LL, 0},
+      {N2, (long long)nc * N3, (long long)N1, 1},
+      {N3, (long long)nc, (long long)N1 * N2, 2}};
+  for (auto& ps : passes) {
+    if (ps.n <= 1) continue;
+    SGeom g;
+    g.Lin = ps.n; g.Lout = ps.n; g.n_img = ps.n; g.mode = 2;
+    g.B = ps.inner >= 16 ? bundle_for(ps.n, esz) : 1;
+    g.inner = ps.inner;
+    g.nbundle = (g.inner + g.B - 1) / g.B;
+    const size_t sh = 2 * (size_t)ps.n * g.B * esz;
+    void* dst = bufs[which];
+    ProfScope prs(p, "k_fft_plain");
+    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * ps.outer)), dim3(256), sh,
+                       p->stream, (const cplx<R>*)src, (cplx<R>*)dst, g, stages_of(p->axN[ps.a]),
+                       (const cplx<R>*)p->axN[ps.a].d_tw, -1);
+    src = dst;
+    which ^= 1;
+  }
+  if (npass == 0)
+    N3D_HIP(hipMemcpyAsync(out, in, total * esz, hipMemcpyDeviceToDevice, p->stream));
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out) {
+  return p->prec == 2 ? fft_plain_t<double>(p, in, nc, out) : fft_plain_t<float>(p, in, nc, out);
 }
 
 int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid) {

@@ -1,0 +1,670 @@
+// Hankel (calibration) Gram of SAKE/PRUNO and the PRUNO null operator.
+//
+// Reference: make_data_matrix (pruno.m:146-179, sake.m:176-224):
+//   AA(c,j,c',k) = sum_v conj(row_j[v,c]) * col_k[v,c'],
+//   row_j = circshift(data, s_j), col_k = circshift(data, s_k),
+//   reshaped to an (nc*nk) x (nc*nk) matrix with the coil index fastest.
+// circshift by s gives out[v] = data[(v - s) mod N] per axis.
+//
+// As a GEMM: X = [V x 2n] real, X[v, p*n + c + nc*j] = Re/Im data_c[v - s_j],
+// C = X^T X (2n x 2n, symmetric), AA = (C11 + C22) + i (C12 - C21).
+// fp32 plans: split-bf16 (hi/lo) MFMA with fp32 accumulation,
+//   x*y ~ xh*yh + xh*yl + xl*yh (the lo*lo term, ~2^-16 relative, dropped);
+//   v_mfma_f32_32x32x16_bf16, 128x128 block tiles, upper tiles only, V split
+//   across work-groups; partial tiles summed in a fixed order in double.
+// fp64 plans: the same Gram in double with plain FMAs (oracle mode).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nufft3d_internal.h"
+
+namespace nufft3d {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct GramGeom {
+  int N1, N2, N3;
+  long long V;
+  int nc, nk, n, n2;      // n = nc*nk, n2 = 2n
+  int ntr;                // tile rows (= tile cols) of C
+  int nsplit;             // V chunks
+  long long vchunk;       // voxels per chunk (multiple of 32)
+};
+
+constexpr int GT = 128;   // block tile edge
+constexpr int GK = 32;    // voxels per K step
+constexpr int GPAD = 8;   // bf16 padding per LDS row
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  // round to nearest even (values are finite)
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f(unsigned short b) {
+  return __uint_as_float(((unsigned)b) << 16);
+}
+
+// value of X[v, col] (v given as x,y,z)
+__device__ __forceinline__ float xval(const float2* __restrict__ data, const int4* __restrict__ sh,
+                                      const GramGeom& g, int col, int x, int y, int z) {
+  if (col >= g.n2) return 0.0f;
+  const int p = col >= g.n;
+  const int a = p ? col - g.n : col;
+  const int c = a % g.nc, j = a / g.nc;
+  const int4 s = sh[j];
+  int xs = x - s.x, ys = y - s.y, zs = z - s.z;
+  xs = xs < 0 ? xs + g.N1 : (xs >= g.N1 ? xs - g.N1 : xs);
+  ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
+  zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
+  const float2 d = data[(long long)c * g.V + xs + (long long)g.N1 * (ys + (long long)g.N2 * zs)];
+  return p ? d.y : d.x;
+}
+
+// grid: x = upper tile pair, y = V chunk. 256 threads = 4 waves (2x2 of 64x64).
+__global__ void __launch_bounds__(256) k_gram_bf16x3(const float2* __restrict__ data,
+                                                     const int4* __restrict__ shifts,
+                                                     const int2* __restrict__ tiles,
+                                                     float* __restrict__ partial, GramGeom g) {
+  __shared__ unsigned short sAh[GT][GK + GPAD], sAl[GT][GK + GPAD];
+  __shared__ unsigned short sBh[GT][GK + GPAD], sBl[GT][GK + GPAD];
+  const int2 tp = tiles[blockIdx.x];
+  const int I0 = tp.x * GT, J0 = tp.y * GT;
+  const long long vbeg = (long long)blockIdx.y * g.vchunk;
+  const long long vend = min(vbeg + g.vchunk, g.V);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  const int vl = tid & (GK - 1);      // this thread's voxel within a K step
+  const int cl0 = tid >> 5;           // first column it loads (stride 8)
+
+  for (long long v0 = vbeg; v0 < vend; v0 += GK) {
+    const long long v = v0 + vl;
+    const bool vin = v < vend;
+    int x = 0, y = 0, z = 0;
+    if (vin) {
+      x = (int)(v % g.N1);
+      const long long r = v / g.N1;
+      y = (int)(r % g.N2);
+      z = (int)(r / g.N2);
+    }
+    __syncthreads();
+    for (int cl = cl0; cl < GT; cl += 8) {
+      const float a = vin ? xval(data, shifts, g, I0 + cl, x, y, z) : 0.0f;
+      const unsigned short ah = bf16_bits(a);
+      sAh[cl][vl] = ah;
+      sAl[cl][vl] = bf16_bits(a - bf16_to_f(ah));
+      const float b = vin ? xval(data, shifts, g, J0 + cl, x, y, z) : 0.0f;
+      const unsigned short bh = bf16_bits(b);
+      sBh[cl][vl] = bh;
+      sBl[cl][vl] = bf16_bits(b - bf16_to_f(bh));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK / 16; ++kk) {
+      const int koff = kk * 16 + 8 * (lane >> 5);
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        const int row = wm * 64 + mb * 32 + (lane & 31);
+        ah[mb] = *reinterpret_cast<const bf16x8*>(&sAh[row][koff]);
+        al[mb] = *reinterpret_cast<const bf16x8*>(&sAl[row][koff]);
+      }
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        const int col = wn * 64 + nb * 32 + (lane & 31);
+        bh[nb] = *reinterpret_cast<const bf16x8*>(&sBh[col][koff]);
+        bl[nb] = *reinterpret_cast<const bf16x8*>(&sBl[col][koff]);
+      }
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mb], bh[nb], acc[mb][nb], 0, 0, 0);
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bl[nb], acc[mb][nb], 0, 0, 0);
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bh[nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+  }
+  // partial[(chunk*ntiles + tile)][128][128], row i (X_I column), col j
+  float* out = partial + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * (GT * GT);
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = wm * 64 + mb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int j = wn * 64 + nb * 32 + (lane & 31);
+        out[i * GT + j] = acc[mb][nb][r];
+      }
+}
+
+// fetch C[r][c] = sum over chunks of the partial tiles (fixed order, double)
+__device__ __forceinline__ double cfetch(const float* __restrict__ partial, const int* __restrict__ tid_of,
+                                         int ntiles, int nsplit, int ntr, int r, int c) {
+  int tr = r / GT, tc = c / GT, lr = r % GT, lc = c % GT;
+  if (tr > tc) { int t = tr; tr = tc; tc = t; t = lr; lr = lc; lc = t; }
+  const int t = tid_of[tr * ntr + tc];
+  double s = 0;
+  for (int k = 0; k < nsplit; ++k) s += (double)partial[((long long)k * ntiles + t) * (GT * GT) + lr * GT + lc];
+  return s;
+}
+
+// AA (column-major complex double n x n) from C
+__global__ void k_gram_assemble(const float* __restrict__ partial, const int* __restrict__ tid_of,
+                                int ntiles, GramGeom g, double2* __restrict__ AA) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)g.n * g.n) return;
+  const int a = (int)(idx % g.n), b = (int)(idx / g.n);
+  const int n = g.n;
+  const double c11 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, a, b);
+  const double c22 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, n + a, n + b);
+  const double c12 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, a, n + b);
+  const double c21 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, n + a, b);
+  double2 o;
+  o.x = c11 + c22;
+  o.y = c12 - c21;
+  AA[idx] = o;
+}
+
+// fp64 Gram: one thread per (a,b) upper entry, complex double directly
+__global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ data,
+                                                   const int4* __restrict__ shifts, GramGeom g,
+                                                   double2* __restrict__ AA) {
+  __shared__ double2 sa[16][64], sb[16][64];
+  const int ta = blockIdx.x * 16, tb = blockIdx.y * 16;
+  if (ta > tb + 15) return;   // strictly below the diagonal band: filled from the upper part
+  const int la = threadIdx.x % 16, lb = threadIdx.x / 16;
+  const int a = ta + la, b = tb + lb;
+  double re = 0, im = 0;
+  for (long long v0 = 0; v0 < g.V; v0 += 64) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * 16 * 64; e += 256) {
+      const int which = e / (16 * 64), rem = e % (16 * 64);
+      const int col = rem / 64, vl = rem % 64;
+      const long long v = v0 + vl;
+      double2 val; val.x = 0; val.y = 0;
+      const int aa = (which ? tb : ta) + col;
+      if (v < g.V && aa < g.n) {
+        const int c = aa % g.nc, j = aa / g.nc;
+        const int4 s = shifts[j];
+        int x = (int)(v % g.N1);
+        const long long r = v / g.N1;
+        int y = (int)(r % g.N2), z = (int)(r / g.N2);
+        x -= s.x; y -= s.y; z -= s.z;
+        x = x < 0 ? x + g.N1 : (x >= g.N1 ? x - g.N1 : x);
+        y = y < 0 ? y + g.N2 : (y >= g.N2 ? y - g.N2 : y);
+        z = z < 0 ? z + g.N3 : (z >= g.N3 ? z - g.N3 : z);
+        val = data[(long long)c * g.V + x + (long long)g.N1 * (y + (long long)g.N2 * z)];
+      }
+      if (which) sb[col][vl] = val; else sa[col][vl] = val;
+    }
+    __syncthreads();
+    const int nv = (int)min((long long)64, g.V - v0);
+    for (int k = 0; k < nv; ++k) {
+      const double2 p = sa[la][k], q = sb[lb][k];
+      re += p.x * q.x + p.y * q.y;
+      im += p.x * q.y - p.y * q.x;
+    }
+  }
+  if (a < g.n && b < g.n && a <= b) {
+    double2 o; o.x = re; o.y = im;
+    AA[a + (long long)g.n * b] = o;
+    if (a != b) { o.y = -im; AA[b + (long long)g.n * a] = o; }
+  }
+}
+
+// ---------------------------------------------------------------- null operator
+// G[d][c][c'] = sum over tap pairs (t, t') with t' - t = d of P[(t,c),(t',c')],
+// P = conj(Kt) Kt^T = conj(Q) with Q = Kt Kt^H; d indexes the (2w-1)^3 offset
+// cube. Reads Q and conjugates.
+__global__ void k_null_G(const double2* __restrict__ Q, const int4* __restrict__ taps, int nk,
+                         int nc, int w, double2* __restrict__ G) {
+  const int D = 2 * w - 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)D * D * D * nc * nc;
+  if (idx >= total) return;
+  const int cc = (int)(idx % (nc * nc));
+  const int d = (int)(idx / (nc * nc));
+  const int c = cc % nc, c2 = cc / nc;
+  const int dx = d % D - (w - 1), dy = (d / D) % D - (w - 1), dz = d / (D * D) - (w - 1);
+  const int n = nk * nc;
+  double re = 0, im = 0;
+  for (int t = 0; t < nk; ++t) {
+    const int4 a = taps[t];
+    for (int t2 = 0; t2 < nk; ++t2) {
+      const int4 b = taps[t2];
+      if (b.x - a.x != dx || b.y - a.y != dy || b.z - a.z != dz) continue;
+      const double2 v = Q[(c + nc * t) + (long long)n * (c2 + nc * t2)];
+      re += v.x;
+      im -= v.y;
+    }
+  }
+  double2 o; o.x = re; o.y = im;
+  G[idx] = o;
+}
+
+// One separable stage of B(x) = sum_d G(d) e^{+2 pi i d.x / N}:
+// in [outer][D][inner] -> out [outer][Nax][inner] with inner contiguous.
+__global__ void k_null_axis(const double2* __restrict__ in, double2* __restrict__ out, int D, int w,
+                            int Nax, long long inner, long long outer) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= outer * Nax * inner) return;
+  const long long q = idx % inner;
+  const int xx = (int)((idx / inner) % Nax);
+  const long long o = idx / (inner * Nax);
+  double re = 0, im = 0;
+  for (int d = 0; d < D; ++d) {
+    const int dd = d - (w - 1);
+    long long ph = ((long long)dd * xx) % Nax;
+    if (ph < 0) ph += Nax;
+    const double ang = 2.0 * M_PI * (double)ph / (double)Nax;
+    const double cs = cos(ang), sn = sin(ang);
+    const double2 v = in[(o * D + d) * inner + q];
+    re += v.x * cs - v.y * sn;
+    im += v.x * sn + v.y * cs;
+  }
+  double2 r; r.x = re; r.y = im;
+  out[(o * Nax + xx) * inner + q] = r;
+}
+
+// B stored per voxel [V][nc][nc] in the plan precision, scaled by lambda^2
+template <typename R>
+__global__ void k_null_store(const double2* __restrict__ Bd, long long V, int nc, double lam2,
+                             cplx<R>* __restrict__ B) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= V * nc * nc) return;
+  // Bd layout after the three stages: [z][y][x][cc] with x fastest of voxels
+  const double2 v = Bd[idx];
+  cplx<R> o; o.x = (R)(lam2 * v.x); o.y = (R)(lam2 * v.y);
+  B[idx] = o;
+}
+
+// y[c][v] += sum_c' B[v][c][c'] x[c'][v]  (B[v][c + nc*c'])
+template <typename R>
+__global__ void k_null_apply(const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ x,
+                             cplx<R>* __restrict__ y, long long V, int nc) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const cplx<R>* b = B + v * nc * nc;
+  for (int c = 0; c < nc; ++c) {
+    R re = 0, im = 0;
+    for (int c2 = 0; c2 < nc; ++c2) {
+      const cplx<R> m = b[c + nc * c2];
+      const cplx<R> xv = x[(long long)c2 * V + v];
+      re += m.x * xv.x - m.y * xv.y;
+      im += m.x * xv.y + m.y * xv.x;
+    }
+    cplx<R> yv = y[(long long)c * V + v];
+    yv.x += re;
+    yv.y += im;
+    y[(long long)c * V + v] = yv;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+// Kernel tap offsets of pruno.m:56-71 / sake.m:61-71 in column-major order
+void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int& w) {
+  const int h = width / 2;   // fix(width/2)
+  w = 2 * h + 1;
+  taps.clear();
+  for (int iz = 0; iz < (is2d ? 1 : w); ++iz)
+    for (int iy = 0; iy < w; ++iy)
+      for (int ix = 0; ix < w; ++ix) {
+        const int x = ix - h, y = iy - h, z = is2d ? 0 : iz - h;
+        bool keep;
+        if (radial) keep = std::sqrt((double)(x * x + y * y + z * z)) <= width / 2.0;
+        else keep = std::abs(x) <= width / 2.0 && std::abs(y) <= width / 2.0 && std::abs(z) <= width / 2.0;
+        if (keep) taps.push_back(make_int4(x, y, z, ix + w * (iy + w * iz)));
+      }
+}
+
+// AA = Hankel Gram of data [N1 N2 N3 nc] (device, plan precision) -> dAA
+// (device complex double, column-major n x n)
+int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
+                double2* dAA) {
+  GramGeom g;
+  g.N1 = dims[0]; g.N2 = dims[1]; g.N3 = dims[2];
+  g.V = (long long)g.N1 * g.N2 * g.N3;
+  g.nc = dims[3];
+  g.nk = (int)taps.size();
+  g.n = g.nc * g.nk;
+  g.n2 = 2 * g.n;
+  int4* dtaps = nullptr;
+  N3D_HIP(hipMalloc(&dtaps, sizeof(int4) * taps.size()));
+  N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
+  int st = NUFFT3D_OK;
+  if (p->prec == 2) {
+    dim3 grid((g.n + 15) / 16, (g.n + 15) / 16);
+    ProfScope ps(p, "k_gram_fp64");
+    hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
+                       (const int4*)dtaps, g, dAA);
+  } else {
+    g.ntr = (g.n2 + GT - 1) / GT;
+    std::vector<int2> tl;
+    std::vector<int> tid_of(g.ntr * g.ntr, -1);
+    for (int i = 0; i < g.ntr; ++i)
+      for (int j = i; j < g.ntr; ++j) { tid_of[i * g.ntr + j] = (int)tl.size(); tl.push_back(make_int2(i, j)); }
+    const int ntiles = (int)tl.size();
+    int nsplit = (2048 + ntiles - 1) / ntiles;
+    long long vchunk = (g.V + nsplit - 1) / nsplit;
+    vchunk = ((vchunk + GK - 1) / GK) * GK;
+    nsplit = (int)((g.V + vchunk - 1) / vchunk);
+    g.nsplit = nsplit;
+    g.vchunk = vchunk;
+    int2* dtl = nullptr;
+    int* dtid = nullptr;
+    float* part = nullptr;
+    N3D_HIP(hipMalloc(&dtl, sizeof(int2) * ntiles));
+    N3D_HIP(hipMalloc(&dtid, sizeof(int) * tid_of.size()));
+    N3D_HIP(hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT));
+    N3D_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, p->stream));
+    N3D_HIP(hipMemcpyAsync(dtid, tid_of.data(), sizeof(int) * tid_of.size(), hipMemcpyHostToDevice, p->stream));
+    {
+      ProfScope ps(p, "k_gram_bf16x3");
+      hipLaunchKernelGGL(k_gram_bf16x3, dim3(ntiles, nsplit), dim3(256), 0, p->stream,
+                         (const float2*)data, (const int4*)dtaps, (const int2*)dtl, part, g);
+    }
+    {
+      ProfScope ps(p, "k_gram_assemble");
+      hipLaunchKernelGGL(k_gram_assemble, dim3(nblk((long long)g.n * g.n)), dim3(256), 0, p->stream,
+                         (const float*)part, (const int*)dtid, ntiles, g, dAA);
+    }
+    if (hipStreamSynchronize(p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+    (void)hipFree(dtl); (void)hipFree(dtid); (void)hipFree(part);
+  }
+  N3D_HIP(hipGetLastError());
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(dtaps);
+  if (st) set_error("device error in the Gram kernel");
+  return st;
+}
+
+// Hermitian eigendecomposition in FP64 on the device (rocSOLVER zheevd):
+// eigenvalues ascending in dW, eigenvectors overwrite dA (column-major)
+int heev_device(nufft3d_plan* p, double2* dA, int n, double* dW) {
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success) {
+    set_error("rocblas_create_handle failed");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  rocblas_set_stream(h, p->stream);
+  double* E = nullptr;
+  rocblas_int* info = nullptr;
+  N3D_HIP(hipMalloc(&E, sizeof(double) * (n > 0 ? n : 1)));
+  N3D_HIP(hipMalloc(&info, sizeof(rocblas_int)));
+  rocblas_status rs = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n,
+                                       (rocblas_double_complex*)dA, n, dW, E, info);
+  rocblas_int hinfo = 0;
+  (void)hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, p->stream);
+  (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(E);
+  (void)hipFree(info);
+  rocblas_destroy_handle(h);
+  if (rs != rocblas_status_success || hinfo != 0) {
+    set_error("rocsolver_zheevd failed (status " + std::to_string((int)rs) + ", info " +
+              std::to_string(hinfo) + ")");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  return NUFFT3D_OK;
+}
+
+// MATLAB hist(y, nbins) counts with equally spaced bins over [min, max]
+static std::vector<int> hist_counts(const std::vector<double>& y, int nb) {
+  std::vector<int> h(std::max(nb, 1), 0);
+  if (y.empty()) return h;
+  double lo = *std::min_element(y.begin(), y.end()), hi = *std::max_element(y.begin(), y.end());
+  if (hi == lo) { h[h.size() / 2] = (int)y.size(); return h; }
+  const double bw = (hi - lo) / h.size();
+  for (double v : y) {
+    int k = (int)std::floor((v - lo) / bw);
+    if (k >= (int)h.size()) k = (int)h.size() - 1;
+    if (k < 0) k = 0;
+    h[k]++;
+  }
+  return h;
+}
+
+static double median_of(std::vector<double> v) {
+  std::sort(v.begin(), v.end());
+  const size_t m = v.size();
+  if (m == 0) return 0;
+  return m % 2 ? v[m / 2] : 0.5 * (v[m / 2 - 1] + v[m / 2]);
+}
+
+// pruno.m:93-138 on the host: eigen-pairs (descending), noise, filter, nv and
+// the null kernels Kt[(t,c), v] = (1 - f_v) V[c + nc*t, v]
+int pruno_select(const std::vector<double>& evals_desc, double noise_in, double density,
+                 int nx, int ny, double tol, double& noise_out, std::vector<double>& f, int& nv) {
+  const int n = (int)evals_desc.size();
+  double noise = noise_in;
+  if (!(noise >= 0)) {   // estimate (pruno.m:97-107)
+    const double W1 = evals_desc[0];
+    const double thr = std::nextafter(n * W1, INFINITY) - n * W1;   // eps(numel(W)*W(1))
+    int hi = 0;
+    for (double v : evals_desc) hi += v > thr;
+    int lo = 1;
+    for (lo = 1; lo <= hi; ++lo) {
+      std::vector<double> seg(evals_desc.begin() + (lo - 1), evals_desc.begin() + hi);
+      const int nb = std::max(1, (int)std::lround(std::sqrt((double)(hi - lo))));
+      std::vector<int> h = hist_counts(seg, nb);
+      const int k = (int)(std::max_element(h.begin(), h.end()) - h.begin()) + 1;
+      if (k > 1) break;
+    }
+    if (lo > hi) lo = hi;
+    std::vector<double> seg(evals_desc.begin() + (lo - 1), evals_desc.begin() + std::max(hi, lo));
+    const double sigma = std::sqrt(median_of(seg));
+    noise = sigma / std::sqrt(density * nx * ny);
+  }
+  noise_out = noise;
+  const double sigma = noise * std::sqrt(density * nx * ny);     // pruno.m:108
+  f.assign(n, 0.0);
+  nv = 0;
+  for (int i = 0; i < n; ++i) {
+    const double Wv = std::sqrt(std::max(evals_desc[i], 0.0));   // pruno.m:111
+    const double fv = Wv > 0 ? std::max(0.0, 1 - sigma * sigma / (Wv * Wv)) : 0.0;
+    f[i] = fv;                                                   // pruno.m:114
+    nv += fv < 1 - tol;                                          // pruno.m:115
+  }
+  return NUFFT3D_OK;
+}
+
+// Per-voxel coupling operator B(x) = lambda^2 sum_v conj(fnull_v) fnull_v^T
+// from the tap-domain null kernels Kt (device complex double [(t,c) x nv]).
+int null_operator(nufft3d_plan* p, const double2* dKt, int nk, int nc, int nv, int w,
+                  const std::vector<int4>& taps, const int N[3], double lambda, void* dB) {
+  const int n = nk * nc;
+  double2 *P = nullptr, *G = nullptr, *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;
+  int4* dtaps = nullptr;
+  N3D_HIP(hipMalloc(&P, sizeof(double2) * (size_t)n * n));
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success) {
+    set_error("rocblas_create_handle failed");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  rocblas_set_stream(h, p->stream);
+  // Q = Kt Kt^H (n x n); P = conj(Kt) Kt^T = conj(Q)
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  rocblas_status rs = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose,
+                                    n, n, nv, &one, (const rocblas_double_complex*)dKt, n,
+                                    (const rocblas_double_complex*)dKt, n, &zero,
+                                    (rocblas_double_complex*)P, n);
+  rocblas_destroy_handle(h);
+  if (rs != rocblas_status_success) {
+    (void)hipFree(P);
+    set_error("rocblas_zgemm failed");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  const int D = 2 * w - 1;
+  const long long nG = (long long)D * D * D * nc * nc;
+  N3D_HIP(hipMalloc(&G, sizeof(double2) * nG));
+  N3D_HIP(hipMalloc(&dtaps, sizeof(int4) * taps.size()));
+  N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_null_G, dim3(nblk(nG)), dim3(256), 0, p->stream, (const double2*)P,
+                     (const int4*)dtaps, nk, nc, w, G);
+  // separable inverse DFT: G [dz][dy][dx][cc] -> B [z][y][x][cc]
+  const long long cc = (long long)nc * nc;
+  const int N1 = N[0], N2 = N[1], N3 = N[2];
+  N3D_HIP(hipMalloc(&T1, sizeof(double2) * (size_t)D * D * N1 * cc));
+  N3D_HIP(hipMalloc(&T2, sizeof(double2) * (size_t)D * N2 * N1 * cc));
+  N3D_HIP(hipMalloc(&T3, sizeof(double2) * (size_t)N3 * N2 * N1 * cc));
+  // x: in [dz*dy][dx][cc] -> [dz*dy][x][cc]
+  hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)D * D * N1 * cc)), dim3(256), 0, p->stream,
+                     (const double2*)G, T1, D, w, N1, cc, (long long)D * D);
+  // y: in [dz][dy][x*cc] -> [dz][y][x*cc]
+  hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)D * N2 * N1 * cc)), dim3(256), 0, p->stream,
+                     (const double2*)T1, T2, D, w, N2, (long long)N1 * cc, (long long)D);
+  // z: in [dz][y*x*cc] -> [z][y*x*cc]
+  hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)N3 * N2 * N1 * cc)), dim3(256), 0, p->stream,
+                     (const double2*)T2, T3, D, w, N3, (long long)N2 * N1 * cc, 1LL);
+  const long long V = (long long)N1 * N2 * N3;
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_null_store<double>), dim3(nblk(V * cc)), dim3(256), 0, p->stream,
+                       (const double2*)T3, V, nc, lambda * lambda, (double2*)dB);
+  else
+    hipLaunchKernelGGL((k_null_store<float>), dim3(nblk(V * cc)), dim3(256), 0, p->stream,
+                       (const double2*)T3, V, nc, lambda * lambda, (float2*)dB);
+  N3D_HIP(hipGetLastError());
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(P); (void)hipFree(G); (void)hipFree(dtaps);
+  (void)hipFree(T1); (void)hipFree(T2); (void)hipFree(T3);
+  return NUFFT3D_OK;
+}
+
+int null_apply(nufft3d_plan* p, const void* B, const void* x, void* y, int nc) {
+  const long long V = (long long)p->N[0] * p->N[1] * p->N[2];
+  ProfScope ps(p, "k_null_apply");
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_null_apply<double>), dim3(nblk(V)), dim3(256), 0, p->stream,
+                       (const double2*)B, (const double2*)x, (double2*)y, V, nc);
+  else
+    hipLaunchKernelGGL((k_null_apply<float>), dim3(nblk(V)), dim3(256), 0, p->stream,
+                       (const float2*)B, (const float2*)x, (float2*)y, V, nc);
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+__global__ void k_count_nonzero(const cplx<R>* __restrict__ d, long long n, unsigned long long* cnt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long mine = 0;
+  if (i < n) { const cplx<R> v = d[i]; mine = (v.x != 0 || v.y != 0); }
+  // wave-level ballot then one atomic per wave (integer: order-independent)
+  const unsigned long long b = __ballot(mine);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+}
+
+// Kt[:, v] = scale_v * E[:, col_v] (eigenvector columns of the ascending
+// decomposition, complex double, column-major n x n)
+__global__ void k_pick_columns(const double2* __restrict__ E, int n, const int* __restrict__ cols,
+                               const double* __restrict__ scale, int nv, double2* __restrict__ Kt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * nv) return;
+  const int r = (int)(i % n), v = (int)(i / n);
+  const double2 e = E[r + (long long)n * cols[v]];
+  double2 o; o.x = scale[v] * e.x; o.y = scale[v] * e.y;
+  Kt[i] = o;
+}
+
+// pruno.m:43-143 for k-space data [N1 N2 N3 nc] on the device: Gram, FP64
+// eigendecomposition, noise floor, filter, nv, null kernels; then the
+// per-voxel coupling operator of iprojection (nufft_3d.m:317-320) with
+// fnull = K*lambda (iNUFT.m:207-208). dB receives [V][nc][nc].
+int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,
+                double lambda, void** dB, int* nv_out, std::vector<double>* sv_desc,
+                double* noise_out) {
+  const int dims[4] = {p->N[0], p->N[1], p->N[2], nc};
+  const long long V = (long long)dims[0] * dims[1] * dims[2];
+  // fully sampled check (pruno.m:77-81)
+  unsigned long long* dcnt = nullptr;
+  N3D_HIP(hipMalloc(&dcnt, sizeof(unsigned long long)));
+  N3D_HIP(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), p->stream));
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_count_nonzero<double>), dim3(nblk(V * nc)), dim3(256), 0, p->stream,
+                       (const double2*)kdata, V * nc, dcnt);
+  else
+    hipLaunchKernelGGL((k_count_nonzero<float>), dim3(nblk(V * nc)), dim3(256), 0, p->stream,
+                       (const float2*)kdata, V * nc, dcnt);
+  unsigned long long cnt = 0;
+  N3D_HIP(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(dcnt);
+  const double density = (double)cnt / (double)(V * nc);
+  if (density < 0.99) { set_error("data must be fully sampled"); return NUFFT3D_ERR_INVALID; }
+
+  std::vector<int4> taps;
+  int w = 0;
+  kernel_taps(o.width, o.radial, dims[2] == 1, taps, w);
+  const int nk = (int)taps.size();
+  const int n = nk * nc;
+  double2* dAA = nullptr;
+  double* dW = nullptr;
+  N3D_HIP(hipMalloc(&dAA, sizeof(double2) * (size_t)n * n));
+  N3D_HIP(hipMalloc(&dW, sizeof(double) * n));
+  int st = gram_device(p, kdata, dims, taps, dAA);
+  if (!st) st = heev_device(p, dAA, n, dW);
+  std::vector<double> wasc(n);
+  if (!st) {
+    N3D_HIP(hipMemcpy(wasc.data(), dW, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  std::vector<double> wdesc(wasc.rbegin(), wasc.rend());
+  std::vector<double> f;
+  int nv = 0;
+  double noise = 0;
+  if (!st) st = pruno_select(wdesc, o.noise, density, dims[0], dims[1], o.tol, noise, f, nv);
+  if (sv_desc) {
+    sv_desc->resize(n);
+    for (int i = 0; i < n; ++i) (*sv_desc)[i] = std::sqrt(std::max(wdesc[i], 0.0));
+  }
+  if (noise_out) *noise_out = noise;
+  if (nv_out) *nv_out = nv;
+  if (!st && dB) {
+    // the nv vectors: literal = first nv (largest, pruno.m:128-130),
+    // otherwise the nv smallest (those with f < 1 - tol)
+    std::vector<int> cols(std::max(nv, 1));
+    std::vector<double> scale(std::max(nv, 1));
+    for (int v = 0; v < nv; ++v) {
+      const int desc = o.literal_first_nv ? v : n - nv + v;
+      cols[v] = n - 1 - desc;
+      scale[v] = 1.0 - f[desc];
+    }
+    int* dcols = nullptr;
+    double* dscale = nullptr;
+    double2* dKt = nullptr;
+    N3D_HIP(hipMalloc(&dcols, sizeof(int) * cols.size()));
+    N3D_HIP(hipMalloc(&dscale, sizeof(double) * scale.size()));
+    N3D_HIP(hipMalloc(&dKt, sizeof(double2) * (size_t)n * std::max(nv, 1)));
+    N3D_HIP(hipMemcpy(dcols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice));
+    N3D_HIP(hipMemcpy(dscale, scale.data(), sizeof(double) * scale.size(), hipMemcpyHostToDevice));
+    if (nv > 0)
+      hipLaunchKernelGGL(k_pick_columns, dim3(nblk((long long)n * nv)), dim3(256), 0, p->stream,
+                         (const double2*)dAA, n, (const int*)dcols, (const double*)dscale, nv, dKt);
+    const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+    N3D_HIP(hipMalloc(dB, esz * (size_t)V * nc * nc));
+    if (nv > 0) st = null_operator(p, dKt, nk, nc, nv, w, taps, p->N, lambda, *dB);
+    else N3D_HIP(hipMemset(*dB, 0, esz * (size_t)V * nc * nc));
+    (void)hipFree(dcols); (void)hipFree(dscale); (void)hipFree(dKt);
+  }
+  (void)hipFree(dAA);
+  (void)hipFree(dW);
+  return st;
+}
+
+}  // namespace nufft3d

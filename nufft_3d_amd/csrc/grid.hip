@@ -36,202 +36,211 @@ __device__ __forceinline__ void load_off(const double* p, int s, double& x, doub
   x = q[0]; y = q[1]; z = q[2];
 }
 
+// Weight of a hit whose support membership was decided at plan time in
+// double (nufft_3d.m:182-191). fp64: the reference value rounded to float
+// and widened (nufft_3d.m:160,205); fp32: evaluated in float.
+__device__ __forceinline__ void decode_hit(unsigned short code, const GridGeom& g, int& dx, int& dy,
+                                           int& dz) {
+  dx = (int)(code & 31u) - g.W1;
+  dy = (int)((code >> 5) & 31u) - g.W2;
+  dz = (int)(code >> 10) - g.W3;
+}
+
+__device__ __forceinline__ double hit_weight_d(int dx, int dy, int dz, const double* f,
+                                               const KBParams& p) {
+#pragma clang fp contract(off)
+  const double ex = (double)dx - f[0], ey = (double)dy - f[1], ez = (double)dz - f[2];
+  if (p.radial) return (double)(float)kb_d(ex * ex + ey * ey + ez * ez, p);
+  return (double)(float)(kb_d(ex * ex, p) * kb_d(ey * ey, p) * kb_d(ez * ez, p));
+}
+
+__device__ __forceinline__ float hit_weight_f(int dx, int dy, int dz, const float4 f,
+                                              const KBParams& p) {
+  const float ex = (float)dx - f.x, ey = (float)dy - f.y, ez = (float)dz - f.z;
+  if (p.radial) return kb_f(ex * ex + ey * ey + ez * ez, p);
+  return kb_f(ex * ex, p) * kb_f(ey * ey, p) * kb_f(ez * ez, p);
+}
+
+// accumulator type: fp64 plans and compensated fp32 plans accumulate in
+// double (an fp32 x fp32 product is exact in double, so the sum carries one
+// rounding per term at 2^-53 instead of 2^-24); plain fp32 plans in float
+template <typename R, bool COMP> struct AccT { using type = double; };
+template <> struct AccT<float, false> { using type = float; };
+
 // ---------------------------------------------------------------------------
-// Forward: k[m,c] = scale * w_m * sum_g H(m,g) G[g,c]. One thread per sorted
-// sample, candidates visited z-y-x so grid reads walk contiguous rows.
-// user_layout: out[orig[s] + c*M] (reference [M nc]); else out[s*nc + c].
-template <typename R, int CG>
-__global__ void __launch_bounds__(256) k_interp(const int4* __restrict__ base,
+// Forward: k[s,c] = scale * w_s * sum_{hits g of s} H(s,g) G[g,c] with the
+// samples in the sorted coil-interleaved layout. LPS lanes share a sample,
+// each owning CPL consecutive coils, so a grid point's coils are read as one
+// contiguous run by neighbouring lanes. Hits come in z-y-x candidate order.
+template <typename R, int CPL, int LPS, bool COMP>
+__global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ spos,
+                                                const unsigned short* __restrict__ fcode,
+                                                const int4* __restrict__ base,
                                                 const double* __restrict__ off,
-                                                const int* __restrict__ orig,
+                                                const float4* __restrict__ off32,
                                                 const cplx<R>* __restrict__ grid,
-                                                cplx<R>* __restrict__ out, int user_layout,
+                                                cplx<R>* __restrict__ out,
                                                 const R* __restrict__ wsamp, R scale,
                                                 GridGeom g, KBParams kp) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  using A = typename AccT<R, COMP>::type;
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)(gt / LPS);
+  const int q = (int)(gt % LPS);
   if (s >= g.nnz) return;
+  const int cl0 = q * CPL;
   const int4 b = base[s];
-  double fx, fy, fz;
-  load_off(off, s, fx, fy, fz);
-  R ar[CG], ai[CG];
+  float4 f32;
+  if constexpr (sizeof(R) == 4) f32 = off32[s];
+  const double* f64 = off + 3LL * s;
+  A ar[CPL], ai[CPL];
 #pragma unroll
-  for (int c = 0; c < CG; ++c) { ar[c] = 0; ai[c] = 0; }
-  for (int dz = -g.W3; dz <= g.W3; ++dz) {
-    const double tz = (double)dz - fz;
-    const double dz2 = tz * tz;
-    const int gz = wrapk(b.z + dz, g.K3);
-    for (int dy = -g.W2; dy <= g.W2; ++dy) {
-      const double ty = (double)dy - fy;
-      const double dy2 = ty * ty;
-      const int gy = wrapk(b.y + dy, g.K2);
-      const long long row = ((long long)gz * g.K2 + gy) * g.K1;
-      for (int dx = -g.W1; dx <= g.W1; ++dx) {
-        const double tx = (double)dx - fx;
-        const double wd = kb_weight<double>(tx * tx, dy2, dz2, kp);
-        if (wd == 0.0) continue;
-        const R w = (R)wd;
-        const int gx = wrapk(b.x + dx, g.K1);
-        const cplx<R>* gp = grid + (row + gx) * g.nc + g.c0;
+  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
+  const long long e0 = spos[s], e1 = spos[s + 1];
+#pragma unroll 4
+  for (long long e = e0; e < e1; ++e) {
+    int dx, dy, dz;
+    decode_hit(fcode[e], g, dx, dy, dz);
+    R w;
+    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, f64, kp);
+    else w = hit_weight_f(dx, dy, dz, f32, kp);
+    const int gx = wrapk(b.x + dx, g.K1), gy = wrapk(b.y + dy, g.K2), gz = wrapk(b.z + dz, g.K3);
+    const cplx<R>* gp = grid + (((long long)gz * g.K2 + gy) * g.K1 + gx) * g.nc + g.c0 + cl0;
 #pragma unroll
-        for (int c = 0; c < CG; ++c) {
-          if (c < g.cg) {
-            const cplx<R> v = gp[c];
-            ar[c] += w * v.x;
-            ai[c] += w * v.y;
-          }
-        }
+    for (int c = 0; c < CPL; ++c) {
+      if (cl0 + c < g.cg) {
+        const cplx<R> v = gp[c];
+        ar[c] += (A)w * (A)v.x;
+        ai[c] += (A)w * (A)v.y;
       }
     }
   }
   R sc = scale;
   if (wsamp) sc *= wsamp[s];
-  if (user_layout) {
-    const long long m = orig[s];
+  cplx<R>* op = out + (long long)s * g.nc + g.c0 + cl0;
 #pragma unroll
-    for (int c = 0; c < CG; ++c)
-      if (c < g.cg) {
-        cplx<R> v; v.x = ar[c] * sc; v.y = ai[c] * sc;
-        out[m + (long long)(g.c0 + c) * g.M] = v;
-      }
-  } else {
-#pragma unroll
-    for (int c = 0; c < CG; ++c)
-      if (c < g.cg) {
-        cplx<R> v; v.x = ar[c] * sc; v.y = ai[c] * sc;
-        out[(long long)s * g.nc + g.c0 + c] = v;
-      }
-  }
+  for (int c = 0; c < CPL; ++c)
+    if (cl0 + c < g.cg) {
+      cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
+      op[c] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint spread, output driven: one work-group per chunk of a tile's sample
-// list, one thread per grid point of the tile. Each thread accumulates the
-// samples of the list in list order, so the summation order is fixed by the
-// plan alone (no atomics, bitwise reproducible, independent of batching).
-// A tile whose list spans several chunks writes partial tiles that
-// k_spread_reduce sums in chunk order.
-struct TileGeom {
-  int T1, T2, T3;       // tile edge per axis
-  int nt1, nt2;         // tiles per axis (x, y)
-  int npts;             // T1*T2*T3
-};
-
-template <typename R>
-struct SampleStage {
-  int bx, by, bz, pad;
-  double fx, fy, fz;
-  R w;
-};
-
-template <typename R, int CG, int SB>
-__global__ void __launch_bounds__(512) k_spread(const SpreadChunk* __restrict__ chunks,
-                                                const int* __restrict__ list,
-                                                const int4* __restrict__ base,
+// Adjoint, output driven: LPS lanes per grid point walk its hit list (sorted
+// sample ids, fixed at plan time) and sum in registers. No atomics: the
+// result is bitwise reproducible and independent of the coil batch. Points
+// with more than rmax hits (the dense k-space centre of radial trajectories)
+// are skipped here and summed by k_spread_heavy + k_spread_hreduce.
+template <typename R, int CPL, int LPS, bool COMP>
+__global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
+                                                const int* __restrict__ hit_s,
+                                                const unsigned short* __restrict__ acode,
                                                 const double* __restrict__ off,
-                                                const int* __restrict__ orig,
-                                                const cplx<R>* __restrict__ samp, int user_layout,
+                                                const float4* __restrict__ off32,
+                                                const cplx<R>* __restrict__ ks,
                                                 const R* __restrict__ wsamp,
-                                                cplx<R>* __restrict__ grid,
-                                                cplx<R>* __restrict__ partial, GridGeom g,
-                                                TileGeom t, KBParams kp) {
-  __shared__ SampleStage<R> sh[SB];
-  __shared__ cplx<R> sv[SB * CG];
-  const SpreadChunk ch = chunks[blockIdx.x];
-  const int tid = threadIdx.x;
-  const int tx = ch.tile % t.nt1;
-  const int ty = (ch.tile / t.nt1) % t.nt2;
-  const int tz = ch.tile / (t.nt1 * t.nt2);
-  const int lx = tid % t.T1;
-  const int ly = (tid / t.T1) % t.T2;
-  const int lz = tid / (t.T1 * t.T2);
-  const int gx = tx * t.T1 + lx;
-  const int gy = ty * t.T2 + ly;
-  const int gz = tz * t.T3 + lz;
-  const bool own = tid < t.npts && gx < g.K1 && gy < g.K2 && gz < g.K3;
-  R ar[CG], ai[CG];
+                                                cplx<R>* __restrict__ grid, long long ngrid,
+                                                int rmax, GridGeom g, KBParams kp) {
+  using A = typename AccT<R, COMP>::type;
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gpt = gt / LPS;
+  const int q = (int)(gt % LPS);
+  if (gpt >= ngrid) return;
+  const long long e0 = rowptr[gpt], e1 = rowptr[gpt + 1];
+  if (e1 - e0 > rmax) return;
+  const int cl0 = q * CPL;
+  A ar[CPL], ai[CPL];
 #pragma unroll
-  for (int c = 0; c < CG; ++c) { ar[c] = 0; ai[c] = 0; }
-
-  for (int e0 = 0; e0 < ch.count; e0 += SB) {
-    const int n = min(SB, ch.count - e0);
-    __syncthreads();
-    for (int i = tid; i < n; i += blockDim.x) {
-      const int s = list[ch.start + e0 + i];
-      const int4 b = base[s];
-      SampleStage<R> st;
-      st.bx = b.x; st.by = b.y; st.bz = b.z; st.pad = 0;
-      load_off(off, s, st.fx, st.fy, st.fz);
-      st.w = wsamp ? wsamp[s] : (R)1;
-      sh[i] = st;
-    }
-    for (int i = tid; i < n * g.cg; i += blockDim.x) {
-      const int e = i / g.cg, c = i % g.cg;
-      const int s = list[ch.start + e0 + e];
-      const long long idx = user_layout ? (long long)orig[s] + (long long)(g.c0 + c) * g.M
-                                        : (long long)s * g.nc + g.c0 + c;
-      sv[e * CG + c] = samp[idx];
-    }
-    __syncthreads();
-    if (own) {
-      for (int i = 0; i < n; ++i) {
-        const SampleStage<R> st = sh[i];
-        int dx = gx - st.bx, dy = gy - st.by, dz = gz - st.bz;
-        if (dx > g.W1) dx -= g.K1; else if (dx < -g.W1) dx += g.K1;
-        if (dy > g.W2) dy -= g.K2; else if (dy < -g.W2) dy += g.K2;
-        if (dz > g.W3) dz -= g.K3; else if (dz < -g.W3) dz += g.K3;
-        if (abs(dx) > g.W1 || abs(dy) > g.W2 || abs(dz) > g.W3) continue;
-        const double ex = (double)dx - st.fx, ey = (double)dy - st.fy, ez = (double)dz - st.fz;
-        const double wd = kb_weight<double>(ex * ex, ey * ey, ez * ez, kp);
-        if (wd == 0.0) continue;
-        const R w = (R)wd * st.w;
+  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
+#pragma unroll 4
+  for (long long e = e0; e < e1; ++e) {
+    const int s = hit_s[e];
+    int dx, dy, dz;
+    decode_hit(acode[e], g, dx, dy, dz);
+    R w;
+    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
+    if (wsamp) w *= wsamp[s];
+    const cplx<R>* v = ks + (long long)s * g.nc + g.c0 + cl0;
 #pragma unroll
-        for (int c = 0; c < CG; ++c) {
-          if (c < g.cg) {
-            const cplx<R> v = sv[i * CG + c];
-            ar[c] += w * v.x;
-            ai[c] += w * v.y;
-          }
-        }
+    for (int c = 0; c < CPL; ++c) {
+      if (cl0 + c < g.cg) {
+        const cplx<R> val = v[c];
+        ar[c] += (A)w * (A)val.x;
+        ai[c] += (A)w * (A)val.y;
       }
     }
   }
-  if (!own) return;
-  if (ch.slot < 0) {
-    cplx<R>* gp = grid + (((long long)gz * g.K2 + gy) * g.K1 + gx) * g.nc + g.c0;
+  cplx<R>* gp = grid + gpt * g.nc + g.c0 + cl0;
 #pragma unroll
-    for (int c = 0; c < CG; ++c)
-      if (c < g.cg) { cplx<R> v; v.x = ar[c]; v.y = ai[c]; gp[c] = v; }
-  } else {
-    cplx<R>* pp = partial + ((long long)ch.slot * t.npts + tid) * g.cg;
-#pragma unroll
-    for (int c = 0; c < CG; ++c)
-      if (c < g.cg) { cplx<R> v; v.x = ar[c]; v.y = ai[c]; pp[c] = v; }
-  }
+  for (int c = 0; c < CPL; ++c)
+    if (cl0 + c < g.cg) { cplx<R> o; o.x = (R)ar[c]; o.y = (R)ai[c]; gp[c] = o; }
 }
 
-// Sum the partial tiles of each split tile in chunk order (deterministic).
-template <typename R>
-__global__ void k_spread_reduce(const SplitTile* __restrict__ split,
-                                const cplx<R>* __restrict__ partial,
-                                cplx<R>* __restrict__ grid, GridGeom g, TileGeom t) {
-  const SplitTile st = split[blockIdx.x];
-  const int tx = st.tile % t.nt1;
-  const int ty = (st.tile / t.nt1) % t.nt2;
-  const int tz = st.tile / (t.nt1 * t.nt2);
-  for (int i = threadIdx.x; i < t.npts * g.cg; i += blockDim.x) {
-    const int p = i / g.cg, c = i % g.cg;
-    const int lx = p % t.T1, ly = (p / t.T1) % t.T2, lz = p / (t.T1 * t.T2);
-    const int gx = tx * t.T1 + lx, gy = ty * t.T2 + ly, gz = tz * t.T3 + lz;
-    if (gx >= g.K1 || gy >= g.K2 || gz >= g.K3) continue;
-    R sr = 0, si = 0;
-    for (int j = 0; j < st.nslots; ++j) {
-      const cplx<R> v = partial[((long long)(st.first_slot + j) * t.npts + p) * g.cg + c];
-      sr += v.x;
-      si += v.y;
+// One segment of a dense point's hit list -> partial[slot][cg]
+template <typename R, int CPL, int LPS, bool COMP>
+__global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
+                                                      const int* __restrict__ hit_s,
+                                                      const unsigned short* __restrict__ acode,
+                                                      const double* __restrict__ off,
+                                                      const float4* __restrict__ off32,
+                                                      const cplx<R>* __restrict__ ks,
+                                                      const R* __restrict__ wsamp,
+                                                      cplx<R>* __restrict__ partial, GridGeom g,
+                                                      KBParams kp) {
+  using A = typename AccT<R, COMP>::type;
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int si = (int)(gt / LPS);
+  const int q = (int)(gt % LPS);
+  if (si >= nseg) return;
+  const HeavySeg sg = segs[si];
+  const int cl0 = q * CPL;
+  A ar[CPL], ai[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
+#pragma unroll 4
+  for (long long e = sg.start; e < sg.end; ++e) {
+    const int s = hit_s[e];
+    int dx, dy, dz;
+    decode_hit(acode[e], g, dx, dy, dz);
+    R w;
+    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
+    if (wsamp) w *= wsamp[s];
+    const cplx<R>* v = ks + (long long)s * g.nc + g.c0 + cl0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      if (cl0 + c < g.cg) {
+        const cplx<R> val = v[c];
+        ar[c] += (A)w * (A)val.x;
+        ai[c] += (A)w * (A)val.y;
+      }
     }
-    cplx<R> o; o.x = sr; o.y = si;
-    grid[(((long long)gz * g.K2 + gy) * g.K1 + gx) * g.nc + g.c0 + c] = o;
   }
+  // partial sums keep the accumulator precision until the final reduction
+  A* pp = reinterpret_cast<A*>(partial) + ((long long)sg.slot * g.cg + cl0) * 2;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+    if (cl0 + c < g.cg) { pp[2 * c] = ar[c]; pp[2 * c + 1] = ai[c]; }
+}
+
+template <typename R, bool COMP>
+__global__ void k_spread_hreduce(const HeavyPt* __restrict__ pts, int npt,
+                                 const cplx<R>* __restrict__ partial, cplx<R>* __restrict__ grid,
+                                 GridGeom g) {
+  using A = typename AccT<R, COMP>::type;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)npt * g.cg) return;
+  const int pi = (int)(i / g.cg), c = (int)(i % g.cg);
+  const HeavyPt hp = pts[pi];
+  const A* pp = reinterpret_cast<const A*>(partial);
+  A re = 0, im = 0;
+  for (int j = 0; j < hp.nslots; ++j) {
+    re += pp[((long long)(hp.first_slot + j) * g.cg + c) * 2];
+    im += pp[((long long)(hp.first_slot + j) * g.cg + c) * 2 + 1];
+  }
+  cplx<R> o; o.x = (R)re; o.y = (R)im;
+  grid[(long long)hp.g * g.nc + g.c0 + c] = o;
 }
 
 template <typename R>
@@ -242,6 +251,17 @@ __global__ void k_zero_oob(const int* __restrict__ oob, long long noob, long lon
   const long long j = i % noob, c = i / noob;
   cplx<R> z; z.x = 0; z.y = 0;
   k[oob[j] + c * M] = z;
+}
+
+// k_user[orig[s] + c*M] = ks[s*nc + c]
+template <typename R>
+__global__ void k_permute_out(const int* __restrict__ orig, int nnz, long long M, int nc,
+                              const cplx<R>* __restrict__ ks, cplx<R>* __restrict__ ku) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nnz * nc) return;
+  const long long s = i / nc;
+  const int c = (int)(i % nc);
+  ku[orig[s] + (long long)c * M] = ks[i];
 }
 
 // ks[s*nc + c] = w_s * k_user[orig[s] + c*M]
@@ -281,6 +301,15 @@ static KBParams make_kb(const nufft3d_plan* p) {
   k.inv_half2f = (float)(1.0 / k.half2);
   k.alphaf = (float)a;
   k.inv_b0f = (float)(1.0 / b0);
+  k.tief = (float)(1e-4 * k.J2);
+  k.large_b0 = a >= 3.75;
+  {
+    const double y = 3.75 / a;
+    const double P0 = 0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 +
+                      y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
+                      y * (-0.1647633e-1 + y * 0.392377e-2)))))));
+    k.inv_P0f = (float)(1.0 / P0);
+  }
   k.radial = p->radial;
   return k;
 }
@@ -295,96 +324,128 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
   return g;
 }
 
-static TileGeom make_tiles(const nufft3d_plan* p) {
-  TileGeom t;
-  t.T1 = p->T[0]; t.T2 = p->T[1]; t.T3 = p->T[2];
-  t.nt1 = p->ntile[0]; t.nt2 = p->ntile[1];
-  t.npts = t.T1 * t.T2 * t.T3;
-  return t;
-}
-
-template <typename R, int CG>
+template <typename R, int CPL, int LPS, bool COMP>
 static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
-                          int user_layout, const void* wsamp, double scale) {
+                          const void* wsamp, double scale) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
-  const int nb = (int)((p->nnz + 255) / 256);
+  const long long nthreads = p->nnz * (long long)LPS;
+  const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
-  hipLaunchKernelGGL((k_interp<R, CG>), dim3(nb), dim3(256), 0, p->stream, p->d_base, (const double*)p->d_off,
-                     p->d_orig, (const cplx<R>*)grid, (cplx<R>*)out, user_layout,
-                     (const R*)wsamp, (R)scale, g, kp);
+  hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP>), dim3(nb), dim3(256), 0, p->stream, p->d_spos,
+                     p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
+                     (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, g, kp);
+}
+
+template <typename R, int CPL, int LPS, bool COMP>
+static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
+                          void* partial, const void* wsamp) {
+  GridGeom g = make_geom(p, nc, c0, cg);
+  KBParams kp = make_kb(p);
+  const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
+  {
+    const long long nthreads = ngrid * LPS;
+    ProfScope ps(p, "k_spread");
+    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP>), dim3((unsigned)((nthreads + 255) / 256)),
+                       dim3(256), 0, p->stream, p->d_rowptr, p->d_hit_s, p->d_acode,
+                       (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
+                       (cplx<R>*)grid, ngrid, p->rmax, g, kp);
+  }
+  if (p->nhseg > 0) {
+    ProfScope ps(p, "k_spread_heavy");
+    const long long nthreads = (long long)p->nhseg * LPS;
+    hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP>), dim3((unsigned)((nthreads + 255) / 256)),
+                       dim3(256), 0, p->stream, p->d_hseg, p->nhseg, p->d_hit_s, p->d_acode,
+                       (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
+                       (cplx<R>*)partial, g, kp);
+    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>),
+                       dim3((unsigned)(((long long)p->nhpt * cg + 255) / 256)), dim3(256), 0,
+                       p->stream, p->d_hpt, p->nhpt, (const cplx<R>*)partial, (cplx<R>*)grid, g);
+  }
+}
+
+// coil groups of up to 16; fp32 groups of more than 4 coils put 2 or 4 lanes
+// on one sample / grid point (4 coils each), fp64 keeps one lane
+template <typename R, bool COMP, bool SPREAD>
+static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
+                       const void* wsamp, double scale) {
+  for (int c0 = 0; c0 < nc; c0 += 16) {
+    const int cg = nc - c0 < 16 ? nc - c0 : 16;
+#define N3D_GO(CPL, LPS)                                                                  \
+    do {                                                                                  \
+      if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP>(p, in, nc, c0, cg, out, partial, wsamp); \
+      else launch_interp<R, CPL, LPS, COMP>(p, in, nc, c0, cg, out, wsamp, scale);        \
+    } while (0)
+    if constexpr (sizeof(R) == 8) {
+      if (cg <= 1) N3D_GO(1, 1);
+      else if (cg <= 2) N3D_GO(2, 1);
+      else if (cg <= 4) N3D_GO(4, 1);
+      else if (cg <= 8) N3D_GO(8, 1);
+      else N3D_GO(16, 1);
+    } else {
+      if (cg <= 1) N3D_GO(1, 1);
+      else if (cg <= 2) N3D_GO(2, 1);
+      else if (cg <= 4) N3D_GO(4, 1);
+      else if (cg <= 8) N3D_GO(4, 2);
+      else N3D_GO(4, 4);
+    }
+#undef N3D_GO
+  }
 }
 
 template <typename R>
-static int interp_t(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
-                    const void* wsamp, double scale) {
+static int interp_t(nufft3d_plan* p, const void* grid, int nc, void* out, const void* wsamp,
+                    double scale) {
   if (p->nnz == 0) return NUFFT3D_OK;
-  for (int c0 = 0; c0 < nc; c0 += 16) {
-    const int cg = nc - c0 < 16 ? nc - c0 : 16;
-    if (cg <= 1) launch_interp<R, 1>(p, grid, nc, c0, cg, out, user_layout, wsamp, scale);
-    else if (cg <= 2) launch_interp<R, 2>(p, grid, nc, c0, cg, out, user_layout, wsamp, scale);
-    else if (cg <= 4) launch_interp<R, 4>(p, grid, nc, c0, cg, out, user_layout, wsamp, scale);
-    else if (cg <= 8) launch_interp<R, 8>(p, grid, nc, c0, cg, out, user_layout, wsamp, scale);
-    else launch_interp<R, 16>(p, grid, nc, c0, cg, out, user_layout, wsamp, scale);
-  }
+  if (sizeof(R) == 4 && p->compensated) run_groups<R, true, false>(p, grid, nc, out, nullptr, wsamp, scale);
+  else run_groups<R, false, false>(p, grid, nc, out, nullptr, wsamp, scale);
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
 
+int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user);
+
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
            const void* wsamp, double scale) {
-  return p->prec == 2 ? interp_t<double>(p, grid, nc, out, user_layout, wsamp, scale)
-                      : interp_t<float>(p, grid, nc, out, user_layout, wsamp, scale);
+  if (user_layout) {
+    // interpolate into the sorted coil-interleaved layout, then scatter rows
+    const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+    N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz));
+    N3D_TRY(p->prec == 2 ? interp_t<double>(p, grid, nc, p->ws_samp.p, wsamp, scale)
+                         : interp_t<float>(p, grid, nc, p->ws_samp.p, wsamp, scale));
+    return permute_out(p, p->ws_samp.p, nc, out);
+  }
+  return p->prec == 2 ? interp_t<double>(p, grid, nc, out, wsamp, scale)
+                      : interp_t<float>(p, grid, nc, out, wsamp, scale);
 }
 
-template <typename R, int CG>
-static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
-                          void* partial, int user_layout, const void* wsamp) {
-  GridGeom g = make_geom(p, nc, c0, cg);
-  KBParams kp = make_kb(p);
-  TileGeom t = make_tiles(p);
-  const int threads = ((t.npts + 63) / 64) * 64;
-  constexpr int SB = sizeof(R) == 8 ? 64 : 128;
-  {
-  ProfScope ps(p, "k_spread");
-  hipLaunchKernelGGL((k_spread<R, CG, SB>), dim3(p->nchunks), dim3(threads), 0, p->stream,
-                     p->d_chunks, p->d_list, p->d_base, (const double*)p->d_off, p->d_orig,
-                     (const cplx<R>*)samp, user_layout, (const R*)wsamp, (cplx<R>*)grid,
-                     (cplx<R>*)partial, g, t, kp);
-  }
-  if (p->nsplit > 0) {
-    ProfScope ps(p, "k_spread_reduce");
-    hipLaunchKernelGGL((k_spread_reduce<R>), dim3(p->nsplit), dim3(256), 0, p->stream,
-                       p->d_split, (const cplx<R>*)partial, (cplx<R>*)grid, g, t);
-  }
-}
-
+// samples in the internal layout ks[s*nc + c] (sorted order)
 template <typename R>
-static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
-                    const void* wsamp) {
-  const int npts = p->T[0] * p->T[1] * p->T[2];
+static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const void* wsamp) {
   const int cgmax = nc < 16 ? nc : 16;
   void* partial = nullptr;
-  if (p->nslots > 0) {
-    N3D_TRY(ensure(p->ws_part, (size_t)p->nslots * npts * cgmax * sizeof(cplx<R>)));
+  if (p->nhseg > 0) {
+    N3D_TRY(ensure(p->ws_part, (size_t)p->nhseg * cgmax * 2 * sizeof(double)));
     partial = p->ws_part.p;
   }
-  for (int c0 = 0; c0 < nc; c0 += 16) {
-    const int cg = nc - c0 < 16 ? nc - c0 : 16;
-    if (cg <= 1) launch_spread<R, 1>(p, samp, nc, c0, cg, grid, partial, user_layout, wsamp);
-    else if (cg <= 2) launch_spread<R, 2>(p, samp, nc, c0, cg, grid, partial, user_layout, wsamp);
-    else if (cg <= 4) launch_spread<R, 4>(p, samp, nc, c0, cg, grid, partial, user_layout, wsamp);
-    else if (cg <= 8) launch_spread<R, 8>(p, samp, nc, c0, cg, grid, partial, user_layout, wsamp);
-    else launch_spread<R, 16>(p, samp, nc, c0, cg, grid, partial, user_layout, wsamp);
-  }
+  if (sizeof(R) == 4 && p->compensated) run_groups<R, true, true>(p, samp, nc, grid, partial, wsamp, 1.0);
+  else run_groups<R, false, true>(p, samp, nc, grid, partial, wsamp, 1.0);
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
 
 int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
            const void* wsamp) {
-  return p->prec == 2 ? spread_t<double>(p, samp, nc, grid, user_layout, wsamp)
-                      : spread_t<float>(p, samp, nc, grid, user_layout, wsamp);
+  if (user_layout) {
+    // gather the caller's [M nc] samples into the sorted coil-interleaved layout
+    const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+    N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz));
+    N3D_TRY(permute_in(p, samp, nc, p->ws_samp.p, wsamp));
+    samp = p->ws_samp.p;
+    wsamp = nullptr;
+  }
+  return p->prec == 2 ? spread_t<double>(p, samp, nc, grid, wsamp)
+                      : spread_t<float>(p, samp, nc, grid, wsamp);
 }
 
 int zero_oob(nufft3d_plan* p, void* k_user, int nc) {
@@ -399,6 +460,21 @@ int zero_oob(nufft3d_plan* p, void* k_user, int nc) {
   else
     hipLaunchKernelGGL((k_zero_oob<float>), dim3(nb), dim3(256), 0, p->stream, p->d_oob, noob,
                        (long long)p->M, nc, (float2*)k_user);
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user) {
+  const long long n = p->nnz * (long long)nc;
+  if (n == 0) return NUFFT3D_OK;
+  const int nb = (int)((n + 255) / 256);
+  ProfScope ps(p, "k_permute_out");
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_permute_out<double>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
+                       (int)p->nnz, (long long)p->M, nc, (const double2*)ks, (double2*)k_user);
+  else
+    hipLaunchKernelGGL((k_permute_out<float>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
+                       (int)p->nnz, (long long)p->M, nc, (const float2*)ks, (float2*)k_user);
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }

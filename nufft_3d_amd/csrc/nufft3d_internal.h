@@ -49,16 +49,15 @@ struct FftAxis {
   void* d_tw = nullptr;   // L roots of unity exp(-2 pi i t / L), precision of the plan
 };
 
-// Work chunk of the deterministic spread: one tile, a slice of its list
-struct SpreadChunk {
-  int tile;    // linear tile id
-  int start;   // first entry in the tile-list array
-  int count;   // entries in this chunk (0 = empty tile, writes zeros)
-  int slot;    // -1: sole chunk of its tile (writes the grid); else partial slot
+// A dense grid point (k-space centre) whose hit list is split into segments;
+// the segments' partial sums are added in segment order
+struct HeavySeg {
+  long long start, end;   // hit range of this segment
+  int g;                  // linear grid index
+  int slot;               // partial slot
 };
-
-struct SplitTile {
-  int tile;
+struct HeavyPt {
+  int g;
   int first_slot;
   int nslots;
   int pad;
@@ -96,6 +95,7 @@ struct nufft3d_plan {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int verbose = 0;
+  int compensated = 1;      // fp32 plans: compensated (TwoSum/TwoProduct) accumulation
 
   // sorted, in-bounds samples
   int* d_orig = nullptr;    // [nnz] original row of sorted sample s
@@ -108,21 +108,33 @@ struct nufft3d_plan {
   void* d_dens = nullptr;   // [nnz] real, sorted order
   std::vector<double> h_U;  // fp64 copy of U (host)
 
-  // spread tiles
+  // sample bins (samples are sorted by bin for locality)
   int T[3] = {8, 8, 8};
   int ntile[3] = {1, 1, 1};
   int ntiles = 0;
-  int* d_list = nullptr;    // tile lists of sorted sample ids
-  int64_t nlist = 0;
-  nufft3d::SpreadChunk* d_chunks = nullptr;
-  int nchunks = 0;
-  nufft3d::SplitTile* d_split = nullptr;
-  int nsplit = 0;
-  int nslots = 0;
-  int chunk_len = 2048;
-  int max_list = 0;
+
+  // interpolation structure (the sparsity pattern of H, weights on the fly):
+  //   forward : per sorted sample s, hits [spos[s], spos[s+1]) as candidate
+  //             codes (dx,dy,dz biased by W, 5 bits each) in z-y-x order
+  //   adjoint : per grid point g, hits [rowptr[g], rowptr[g+1]) as sorted
+  //             sample ids (ascending) with the same codes alongside
+  int64_t nhits = 0;
+  long long* d_spos = nullptr;      // [nnz + 1]
+  unsigned short* d_fcode = nullptr;  // [nhits] forward codes (sample-major)
+  long long* d_rowptr = nullptr;    // [K1*K2*K3 + 1]
+  int* d_hit_s = nullptr;           // [nhits] sample ids (grid-major)
+  unsigned short* d_acode = nullptr;  // [nhits] codes (grid-major)
+  float4* d_off32 = nullptr;        // [nnz] (fx, fy, fz, 0)
+  int rmax = 512;                   // grid points with more hits are split
+  nufft3d::HeavySeg* d_hseg = nullptr;
+  int nhseg = 0;
+  nufft3d::HeavyPt* d_hpt = nullptr;
+  int nhpt = 0;
+  int chunk_len = 512;              // hits per heavy segment
+  int max_hits = 0;
 
   nufft3d::FftAxis ax[3];
+  nufft3d::FftAxis axN[3];  // image-size axes (unshifted FFT of the PRUNO calibration)
   nufft3d::Profiler prof;
 
   // scratch, grown on demand
@@ -157,6 +169,7 @@ int fft_setup_axis(nufft3d_plan* p, int axis);
 int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid);   // deap, pad, FFT
 int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                 const void* img_add, double add_coef);                    // IFFT, crop, deap
+int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out);       // unshifted N-size FFT
 
 // solve.hip
 int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
@@ -164,5 +177,16 @@ int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
 int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void* wsamp,
                  void* y);
 int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out);
+int minres_coupled(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                   const void* wsamp, const void* B, double tol, void* x, nufft3d_info* info);
+
+// gram.hip
+int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,
+                double lambda, void** dB, int* nv_out, std::vector<double>* sv_desc,
+                double* noise_out);
+int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
+                double2* dAA);
+void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int& w);
+int null_apply(nufft3d_plan* p, const void* B, const void* x, void* y, int nc);
 
 }  // namespace nufft3d

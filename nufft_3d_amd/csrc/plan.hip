@@ -258,53 +258,86 @@ __global__ void k_gather(const double* __restrict__ om, const unsigned* __restri
   o[0] = x - rx; o[1] = y - ry; o[2] = z - rz;
 }
 
-// distinct tiles touched along one axis by grid points b-W..b+W (wrapped)
-__device__ __forceinline__ int axis_tiles(int b, int W, int K, int T, int* t) {
+// The interpolation pattern of H: candidates round(u*om)+d, |d| <= ceil(J/2)
+// per axis (nufft_3d.m:154,168-170), kept when 4*dist2 <= J^2 (radial) or
+// 4*d^2 <= J^2 per axis (separable), evaluated in double exactly as
+// nufft_3d.m:173-191 (dx = d - f with f = u*om - round(u*om) exact).
+struct HitGeom {
+  int K1, K2, K3, W1, W2, W3;
+  double J2;
+  int radial;
+};
+
+__device__ __forceinline__ bool hit_test(double tx, double ty, double tz, const HitGeom& g) {
+#pragma clang fp contract(off)
+  const double dx2 = tx * tx, dy2 = ty * ty, dz2 = tz * tz;
+  if (g.radial) return 4.0 * (dx2 + dy2 + dz2) <= g.J2;
+  return 4.0 * dx2 <= g.J2 && 4.0 * dy2 <= g.J2 && 4.0 * dz2 <= g.J2;
+}
+
+__device__ __forceinline__ int wrapg(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
+
+__global__ void k_hits_count(const int4* __restrict__ base, const double* __restrict__ off,
+                             long long nnz, HitGeom g, int* __restrict__ cnt_s,
+                             int* __restrict__ cnt_g) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int4 b = base[s];
+  const double fx = off[3 * s], fy = off[3 * s + 1], fz = off[3 * s + 2];
   int n = 0;
-  for (int d = -W; d <= W; ++d) {
-    int g = b + d;
-    g = g < 0 ? g + K : (g >= K ? g - K : g);
-    const int tt = g / T;
-    bool seen = false;
-    for (int i = 0; i < n; ++i) seen = seen || t[i] == tt;
-    if (!seen) t[n++] = tt;
-  }
-  return n;
+  for (int dz = -g.W3; dz <= g.W3; ++dz)
+    for (int dy = -g.W2; dy <= g.W2; ++dy)
+      for (int dx = -g.W1; dx <= g.W1; ++dx) {
+        if (!hit_test((double)dx - fx, (double)dy - fy, (double)dz - fz, g)) continue;
+        const long long gi = ((long long)wrapg(b.z + dz, g.K3) * g.K2 + wrapg(b.y + dy, g.K2)) * g.K1 +
+                             wrapg(b.x + dx, g.K1);
+        atomicAdd(&cnt_g[gi], 1);
+        ++n;
+      }
+  cnt_s[s] = n;
 }
 
-__global__ void k_touch_count(const int4* __restrict__ base, long long nnz, BinGeom g,
-                              int* __restrict__ ntouch, int* __restrict__ tile_count) {
+// hits in sample-major, z-y-x candidate order: key = grid index, value =
+// (sample << 16) | code, code = (dx+W1) | (dy+W2) << 5 | (dz+W3) << 10
+__global__ void k_hits_fill(const int4* __restrict__ base, const double* __restrict__ off,
+                            long long nnz, HitGeom g, const long long* __restrict__ spos,
+                            unsigned* __restrict__ keys, unsigned long long* __restrict__ vals,
+                            unsigned short* __restrict__ fcode) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
   const int4 b = base[s];
-  int tx[16], ty[16], tz[16];
-  const int nx = axis_tiles(b.x, g.W1, g.K1, g.T1, tx);
-  const int ny = axis_tiles(b.y, g.W2, g.K2, g.T2, ty);
-  const int nz = axis_tiles(b.z, g.W3, g.K3, g.T3, tz);
-  ntouch[s] = nx * ny * nz;
-  for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) atomicAdd(&tile_count[(tz[k] * g.nt2 + ty[j]) * g.nt1 + tx[i]], 1);
-}
-
-__global__ void k_touch_fill(const int4* __restrict__ base, long long nnz, BinGeom g,
-                             const int* __restrict__ toff, unsigned* __restrict__ keys,
-                             int* __restrict__ vals) {
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nnz) return;
-  const int4 b = base[s];
-  int tx[16], ty[16], tz[16];
-  const int nx = axis_tiles(b.x, g.W1, g.K1, g.T1, tx);
-  const int ny = axis_tiles(b.y, g.W2, g.K2, g.T2, ty);
-  const int nz = axis_tiles(b.z, g.W3, g.K3, g.T3, tz);
-  int o = toff[s];
-  for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        keys[o] = (unsigned)((tz[k] * g.nt2 + ty[j]) * g.nt1 + tx[i]);
-        vals[o] = (int)s;
+  const double fx = off[3 * s], fy = off[3 * s + 1], fz = off[3 * s + 2];
+  long long o = spos[s];
+  for (int dz = -g.W3; dz <= g.W3; ++dz)
+    for (int dy = -g.W2; dy <= g.W2; ++dy)
+      for (int dx = -g.W1; dx <= g.W1; ++dx) {
+        if (!hit_test((double)dx - fx, (double)dy - fy, (double)dz - fz, g)) continue;
+        const long long gi = ((long long)wrapg(b.z + dz, g.K3) * g.K2 + wrapg(b.y + dy, g.K2)) * g.K1 +
+                             wrapg(b.x + dx, g.K1);
+        const unsigned code = (unsigned)(dx + g.W1) | ((unsigned)(dy + g.W2) << 5) |
+                              ((unsigned)(dz + g.W3) << 10);
+        keys[o] = (unsigned)gi;
+        vals[o] = ((unsigned long long)s << 16) | code;
+        fcode[o] = (unsigned short)code;
         ++o;
       }
+}
+
+__global__ void k_split_vals(const unsigned long long* __restrict__ vals, long long n,
+                             int* __restrict__ hit_s, unsigned short* __restrict__ acode) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long v = vals[i];
+  hit_s[i] = (int)(v >> 16);
+  acode[i] = (unsigned short)(v & 0xffffu);
+}
+
+__global__ void k_off32(const double* __restrict__ off, long long nnz, float4* __restrict__ o32) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  float4 v;
+  v.x = (float)off[3 * s]; v.y = (float)off[3 * s + 1]; v.z = (float)off[3 * s + 2]; v.w = 0.f;
+  o32[s] = v;
 }
 
 // ---------------------------------------------------------------- density kernels
@@ -512,78 +545,112 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   (void)hipFree(keys); (void)hipFree(skeys); (void)hipFree(vals); (void)hipFree(svals);
   (void)hipFree(om);
 
-  // tile lists: every sample is listed in each tile its footprint touches
-  int *ntouch = nullptr, *toff = nullptr;
-  N3D_HIP(hipMalloc(&ntouch, sizeof(int) * (nnz + 1)));
-  N3D_HIP(hipMalloc(&toff, sizeof(int) * (nnz + 1)));
-  N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (p->ntiles + 1), p->stream));
-  N3D_HIP(hipMemsetAsync(ntouch, 0, sizeof(int) * (nnz + 1), p->stream));
+  if (p->prec == 1) {
+    N3D_HIP(hipMalloc(&p->d_off32, sizeof(float4) * (nnz > 0 ? nnz : 1)));
+    if (nnz > 0)
+      hipLaunchKernelGGL(k_off32, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
+                         (const double*)p->d_off, nnz, p->d_off32);
+  }
+
+  // interpolation pattern: hit lists per sample (forward) and per grid point
+  // (adjoint); the adjoint lists are sorted by grid index with a stable sort
+  // of the sample-major list, so each point sees its samples in ascending order
+  HitGeom hg;
+  hg.K1 = p->K[0]; hg.K2 = p->K[1]; hg.K3 = p->K[2];
+  hg.W1 = p->W[0]; hg.W2 = p->W[1]; hg.W3 = p->W[2];
+  hg.J2 = p->J * p->J;
+  hg.radial = p->radial;
+  const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
+  int *cnt_s = nullptr, *cnt_g = nullptr;
+  N3D_HIP(hipMalloc(&cnt_s, sizeof(int) * (nnz + 1)));
+  N3D_HIP(hipMalloc(&cnt_g, sizeof(int) * (ngrid + 1)));
+  N3D_HIP(hipMalloc(&p->d_spos, sizeof(long long) * (nnz + 1)));
+  N3D_HIP(hipMalloc(&p->d_rowptr, sizeof(long long) * (ngrid + 1)));
+  N3D_HIP(hipMemsetAsync(cnt_s, 0, sizeof(int) * (nnz + 1), p->stream));
+  N3D_HIP(hipMemsetAsync(cnt_g, 0, sizeof(int) * (ngrid + 1), p->stream));
   if (nnz > 0)
-    hipLaunchKernelGGL(k_touch_count, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
-                       nnz, g, ntouch, counts);
+    hipLaunchKernelGGL(k_hits_count, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
+                       (const double*)p->d_off, nnz, hg, cnt_s, cnt_g);
   tb = 0;
-  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ntouch, toff, (int)(nnz + 1), p->stream));
-  N3D_HIP(hipMalloc(&tmp, tb + 16));
-  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, ntouch, toff, (int)(nnz + 1), p->stream));
-  int nlist = 0;
-  N3D_HIP(hipMemcpyAsync(&nlist, toff + nnz, sizeof(int), hipMemcpyDeviceToHost, p->stream));
-  std::vector<int> hcount(p->ntiles);
-  N3D_HIP(hipMemcpyAsync(hcount.data(), counts, sizeof(int) * p->ntiles, hipMemcpyDeviceToHost,
+  size_t tb2 = 0;
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt_s, p->d_spos, (long long)(nnz + 1),
+                                           p->stream));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt_g, p->d_rowptr, (long long)(ngrid + 1),
+                                           p->stream));
+  N3D_HIP(hipMalloc(&tmp, std::max(tb, tb2) + 16));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt_s, p->d_spos, (long long)(nnz + 1),
+                                           p->stream));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt_g, p->d_rowptr, (long long)(ngrid + 1),
+                                           p->stream));
+  long long nhits = 0;
+  N3D_HIP(hipMemcpyAsync(&nhits, p->d_spos + nnz, sizeof(long long), hipMemcpyDeviceToHost,
                          p->stream));
+  std::vector<int> hcnt(ngrid);
+  N3D_HIP(hipMemcpyAsync(hcnt.data(), cnt_g, sizeof(int) * ngrid, hipMemcpyDeviceToHost, p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(tmp);
-  p->nlist = nlist;
-  unsigned *lk = nullptr, *lks = nullptr;
-  int* lv = nullptr;
-  const long long nl = nlist > 0 ? nlist : 1;
-  N3D_HIP(hipMalloc(&lk, sizeof(unsigned) * nl));
-  N3D_HIP(hipMalloc(&lks, sizeof(unsigned) * nl));
-  N3D_HIP(hipMalloc(&lv, sizeof(int) * nl));
-  N3D_HIP(hipMalloc(&p->d_list, sizeof(int) * nl));
+  p->nhits = nhits;
+  const long long nh = nhits > 0 ? nhits : 1;
+  unsigned *hk = nullptr, *hks = nullptr;
+  unsigned long long *hv = nullptr, *hvs = nullptr;
+  N3D_HIP(hipMalloc(&p->d_fcode, sizeof(unsigned short) * nh));
+  N3D_HIP(hipMalloc(&p->d_hit_s, sizeof(int) * nh));
+  N3D_HIP(hipMalloc(&p->d_acode, sizeof(unsigned short) * nh));
+  N3D_HIP(hipMalloc(&hk, sizeof(unsigned) * nh));
+  N3D_HIP(hipMalloc(&hks, sizeof(unsigned) * nh));
+  N3D_HIP(hipMalloc(&hv, sizeof(unsigned long long) * nh));
+  N3D_HIP(hipMalloc(&hvs, sizeof(unsigned long long) * nh));
   if (nnz > 0)
-    hipLaunchKernelGGL(k_touch_fill, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base, nnz,
-                       g, toff, lk, lv);
+    hipLaunchKernelGGL(k_hits_fill, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
+                       (const double*)p->d_off, nnz, hg, (const long long*)p->d_spos, hk, hv,
+                       p->d_fcode);
+  int key_bits = 1;
+  while ((1LL << key_bits) < ngrid) ++key_bits;
   tb = 0;
-  N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, lk, lks, lv, p->d_list, nlist, 0,
-                                             end_bit, p->stream));
+  N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hks, hv, hvs, nhits, 0, key_bits,
+                                             p->stream));
   N3D_HIP(hipMalloc(&tmp, tb + 16));
-  if (nlist > 0)
-    N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, lk, lks, lv, p->d_list, nlist, 0, end_bit,
+  if (nhits > 0) {
+    N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hk, hks, hv, hvs, nhits, 0, key_bits,
                                                p->stream));
-  N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(tmp); (void)hipFree(lk); (void)hipFree(lks); (void)hipFree(lv);
-  (void)hipFree(ntouch); (void)hipFree(toff); (void)hipFree(counts);
-
-  // chunk table (host): split long lists so no work-group owns a hot tile alone
-  std::vector<SpreadChunk> chunks;
-  std::vector<SplitTile> split;
-  int start = 0, slots = 0, maxl = 0;
-  const int CH = p->chunk_len;
-  for (int t = 0; t < p->ntiles; ++t) {
-    const int c = hcount[t];
-    maxl = std::max(maxl, c);
-    if (c <= CH) {
-      chunks.push_back({t, start, c, -1});
-    } else {
-      const int nch = (c + CH - 1) / CH;
-      split.push_back({t, slots, nch, 0});
-      for (int j = 0; j < nch; ++j)
-        chunks.push_back({t, start + j * CH, std::min(CH, c - j * CH), slots + j});
-      slots += nch;
-    }
-    start += c;
+    hipLaunchKernelGGL(k_split_vals, dim3(nblocks(nhits)), dim3(256), 0, p->stream,
+                       (const unsigned long long*)hvs, nhits, p->d_hit_s, p->d_acode);
   }
-  p->max_list = maxl;
-  p->nchunks = (int)chunks.size();
-  p->nsplit = (int)split.size();
-  p->nslots = slots;
-  N3D_HIP(hipMalloc(&p->d_chunks, sizeof(SpreadChunk) * std::max<size_t>(1, chunks.size())));
-  N3D_HIP(hipMemcpy(p->d_chunks, chunks.data(), sizeof(SpreadChunk) * chunks.size(),
-                    hipMemcpyHostToDevice));
-  N3D_HIP(hipMalloc(&p->d_split, sizeof(SplitTile) * std::max<size_t>(1, split.size())));
-  if (!split.empty())
-    N3D_HIP(hipMemcpy(p->d_split, split.data(), sizeof(SplitTile) * split.size(),
-                      hipMemcpyHostToDevice));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(tmp); (void)hipFree(hk); (void)hipFree(hks); (void)hipFree(hv); (void)hipFree(hvs);
+  (void)hipFree(cnt_s); (void)hipFree(cnt_g); (void)hipFree(counts);
+
+  // dense grid points (the k-space centre of radial trajectories) are split
+  // into segments of chunk_len hits whose partial sums add in segment order
+  std::vector<HeavySeg> hseg;
+  std::vector<HeavyPt> hpt;
+  {
+    long long pos = 0;
+    int maxc = 0;
+    for (long long gi = 0; gi < ngrid; ++gi) {
+      const int c = hcnt[gi];
+      maxc = std::max(maxc, c);
+      if (c > p->rmax) {
+        const int nseg = (c + p->chunk_len - 1) / p->chunk_len;
+        hpt.push_back({(int)gi, (int)hseg.size(), nseg, 0});
+        for (int j = 0; j < nseg; ++j) {
+          const long long a = pos + (long long)j * p->chunk_len;
+          const long long e = std::min(pos + c, a + p->chunk_len);
+          hseg.push_back({a, e, (int)gi, (int)hseg.size()});
+        }
+      }
+      pos += c;
+    }
+    p->max_hits = maxc;
+  }
+  p->nhseg = (int)hseg.size();
+  p->nhpt = (int)hpt.size();
+  N3D_HIP(hipMalloc(&p->d_hseg, sizeof(HeavySeg) * std::max<size_t>(1, hseg.size())));
+  N3D_HIP(hipMalloc(&p->d_hpt, sizeof(HeavyPt) * std::max<size_t>(1, hpt.size())));
+  if (!hseg.empty()) {
+    N3D_HIP(hipMemcpy(p->d_hseg, hseg.data(), sizeof(HeavySeg) * hseg.size(), hipMemcpyHostToDevice));
+    N3D_HIP(hipMemcpy(p->d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
+  }
 
   // deapodization (deap.m) and FFT twiddles
   deap_host(p);
@@ -606,11 +673,14 @@ static void destroy_plan(nufft3d_plan* p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
-                  p->d_list, p->d_chunks, p->d_split};
+                  p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit_s, p->d_acode,
+                  p->d_off32, p->d_hseg, p->d_hpt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < 3; ++a) {
     if (p->ax[a].d_tw) (void)hipFree(p->ax[a].d_tw);
+    if (p->axN[a].d_tw) (void)hipFree(p->axN[a].d_tw);
+  }
   for (auto& r : p->prof.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
   release(p->ws_grid); release(p->ws_t1); release(p->ws_t2); release(p->ws_samp);
@@ -637,6 +707,7 @@ int nufft3d_opts_default(nufft3d_opts* o) {
   o->alpha = 0;
   o->radial = 1;
   o->precision = 1;
+  o->compensated = 1;
   return NUFFT3D_OK;
 }
 
@@ -693,8 +764,9 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   for (int a = 0; a < 3; ++a) p->N[a] = N[a];
   p->device = o.device;
   p->verbose = o.verbose;
+  p->compensated = o.compensated ? 1 : 0;
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
-  if (o.chunk > 0) p->chunk_len = o.chunk;
+  if (o.chunk > 0) { p->chunk_len = o.chunk; p->rmax = o.chunk; }
   if (p->T[0] * p->T[1] * p->T[2] > 512) {
     set_error("tile too large (at most 512 grid points per tile)");
     delete p;
@@ -736,8 +808,9 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
     printf(" Created plan in %.3f s: N=[%d %d %d] K=[%d %d %d] J=%g u=%g alpha=%.4f radial=%d\n",
            sec, p->N[0], p->N[1], p->N[2], p->K[0], p->K[1], p->K[2], p->J, p->u, p->alpha,
            p->radial);
-    printf("\ttiles %d (%dx%dx%d) list %lld chunks %d split %d max list %d\n", p->ntiles, p->T[0],
-           p->T[1], p->T[2], (long long)p->nlist, p->nchunks, p->nsplit, p->max_list);
+    printf("\tbins %d (%dx%dx%d) hits %lld (%.1f/sample) max hits/point %d dense points %d "
+           "segments %d\n", p->ntiles, p->T[0], p->T[1], p->T[2], (long long)p->nhits,
+           p->nnz ? (double)p->nhits / p->nnz : 0.0, p->max_hits, p->nhpt, p->nhseg);
   }
   *out = p;
   return NUFFT3D_OK;
@@ -1022,6 +1095,127 @@ int nufft3d_density(nufft3d_plan* p, void* d) {
   int st = finish_out(p, d, esz * p->M, o);
   N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_out);
+  return st;
+}
+
+int nufft3d_pruno_opts_default(nufft3d_pruno_opts* o) {
+  if (!o) { set_error("opts is NULL"); return NUFFT3D_ERR_INVALID; }
+  o->width = 5;     // pruno.m:27-30
+  o->radial = 1;
+  o->noise = -1.0;
+  o->tol = 0.01;
+  o->literal_first_nv = 0;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_gram(nufft3d_plan* p, const void* data, const int32_t dims[4], int32_t width,
+                 int32_t radial, void* AA, int32_t* nout) {
+  N3D_PLAN_GUARD(p);
+  if (!data || !dims || !AA || width < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
+  for (int a = 0; a < 4; ++a)
+    if (dims[a] < 1) { set_error("dims must be positive"); return NUFFT3D_ERR_INVALID; }
+  std::vector<int4> taps;
+  int w = 0;
+  kernel_taps(width, radial, dims[2] == 1, taps, w);
+  const int dd[4] = {dims[0], dims[1], dims[2], dims[3]};
+  const long long n = (long long)taps.size() * dims[3];
+  if (nout) *nout = (int32_t)n;
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t bytes = esz * (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  Staged di, ao;
+  Workspace tmp_in, tmp_out;
+  N3D_TRY(stage_in(p, data, bytes, tmp_in, di));
+  int st = stage_out(p, AA, sizeof(double2) * n * n, tmp_out, ao);
+  if (!st) st = gram_device(p, di.dev, dd, taps, (double2*)ao.dev);
+  if (!st) st = finish_out(p, AA, sizeof(double2) * n * n, ao);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  release(tmp_in);
+  release(tmp_out);
+  return st;
+}
+
+int nufft3d_pruno(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_pruno_opts* opts,
+                  int32_t* nv, double* sv, double* noise) {
+  N3D_PLAN_GUARD(p);
+  if (!data || nc < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
+  nufft3d_pruno_opts o;
+  if (opts) o = *opts; else nufft3d_pruno_opts_default(&o);
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t bytes = esz * (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  Staged di;
+  Workspace tmp_in;
+  N3D_TRY(stage_in(p, data, bytes, tmp_in, di));
+  std::vector<double> svd;
+  int nvv = 0;
+  double nz = 0;
+  int st = pruno_build(p, di.dev, nc, o, 1.0, nullptr, &nvv, &svd, &nz);
+  if (!st) {
+    if (nv) *nv = nvv;
+    if (noise) *noise = nz;
+    if (sv) std::memcpy(sv, svd.data(), sizeof(double) * svd.size());
+  }
+  release(tmp_in);
+  return st;
+}
+
+int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit, double damp,
+                          const double* W, int64_t nW, double lambda,
+                          const nufft3d_pruno_opts* opts, int32_t iters, void* im,
+                          nufft3d_info* info) {
+  N3D_PLAN_GUARD(p);
+  if (!raw || !im || nc < 2) {
+    set_error("parallel-imaging requires multiple coils");               // iNUFT.m:84
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (maxit < 0 || iters < 0 || !(damp >= 0) || !(lambda >= 0) || !std::isfinite(lambda)) {
+    set_error("invalid maxit/iters/damp/lambda");
+    return NUFFT3D_ERR_INVALID;
+  }
+  nufft3d_pruno_opts o;
+  if (opts) o = *opts; else nufft3d_pruno_opts_default(&o);
+  const double tol = 1e-4;                                                 // iNUFT.m:114
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  void* wsamp = nullptr;
+  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  Workspace tmp_out, tmp_b, tmp_x0, tmp_k;
+  Staged ri, xo;
+  void* dB = nullptr;
+  int st = stage_in(p, raw, nk * esz, p->ws_img, ri);
+  if (!st) st = stage_out(p, im, nimg * esz, tmp_out, xo);
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  if (!st) st = ensure(p->ws_grid, ngrid * esz);
+  if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
+  if (!st) st = ensure(tmp_b, nimg * esz);
+  if (!st) st = ensure(tmp_x0, nimg * esz);
+  if (!st) st = ensure(tmp_k, nimg * esz);
+  // b = aNUFT(W.*raw)
+  if (!st) st = permute_in(p, ri.dev, nc, p->ws_samp.p, wsamp);
+  if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr);
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, tmp_b.p, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
+  // calibration image (iNUFT.m:119-136), then k-space (iNUFT.m:203-204)
+  const void* x0 = tmp_b.p;
+  if (!st && maxit > 0) {
+    st = minres_solve(p, tmp_b.p, nc, maxit, damp, wsamp, tol, tmp_x0.p, nullptr);
+    x0 = tmp_x0.p;
+  }
+  if (!st) st = fft_plain(p, x0, nc, tmp_k.p);
+  int nv = 0;
+  if (!st) st = pruno_build(p, tmp_k.p, nc, o, lambda, &dB, &nv, nullptr, nullptr);
+  // coupled least squares (iNUFT.m:210-218)
+  if (!st) st = minres_coupled(p, tmp_b.p, nc, iters, damp, wsamp, dB, tol, xo.dev, info);
+  if (!st) st = finish_out(p, im, nimg * esz, xo);
+  if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error in iNUFT (pruno)");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  if (dB) (void)hipFree(dB);
+  release(tmp_out); release(tmp_b); release(tmp_x0); release(tmp_k);
+  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
 

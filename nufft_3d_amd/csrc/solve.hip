@@ -13,6 +13,7 @@
 // Inner products and norms accumulate in double in a fixed order, so the
 // solve is bitwise reproducible.
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "nufft3d_internal.h"
@@ -263,11 +264,13 @@ int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void
   return NUFFT3D_OK;
 }
 
+// MINRES on nc right-hand sides of length n stored [nc][n]; op(v, y) applies
+// the Hermitian operator to all right-hand sides at once.
 template <typename R>
-static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
-                    const void* wsamp, double tol, void* xout, nufft3d_info* info) {
+static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int maxit, double tol,
+                       void* xout, nufft3d_info* info,
+                       const std::function<int(const void*, void*)>& op) {
   using C = cplx<R>;
-  const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
   const size_t vec = (size_t)n * nc * sizeof(C);
   // vectors: v, r1, r2, y, w, w2 ; x is the caller's output
   N3D_TRY(ensure(p->ws_solve, 6 * vec));
@@ -294,7 +297,7 @@ static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double da
   N3D_HIP(hipMemsetAsync(w, 0, vec, p->stream));
   N3D_HIP(hipMemsetAsync(w2, 0, vec, p->stream));
   for (int itn = 1; itn <= maxit; ++itn) {
-    N3D_TRY(apply_normal(p, v, nc, damp, wsamp, y));
+    N3D_TRY(op(v, y));
     hipLaunchKernelGGL((k_mr_phaseA<R>), grid, dim3(kRedThreads), 0, p->stream, y, (const C*)r1,
                        (const C*)v, n, (const MrState*)st, itn, part);
     hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
@@ -337,6 +340,34 @@ static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double da
     }
   }
   return NUFFT3D_OK;
+}
+
+template <typename R>
+static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                    const void* wsamp, double tol, void* xout, nufft3d_info* info) {
+  const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
+  return minres_core<R>(p, b, n, nc, maxit, tol, xout, info, [&](const void* v, void* y) {
+    return apply_normal(p, v, nc, damp, wsamp, y);
+  });
+}
+
+// Coils coupled through the PRUNO null term: one right-hand side of length
+// N^3*nc (iNUFT.m:214-218, nufft_3d.m:317-320).
+template <typename R>
+static int minres_coupled_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                            const void* wsamp, const void* B, double tol, void* xout,
+                            nufft3d_info* info) {
+  const long long n = (long long)p->N[0] * p->N[1] * p->N[2] * nc;
+  return minres_core<R>(p, b, n, 1, maxit, tol, xout, info, [&](const void* v, void* y) {
+    N3D_TRY(apply_normal(p, v, nc, damp, wsamp, y));
+    return null_apply(p, B, v, y, nc);
+  });
+}
+
+int minres_coupled(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
+                   const void* wsamp, const void* B, double tol, void* x, nufft3d_info* info) {
+  return p->prec == 2 ? minres_coupled_t<double>(p, b, nc, maxit, damp, wsamp, B, tol, x, info)
+                      : minres_coupled_t<float>(p, b, nc, maxit, damp, wsamp, B, tol, x, info);
 }
 
 int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, const void* wsamp,

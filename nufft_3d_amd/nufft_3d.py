@@ -41,7 +41,7 @@ class nufft_3d:
     """Non-uniform 3-D FFT operator (Fessler-style, sparse formulation)."""
 
     def __init__(self, om, N=None, J=4, u=2, alpha=0, radial=1, gpu=1, d=None,
-                 tile=0, chunk=0, device=0, verbose=False):
+                 tile=0, chunk=0, device=0, verbose=False, compensated=True):
         if gpu not in (1, 2):
             raise ValueError("gpu must be 1 (single) or 2 (double): the HIP library has no "
                              "CPU path")
@@ -69,6 +69,7 @@ class nufft_3d:
         opts.chunk = int(chunk)
         opts.device = int(device)
         opts.verbose = int(bool(verbose))
+        opts.compensated = int(bool(compensated))
         self._dkeep = None
         if d is not None:
             dd = np.ascontiguousarray(np.reshape(np.asarray(d, dtype=np.float64), (-1,)))
@@ -191,7 +192,7 @@ class nufft_3d:
         return y
 
     def iNUFT(self, raw, maxit=20, damp=0.0, W=None, constraint=None, lambda_=None,
-              tol=1e-4, return_info=False):
+              tol=1e-4, return_info=False, pruno_opts=None, pruno_iters=100):
         """Inverse by MINRES on A'WA + damp^2 I (iNUFT.m:2-138,225)."""
         if maxit is None:
             maxit = 20
@@ -201,12 +202,18 @@ class nufft_3d:
             raise ValueError("maxit must be a nonnegative integer")
         if not (np.isfinite(damp) and damp >= 0):
             raise ValueError("damp must be finite and nonnegative")
-        if constraint:
+        pruno_mode = False
+        if constraint:                                          # iNUFT.m:75-91
             if maxit <= 1:
                 warnings.warn("phase constraint is only effective when maxit>1")
+                constraint = None
             elif constraint not in ("phase-constraint", "compressed-sensing",
                                     "parallel-imaging-sake", "parallel-imaging-pruno"):
                 raise ValueError("unknown constraint")
+            elif lambda_ is None:
+                raise ValueError("lambda must be supplied with %s" % constraint)
+            elif constraint == "parallel-imaging-pruno":
+                pruno_mode = True
             else:
                 raise NotImplementedError(
                     "constraint %r is not implemented on the HIP path" % constraint)
@@ -218,6 +225,9 @@ class nufft_3d:
             Wp, nW = Wa, Wa.size
         info = _lib.Info()
         wptr = Wp.ctypes.data if Wp is not None else None
+        if pruno_mode:
+            return self._inufft_pruno(raw, maxit, damp, wptr, nW, lambda_, pruno_opts or {},
+                                      pruno_iters, return_info)
         if _is_torch(raw):
             import torch
             rt = self._torch_in(raw, self.M, "raw")
@@ -244,6 +254,68 @@ class nufft_3d:
             n = min(nc, 64)
             return x, {"iters": list(info.iters[:n]), "relres": list(info.relres[:n]),
                        "seconds": info.seconds}
+        return x
+
+    # ------------------------------------------------------------ SAKE/PRUNO
+    def _popts(self, opts):
+        o = _lib.PrunoOpts()
+        _lib.check(self._lib.nufft3d_pruno_opts_default(C.byref(o)))
+        for k, v in (opts or {}).items():
+            if k == "noise":
+                v = -1.0 if v is None else float(v)
+            if not hasattr(o, k):
+                warnings.warn("'%s' is not a valid option." % k)   # pruno.m:37-39
+                continue
+            setattr(o, k, v)
+        return o
+
+    def _host_c(self, a, shape):
+        return np.asfortranarray(np.reshape(np.asarray(a), shape, order="F"), dtype=self.cdtype)
+
+    def gram(self, data, width=5, radial=1):
+        """AA = make_data_matrix(data) (pruno.m:146-179): complex128 [n n]."""
+        data = np.asarray(data)
+        if data.ndim != 4:
+            raise ValueError("'data' must be a 4d float array")
+        db = self._host_c(data, data.shape)
+        dims = (C.c_int32 * 4)(*data.shape)
+        n = C.c_int32()
+        # size query from the tap count
+        from . import taps as _taps
+        nk = len(_taps.kernel_taps(width, radial, data.shape[2] == 1))
+        AA = np.empty((nk * data.shape[3], nk * data.shape[3]), dtype=np.complex128, order="F")
+        _lib.check(self._lib.nufft3d_gram(self._h, db.ctypes.data, dims, int(width), int(radial),
+                                          AA.ctypes.data, C.byref(n)))
+        assert n.value == AA.shape[0]
+        return AA
+
+    def pruno(self, data, **opts):
+        """pruno(obj,data,...) (pruno.m:2-143): returns dict(nv, sv, noise)."""
+        data = np.asarray(data)
+        nc = data.size // (self.N[0] * self.N[1] * self.N[2])
+        db = self._host_c(data, self.N + (nc,))
+        o = self._popts(opts)
+        from . import taps as _taps
+        n = len(_taps.kernel_taps(o.width, o.radial, self.N[2] == 1)) * nc
+        sv = np.empty(n, dtype=np.float64)
+        nv = C.c_int32()
+        noise = C.c_double()
+        _lib.check(self._lib.nufft3d_pruno(self._h, db.ctypes.data, nc, C.byref(o), C.byref(nv),
+                                           sv.ctypes.data, C.byref(noise)))
+        return {"nv": nv.value, "sv": sv, "noise": noise.value}
+
+    def _inufft_pruno(self, raw, maxit, damp, wptr, nW, lam, opts, iters, return_info):
+        ra = np.asarray(raw)
+        nc = ra.size // self.M
+        rb = np.asfortranarray(np.reshape(ra, (self.M, nc), order="F"), dtype=self.cdtype)
+        x = np.empty((self.N[0], self.N[1], self.N[2], nc), dtype=self.cdtype, order="F")
+        info = _lib.Info()
+        o = self._popts(opts)
+        _lib.check(self._lib.nufft3d_inverse_pruno(self._h, rb.ctypes.data, nc, int(maxit),
+                                                   float(damp), wptr, nW, float(lam), C.byref(o),
+                                                   int(iters), x.ctypes.data, C.byref(info)))
+        if return_info:
+            return x, {"iters": [info.iters[0]], "relres": [info.relres[0]], "seconds": info.seconds}
         return x
 
     # ------------------------------------------------------------ properties

@@ -1,0 +1,192 @@
+"""FP64 oracle of the SAKE/PRUNO calibration path (test infrastructure only).
+
+Restates, in NumPy double precision:
+* ``kernel_taps``       -> kernel offsets of ``pruno.m:56-71`` / ``sake.m:61-71``
+  (``ndgrid(-fix(w/2):fix(w/2))`` masked, column-major order)
+* ``make_data_matrix``  -> ``pruno.m:146-179`` literally (circshift copies,
+  j<=k products, conjugate-symmetric fill)
+* ``gram_fft``          -> the same matrix from circular cross-correlations
+  (pinned against ``make_data_matrix`` on small inputs)
+* ``pruno``             -> ``pruno.m:43-143`` (svd of AA, noise floor,
+  filter, nv, null kernels, image-space transform)
+* ``null_term``         -> the coupled term of ``nufft_3d.m:317-320``
+* ``inufft_pruno``      -> ``iNUFT.m:200-218`` with no preconditioner (the
+  reference passes an undefined ``M``)
+
+Pinning: ``make_data_matrix`` is the literal loop; ``gram_fft`` and the
+device Gram are checked against it; the null projector is compared as the
+phase-invariant ``sum_v (1-f_v)^2 v v^H``; the null operator is checked
+against its direct definition with image-space kernels.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .nufft_oracle import minres
+
+
+def kernel_taps(width=5, radial=1, is2d=False):
+    h = int(width) // 2                          # fix(width/2)
+    r = np.arange(-h, h + 1)
+    if is2d:
+        x, y = np.meshgrid(r, r, indexing='ij')
+        z = np.zeros_like(x)
+    else:
+        x, y, z = np.meshgrid(r, r, r, indexing='ij')
+    if radial:
+        k = np.sqrt(x ** 2 + y ** 2 + z ** 2) <= width / 2
+    else:
+        k = (np.abs(x) <= width / 2) & (np.abs(y) <= width / 2) & (np.abs(z) <= width / 2)
+    kf = k.ravel(order='F')
+    taps = np.stack([x.ravel(order='F')[kf], y.ravel(order='F')[kf], z.ravel(order='F')[kf]], 1)
+    return taps, k
+
+
+def make_data_matrix(data, taps):
+    """pruno.m:146-179: AA(:,j,:,k) = row'*col with circshift copies."""
+    nx, ny, nz, nc = data.shape
+    nk = len(taps)
+    AA = np.zeros((nc, nk, nc, nk), dtype=np.complex128)
+    for j in range(nk):
+        row = np.roll(data, tuple(taps[j]), axis=(0, 1, 2)).reshape(-1, nc, order='F')
+        for k in range(j, nk):
+            col = np.roll(data, tuple(taps[k]), axis=(0, 1, 2)).reshape(-1, nc, order='F')
+            blk = row.conj().T @ col
+            AA[:, j, :, k] = blk
+            AA[:, k, :, j] = blk.conj().T
+    return AA.reshape(nc * nk, nc * nk, order='F')
+
+
+def gram_fft(data, taps):
+    """AA from R_cc'(d) = sum_v conj(D_c[v]) D_c'[v+d], AA = R(s_j - s_k)."""
+    nx, ny, nz, nc = data.shape
+    F = np.fft.fftn(data, axes=(0, 1, 2))
+    nk = len(taps)
+    AA = np.zeros((nc, nk, nc, nk), dtype=np.complex128)
+    for c in range(nc):
+        for c2 in range(nc):
+            R = np.fft.ifftn(np.conj(F[..., c]) * F[..., c2])   # R[d] = sum conj(D_c[v]) D_c2[v+d]
+            for j in range(nk):
+                d = taps[j] - taps       # s_j - s_k for all k
+                AA[c, j, c2, :] = R[d[:, 0] % nx, d[:, 1] % ny, d[:, 2] % nz]
+    return AA.reshape(nc * nk, nc * nk, order='F')
+
+
+def _hist(y, nb):
+    nb = max(1, int(nb))
+    lo, hi = float(np.min(y)), float(np.max(y))
+    if hi == lo:
+        h = np.zeros(nb, int)
+        h[nb // 2] = len(y)
+        return h
+    k = np.floor((np.asarray(y) - lo) / ((hi - lo) / nb)).astype(int)
+    k = np.clip(k, 0, nb - 1)
+    return np.bincount(k, minlength=nb)
+
+
+def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=False, AA=None):
+    """pruno.m:43-143. Returns dict(K, Kt, nv, sv, f, noise, taps, mask)."""
+    nx, ny, nz, nc = data.shape
+    taps, mask = kernel_taps(width, radial, nz == 1)
+    nk = len(taps)
+    density = np.count_nonzero(data) / data.size
+    if density < 0.99:
+        raise ValueError('data must be fully sampled')
+    if AA is None:
+        AA = gram_fft(data, taps)
+    w, V = np.linalg.eigh(AA)
+    order = np.argsort(w)[::-1]
+    W = w[order]
+    V = V[:, order]
+    if noise is None:
+        n = len(W)
+        thr = np.spacing(n * W[0])
+        hi = int(np.count_nonzero(W > thr))
+        lo = 1
+        while lo <= hi:
+            h = _hist(W[lo - 1:hi], round(math.sqrt(hi - lo)))
+            if int(np.argmax(h)) + 1 > 1:
+                break
+            lo += 1
+        lo = min(lo, hi)
+        sigma = math.sqrt(float(np.median(W[lo - 1:max(hi, lo)])))
+        noise = sigma / math.sqrt(density * nx * ny)
+    sigma = noise * math.sqrt(density * nx * ny)              # pruno.m:108
+    Ws = np.sqrt(np.maximum(W, 0))                            # pruno.m:111
+    with np.errstate(divide='ignore'):
+        f = np.where(Ws > 0, np.maximum(0, 1 - sigma ** 2 / np.where(Ws > 0, Ws, 1) ** 2), 0.0)
+    nv = int(np.count_nonzero(f < 1 - tol))                   # pruno.m:115
+    n = nc * nk
+    sel = np.arange(nv) if literal_first_nv else np.arange(n - nv, n)
+    Kt = V[:, sel] * (1 - f[sel])[None, :]                    # [(c + nc*t), v]
+    # pruno.m:124-143: taps into the w^3 cube, zero-padded ifft to N, x N
+    wdt = mask.shape[0]
+    Kc = np.zeros((wdt, wdt, mask.shape[2] if mask.ndim == 3 else 1, nc, nv), dtype=np.complex128)
+    flat = np.zeros((mask.size, nc, nv), dtype=np.complex128)
+    Vr = Kt.reshape(nc, nk, nv, order='F').transpose(1, 0, 2)   # [t, c, v]
+    flat[mask.ravel(order='F')] = Vr
+    Kc = flat.reshape(Kc.shape, order='F')
+    K = Kc
+    for ax in range(3):
+        K = np.fft.ifft(K, n=N[ax], axis=ax) * N[ax]
+    return dict(K=K, Kt=Kt, nv=nv, sv=Ws, f=f, noise=noise, taps=taps, mask=mask)
+
+
+def null_term(fnull, x):
+    """nufft_3d.m:317-320 with anull = conj(fnull): sum_v anull*sum_c fnull*x."""
+    r = np.sum(fnull * x[..., None], axis=3)                  # [N1 N2 N3 nv]
+    return np.sum(np.conj(fnull) * r[:, :, :, None, :], axis=4)
+
+
+def inufft_pruno(orc, raw, maxit, damp, lam, width=5, radial=1, noise=None, tol=0.01,
+                 literal_first_nv=False, iters=100):
+    """iNUFT.m:119-136 then 200-218 (coupled MINRES, no preconditioner)."""
+    N = orc.N
+    raw = np.reshape(np.asarray(raw, dtype=np.complex128), (orc.M, -1), order='F')
+    nc = raw.shape[1]
+    W = orc.d
+    b = orc.aNUFT(W[:, None] * raw)
+    if maxit == 0:
+        x = b
+    else:
+        x, _ = minres(lambda v: orc.iprojection(v, damp, W), b.reshape(-1, nc, order='F'),
+                      1e-4, maxit)
+    x = np.reshape(x, N + (nc,), order='F')
+    xk = np.fft.fft(np.fft.fft(np.fft.fft(x, axis=0), axis=1), axis=2)
+    pr = pruno(xk, N, width, radial, noise, tol, literal_first_nv)
+    fnull = pr['K'] * lam
+
+    def A(v):
+        v4 = np.reshape(v, N + (nc,), order='F')
+        y = orc.iprojection(v4, damp, W).reshape(N + (nc,), order='F')
+        y = y + null_term(fnull, v4)
+        return y.reshape(-1, 1, order='F')
+
+    xs, info = minres(A, b.reshape(-1, 1, order='F'), 1e-4, iters)
+    return xs.reshape(N + (nc,), order='F'), pr, info
+
+
+def null_B(Kt, taps, nc, N, lam):
+    """Per-voxel B(x)[c,c'] = lam^2 sum_v conj(fnull_c,v(x)) fnull_c',v(x) built
+    from the tap-domain kernels via G(d) = sum_{t'-t=d} conj(K_t) K_t'^T and a
+    separable inverse DFT (the reformulation the device path uses)."""
+    nk = len(taps)
+    w = int(taps.max() - taps.min()) + 1 if len(taps) else 1
+    h = (w - 1) // 2
+    D = 2 * w - 1
+    P = np.conj(Kt) @ Kt.T                       # [(t,c),(t',c')]
+    G = np.zeros((D, D, D, nc, nc), dtype=np.complex128)
+    for t in range(nk):
+        for t2 in range(nk):
+            d = taps[t2] - taps[t] + (w - 1)
+            G[d[0], d[1], d[2]] += P[t * nc:(t + 1) * nc, t2 * nc:(t2 + 1) * nc]
+    # ifft kernels sit at cube index (tap + h): d = (t2+h) - (t+h) = t2 - t
+    B = G
+    for ax in range(3):
+        n = N[ax]
+        dd = np.arange(D) - (w - 1)
+        E = np.exp(2j * np.pi * np.outer(np.arange(n), dd) / n)     # [x, d]
+        B = np.moveaxis(np.tensordot(E, np.moveaxis(B, ax, 0), axes=(1, 0)), 0, ax)
+    return lam ** 2 * B                          # [N1 N2 N3 nc nc], B[..., c, c']

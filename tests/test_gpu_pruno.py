@@ -1,0 +1,81 @@
+"""SAKE/PRUNO calibration on the GPU vs the FP64 oracle: Hankel Gram (FP64
+path at 1e-12, split-BF16 MFMA path at 1e-4), PRUNO spectrum and nv, and the
+coupled PRUNO reconstruction."""
+import numpy as np
+import pytest
+
+from nufft_3d_amd import nufft_3d, synth
+from oracle import pruno_oracle as P
+from oracle.nufft_oracle import NufftOracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _data(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.normal(size=shape) + 1j * r.normal(size=shape)
+
+
+@pytest.fixture(scope="module")
+def plans():
+    om = synth.koosh_ball(8, 40, 16)
+    return om, nufft_3d(om, 8, gpu=1), nufft_3d(om, 8, gpu=2)
+
+
+@pytest.mark.parametrize("width,radial", [(3, 1), (3, 0), (4, 1)])
+def test_gram_small(plans, width, radial):
+    om, g32, g64 = plans
+    d = _data((8, 6, 5, 2), 1)
+    taps, _ = P.kernel_taps(width, radial)
+    want = P.make_data_matrix(d, taps)
+    assert rel(g64.gram(d, width, radial), want) < 1e-12
+    got = g32.gram(d.astype(np.complex64), width, radial)
+    assert rel(got, want.astype(np.complex64).astype(np.complex128)) < 1e-4
+    assert np.allclose(got, got.conj().T, rtol=0, atol=1e-6 * np.abs(want).max())
+
+
+def test_gram_bf16x3_many_tiles(plans):
+    # n = 4 coils x 81 taps = 324 -> a 648x648 real Gram: 6 tile rows, V split
+    om, g32, g64 = plans
+    d = _data((24, 24, 24, 4), 2).astype(np.complex64)
+    taps, _ = P.kernel_taps(5, 1)
+    want = P.gram_fft(d.astype(np.complex128), taps)
+    got = g32.gram(d, 5, 1)
+    e = rel(got, want)
+    print("bf16x3 gram rel err %.3e" % e, flush=True)
+    assert e < 1e-4
+
+
+def test_pruno_spectrum_and_nv(plans):
+    om, g32, g64 = plans
+    d = _data((8, 8, 8, 3), 3)
+    ref = P.pruno(d, (8, 8, 8), width=3, noise=0.05)
+    got = g64.pruno(d, width=3, noise=0.05)
+    assert got["nv"] == ref["nv"]
+    assert rel(got["sv"], ref["sv"]) < 1e-10
+    est_ref = P.pruno(d, (8, 8, 8), width=3)
+    est = g64.pruno(d, width=3)
+    assert est["noise"] == pytest.approx(est_ref["noise"], rel=1e-9)
+    got32 = g32.pruno(d.astype(np.complex64), width=3, noise=0.05)
+    assert rel(got32["sv"], ref["sv"]) < 1e-4
+
+
+@pytest.mark.parametrize("gpu,tol", [(2, 1e-8), (1, 1e-3)])
+def test_inverse_pruno(gpu, tol):
+    N, nc = 8, 3
+    om = synth.koosh_ball(N, 60, 16)
+    orc = NufftOracle(om, N)
+    maps = synth.coil_maps(N, nc, seed=1)
+    img = synth.phantom_complex(N)[..., None] * maps
+    raw = orc.fNUFT(img) + 1e-3 * _data((orc.M, nc), 4)
+    want, pr, winfo = P.inufft_pruno(orc, raw, 4, 0.01, 0.3, width=3, noise=1e-3, iters=12)
+    g = nufft_3d(om, N, gpu=gpu)
+    got, info = g.iNUFT(raw, 4, 0.01, constraint="parallel-imaging-pruno", lambda_=0.3,
+                        pruno_opts=dict(width=3, noise=1e-3), pruno_iters=12, return_info=True)
+    e = rel(got, want)
+    print("pruno inverse gpu=%d rel err %.3e nv=%d" % (gpu, e, pr["nv"]), flush=True)
+    assert e < tol

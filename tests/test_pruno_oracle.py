@@ -1,0 +1,84 @@
+"""Pins the PRUNO/SAKE calibration oracle (CPU)."""
+import numpy as np
+import pytest
+
+from oracle import pruno_oracle as P
+from oracle.nufft_oracle import NufftOracle
+from nufft_3d_amd import synth
+
+
+def _data(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.normal(size=shape) + 1j * r.normal(size=shape)
+
+
+def test_tap_counts():
+    # w=5: 81 taps inside the radius-2.5 ball, 125 in the cube; w=4 radial: 33
+    assert len(P.kernel_taps(5, 1)[0]) == 81
+    assert len(P.kernel_taps(5, 0)[0]) == 125
+    assert len(P.kernel_taps(4, 1)[0]) == 33
+    t, m = P.kernel_taps(5, 1)
+    assert tuple(t[0]) == (-1, -1, -2)      # column-major: lowest z first, then y, then x
+
+
+@pytest.mark.parametrize("radial", [1, 0])
+def test_gram_fft_equals_literal_loop(radial):
+    d = _data((6, 5, 4, 3), 1)
+    taps, _ = P.kernel_taps(3, radial)
+    A1 = P.make_data_matrix(d, taps)
+    A2 = P.gram_fft(d, taps)
+    assert np.allclose(A1, A2, rtol=1e-12, atol=1e-10)
+    assert np.allclose(A1, A1.conj().T, atol=1e-10)          # Hermitian
+    w = np.linalg.eigvalsh(A1)
+    assert w.min() > -1e-8 * w.max()                          # PSD
+
+
+def test_gram_block_depends_on_offset_difference():
+    d = _data((5, 5, 5, 2), 2)
+    taps, _ = P.kernel_taps(3, 0)
+    A = P.make_data_matrix(d, taps).reshape(2, len(taps), 2, len(taps), order='F')
+    # blocks (j,k) with equal s_j - s_k are equal
+    seen = {}
+    for j in range(len(taps)):
+        for k in range(len(taps)):
+            key = tuple(taps[j] - taps[k])
+            if key in seen:
+                assert np.allclose(A[:, j, :, k], seen[key], atol=1e-10)
+            else:
+                seen[key] = A[:, j, :, k]
+
+
+def test_null_operator_reformulation_matches_reference_formula():
+    d = _data((6, 6, 6, 3), 3)
+    pr = P.pruno(d, (6, 6, 6), width=3, radial=1, noise=0.5)
+    assert pr['nv'] > 0
+    lam = 0.7
+    fnull = pr['K'] * lam
+    x = _data((6, 6, 6, 3), 4)
+    ref = P.null_term(fnull, x)
+    B = P.null_B(pr['Kt'], pr['taps'], 3, (6, 6, 6), lam)
+    got = np.einsum('xyzab,xyzb->xyza', B, x)
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-10)
+
+
+def test_pruno_selection_literal_vs_null():
+    d = _data((6, 6, 6, 2), 5)
+    a0 = P.pruno(d, (6, 6, 6), width=3, noise=1.0)
+    noise = float(np.median(a0["sv"])) / 60.0         # f < 1-tol for W < 10*sigma: mid-spectrum
+    a = P.pruno(d, (6, 6, 6), width=3, noise=noise)
+    b = P.pruno(d, (6, 6, 6), width=3, noise=noise, literal_first_nv=True)
+    assert a['nv'] == b['nv'] and 0 < a['nv'] < len(a['sv'])
+    # the literal choice picks the largest vectors, whose weights 1-f are small
+    assert np.linalg.norm(b['Kt']) < np.linalg.norm(a['Kt'])
+
+
+def test_inufft_pruno_runs_and_reduces_residual():
+    N = 8
+    om = synth.koosh_ball(N, 40, 16)
+    orc = NufftOracle(om, N)
+    maps = synth.coil_maps(N, 3, seed=1)
+    img = synth.phantom_complex(N)[..., None] * maps
+    raw = orc.fNUFT(img)
+    x, pr, info = P.inufft_pruno(orc, raw, 3, 0.0, 0.1, width=3, noise=1e-3, iters=10)
+    assert x.shape == (N, N, N, 3)
+    assert info['relres'][0] < 1.0

@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--chunk", type=int, default=0, help="spread slice length (0 = library default)")
     ap.add_argument("--tile", type=int, default=0, help="spread tile edge (0 = library default)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="bytes of one device-to-device copy run first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the NUFFT on a single MI355X "
@@ -126,6 +128,15 @@ def main():
     log("plan %.3f s  K=%s nnz=%d" % (plan_s, plan.K, plan.nnz))
     K = plan.K[0]
 
+    if args.copy_probe > 0:
+        # one plain copy of known size: its FETCH_SIZE/WRITE_SIZE readings
+        # calibrate the per-dispatch HBM counters of the NUFFT kernels
+        src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     x = torch.from_numpy(x_host).to(dev)
     k_clean = plan.fNUFT(x)
     torch.cuda.synchronize()

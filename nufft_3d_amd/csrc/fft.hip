@@ -102,59 +102,73 @@ struct Bfly<R, 3> {
   }
 };
 
-// One Stockham stage (Ns = product of the earlier radices).
+// Elements held in registers per thread and stage (L*B <= 256*MaxE<R>)
+template <typename R> struct MaxE { static constexpr int v = sizeof(R) == 8 ? 16 : 32; };
+
+// One in-place Stockham stage (Ns = product of the earlier radices): every
+// thread first loads the inputs of its butterflies into registers, the block
+// synchronises, then the outputs are written to their autosorted places.
 template <typename R, int RAD>
-__device__ __forceinline__ void stockham_stage(const cplx<R>* src, cplx<R>* dst, int L, int B,
-                                               int Ns, const cplx<R>* __restrict__ tw, int sign) {
+__device__ __forceinline__ void stockham_stage(cplx<R>* buf, int L, int B, int Ns,
+                                               const cplx<R>* __restrict__ tw, int sign) {
+  constexpr int NB = (MaxE<R>::v + RAD - 1) / RAD;   // butterflies per thread (upper bound)
   const int Lr = L / RAD;
   const int tstep = L / (Ns * RAD);
-  for (int idx = threadIdx.x; idx < Lr * B; idx += blockDim.x) {
-    const int line = idx % B;
-    const int j = idx / B;
-    const int kk = j % Ns;
-    cplx<R> v[RAD];
+  const int total = Lr * B;
+  cplx<R> v[NB][RAD];
 #pragma unroll
-    for (int r = 0; r < RAD; ++r) v[r] = src[(j + r * Lr) * B + line];
-    if (Ns > 1) {
+  for (int k = 0; k < NB; ++k) {
+    const int idx = threadIdx.x + k * blockDim.x;
+    if (idx < total) {
+      const int line = idx % B;
+      const int j = idx / B;
 #pragma unroll
-      for (int r = 1; r < RAD; ++r) {
-        cplx<R> w = tw[kk * r * tstep];
-        if (sign > 0) w.y = -w.y;
-        v[r] = cmul<R>(v[r], w);
-      }
+      for (int r = 0; r < RAD; ++r) v[k][r] = buf[(j + r * Lr) * B + line];
     }
-    Bfly<R, RAD>::run(v, sign, tw, L);
-    const int idxD = (j / Ns) * Ns * RAD + kk;
-#pragma unroll
-    for (int r = 0; r < RAD; ++r) dst[(idxD + r * Ns) * B + line] = v[r];
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int idx = threadIdx.x + k * blockDim.x;
+    if (idx < total) {
+      const int line = idx % B;
+      const int j = idx / B;
+      const int kk = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+          cplx<R> w = tw[kk * r * tstep];
+          if (sign > 0) w.y = -w.y;
+          v[k][r] = cmul<R>(v[k][r], w);
+        }
+      }
+      Bfly<R, RAD>::run(v[k], sign, tw, L);
+      const int idxD = (j / Ns) * Ns * RAD + kk;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) buf[(idxD + r * Ns) * B + line] = v[k][r];
+    }
+  }
+  __syncthreads();
 }
 
-// In-LDS Stockham FFT of B lines of length L stored a[i*B + line].
-// Returns the buffer holding the result.
+// In-LDS in-place Stockham FFT of B lines of length L stored a[i*B + line]
+// (L*B <= blockDim.x * MaxE).
 template <typename R>
-__device__ cplx<R>* lds_fft(cplx<R>* a, cplx<R>* b, int B, const Stages& st,
-                            const cplx<R>* __restrict__ tw, int sign) {
+__device__ void lds_fft(cplx<R>* a, int B, const Stages& st, const cplx<R>* __restrict__ tw,
+                        int sign) {
   const int L = st.L;
   int Ns = 1;
-  cplx<R>* src = a;
-  cplx<R>* dst = b;
   for (int k = 0; k < st.n; ++k) {
     const int rad = st.r[k];
     switch (rad) {
-      case 4: stockham_stage<R, 4>(src, dst, L, B, Ns, tw, sign); break;
-      case 2: stockham_stage<R, 2>(src, dst, L, B, Ns, tw, sign); break;
-      case 3: stockham_stage<R, 3>(src, dst, L, B, Ns, tw, sign); break;
-      case 5: stockham_stage<R, 5>(src, dst, L, B, Ns, tw, sign); break;
-      case 7: stockham_stage<R, 7>(src, dst, L, B, Ns, tw, sign); break;
-      case 11: stockham_stage<R, 11>(src, dst, L, B, Ns, tw, sign); break;
-      default: stockham_stage<R, 13>(src, dst, L, B, Ns, tw, sign); break;
+      case 4: stockham_stage<R, 4>(a, L, B, Ns, tw, sign); break;
+      case 2: stockham_stage<R, 2>(a, L, B, Ns, tw, sign); break;
+      case 3: stockham_stage<R, 3>(a, L, B, Ns, tw, sign); break;
+      case 5: stockham_stage<R, 5>(a, L, B, Ns, tw, sign); break;
+      default: stockham_stage<R, 7>(a, L, B, Ns, tw, sign); break;
     }
-    __syncthreads();
-    cplx<R>* t = src; src = dst; dst = t;
     Ns *= rad;
   }
-  return src;
 }
 
 // grid position of image index i on an axis of image size n, grid size L
@@ -192,7 +206,8 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ i
                                                    const cplx<R>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
-  cplx<R>* b = a + (size_t)g.K1 * g.B;
+  cplx<R>* stw = a + (size_t)g.K1 * g.B;
+  for (int e = threadIdx.x; e < g.K1; e += blockDim.x) stw[e] = tw[e];
   const long long l0 = (long long)blockIdx.x * g.B;
   for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) { a[e].x = 0; a[e].y = 0; }
   __syncthreads();
@@ -208,7 +223,8 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ i
     a[padpos(x, g.N1, g.K1) * g.B + li] = v;
   }
   __syncthreads();
-  cplx<R>* r = lds_fft<R>(a, b, g.B, st, tw, -1);
+  lds_fft<R>(a, g.B, st, stw, -1);
+  const cplx<R>* r = a;
   for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) {
     int li, kx;
     xline_of<R>(e, g, li, kx);
@@ -228,7 +244,8 @@ __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ i
                                                    const cplx<R>* __restrict__ add, R add_coef) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
-  cplx<R>* b = a + (size_t)g.K1 * g.B;
+  cplx<R>* stw = a + (size_t)g.K1 * g.B;
+  for (int e = threadIdx.x; e < g.K1; e += blockDim.x) stw[e] = tw[e];
   const long long l0 = (long long)blockIdx.x * g.B;
   for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) {
     int li, kx;
@@ -243,7 +260,8 @@ __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ i
     a[kx * g.B + li] = v;
   }
   __syncthreads();
-  cplx<R>* r = lds_fft<R>(a, b, g.B, st, tw, +1);
+  lds_fft<R>(a, g.B, st, stw, +1);
+  const cplx<R>* r = a;
   for (int e = threadIdx.x; e < g.N1 * g.B; e += blockDim.x) {
     const int li = e / g.N1, x = e % g.N1;
     const long long l = l0 + li;
@@ -279,7 +297,8 @@ __global__ void __launch_bounds__(256) k_fft_strided(const cplx<R>* __restrict__
   extern __shared__ unsigned char smem_raw[];
   const int L = st.L;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
-  cplx<R>* b = a + (size_t)L * g.B;
+  cplx<R>* stw = a + (size_t)L * g.B;
+  for (int e = threadIdx.x; e < L; e += blockDim.x) stw[e] = tw[e];
   const long long outer = blockIdx.x / g.nbundle;
   const long long col0 = (blockIdx.x % g.nbundle) * g.B;
   const cplx<R>* src = in + outer * g.Lin * g.inner;
@@ -297,7 +316,8 @@ __global__ void __launch_bounds__(256) k_fft_strided(const cplx<R>* __restrict__
     a[p * g.B + li] = v;
   }
   __syncthreads();
-  cplx<R>* r = lds_fft<R>(a, b, g.B, st, tw, sign);
+  lds_fft<R>(a, g.B, st, stw, sign);
+  const cplx<R>* r = a;
   for (int e = threadIdx.x; e < g.Lout * g.B; e += blockDim.x) {
     const int i = e / g.B, li = e % g.B;
     const long long col = col0 + li;
@@ -313,7 +333,7 @@ static void factorize(int L, Stages& st) {
   st.n = 0;
   int m = L;
   while (m % 4 == 0) { st.r[st.n++] = 4; m /= 4; }
-  const int primes[] = {2, 3, 5, 7, 11, 13};
+  const int primes[] = {2, 3, 5, 7};
   for (int q : primes)
     while (m % q == 0) { st.r[st.n++] = q; m /= q; }
   if (m != 1) st.n = -1;
@@ -327,7 +347,7 @@ int fft_setup_axis(nufft3d_plan* p, int axis) {
   factorize(L, st);
   if (st.n < 0) {
     set_error("oversampled size K=" + std::to_string(L) +
-              " has a prime factor above 13 (radices 2,3,4,5,7,11,13 supported)");
+              " has a prime factor above 7 (radices 2, 3, 4, 5, 7 supported)");
     return NUFFT3D_ERR_UNSUPPORTED;
   }
   a.nstage = st.n;
@@ -358,10 +378,19 @@ static Stages stages_of(const FftAxis& a) {
   return st;
 }
 
-// lines per LDS bundle so that two buffers fit a 64 KiB budget
+// lines per LDS bundle: L*B elements staged in registers by 256 threads
+// (at most MaxE each) and held in one LDS buffer of at most 64 KiB
+static int g_fft_bundle = 0;   // requested lines per work-group (0 = automatic)
+
 static int bundle_for(int L, size_t esz) {
-  const size_t budget = 65536;
-  int B = (int)(budget / (2 * (size_t)L * esz));
+  int B = 256 * (esz == sizeof(double2) ? MaxE<double>::v : MaxE<float>::v) / L;
+  const int bl = (int)(65536 / ((size_t)L * esz));
+  if (bl < B) B = bl;
+  if (g_fft_bundle > 0 && g_fft_bundle <= B) {
+    int b = 1;
+    while (b * 2 <= g_fft_bundle) b *= 2;
+    return b;
+  }
   if (B >= 16) return 16;
   if (B >= 8) return 8;
   if (B >= 4) return 4;
@@ -379,6 +408,7 @@ static int bundle_x(int L, size_t esz, int nc) {
 
 template <typename R>
 static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
+  g_fft_bundle = p->fft_bundle;
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -397,7 +427,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K1 * g.B * esz;
+    const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_fwd");
     hipLaunchKernelGGL((k_fft_x_fwd<R>), dim3((unsigned)nb), dim3(256), sh, p->stream,
                        (const cplx<R>*)img, (const R*)p->d_U, (cplx<R>*)T1, g,
@@ -409,7 +439,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     g.B = bundle_for(K2, esz);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K2 * g.B * esz;
+    const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_fwd");
     hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * N3)), dim3(256), sh,
                        p->stream, (const cplx<R>*)T1, (cplx<R>*)T2, g, stages_of(p->ax[1]),
@@ -421,7 +451,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     g.B = bundle_for(K3, esz);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K3 * g.B * esz;
+    const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_fwd");
     hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)g.nbundle), dim3(256), sh, p->stream,
                        (const cplx<R>*)T2, (cplx<R>*)grid, g, stages_of(p->ax[2]),
@@ -434,6 +464,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
 template <typename R>
 static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                          const void* add, double add_coef) {
+  g_fft_bundle = p->fft_bundle;
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -448,7 +479,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     g.B = bundle_for(K3, esz);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K3 * g.B * esz;
+    const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_adj");
     hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)g.nbundle), dim3(256), sh, p->stream,
                        (const cplx<R>*)grid, (cplx<R>*)p->ws_t2.p, g, stages_of(p->ax[2]),
@@ -461,7 +492,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     g.B = bundle_for(K2, esz);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K2 * g.B * esz;
+    const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_adj");
     hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * N3)), dim3(256), sh,
                        p->stream, (const cplx<R>*)T2, (cplx<R>*)T1, g, stages_of(p->ax[1]),
@@ -474,7 +505,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)K1 * g.B * esz;
+    const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_adj");
     hipLaunchKernelGGL((k_fft_x_adj<R>), dim3((unsigned)nb), dim3(256), sh, p->stream,
                        (const cplx<R>*)T1, (const R*)p->d_U, (cplx<R>*)img, g,
@@ -522,7 +553,7 @@ static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
     g.B = ps.inner >= 16 ? bundle_for(ps.n, esz) : 1;
     g.inner = ps.inner;
     g.nbundle = (g.inner + g.B - 1) / g.B;
-    const size_t sh = 2 * (size_t)ps.n * g.B * esz;
+    const size_t sh = (size_t)ps.n * (g.B + 1) * esz;
     void* dst = bufs[which];
     ProfScope prs(p, "k_fft_plain");
     hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * ps.outer)), dim3(256), sh,

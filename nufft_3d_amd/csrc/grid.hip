@@ -13,6 +13,7 @@
 namespace nufft3d {
 
 struct GridGeom {
+  int flags;  // bit 0: XCD-contiguous block renumbering; bit 1: brick-order spread
   int K1, K2, K3;
   int W1, W2, W3;
   int nc;     // coils in the grid / sample arrays
@@ -21,6 +22,18 @@ struct GridGeom {
   int nnz;
   long long M;
 };
+
+// Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2. Renumber blocks so that XCD x receives
+// one contiguous run of logical blocks: neighbouring samples / grid points,
+// which share grid rows and samples, then meet in the same L2.
+__device__ __forceinline__ long long xcd_block(int remap) {
+  if (!remap) return blockIdx.x;
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb / 8, r = nb % 8, x = b % 8;
+  const unsigned first = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return (long long)first + b / 8;
+}
 
 __device__ __forceinline__ int wrapk(int g, int K) {
   return g < 0 ? g + K : (g >= K ? g - K : g);
@@ -83,7 +96,7 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
                                                 const R* __restrict__ wsamp, R scale,
                                                 GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long gt = xcd_block(g.flags & 1) * blockDim.x + threadIdx.x;
   const int s = (int)(gt / LPS);
   const int q = (int)(gt % LPS);
   if (s >= g.nnz) return;
@@ -142,10 +155,20 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
                                                 cplx<R>* __restrict__ grid, long long ngrid,
                                                 int rmax, GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long gpt = gt / LPS;
+  const long long gt = xcd_block(g.flags & 1) * blockDim.x + threadIdx.x;
+  const long long lin = gt / LPS;
   const int q = (int)(gt % LPS);
-  if (gpt >= ngrid) return;
+  if (lin >= ngrid) return;
+  long long gpt = lin;                     // linear order (x fastest)
+  if (g.flags & 2) {                       // 8x8x8 brick order
+    const int bx = (g.K1 + 7) / 8, by = (g.K2 + 7) / 8;
+    const long long brick = lin / 512;
+    const int l = (int)(lin % 512);
+    const int tx = (int)(brick % bx), ty = (int)((brick / bx) % by), tz = (int)(brick / ((long long)bx * by));
+    const int x = tx * 8 + (l & 7), y = ty * 8 + ((l >> 3) & 7), z = tz * 8 + (l >> 6);
+    if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
+    gpt = ((long long)z * g.K2 + y) * g.K1 + x;
+  }
   const long long e0 = rowptr[gpt], e1 = rowptr[gpt + 1];
   if (e1 - e0 > rmax) return;
   const int cl0 = q * CPL;
@@ -321,6 +344,7 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
   g.nc = nc; g.c0 = c0; g.cg = cg;
   g.nnz = (int)p->nnz;
   g.M = p->M;
+  g.flags = p->tune_flags;
   return g;
 }
 
@@ -344,12 +368,15 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   KBParams kp = make_kb(p);
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
   {
-    const long long nthreads = ngrid * LPS;
+    // brick order covers whole 8x8x8 bricks (points outside the grid exit)
+    const long long nbrick = (long long)((p->K[0] + 7) / 8) * ((p->K[1] + 7) / 8) * ((p->K[2] + 7) / 8);
+    const long long npts = (p->tune_flags & 2) ? nbrick * 512 : ngrid;
+    const long long nthreads = npts * LPS;
     ProfScope ps(p, "k_spread");
     hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP>), dim3((unsigned)((nthreads + 255) / 256)),
                        dim3(256), 0, p->stream, p->d_rowptr, p->d_hit_s, p->d_acode,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
-                       (cplx<R>*)grid, ngrid, p->rmax, g, kp);
+                       (cplx<R>*)grid, npts, p->rmax, g, kp);
   }
   if (p->nhseg > 0) {
     ProfScope ps(p, "k_spread_heavy");

@@ -95,7 +95,9 @@ struct nufft3d_plan {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int verbose = 0;
-  int compensated = 1;      // fp32 plans: compensated (TwoSum/TwoProduct) accumulation
+  int compensated = 1;      // fp32 plans: products accumulated in double
+  int tune_flags = 2;       // gridding launch layout (see GridGeom::flags)
+  int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
 
   // sorted, in-bounds samples
   int* d_orig = nullptr;    // [nnz] original row of sorted sample s

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -765,6 +766,8 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   p->device = o.device;
   p->verbose = o.verbose;
   p->compensated = o.compensated ? 1 : 0;
+  if (const char* tv = getenv("NUFFT3D_TUNE")) p->tune_flags = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_FFTB")) p->fft_bundle = atoi(tv);
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
   if (o.chunk > 0) { p->chunk_len = o.chunk; p->rmax = o.chunk; }
   if (p->T[0] * p->T[1] * p->T[2] > 512) {

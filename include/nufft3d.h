@@ -103,6 +103,16 @@ int64_t nufft3d_nnz(const nufft3d_plan* plan);
 int nufft3d_get_params(const nufft3d_plan* plan, int32_t K[3], int32_t N[3], double* J,
                        double* u, double* alpha, int32_t* radial, int32_t* precision);
 
+/* Device memory held by a plan: the resident plan arrays (sorted samples,
+ * hit lists, density, U, twiddles) and the workspaces grown by calls so far
+ * (they scale with the largest coil count seen). No counterpart in the
+ * reference: a serving loop admits work against an HBM budget with it. */
+int nufft3d_device_bytes(const nufft3d_plan* plan, int64_t* plan_bytes, int64_t* workspace_bytes);
+
+/* Free the plan's workspaces (they regrow on the next call). No counterpart
+ * in the reference: lets a serving loop give back memory after a large batch. */
+int nufft3d_release_workspaces(nufft3d_plan* plan);
+
 /* Bind subsequent work of this plan to a HIP stream (hipStream_t as void*). */
 int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
 

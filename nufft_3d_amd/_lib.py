@@ -56,6 +56,8 @@ SIGNATURES = {
                                      C.POINTER(C.c_double), C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32)]),
     "nufft3d_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "nufft3d_device_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nufft3d_release_workspaces": (C.c_int, [C.c_void_p]),
     "nufft3d_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nufft3d_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "nufft3d_pruno_opts_default": (C.c_int, [C.POINTER(PrunoOpts)]),

@@ -380,15 +380,14 @@ static Stages stages_of(const FftAxis& a) {
 
 // lines per LDS bundle: L*B elements staged in registers by 256 threads
 // (at most MaxE each) and held in one LDS buffer of at most 64 KiB
-static int g_fft_bundle = 0;   // requested lines per work-group (0 = automatic)
-
-static int bundle_for(int L, size_t esz) {
+// `want` = requested lines per work-group (0 = automatic), from the plan
+static int bundle_for(int L, size_t esz, int want) {
   int B = 256 * (esz == sizeof(double2) ? MaxE<double>::v : MaxE<float>::v) / L;
   const int bl = (int)(65536 / ((size_t)L * esz));
   if (bl < B) B = bl;
-  if (g_fft_bundle > 0 && g_fft_bundle <= B) {
+  if (want > 0 && want <= B) {
     int b = 1;
-    while (b * 2 <= g_fft_bundle) b *= 2;
+    while (b * 2 <= want) b *= 2;
     return b;
   }
   if (B >= 16) return 16;
@@ -398,8 +397,8 @@ static int bundle_for(int L, size_t esz) {
   return 1;
 }
 
-static int bundle_x(int L, size_t esz, int nc) {
-  const int Bmax = bundle_for(L, esz);
+static int bundle_x(int L, size_t esz, int nc, int want) {
+  const int Bmax = bundle_for(L, esz, want);
   if (nc <= Bmax) return nc * (Bmax / nc);
   for (int B = Bmax; B >= 1; --B)
     if (nc % B == 0) return B;
@@ -408,7 +407,6 @@ static int bundle_x(int L, size_t esz, int nc) {
 
 template <typename R>
 static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
-  g_fft_bundle = p->fft_bundle;
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -423,7 +421,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   {  // x-pass
     XGeom g;
     g.N1 = N1; g.K1 = K1; g.nc = nc;
-    g.B = bundle_x(K1, esz, nc);
+    g.B = bundle_x(K1, esz, nc, p->fft_bundle);
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
@@ -436,7 +434,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   {  // y-pass: [N3][N2][K1*nc] -> [N3][K2][K1*nc]
     SGeom g;
     g.Lin = N2; g.Lout = K2; g.n_img = N2; g.mode = 0;
-    g.B = bundle_for(K2, esz);
+    g.B = bundle_for(K2, esz, p->fft_bundle);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
@@ -448,7 +446,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   if (K3 > 1) {  // z-pass: [N3][K2*K1*nc] -> [K3][K2*K1*nc]
     SGeom g;
     g.Lin = N3; g.Lout = K3; g.n_img = N3; g.mode = 0;
-    g.B = bundle_for(K3, esz);
+    g.B = bundle_for(K3, esz, p->fft_bundle);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
@@ -464,7 +462,6 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
 template <typename R>
 static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                          const void* add, double add_coef) {
-  g_fft_bundle = p->fft_bundle;
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -476,7 +473,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     N3D_TRY(ensure(p->ws_t2, t2 * esz));
     SGeom g;
     g.Lin = K3; g.Lout = N3; g.n_img = N3; g.mode = 1;
-    g.B = bundle_for(K3, esz);
+    g.B = bundle_for(K3, esz, p->fft_bundle);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
@@ -489,7 +486,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   {  // y-pass: [N3][K2][K1*nc] -> [N3][N2][K1*nc]
     SGeom g;
     g.Lin = K2; g.Lout = N2; g.n_img = N2; g.mode = 1;
-    g.B = bundle_for(K2, esz);
+    g.B = bundle_for(K2, esz, p->fft_bundle);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
@@ -501,7 +498,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   {  // x-pass with crop, U, scale and the optional add term
     XGeom g;
     g.N1 = N1; g.K1 = K1; g.nc = nc;
-    g.B = bundle_x(K1, esz, nc);
+    g.B = bundle_x(K1, esz, nc, p->fft_bundle);
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
@@ -550,7 +547,7 @@ static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
     if (ps.n <= 1) continue;
     SGeom g;
     g.Lin = ps.n; g.Lout = ps.n; g.n_img = ps.n; g.mode = 2;
-    g.B = ps.inner >= 16 ? bundle_for(ps.n, esz) : 1;
+    g.B = ps.inner >= 16 ? bundle_for(ps.n, esz, p->fft_bundle) : 1;
     g.inner = ps.inner;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)ps.n * (g.B + 1) * esz;

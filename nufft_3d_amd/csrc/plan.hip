@@ -826,6 +826,45 @@ int nufft3d_plan_destroy(nufft3d_plan* p) {
 
 int64_t nufft3d_nnz(const nufft3d_plan* p) { return p ? p->nnz : -1; }
 
+int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* workspace_bytes) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  const int64_t nnz = std::max<int64_t>(p->nnz, 1), noob = std::max<int64_t>(p->M - p->nnz, 1);
+  const int64_t ngrid = (int64_t)p->K[0] * p->K[1] * p->K[2];
+  const int64_t nimg = (int64_t)p->N[0] * p->N[1] * p->N[2];
+  const int64_t rsz = p->prec == 2 ? 8 : 4;
+  int64_t b = 4 * nnz + 4 * noob + 16 * nnz + 24 * nnz;              // orig, oob, base, off
+  b += rsz * nimg + rsz * nnz;                                        // U, density
+  b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
+  b += (2 + 4 + 2) * std::max<int64_t>(p->nhits, 1);                  // fcode, hit_s, acode
+  if (p->d_off32) b += 16 * nnz;
+  b += (int64_t)sizeof(HeavySeg) * std::max(1, p->nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, p->nhpt);
+  for (int a = 0; a < 3; ++a) {
+    if (p->ax[a].d_tw) b += 2 * rsz * p->ax[a].L;
+    if (p->axN[a].d_tw) b += 2 * rsz * p->axN[a].L;
+  }
+  const Workspace* ws[] = {&p->ws_grid, &p->ws_t1, &p->ws_t2, &p->ws_samp, &p->ws_part,
+                           &p->ws_img, &p->ws_solve, &p->ws_red};
+  int64_t w = 0;
+  for (const Workspace* x : ws) w += (int64_t)x->bytes;
+  if (plan_bytes) *plan_bytes = b;
+  if (workspace_bytes) *workspace_bytes = w;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_release_workspaces(nufft3d_plan* p) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error while releasing workspaces");
+    return NUFFT3D_ERR_DEVICE;
+  }
+  Workspace* ws[] = {&p->ws_grid, &p->ws_t1, &p->ws_t2, &p->ws_samp, &p->ws_part,
+                     &p->ws_img, &p->ws_solve, &p->ws_red};
+  for (Workspace* x : ws) release(*x);
+  return NUFFT3D_OK;
+}
+
 int nufft3d_get_params(const nufft3d_plan* p, int32_t K[3], int32_t N[3], double* J, double* u,
                        double* alpha, int32_t* radial, int32_t* precision) {
   if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }

@@ -109,6 +109,16 @@ class nufft_3d:
         except Exception:
             pass
 
+    def device_bytes(self):
+        """(plan bytes, workspace bytes) of device memory held by this plan."""
+        pb, wb = C.c_int64(), C.c_int64()
+        _lib.check(self._lib.nufft3d_device_bytes(self._h, C.byref(pb), C.byref(wb)))
+        return pb.value, wb.value
+
+    def release_workspaces(self):
+        """Free the per-call workspaces (they regrow on the next call)."""
+        _lib.check(self._lib.nufft3d_release_workspaces(self._h))
+
     # ------------------------------------------------------------ helpers
     @property
     def cdtype(self):

@@ -13,7 +13,8 @@
 namespace nufft3d {
 
 struct GridGeom {
-  int flags;  // bit 0: XCD-contiguous block renumbering; bit 1: brick-order spread
+  int flags;  // bit 1: brick-order spread
+  int xchunk; // XCD block renumbering run length (0 = off)
   int K1, K2, K3;
   int W1, W2, W3;
   int nc;     // coils in the grid / sample arrays
@@ -24,15 +25,19 @@ struct GridGeom {
 };
 
 // Work-groups are dealt to the 8 XCDs round-robin (block b runs on XCD
-// b % 8), and each XCD has its own L2. Renumber blocks so that XCD x receives
-// one contiguous run of logical blocks: neighbouring samples / grid points,
-// which share grid rows and samples, then meet in the same L2.
-__device__ __forceinline__ long long xcd_block(int remap) {
-  if (!remap) return blockIdx.x;
+// b % 8), and each XCD has its own L2. With a run length C > 0, blocks are
+// renumbered so that XCD x receives runs of C consecutive logical blocks
+// (runs x, x+8, x+16, ...): the blocks an XCD holds at once cover one compact
+// region of samples / grid points, which share grid rows and samples in its
+// L2, while every XCD still gets a share of the dense k-space centre. Blocks
+// past the last whole round of 8C keep their number.
+__device__ __forceinline__ long long xcd_block(int chunk) {
+  if (chunk <= 0) return blockIdx.x;
   const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned q = nb / 8, r = nb % 8, x = b % 8;
-  const unsigned first = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-  return (long long)first + b / 8;
+  const unsigned span = 8u * (unsigned)chunk;
+  if (b >= nb / span * span) return b;
+  const unsigned x = b % 8, k = b / 8;
+  return (long long)((k / chunk) * 8 + x) * chunk + k % chunk;
 }
 
 __device__ __forceinline__ int wrapk(int g, int K) {
@@ -81,11 +86,78 @@ template <typename R, bool COMP> struct AccT { using type = double; };
 template <> struct AccT<float, false> { using type = float; };
 
 // ---------------------------------------------------------------------------
+// Lane groups. LPS consecutive lanes (1, 2 or 4; always inside one DPP quad)
+// share a sample (interp) or grid point (spread), each owning CPL coils.
+// They walk the hit list LPS hits at a time: lane q evaluates the weight and
+// address of hit e+q only, and quad-permute broadcasts hand every lane of the
+// group all LPS weights and addresses. Each lane then accumulates its coils
+// over the hits in list order, so a coil's sum is the same whatever LPS,
+// CPL or the batch around it. All lanes of a group take the same branches
+// (same hit list), so a broadcast never reads an inactive lane.
+template <int LPS, int J>
+__device__ __forceinline__ int grp_bcast_i(int v) {
+  if constexpr (LPS == 1) {
+    return v;
+  } else if constexpr (LPS == 2) {
+    return __builtin_amdgcn_mov_dpp(v, J | (J << 2) | ((J + 2) << 4) | ((J + 2) << 6), 0xF, 0xF,
+                                    false);
+  } else {
+    static_assert(LPS == 4, "lane groups are 1, 2 or 4 lanes");
+    return __builtin_amdgcn_mov_dpp(v, J * 0x55, 0xF, 0xF, false);
+  }
+}
+
+template <int LPS, int J, typename T>
+__device__ __forceinline__ T grp_bcast(T v) {
+  if constexpr (LPS == 1) {
+    return v;
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, grp_bcast_i<LPS, J>(__builtin_bit_cast(int, v)));
+  } else {
+    static_assert(sizeof(T) == 8, "4- or 8-byte values");
+    int2 h = __builtin_bit_cast(int2, v);
+    h.x = grp_bcast_i<LPS, J>(h.x);
+    h.y = grp_bcast_i<LPS, J>(h.y);
+    return __builtin_bit_cast(T, h);
+  }
+}
+
+// ar/ai[c] += w * row[cl0 + c] for the lane's CPL coils. FULL: the group's
+// coils fill every lane and the row start is 16-byte aligned (even coil
+// count and offset), so the coils come in as 16-byte loads. Otherwise coil
+// indices past the group are clamped to its last coil (the row stays in
+// bounds; those accumulators are never stored).
+template <typename R, int CPL, bool FULL, typename A>
+__device__ __forceinline__ void acc_coils(A* ar, A* ai, R w, const cplx<R>* __restrict__ row,
+                                          int cl0, int cg) {
+  cplx<R> val[CPL];
+  if constexpr (FULL && sizeof(R) == 4 && CPL % 2 == 0) {
+    const float4* p4 = reinterpret_cast<const float4*>(row + cl0);
+#pragma unroll
+    for (int c = 0; c < CPL / 2; ++c) {
+      const float4 t = p4[c];
+      val[2 * c].x = t.x; val[2 * c].y = t.y;
+      val[2 * c + 1].x = t.z; val[2 * c + 1].y = t.w;
+    }
+  } else if constexpr (FULL) {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) val[c] = row[cl0 + c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) val[c] = row[cl0 + c < cg ? cl0 + c : cg - 1];
+  }
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    ar[c] += (A)w * (A)val[c].x;
+    ai[c] += (A)w * (A)val[c].y;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Forward: k[s,c] = scale * w_s * sum_{hits g of s} H(s,g) G[g,c] with the
-// samples in the sorted coil-interleaved layout. LPS lanes share a sample,
-// each owning CPL consecutive coils, so a grid point's coils are read as one
-// contiguous run by neighbouring lanes. Hits come in z-y-x candidate order.
-template <typename R, int CPL, int LPS, bool COMP>
+// samples in the sorted coil-interleaved layout. Hits come in z-y-x candidate
+// order.
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ spos,
                                                 const unsigned short* __restrict__ fcode,
                                                 const int4* __restrict__ base,
@@ -96,7 +168,7 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
                                                 const R* __restrict__ wsamp, R scale,
                                                 GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
-  const long long gt = xcd_block(g.flags & 1) * blockDim.x + threadIdx.x;
+  const long long gt = xcd_block(g.xchunk) * blockDim.x + threadIdx.x;
   const int s = (int)(gt / LPS);
   const int q = (int)(gt % LPS);
   if (s >= g.nnz) return;
@@ -109,23 +181,28 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
   const long long e0 = spos[s], e1 = spos[s + 1];
-#pragma unroll 4
-  for (long long e = e0; e < e1; ++e) {
+  // lane q takes hit e+q of each step; past the end of the list it repeats
+  // the step's first hit with weight 0 (a point the sample does touch).
+  // Codes are loaded one step ahead.
+  unsigned short code = e0 < e1 ? fcode[e0 + q < e1 ? e0 + q : e0] : 0;
+  for (long long e = e0; e < e1; e += LPS) {
+    const unsigned short cur = code;
+    if (e + LPS < e1) code = fcode[e + LPS + q < e1 ? e + LPS + q : e + LPS];
     int dx, dy, dz;
-    decode_hit(fcode[e], g, dx, dy, dz);
+    decode_hit(cur, g, dx, dy, dz);
     R w;
     if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, f64, kp);
     else w = hit_weight_f(dx, dy, dz, f32, kp);
+    if (e + q >= e1) w = 0;
     const int gx = wrapk(b.x + dx, g.K1), gy = wrapk(b.y + dy, g.K2), gz = wrapk(b.z + dz, g.K3);
-    const cplx<R>* gp = grid + (((long long)gz * g.K2 + gy) * g.K1 + gx) * g.nc + g.c0 + cl0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (cl0 + c < g.cg) {
-        const cplx<R> v = gp[c];
-        ar[c] += (A)w * (A)v.x;
-        ai[c] += (A)w * (A)v.y;
-      }
+    const int gi = (gz * g.K2 + gy) * g.K1 + gx;
+#define N3D_STEP(J)                                                                          \
+    if constexpr (J < LPS) {                                                                 \
+      const R wj = grp_bcast<LPS, J>(w);                                                     \
+      const int gj = grp_bcast_i<LPS, J>(gi);                                                \
+      acc_coils<R, CPL, FULL>(ar, ai, wj, grid + (long long)gj * g.nc + g.c0, cl0, g.cg);    \
     }
+    N3D_STEP(0) N3D_STEP(1) N3D_STEP(2) N3D_STEP(3)
   }
   R sc = scale;
   if (wsamp) sc *= wsamp[s];
@@ -138,13 +215,55 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
     }
 }
 
+// Sum over hits [e0, e1) of one grid point's list (sorted sample ids, codes
+// alongside), weights times wsamp when given, into ar/ai (LPS-lane group).
+template <typename R, int CPL, int LPS, bool FULL, typename A>
+__device__ __forceinline__ void spread_hits(long long e0, long long e1, int q, int cl0,
+                                            const int* __restrict__ hit_s,
+                                            const unsigned short* __restrict__ acode,
+                                            const double* __restrict__ off,
+                                            const float4* __restrict__ off32,
+                                            const cplx<R>* __restrict__ ks,
+                                            const R* __restrict__ wsamp, const GridGeom& g,
+                                            const KBParams& kp, A* ar, A* ai) {
+  long long h = e0 + q < e1 ? e0 + q : e0;
+  int sn = e0 < e1 ? hit_s[h] : 0;
+  unsigned short cn = e0 < e1 ? acode[h] : 0;
+  for (long long e = e0; e < e1; e += LPS) {
+    const int s = sn;
+    const unsigned short cur = cn;
+    if (e + LPS < e1) {
+      h = e + LPS + q < e1 ? e + LPS + q : e + LPS;
+      sn = hit_s[h];
+      cn = acode[h];
+    }
+    int dx, dy, dz;
+    decode_hit(cur, g, dx, dy, dz);
+    R w;
+    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
+    if (wsamp) w *= wsamp[s];
+    if (e + q >= e1) w = 0;
+#define N3D_SSTEP(J)                                                                       \
+    if constexpr (J < LPS) {                                                               \
+      const R wj = grp_bcast<LPS, J>(w);                                                   \
+      const int sj = grp_bcast_i<LPS, J>(s);                                               \
+      acc_coils<R, CPL, FULL>(ar, ai, wj, ks + (long long)sj * g.nc + g.c0, cl0, g.cg);    \
+    }
+    N3D_SSTEP(0) N3D_SSTEP(1) N3D_SSTEP(2) N3D_SSTEP(3)
+  }
+}
+#undef N3D_STEP
+#undef N3D_SSTEP
+
 // ---------------------------------------------------------------------------
-// Adjoint, output driven: LPS lanes per grid point walk its hit list (sorted
-// sample ids, fixed at plan time) and sum in registers. No atomics: the
-// result is bitwise reproducible and independent of the coil batch. Points
-// with more than rmax hits (the dense k-space centre of radial trajectories)
-// are skipped here and summed by k_spread_heavy + k_spread_hreduce.
-template <typename R, int CPL, int LPS, bool COMP>
+// Adjoint, output driven: an LPS-lane group per grid point walks its hit list
+// (sorted sample ids, fixed at plan time) and sums in registers. No atomics:
+// the result is bitwise reproducible and independent of the coil batch.
+// Points with more than rmax hits (the dense k-space centre of radial
+// trajectories) are skipped here and summed by k_spread_heavy +
+// k_spread_hreduce.
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
                                                 const int* __restrict__ hit_s,
                                                 const unsigned short* __restrict__ acode,
@@ -153,18 +272,18 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
                                                 const cplx<R>* __restrict__ ks,
                                                 const R* __restrict__ wsamp,
                                                 cplx<R>* __restrict__ grid, long long ngrid,
+                                                const int* __restrict__ bricks,
                                                 int rmax, GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
-  const long long gt = xcd_block(g.flags & 1) * blockDim.x + threadIdx.x;
+  const long long gt = xcd_block(g.xchunk) * blockDim.x + threadIdx.x;
   const long long lin = gt / LPS;
   const int q = (int)(gt % LPS);
   if (lin >= ngrid) return;
   long long gpt = lin;                     // linear order (x fastest)
-  if (g.flags & 2) {                       // 8x8x8 brick order
-    const int bx = (g.K1 + 7) / 8, by = (g.K2 + 7) / 8;
-    const long long brick = lin / 512;
+  if (g.flags & 2) {                       // 8x8x8 bricks in the plan's brick order
+    const int t = bricks[lin / 512];
     const int l = (int)(lin % 512);
-    const int tx = (int)(brick % bx), ty = (int)((brick / bx) % by), tz = (int)(brick / ((long long)bx * by));
+    const int tx = t & 1023, ty = (t >> 10) & 1023, tz = t >> 20;
     const int x = tx * 8 + (l & 7), y = ty * 8 + ((l >> 3) & 7), z = tz * 8 + (l >> 6);
     if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
     gpt = ((long long)z * g.K2 + y) * g.K1 + x;
@@ -175,25 +294,7 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-#pragma unroll 4
-  for (long long e = e0; e < e1; ++e) {
-    const int s = hit_s[e];
-    int dx, dy, dz;
-    decode_hit(acode[e], g, dx, dy, dz);
-    R w;
-    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
-    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
-    if (wsamp) w *= wsamp[s];
-    const cplx<R>* v = ks + (long long)s * g.nc + g.c0 + cl0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (cl0 + c < g.cg) {
-        const cplx<R> val = v[c];
-        ar[c] += (A)w * (A)val.x;
-        ai[c] += (A)w * (A)val.y;
-      }
-    }
-  }
+  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, cl0, hit_s, acode, off, off32, ks, wsamp, g, kp, ar, ai);
   cplx<R>* gp = grid + gpt * g.nc + g.c0 + cl0;
 #pragma unroll
   for (int c = 0; c < CPL; ++c)
@@ -201,7 +302,7 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
 }
 
 // One segment of a dense point's hit list -> partial[slot][cg]
-template <typename R, int CPL, int LPS, bool COMP>
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const int* __restrict__ hit_s,
                                                       const unsigned short* __restrict__ acode,
@@ -221,25 +322,8 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-#pragma unroll 4
-  for (long long e = sg.start; e < sg.end; ++e) {
-    const int s = hit_s[e];
-    int dx, dy, dz;
-    decode_hit(acode[e], g, dx, dy, dz);
-    R w;
-    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
-    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
-    if (wsamp) w *= wsamp[s];
-    const cplx<R>* v = ks + (long long)s * g.nc + g.c0 + cl0;
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) {
-      if (cl0 + c < g.cg) {
-        const cplx<R> val = v[c];
-        ar[c] += (A)w * (A)val.x;
-        ai[c] += (A)w * (A)val.y;
-      }
-    }
-  }
+  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, cl0, hit_s, acode, off, off32, ks, wsamp, g, kp,
+                           ar, ai);
   // partial sums keep the accumulator precision until the final reduction
   A* pp = reinterpret_cast<A*>(partial) + ((long long)sg.slot * g.cg + cl0) * 2;
 #pragma unroll
@@ -324,7 +408,6 @@ static KBParams make_kb(const nufft3d_plan* p) {
   k.inv_half2f = (float)(1.0 / k.half2);
   k.alphaf = (float)a;
   k.inv_b0f = (float)(1.0 / b0);
-  k.tief = (float)(1e-4 * k.J2);
   k.large_b0 = a >= 3.75;
   {
     const double y = 3.75 / a;
@@ -345,10 +428,11 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
   g.nnz = (int)p->nnz;
   g.M = p->M;
   g.flags = p->tune_flags;
+  g.xchunk = p->xcd_chunk;
   return g;
 }
 
-template <typename R, int CPL, int LPS, bool COMP>
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
                           const void* wsamp, double scale) {
   GridGeom g = make_geom(p, nc, c0, cg);
@@ -356,12 +440,12 @@ static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int
   const long long nthreads = p->nnz * (long long)LPS;
   const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
-  hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP>), dim3(nb), dim3(256), 0, p->stream, p->d_spos,
+  hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL>), dim3(nb), dim3(256), 0, p->stream, p->d_spos,
                      p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
                      (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, g, kp);
 }
 
-template <typename R, int CPL, int LPS, bool COMP>
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
                           void* partial, const void* wsamp) {
   GridGeom g = make_geom(p, nc, c0, cg);
@@ -369,19 +453,18 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
   {
     // brick order covers whole 8x8x8 bricks (points outside the grid exit)
-    const long long nbrick = (long long)((p->K[0] + 7) / 8) * ((p->K[1] + 7) / 8) * ((p->K[2] + 7) / 8);
-    const long long npts = (p->tune_flags & 2) ? nbrick * 512 : ngrid;
+    const long long npts = (p->tune_flags & 2) ? p->nbrick * 512 : ngrid;
     const long long nthreads = npts * LPS;
     ProfScope ps(p, "k_spread");
-    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP>), dim3((unsigned)((nthreads + 255) / 256)),
+    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
                        dim3(256), 0, p->stream, p->d_rowptr, p->d_hit_s, p->d_acode,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
-                       (cplx<R>*)grid, npts, p->rmax, g, kp);
+                       (cplx<R>*)grid, npts, p->d_brick, p->rmax, g, kp);
   }
   if (p->nhseg > 0) {
     ProfScope ps(p, "k_spread_heavy");
     const long long nthreads = (long long)p->nhseg * LPS;
-    hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP>), dim3((unsigned)((nthreads + 255) / 256)),
+    hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
                        dim3(256), 0, p->stream, p->d_hseg, p->nhseg, p->d_hit_s, p->d_acode,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
                        (cplx<R>*)partial, g, kp);
@@ -398,10 +481,17 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
                        const void* wsamp, double scale) {
   for (int c0 = 0; c0 < nc; c0 += 16) {
     const int cg = nc - c0 < 16 ? nc - c0 : 16;
-#define N3D_GO(CPL, LPS)                                                                  \
-    do {                                                                                  \
-      if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP>(p, in, nc, c0, cg, out, partial, wsamp); \
-      else launch_interp<R, CPL, LPS, COMP>(p, in, nc, c0, cg, out, wsamp, scale);        \
+    // full groups with an even coil stride and offset take aligned vector loads
+    const bool even = (nc % 2 == 0) && (c0 % 2 == 0);
+#define N3D_GO2(CPL, LPS, FULL)                                                                  \
+    do {                                                                                         \
+      if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, partial, wsamp); \
+      else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale);        \
+    } while (0)
+#define N3D_GO(CPL, LPS)                                                                         \
+    do {                                                                                         \
+      if (cg == (CPL) * (LPS) && ((CPL) == 1 || even)) N3D_GO2(CPL, LPS, true);                  \
+      else N3D_GO2(CPL, LPS, false);                                                             \
     } while (0)
     if constexpr (sizeof(R) == 8) {
       if (cg <= 1) N3D_GO(1, 1);
@@ -417,6 +507,7 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
       else N3D_GO(4, 4);
     }
 #undef N3D_GO
+#undef N3D_GO2
   }
 }
 

@@ -96,7 +96,8 @@ struct nufft3d_plan {
   bool own_stream = false;
   int verbose = 0;
   int compensated = 1;      // fp32 plans: products accumulated in double
-  int tune_flags = 2;       // gridding launch layout (see GridGeom::flags)
+  int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order)
+  int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
 
   // sorted, in-bounds samples
@@ -133,6 +134,8 @@ struct nufft3d_plan {
   nufft3d::HeavyPt* d_hpt = nullptr;
   int nhpt = 0;
   int chunk_len = 512;              // hits per heavy segment
+  int* d_brick = nullptr;           // [nbrick] 8x8x8 spread bricks in traversal order (tx|ty<<10|tz<<20)
+  long long nbrick = 0;
   int max_hits = 0;
 
   nufft3d::FftAxis ax[3];

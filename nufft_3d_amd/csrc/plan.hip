@@ -199,7 +199,18 @@ struct BinGeom {
   int ntiles;
   int W1, W2, W3;
   int is2d;
+  int mbits;    // bits per axis of the Morton tile key (0 = linear tile order)
 };
+
+// Interleave the low `bits` bits of (x, y, z) as z y x ... z y x (x lowest):
+// the Z-order curve, under which consecutive keys stay in compact 3-D blocks
+__host__ __device__ __forceinline__ unsigned morton3(unsigned x, unsigned y, unsigned z, int bits) {
+  unsigned k = 0;
+  for (int b = 0; b < bits; ++b)
+    k |= (((x >> b) & 1u) << (3 * b)) | (((y >> b) & 1u) << (3 * b + 1)) |
+         (((z >> b) & 1u) << (3 * b + 2));
+  return k;
+}
 
 __device__ __forceinline__ bool sample_ok(const double* om, long long m, const BinGeom& g,
                                           double& x, double& y, double& z) {
@@ -224,14 +235,18 @@ __global__ void k_keys(const double* __restrict__ om, long long M, BinGeom g,
   const long long m = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   double x, y, z;
-  unsigned key = (unsigned)g.ntiles;
+  // out-of-bounds samples sort last
+  unsigned key = g.mbits > 0 ? (1u << (3 * g.mbits)) : (unsigned)g.ntiles;
   if (sample_ok(om, m, g, x, y, z)) {
     const int bx = wrapb(round(x), g.K1), by = wrapb(round(y), g.K2), bz = wrapb(round(z), g.K3);
-    key = (unsigned)((bz / g.T3) * g.nt2 * g.nt1 + (by / g.T2) * g.nt1 + bx / g.T1);
+    const int tx = bx / g.T1, ty = by / g.T2, tz = bz / g.T3;
+    key = g.mbits > 0 ? morton3(tx, ty, tz, g.mbits)
+                      : (unsigned)(tz * g.nt2 * g.nt1 + ty * g.nt1 + tx);
+  } else {
+    atomicAdd(counts, 1);
   }
   keys[m] = key;
   vals[m] = (int)m;
-  atomicAdd(&counts[key], 1);
 }
 
 __global__ void k_gather(const double* __restrict__ om, const unsigned* __restrict__ skeys,
@@ -504,6 +519,13 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   g.ntiles = p->ntiles;
   g.W1 = p->W[0]; g.W2 = p->W[1]; g.W3 = p->W[2];
   g.is2d = p->N[2] == 1;
+  g.mbits = 0;
+  if (!(p->tune_flags & 4)) {   // Morton tile order unless linear order is asked for
+    int nt = std::max(p->ntile[0], std::max(p->ntile[1], p->ntile[2]));
+    g.mbits = 1;
+    while ((1 << g.mbits) < nt) ++g.mbits;
+  }
+  const long long keymax = g.mbits > 0 ? (1LL << (3 * g.mbits)) : (long long)p->ntiles;
 
   unsigned *keys = nullptr, *skeys = nullptr;
   int *vals = nullptr, *svals = nullptr, *counts = nullptr;
@@ -512,13 +534,13 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   N3D_HIP(hipMalloc(&skeys, sizeof(unsigned) * Mm));
   N3D_HIP(hipMalloc(&vals, sizeof(int) * Mm));
   N3D_HIP(hipMalloc(&svals, sizeof(int) * Mm));
-  N3D_HIP(hipMalloc(&counts, sizeof(int) * (p->ntiles + 1)));
-  N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int) * (p->ntiles + 1), p->stream));
+  N3D_HIP(hipMalloc(&counts, sizeof(int)));
+  N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int), p->stream));
   if (M > 0)
     hipLaunchKernelGGL(k_keys, dim3(nblocks(M)), dim3(256), 0, p->stream, om, (long long)M, g,
                        keys, vals, counts);
   int end_bit = 1;
-  while ((1LL << end_bit) <= p->ntiles) ++end_bit;
+  while ((1LL << end_bit) <= keymax) ++end_bit;
   size_t tb = 0;
   N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, vals, svals, (int)M, 0,
                                              end_bit, p->stream));
@@ -528,7 +550,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, vals, svals, (int)M, 0,
                                                end_bit, p->stream));
   int noob = 0;
-  N3D_HIP(hipMemcpyAsync(&noob, counts + p->ntiles, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipMemcpyAsync(&noob, counts, sizeof(int), hipMemcpyDeviceToHost, p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(tmp);
   p->nnz = M - noob;
@@ -653,6 +675,28 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     N3D_HIP(hipMemcpy(p->d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
   }
 
+  // spread traversal: 8x8x8 bricks of the grid, Z-order unless linear order
+  // is asked for, so consecutive work-groups cover compact regions
+  {
+    const int bx = (p->K[0] + 7) / 8, by = (p->K[1] + 7) / 8, bz = (p->K[2] + 7) / 8;
+    int bits = 1;
+    while ((1 << bits) < std::max(bx, std::max(by, bz))) ++bits;
+    std::vector<std::pair<unsigned, int>> order;
+    order.reserve((size_t)bx * by * bz);
+    for (int z = 0; z < bz; ++z)
+      for (int y = 0; y < by; ++y)
+        for (int x = 0; x < bx; ++x) {
+          const unsigned key = (p->tune_flags & 4) ? (unsigned)order.size() : morton3(x, y, z, bits);
+          order.push_back({key, x | (y << 10) | (z << 20)});
+        }
+    std::sort(order.begin(), order.end());
+    std::vector<int> tab(order.size());
+    for (size_t i = 0; i < order.size(); ++i) tab[i] = order[i].second;
+    p->nbrick = (long long)tab.size();
+    N3D_HIP(hipMalloc(&p->d_brick, sizeof(int) * tab.size()));
+    N3D_HIP(hipMemcpy(p->d_brick, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+  }
+
   // deapodization (deap.m) and FFT twiddles
   deap_host(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2];
@@ -675,7 +719,7 @@ static void destroy_plan(nufft3d_plan* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit_s, p->d_acode,
-                  p->d_off32, p->d_hseg, p->d_hpt};
+                  p->d_off32, p->d_hseg, p->d_hpt, p->d_brick};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int a = 0; a < 3; ++a) {
@@ -768,6 +812,7 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   p->compensated = o.compensated ? 1 : 0;
   if (const char* tv = getenv("NUFFT3D_TUNE")) p->tune_flags = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FFTB")) p->fft_bundle = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
   if (o.chunk > 0) { p->chunk_len = o.chunk; p->rmax = o.chunk; }
   if (p->T[0] * p->T[1] * p->T[2] > 512) {
@@ -838,6 +883,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += (2 + 4 + 2) * std::max<int64_t>(p->nhits, 1);                  // fcode, hit_s, acode
   if (p->d_off32) b += 16 * nnz;
   b += (int64_t)sizeof(HeavySeg) * std::max(1, p->nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, p->nhpt);
+  b += 4 * p->nbrick;
   for (int a = 0; a < 3; ++a) {
     if (p->ax[a].d_tw) b += 2 * rsz * p->ax[a].L;
     if (p->axN[a].d_tw) b += 2 * rsz * p->axN[a].L;

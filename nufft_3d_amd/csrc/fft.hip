@@ -13,9 +13,10 @@
 //             z-pass T2 -> G[kz][ky][kx][c]
 //   adjoint : the reverse, cropping as early as possible, U and 1/sqrt(nnz)
 //             (and the damp^2 x term of iprojection) fused into the x-pass.
-// Each work-group transforms a bundle of B lines staged in LDS with
-// element-major layout (line fastest) and runs a mixed-radix (4,2,3,5,7,..)
-// Stockham autosort between two LDS buffers.
+// Each work-group transforms a bundle of B lines with a mixed-radix
+// (16, 8, 4, 2, 3, 5, 7) Stockham autosort whose stages exchange data in LDS
+// (element-major layout, line fastest); the first stage reads global memory
+// and the last writes it wherever the access pattern stays coalesced.
 #include <cmath>
 #include <vector>
 
@@ -102,20 +103,88 @@ struct Bfly<R, 3> {
   }
 };
 
-// Elements held in registers per thread and stage (L*B <= 256*MaxE<R>)
-template <typename R> struct MaxE { static constexpr int v = sizeof(R) == 8 ? 16 : 32; };
+// Constant roots of unity e^{2 pi i k/16}: cos and sin of 2 pi k / 16
+__device__ __constant__ const double kC16[10] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+    -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613, -1.0,
+    -0.92387953251128675613};
+__device__ __constant__ const double kS16[10] = {
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613, 1.0,
+    0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173, 0.0,
+    -0.38268343236508977173};
 
-// One in-place Stockham stage (Ns = product of the earlier radices): every
-// thread first loads the inputs of its butterflies into registers, the block
-// synchronises, then the outputs are written to their autosorted places.
-template <typename R, int RAD>
-__device__ __forceinline__ void stockham_stage(cplx<R>* buf, int L, int B, int Ns,
-                                               const cplx<R>* __restrict__ tw, int sign) {
+// v * e^{sign 2 pi i k / 16}
+template <typename R>
+__device__ __forceinline__ cplx<R> rot16(cplx<R> v, int k, int sign) {
+  if (k == 0) return v;
+  cplx<R> w;
+  w.x = (R)kC16[k];
+  w.y = (R)(sign < 0 ? -kS16[k] : kS16[k]);
+  return cmul<R>(v, w);
+}
+
+// DFT of size P*Q as Q-point DFTs over r2 of v[r1 + P r2], twiddles
+// w^{r1 q2} (w = e^{sign 2 pi i/(PQ)}), then P-point DFTs over r1:
+// X[q2 + Q q1]. Used for 16 = 4x4 and 8 = 2x4.
+template <typename R, int P, int Q>
+__device__ __forceinline__ void bfly_composite(cplx<R>* v, int sign) {
+  cplx<R> y[P][Q];
+#pragma unroll
+  for (int r1 = 0; r1 < P; ++r1) {
+    cplx<R> t[Q];
+#pragma unroll
+    for (int r2 = 0; r2 < Q; ++r2) t[r2] = v[r1 + P * r2];
+    Bfly<R, Q>::run(t, sign, nullptr, 0);
+#pragma unroll
+    for (int q2 = 0; q2 < Q; ++q2) y[r1][q2] = rot16<R>(t[q2], (16 / (P * Q)) * r1 * q2, sign);
+  }
+#pragma unroll
+  for (int q2 = 0; q2 < Q; ++q2) {
+    cplx<R> t[P];
+#pragma unroll
+    for (int r1 = 0; r1 < P; ++r1) t[r1] = y[r1][q2];
+    Bfly<R, P>::run(t, sign, nullptr, 0);
+#pragma unroll
+    for (int q1 = 0; q1 < P; ++q1) v[q2 + Q * q1] = t[q1];
+  }
+}
+
+template <typename R>
+struct Bfly<R, 8> {
+  static __device__ __forceinline__ void run(cplx<R>* v, int sign, const cplx<R>*, int) {
+    bfly_composite<R, 2, 4>(v, sign);
+  }
+};
+
+template <typename R>
+struct Bfly<R, 16> {
+  static __device__ __forceinline__ void run(cplx<R>* v, int sign, const cplx<R>*, int) {
+    bfly_composite<R, 4, 4>(v, sign);
+  }
+};
+
+// Elements held in registers per thread and stage (L*B <= 256*MaxE<R>)
+template <typename R> struct MaxE { static constexpr int v = 16; };
+
+// One Stockham stage over B lines of length L (Ns = product of the earlier
+// radices). Every thread loads the inputs of its butterflies into registers
+// - from LDS (element-major, a[pos*B + line]) or, for the first stage when
+// FG, straight from global memory through ld(line, pos) - then twiddles,
+// transforms and writes the outputs to their autosorted places - in LDS, or
+// for the last stage when LG straight to global memory through
+// sto(line, pos, value). Barriers: before reading LDS written by the previous
+// stage, and between the reads and the in-place writes of one stage.
+template <typename R, int RAD, bool FG, bool LG, typename LD, typename ST>
+__device__ __forceinline__ void fft_stage(cplx<R>* buf, int L, int B, int Ns,
+                                          const cplx<R>* __restrict__ tw, int sign, bool first,
+                                          bool last, const LD& ld, const ST& sto) {
   constexpr int NB = (MaxE<R>::v + RAD - 1) / RAD;   // butterflies per thread (upper bound)
   const int Lr = L / RAD;
   const int tstep = L / (Ns * RAD);
   const int total = Lr * B;
+  const bool gin = FG && first, gout = LG && last;
   cplx<R> v[NB][RAD];
+  if (!first) __syncthreads();
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const int idx = threadIdx.x + k * blockDim.x;
@@ -123,10 +192,13 @@ __device__ __forceinline__ void stockham_stage(cplx<R>* buf, int L, int B, int N
       const int line = idx % B;
       const int j = idx / B;
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) v[k][r] = buf[(j + r * Lr) * B + line];
+      for (int r = 0; r < RAD; ++r) {
+        if (gin) v[k][r] = ld(line, j + r * Lr);
+        else v[k][r] = buf[(j + r * Lr) * B + line];
+      }
     }
   }
-  __syncthreads();
+  if (!gin && !gout) __syncthreads();
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     const int idx = threadIdx.x + k * blockDim.x;
@@ -145,29 +217,65 @@ __device__ __forceinline__ void stockham_stage(cplx<R>* buf, int L, int B, int N
       Bfly<R, RAD>::run(v[k], sign, tw, L);
       const int idxD = (j / Ns) * Ns * RAD + kk;
 #pragma unroll
-      for (int r = 0; r < RAD; ++r) buf[(idxD + r * Ns) * B + line] = v[k][r];
+      for (int r = 0; r < RAD; ++r) {
+        if (gout) sto(line, idxD + r * Ns, v[k][r]);
+        else buf[(idxD + r * Ns) * B + line] = v[k][r];
+      }
     }
   }
-  __syncthreads();
 }
 
-// In-LDS in-place Stockham FFT of B lines of length L stored a[i*B + line]
-// (L*B <= blockDim.x * MaxE).
-template <typename R>
-__device__ void lds_fft(cplx<R>* a, int B, const Stages& st, const cplx<R>* __restrict__ tw,
-                        int sign) {
+// Mixed-radix Stockham FFT of B lines of length L (L*B <= blockDim.x*MaxE).
+// FG: the first stage reads ld(line, pos); otherwise the input is in buf,
+// written and synchronised by the caller. LG: the last stage writes
+// sto(line, pos, v); otherwise the output is left in buf (the caller
+// synchronises before reading it). The LDS twiddles tw must be visible.
+template <typename R, bool FG, bool LG, typename LD, typename ST>
+__device__ void fft_lines(cplx<R>* buf, int B, const Stages& st, const cplx<R>* __restrict__ tw,
+                          int sign, const LD& ld, const ST& sto) {
   const int L = st.L;
   int Ns = 1;
   for (int k = 0; k < st.n; ++k) {
     const int rad = st.r[k];
+    const bool first = k == 0, last = k == st.n - 1;
     switch (rad) {
-      case 4: stockham_stage<R, 4>(a, L, B, Ns, tw, sign); break;
-      case 2: stockham_stage<R, 2>(a, L, B, Ns, tw, sign); break;
-      case 3: stockham_stage<R, 3>(a, L, B, Ns, tw, sign); break;
-      case 5: stockham_stage<R, 5>(a, L, B, Ns, tw, sign); break;
-      default: stockham_stage<R, 7>(a, L, B, Ns, tw, sign); break;
+      case 16: fft_stage<R, 16, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      case 8: fft_stage<R, 8, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      case 4: fft_stage<R, 4, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      case 2: fft_stage<R, 2, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      case 3: fft_stage<R, 3, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      case 5: fft_stage<R, 5, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
+      default: fft_stage<R, 7, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
     }
     Ns *= rad;
+  }
+}
+
+// Stage plans compiled as straight-line code for the common grid sizes
+// (a runtime loop over stages makes the register allocation cover every
+// radix at once). Plan 0 is the runtime loop for any other size.
+template <int PL> struct PlanRadix { static constexpr int r0 = 1, r1 = 1, r2 = 1; };
+template <> struct PlanRadix<1> { static constexpr int r0 = 16, r1 = 16, r2 = 1; };  // 256
+template <> struct PlanRadix<2> { static constexpr int r0 = 16, r1 = 8, r2 = 1; };   // 128
+template <> struct PlanRadix<3> { static constexpr int r0 = 16, r1 = 8, r2 = 3; };   // 384
+template <> struct PlanRadix<4> { static constexpr int r0 = 16, r1 = 4, r2 = 5; };   // 320
+template <> struct PlanRadix<5> { static constexpr int r0 = 16, r1 = 4, r2 = 1; };   // 64
+template <> struct PlanRadix<6> { static constexpr int r0 = 16, r1 = 8, r2 = 4; };   // 512
+template <> struct PlanRadix<7> { static constexpr int r0 = 16, r1 = 4, r2 = 3; };   // 192
+constexpr int kNumPlans = 8;
+
+template <typename R, bool FG, bool LG, int PL, typename LD, typename ST>
+__device__ __forceinline__ void fft_lines_pl(cplx<R>* buf, int B, const Stages& st,
+                                             const cplx<R>* __restrict__ tw, int sign,
+                                             const LD& ld, const ST& sto) {
+  if constexpr (PL == 0) {
+    fft_lines<R, FG, LG>(buf, B, st, tw, sign, ld, sto);
+  } else {
+    constexpr int r0 = PlanRadix<PL>::r0, r1 = PlanRadix<PL>::r1, r2 = PlanRadix<PL>::r2;
+    constexpr int L = r0 * r1 * r2;
+    fft_stage<R, r0, FG, LG>(buf, L, B, 1, tw, sign, true, false, ld, sto);
+    fft_stage<R, r1, FG, LG>(buf, L, B, r0, tw, sign, false, r2 == 1, ld, sto);
+    if constexpr (r2 > 1) fft_stage<R, r2, FG, LG>(buf, L, B, r0 * r1, tw, sign, false, true, ld, sto);
   }
 }
 
@@ -175,6 +283,19 @@ __device__ void lds_fft(cplx<R>* a, int B, const Stages& st, const cplx<R>* __re
 __device__ __forceinline__ int padpos(int i, int n, int L) {
   int x = i - n / 2;
   return x < 0 ? x + L : x;
+}
+
+// image index at grid position p (inverse of padpos), or -1 in the padding
+__device__ __forceinline__ int unpad(int p, int n, int L) {
+  const int h = n / 2;
+  if (p < n - h) return p + h;
+  if (p >= L - h) return p - L + h;
+  return -1;
+}
+
+template <typename R>
+__device__ __forceinline__ void stage_twiddles(cplx<R>* stw, const cplx<R>* __restrict__ tw, int L) {
+  for (int e = threadIdx.x; e < L; e += blockDim.x) stw[e] = tw[e];
 }
 
 // ---------------------------------------------------------------- x-pass
@@ -186,20 +307,9 @@ struct XGeom {
   long long nlines;   // nyz*nc
 };
 
-template <typename R>
-__device__ __forceinline__ void xline_of(int e, const XGeom& g, int& li, int& kx) {
-  if (g.B % g.nc == 0) {
-    const int per = g.K1 * g.nc;
-    const int j = e / per, rem = e % per;
-    kx = rem / g.nc;
-    li = j * g.nc + rem % g.nc;
-  } else {
-    kx = e / g.B;
-    li = e % g.B;
-  }
-}
-
-template <typename R>
+// Forward: image rows (x fastest) are staged through LDS with coalesced
+// reads; the last stage writes [yz][kx][c] directly (lines = coils fastest).
+template <typename R, int PL>
 __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ img,
                                                    const R* __restrict__ U,
                                                    cplx<R>* __restrict__ out, XGeom g, Stages st,
@@ -207,7 +317,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ i
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
   cplx<R>* stw = a + (size_t)g.K1 * g.B;
-  for (int e = threadIdx.x; e < g.K1; e += blockDim.x) stw[e] = tw[e];
+  stage_twiddles<R>(stw, tw, g.K1);
   const long long l0 = (long long)blockIdx.x * g.B;
   for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) { a[e].x = 0; a[e].y = 0; }
   __syncthreads();
@@ -223,20 +333,19 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ i
     a[padpos(x, g.N1, g.K1) * g.B + li] = v;
   }
   __syncthreads();
-  lds_fft<R>(a, g.B, st, stw, -1);
-  const cplx<R>* r = a;
-  for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) {
-    int li, kx;
-    xline_of<R>(e, g, li, kx);
+  auto ld = [](int, int) { cplx<R> z; z.x = 0; z.y = 0; return z; };
+  auto sto = [=](int li, int kx, cplx<R> v) {
     const long long l = l0 + li;
-    if (l >= g.nlines) continue;
-    const long long yz = l / g.nc;
-    const int c = (int)(l % g.nc);
-    out[(yz * g.K1 + kx) * g.nc + c] = r[kx * g.B + li];
-  }
+    if (l >= g.nlines) return;
+    out[((l / g.nc) * g.K1 + kx) * g.nc + (l % g.nc)] = v;
+  };
+  fft_lines_pl<R, false, true, PL>(a, g.B, st, stw, -1, ld, sto);
 }
 
-template <typename R>
+// Adjoint: the first stage reads [yz][kx][c] directly; the result is cropped
+// out of LDS with coalesced image-row writes (x fastest), with U, the scale
+// and the optional add term.
+template <typename R, int PL>
 __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ in,
                                                    const R* __restrict__ U,
                                                    cplx<R>* __restrict__ img, XGeom g, Stages st,
@@ -245,22 +354,18 @@ __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ i
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
   cplx<R>* stw = a + (size_t)g.K1 * g.B;
-  for (int e = threadIdx.x; e < g.K1; e += blockDim.x) stw[e] = tw[e];
+  stage_twiddles<R>(stw, tw, g.K1);
+  __syncthreads();
   const long long l0 = (long long)blockIdx.x * g.B;
-  for (int e = threadIdx.x; e < g.K1 * g.B; e += blockDim.x) {
-    int li, kx;
-    xline_of<R>(e, g, li, kx);
+  auto ld = [=](int li, int kx) {
     const long long l = l0 + li;
     cplx<R> v; v.x = 0; v.y = 0;
-    if (l < g.nlines) {
-      const long long yz = l / g.nc;
-      const int c = (int)(l % g.nc);
-      v = in[(yz * g.K1 + kx) * g.nc + c];
-    }
-    a[kx * g.B + li] = v;
-  }
+    if (l < g.nlines) v = in[((l / g.nc) * g.K1 + kx) * g.nc + (l % g.nc)];
+    return v;
+  };
+  auto sto = [](int, int, cplx<R>) {};
+  fft_lines_pl<R, true, false, PL>(a, g.B, st, stw, +1, ld, sto);
   __syncthreads();
-  lds_fft<R>(a, g.B, st, stw, +1);
   const cplx<R>* r = a;
   for (int e = threadIdx.x; e < g.N1 * g.B; e += blockDim.x) {
     const int li = e / g.N1, x = e % g.N1;
@@ -279,7 +384,10 @@ __global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ i
 
 // ---------------------------------------------------------------- y/z pass
 // in [outer][Lin][inner] -> out [outer][Lout][inner]; one work-group per
-// (outer, bundle of B consecutive inner columns).
+// (outer, bundle of B consecutive inner columns). The first stage reads the
+// columns from global memory (zero in the padding) and the last stage writes
+// them back (cropped rows skipped), so the only LDS traffic is between
+// stages.
 struct SGeom {
   int Lin, Lout, n_img;   // n_img: image size on this axis (pad/crop map)
   int mode;               // 0: pad (Lin = n_img, Lout = L), 1: crop (Lin = L, Lout = n_img),
@@ -289,7 +397,7 @@ struct SGeom {
   long long nbundle;      // ceil(inner / B)
 };
 
-template <typename R>
+template <typename R, int PL>
 __global__ void __launch_bounds__(256) k_fft_strided(const cplx<R>* __restrict__ in,
                                                      cplx<R>* __restrict__ out, SGeom g,
                                                      Stages st, const cplx<R>* __restrict__ tw,
@@ -298,45 +406,68 @@ __global__ void __launch_bounds__(256) k_fft_strided(const cplx<R>* __restrict__
   const int L = st.L;
   cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
   cplx<R>* stw = a + (size_t)L * g.B;
-  for (int e = threadIdx.x; e < L; e += blockDim.x) stw[e] = tw[e];
+  stage_twiddles<R>(stw, tw, L);
+  __syncthreads();
   const long long outer = blockIdx.x / g.nbundle;
   const long long col0 = (blockIdx.x % g.nbundle) * g.B;
   const cplx<R>* src = in + outer * g.Lin * g.inner;
   cplx<R>* dst = out + outer * g.Lout * g.inner;
-  if (g.mode == 0) {
-    for (int e = threadIdx.x; e < L * g.B; e += blockDim.x) { a[e].x = 0; a[e].y = 0; }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < g.Lin * g.B; e += blockDim.x) {
-    const int i = e / g.B, li = e % g.B;
+  auto ld = [=](int li, int p) {
     const long long col = col0 + li;
-    if (col >= g.inner) continue;
-    const cplx<R> v = src[(long long)i * g.inner + col];
-    const int p = g.mode == 0 ? padpos(i, g.n_img, L) : i;
-    a[p * g.B + li] = v;
-  }
-  __syncthreads();
-  lds_fft<R>(a, g.B, st, stw, sign);
-  const cplx<R>* r = a;
-  for (int e = threadIdx.x; e < g.Lout * g.B; e += blockDim.x) {
-    const int i = e / g.B, li = e % g.B;
+    const int i = g.mode == 0 ? unpad(p, g.n_img, L) : p;
+    cplx<R> v; v.x = 0; v.y = 0;
+    if (i >= 0 && col < g.inner) v = src[(long long)i * g.inner + col];
+    return v;
+  };
+  auto sto = [=](int li, int p, cplx<R> v) {
     const long long col = col0 + li;
-    if (col >= g.inner) continue;
-    const int p = g.mode == 1 ? padpos(i, g.n_img, L) : i;
-    dst[(long long)i * g.inner + col] = r[p * g.B + li];
-  }
+    const int i = g.mode == 1 ? unpad(p, g.n_img, L) : p;
+    if (i >= 0 && col < g.inner) dst[(long long)i * g.inner + col] = v;
+  };
+  fft_lines_pl<R, true, true, PL>(a, g.B, st, stw, sign, ld, sto);
 }
 
 // ---------------------------------------------------------------- host side
+#define N3D_PLAN_SWITCH(ID, LAUNCH)     \
+  switch (ID) {                         \
+    case 1: LAUNCH(1); break;           \
+    case 2: LAUNCH(2); break;           \
+    case 3: LAUNCH(3); break;           \
+    case 4: LAUNCH(4); break;           \
+    case 5: LAUNCH(5); break;           \
+    case 6: LAUNCH(6); break;           \
+    case 7: LAUNCH(7); break;           \
+    default: LAUNCH(0); break;          \
+  }
+// Radices 16 first (the fewest stages, hence LDS exchanges), the leftover
+// power of two as one 2, 4 or 8 (2^(4k+1) as 8*4 instead of 16*2), then the
+// odd primes 3, 5, 7.
 static void factorize(int L, Stages& st) {
   st.L = L;
   st.n = 0;
-  int m = L;
-  while (m % 4 == 0) { st.r[st.n++] = 4; m /= 4; }
-  const int primes[] = {2, 3, 5, 7};
+  int m = L, p2 = 0;
+  while (m % 2 == 0) { m /= 2; ++p2; }
+  int n16 = p2 / 4, rest = p2 % 4;
+  const bool split = rest == 1 && n16 > 0;   // 2^(4k+1) = 16^(k-1) * 8 * 4
+  if (split) --n16;
+  for (int k = 0; k < n16; ++k) st.r[st.n++] = 16;
+  if (split) { st.r[st.n++] = 8; st.r[st.n++] = 4; }
+  else if (rest > 0) st.r[st.n++] = 1 << rest;
+  const int primes[] = {3, 5, 7};
   for (int q : primes)
     while (m % q == 0) { st.r[st.n++] = q; m /= q; }
   if (m != 1) st.n = -1;
+}
+
+// compile-time plan matching a factorization (0: the runtime loop)
+static int plan_id_of(const Stages& st) {
+  const int r0 = st.n > 0 ? st.r[0] : 1, r1 = st.n > 1 ? st.r[1] : 1, r2 = st.n > 2 ? st.r[2] : 1;
+  if (st.n < 2 || st.n > 3) return 0;
+  const int ids[][3] = {{0, 0, 0}, {16, 16, 1}, {16, 8, 1}, {16, 8, 3}, {16, 4, 5}, {16, 4, 1},
+                        {16, 8, 4}, {16, 4, 3}};
+  for (int i = 1; i < kNumPlans; ++i)
+    if (ids[i][0] == r0 && ids[i][1] == r1 && ids[i][2] == r2) return i;
+  return 0;
 }
 
 int fft_setup_axis(nufft3d_plan* p, int axis) {
@@ -352,6 +483,7 @@ int fft_setup_axis(nufft3d_plan* p, int axis) {
   }
   a.nstage = st.n;
   for (int i = 0; i < st.n; ++i) a.radix[i] = st.r[i];
+  a.plan_id = plan_id_of(st);
   const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
   std::vector<unsigned char> host(esz * L);
   for (int t = 0; t < L; ++t) {
@@ -427,9 +559,12 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_fwd");
-    hipLaunchKernelGGL((k_fft_x_fwd<R>), dim3((unsigned)nb), dim3(256), sh, p->stream,
-                       (const cplx<R>*)img, (const R*)p->d_U, (cplx<R>*)T1, g,
-                       stages_of(p->ax[0]), (const cplx<R>*)p->ax[0].d_tw);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_fwd<R, PL>), dim3((unsigned)nb), dim3(256), sh,   \
+                                    p->stream, (const cplx<R>*)img, (const R*)p->d_U,           \
+                                    (cplx<R>*)T1, g, stages_of(p->ax[0]),                       \
+                                    (const cplx<R>*)p->ax[0].d_tw)
+    N3D_PLAN_SWITCH(p->ax[0].plan_id, N3D_L);
+#undef N3D_L
   }
   {  // y-pass: [N3][N2][K1*nc] -> [N3][K2][K1*nc]
     SGeom g;
@@ -439,9 +574,11 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_fwd");
-    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * N3)), dim3(256), sh,
-                       p->stream, (const cplx<R>*)T1, (cplx<R>*)T2, g, stages_of(p->ax[1]),
-                       (const cplx<R>*)p->ax[1].d_tw, -1);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * N3)),  \
+                                    dim3(256), sh, p->stream, (const cplx<R>*)T1, (cplx<R>*)T2, \
+                                    g, stages_of(p->ax[1]), (const cplx<R>*)p->ax[1].d_tw, -1)
+    N3D_PLAN_SWITCH(p->ax[1].plan_id, N3D_L);
+#undef N3D_L
   }
   if (K3 > 1) {  // z-pass: [N3][K2*K1*nc] -> [K3][K2*K1*nc]
     SGeom g;
@@ -451,9 +588,11 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_fwd");
-    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)g.nbundle), dim3(256), sh, p->stream,
-                       (const cplx<R>*)T2, (cplx<R>*)grid, g, stages_of(p->ax[2]),
-                       (const cplx<R>*)p->ax[2].d_tw, -1);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(256), \
+                                    sh, p->stream, (const cplx<R>*)T2, (cplx<R>*)grid, g,        \
+                                    stages_of(p->ax[2]), (const cplx<R>*)p->ax[2].d_tw, -1)
+    N3D_PLAN_SWITCH(p->ax[2].plan_id, N3D_L);
+#undef N3D_L
   }
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
@@ -478,9 +617,11 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_adj");
-    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)g.nbundle), dim3(256), sh, p->stream,
-                       (const cplx<R>*)grid, (cplx<R>*)p->ws_t2.p, g, stages_of(p->ax[2]),
-                       (const cplx<R>*)p->ax[2].d_tw, +1);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(256), \
+                                    sh, p->stream, (const cplx<R>*)grid, (cplx<R>*)p->ws_t2.p,   \
+                                    g, stages_of(p->ax[2]), (const cplx<R>*)p->ax[2].d_tw, +1)
+    N3D_PLAN_SWITCH(p->ax[2].plan_id, N3D_L);
+#undef N3D_L
     T2 = p->ws_t2.p;
   }
   {  // y-pass: [N3][K2][K1*nc] -> [N3][N2][K1*nc]
@@ -491,9 +632,11 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_adj");
-    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * N3)), dim3(256), sh,
-                       p->stream, (const cplx<R>*)T2, (cplx<R>*)T1, g, stages_of(p->ax[1]),
-                       (const cplx<R>*)p->ax[1].d_tw, +1);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * N3)),  \
+                                    dim3(256), sh, p->stream, (const cplx<R>*)T2, (cplx<R>*)T1, \
+                                    g, stages_of(p->ax[1]), (const cplx<R>*)p->ax[1].d_tw, +1)
+    N3D_PLAN_SWITCH(p->ax[1].plan_id, N3D_L);
+#undef N3D_L
   }
   {  // x-pass with crop, U, scale and the optional add term
     XGeom g;
@@ -504,10 +647,13 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_adj");
-    hipLaunchKernelGGL((k_fft_x_adj<R>), dim3((unsigned)nb), dim3(256), sh, p->stream,
-                       (const cplx<R>*)T1, (const R*)p->d_U, (cplx<R>*)img, g,
-                       stages_of(p->ax[0]), (const cplx<R>*)p->ax[0].d_tw, (R)scale,
-                       (const cplx<R>*)add, (R)add_coef);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_adj<R, PL>), dim3((unsigned)nb), dim3(256), sh,   \
+                                    p->stream, (const cplx<R>*)T1, (const R*)p->d_U,           \
+                                    (cplx<R>*)img, g, stages_of(p->ax[0]),                     \
+                                    (const cplx<R>*)p->ax[0].d_tw, (R)scale,                   \
+                                    (const cplx<R>*)add, (R)add_coef)
+    N3D_PLAN_SWITCH(p->ax[0].plan_id, N3D_L);
+#undef N3D_L
   }
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
@@ -553,9 +699,11 @@ static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
     const size_t sh = (size_t)ps.n * (g.B + 1) * esz;
     void* dst = bufs[which];
     ProfScope prs(p, "k_fft_plain");
-    hipLaunchKernelGGL((k_fft_strided<R>), dim3((unsigned)(g.nbundle * ps.outer)), dim3(256), sh,
-                       p->stream, (const cplx<R>*)src, (cplx<R>*)dst, g, stages_of(p->axN[ps.a]),
-                       (const cplx<R>*)p->axN[ps.a].d_tw, -1);
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * ps.outer)), \
+                                    dim3(256), sh, p->stream, (const cplx<R>*)src, (cplx<R>*)dst,  \
+                                    g, stages_of(p->axN[ps.a]), (const cplx<R>*)p->axN[ps.a].d_tw, -1)
+    N3D_PLAN_SWITCH(p->axN[ps.a].plan_id, N3D_L);
+#undef N3D_L
     src = dst;
     which ^= 1;
   }

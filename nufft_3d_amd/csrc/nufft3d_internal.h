@@ -46,6 +46,7 @@ struct FftAxis {
   int L = 1;
   int nstage = 0;
   int radix[24] = {0};
+  int plan_id = 0;        // compiled stage plan (0 = generic stage loop)
   void* d_tw = nullptr;   // L roots of unity exp(-2 pi i t / L), precision of the plan
 };
 

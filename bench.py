@@ -68,6 +68,43 @@ def make_inputs(w, seed=2):
     return om, x
 
 
+def precision_study(p32c, p64, om, N, w, x, data, iters):
+    """Relative L2 error against the FP64 path, per stage, of FP32 with
+    compensated (double) and plain (float) gridding accumulation: fNUFT,
+    aNUFT, the normal operator A'DA, and an iNUFT of `iters` MINRES
+    iterations (damp 0) on the noisy data."""
+    import torch
+    from nufft_3d_amd import nufft_3d
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a.to(torch.complex128) - b) / torch.linalg.vector_norm(b))
+
+    x64 = x.to(torch.complex128)
+    d64 = data.to(torch.complex128)
+    ref = {"fNUFT": p64.fNUFT(x64), "aNUFT": p64.aNUFT(d64), "normal": p64.normal(x64)}
+    t0 = time.perf_counter()
+    ref["iNUFT"], info64 = p64.iNUFT(d64, iters, 0.0, return_info=True)
+    torch.cuda.synchronize()
+    log("fp64 iNUFT(%d) %.2f s" % (iters, time.perf_counter() - t0))
+    out = {"iters": iters, "reference": "FP64 HIP path (pinned to the FP64 oracle at 1e-12 by the "
+                                        "parity tests)",
+           "fp64_relres": info64["relres"][:w["nc"]]}
+    p32p = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1, compensated=False)
+    for name, pl in (("compensated", p32c), ("plain", p32p)):
+        r = {"fNUFT": rel(pl.fNUFT(x), ref["fNUFT"]), "aNUFT": rel(pl.aNUFT(data), ref["aNUFT"]),
+             "normal": rel(pl.normal(x), ref["normal"])}
+        t0 = time.perf_counter()
+        xi, info = pl.iNUFT(data, iters, 0.0, return_info=True)
+        torch.cuda.synchronize()
+        r["iNUFT"] = rel(xi, ref["iNUFT"])
+        r["iNUFT_s"] = round(time.perf_counter() - t0, 4)
+        r["relres"] = info["relres"][:w["nc"]]
+        out[name] = r
+        log("precision", name, r)
+    p32p.close()
+    return out
+
+
 def cpu_baseline(om, x_t, w, budget_s=30.0):
     """Time the FP64 oracle's fNUFT+aNUFT on one coil of the same workload."""
     from oracle.nufft_oracle import NufftOracle
@@ -170,6 +207,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches (streams)")
     ap.add_argument("--max-batch-coils", type=int, default=64)
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: HBM budget in GB")
+    ap.add_argument("--study", action="store_true",
+                    help="per-stage FP32 (plain / compensated) errors against FP64")
+    ap.add_argument("--study-iters", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
@@ -273,6 +313,7 @@ def main():
     achieved = dom_bytes / (dom_avg_ms / 1e3) / 1e9
 
     rel = {}
+    study = None
     if not args.no_check:
         p64 = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=2)
         x64 = x.to(torch.complex128)
@@ -283,6 +324,8 @@ def main():
                              / torch.linalg.vector_norm(k64))
         rel["aNUFT"] = float(torch.linalg.vector_norm(a_out.to(torch.complex128) - a64)
                              / torch.linalg.vector_norm(a64))
+        if args.study:
+            study = precision_study(plan, p64, om, N, w, x, data, args.study_iters)
         del p64, x64, k64, a64
         torch.cuda.empty_cache()
         log("rel err vs fp64:", rel)
@@ -321,6 +364,8 @@ def main():
         "kernels": kstats,
         "cpu_baseline": cpu,
     }
+    if study is not None:
+        out["precision_study"] = study
     print(json.dumps(out), flush=True)
 
 

@@ -122,30 +122,47 @@ __device__ __forceinline__ T grp_bcast(T v) {
   }
 }
 
-// ar/ai[c] += w * row[cl0 + c] for the lane's CPL coils. FULL: the group's
-// coils fill every lane and the row start is 16-byte aligned (even coil
-// count and offset), so the coils come in as 16-byte loads. Otherwise coil
-// indices past the group are clamped to its last coil (the row stays in
-// bounds; those accumulators are never stored).
-template <typename R, int CPL, bool FULL, typename A>
-__device__ __forceinline__ void acc_coils(A* ar, A* ai, R w, const cplx<R>* __restrict__ row,
-                                          int cl0, int cg) {
-  cplx<R> val[CPL];
+// Coils of a lane. Lane q of an LPS-lane group owns CPL of the group's coils:
+// with FULL groups of float data and an even CPL, coil pairs are dealt
+// round-robin over the lanes (lane q: pairs q, q+LPS, ...) so that one
+// 16-byte load per lane makes the group read one contiguous run of 16*LPS
+// bytes of a row; otherwise lane q owns coils q*CPL .. q*CPL+CPL-1.
+template <typename R, int CPL, int LPS, bool FULL>
+__device__ __forceinline__ int coil_of(int q, int c) {
+  if constexpr (FULL && sizeof(R) == 4 && CPL % 2 == 0) return ((c / 2) * LPS + q) * 2 + (c & 1);
+  else return q * CPL + c;
+}
+
+// val[c] = row[coil_of(q, c)] (load_coils), then ar/ai[c] += w * val[c]
+// (fma_coils). FULL: the group's coils fill every lane and the row start is
+// 16-byte aligned (even coil count and offset), so the coils come in as
+// 16-byte loads. Otherwise coil indices past the group are clamped to its
+// last coil (the row stays in bounds; those accumulators are never stored).
+template <typename R, int CPL, int LPS, bool FULL>
+__device__ __forceinline__ void load_coils(cplx<R>* val, const cplx<R>* __restrict__ row, int q,
+                                           int cg) {
   if constexpr (FULL && sizeof(R) == 4 && CPL % 2 == 0) {
-    const float4* p4 = reinterpret_cast<const float4*>(row + cl0);
+    const float4* p4 = reinterpret_cast<const float4*>(row);
 #pragma unroll
     for (int c = 0; c < CPL / 2; ++c) {
-      const float4 t = p4[c];
+      const float4 t = p4[c * LPS + q];
       val[2 * c].x = t.x; val[2 * c].y = t.y;
       val[2 * c + 1].x = t.z; val[2 * c + 1].y = t.w;
     }
   } else if constexpr (FULL) {
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) val[c] = row[cl0 + c];
+    for (int c = 0; c < CPL; ++c) val[c] = row[q * CPL + c];
   } else {
 #pragma unroll
-    for (int c = 0; c < CPL; ++c) val[c] = row[cl0 + c < cg ? cl0 + c : cg - 1];
+    for (int c = 0; c < CPL; ++c) {
+      const int k = q * CPL + c;
+      val[c] = row[k < cg ? k : cg - 1];
+    }
   }
+}
+
+template <typename R, int CPL, typename A>
+__device__ __forceinline__ void fma_coils(A* ar, A* ai, R w, const cplx<R>* val) {
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     ar[c] += (A)w * (A)val[c].x;
@@ -172,7 +189,6 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
   const int s = (int)(gt / LPS);
   const int q = (int)(gt % LPS);
   if (s >= g.nnz) return;
-  const int cl0 = q * CPL;
   const int4 b = base[s];
   float4 f32;
   if constexpr (sizeof(R) == 4) f32 = off32[s];
@@ -184,10 +200,15 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
   // lane q takes hit e+q of each step; past the end of the list it repeats
   // the step's first hit with weight 0 (a point the sample does touch).
   // Codes are loaded one step ahead.
+  // (the prefetch is issued on every path, index clamped into the list, so
+  // the wait counts do not depend on the branch taken)
   unsigned short code = e0 < e1 ? fcode[e0 + q < e1 ? e0 + q : e0] : 0;
   for (long long e = e0; e < e1; e += LPS) {
     const unsigned short cur = code;
-    if (e + LPS < e1) code = fcode[e + LPS + q < e1 ? e + LPS + q : e + LPS];
+    {
+      const long long en = e + LPS < e1 ? e + LPS : e;
+      code = fcode[en + q < e1 ? en + q : en];
+    }
     int dx, dy, dz;
     decode_hit(cur, g, dx, dy, dz);
     R w;
@@ -196,29 +217,38 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
     if (e + q >= e1) w = 0;
     const int gx = wrapk(b.x + dx, g.K1), gy = wrapk(b.y + dy, g.K2), gz = wrapk(b.z + dz, g.K3);
     const int gi = (gz * g.K2 + gy) * g.K1 + gx;
-#define N3D_STEP(J)                                                                          \
-    if constexpr (J < LPS) {                                                                 \
-      const R wj = grp_bcast<LPS, J>(w);                                                     \
-      const int gj = grp_bcast_i<LPS, J>(gi);                                                \
-      acc_coils<R, CPL, FULL>(ar, ai, wj, grid + (long long)gj * g.nc + g.c0, cl0, g.cg);    \
+    R wj[LPS];
+    cplx<R> val[LPS][CPL];
+#define N3D_STEP(J)                                                                   \
+    if constexpr (J < LPS) {                                                          \
+      wj[J] = grp_bcast<LPS, J>(w);                                                   \
+      const int gj = grp_bcast_i<LPS, J>(gi);                                         \
+      load_coils<R, CPL, LPS, FULL>(val[J], grid + (long long)gj * g.nc + g.c0, q, g.cg); \
     }
     N3D_STEP(0) N3D_STEP(1) N3D_STEP(2) N3D_STEP(3)
+#pragma unroll
+    for (int j = 0; j < LPS; ++j) fma_coils<R, CPL>(ar, ai, wj[j], val[j]);
   }
   R sc = scale;
   if (wsamp) sc *= wsamp[s];
-  cplx<R>* op = out + (long long)s * g.nc + g.c0 + cl0;
+  cplx<R>* op = out + (long long)s * g.nc + g.c0;
 #pragma unroll
-  for (int c = 0; c < CPL; ++c)
-    if (cl0 + c < g.cg) {
+  for (int c = 0; c < CPL; ++c) {
+    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+    if (k < g.cg) {
       cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
-      op[c] = v;
+      op[k] = v;
     }
+  }
 }
 
 // Sum over hits [e0, e1) of one grid point's list (sorted sample ids, codes
 // alongside), weights times wsamp when given, into ar/ai (LPS-lane group).
+// Software pipeline over steps of LPS hits: the list entries are loaded two
+// steps ahead and the sample offsets one step ahead, so a step waits for one
+// round trip (its sample values) instead of three dependent ones.
 template <typename R, int CPL, int LPS, bool FULL, typename A>
-__device__ __forceinline__ void spread_hits(long long e0, long long e1, int q, int cl0,
+__device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
                                             const int* __restrict__ hit_s,
                                             const unsigned short* __restrict__ acode,
                                             const double* __restrict__ off,
@@ -226,35 +256,55 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q, i
                                             const cplx<R>* __restrict__ ks,
                                             const R* __restrict__ wsamp, const GridGeom& g,
                                             const KBParams& kp, A* ar, A* ai) {
-  long long h = e0 + q < e1 ? e0 + q : e0;
-  int sn = e0 < e1 ? hit_s[h] : 0;
-  unsigned short cn = e0 < e1 ? acode[h] : 0;
+  if (e0 >= e1) return;
+  // lane q's entry of the step starting at e (the step's first entry past the end)
+  auto ent = [&](long long e) { return e + q < e1 ? e + q : e; };
+  int s0 = hit_s[ent(e0)];
+  unsigned short c0 = acode[ent(e0)];
+  float4 f0;
+  if constexpr (sizeof(R) == 4) f0 = off32[s0];
+  R m0 = wsamp ? wsamp[s0] : (R)1;
+  const long long e1st = e0 + LPS < e1 ? e0 + LPS : e0;
+  int s1 = hit_s[ent(e1st)];
+  unsigned short c1 = acode[ent(e1st)];
   for (long long e = e0; e < e1; e += LPS) {
-    const int s = sn;
-    const unsigned short cur = cn;
-    if (e + LPS < e1) {
-      h = e + LPS + q < e1 ? e + LPS + q : e + LPS;
-      sn = hit_s[h];
-      cn = acode[h];
-    }
+    const int s = s0;
     int dx, dy, dz;
-    decode_hit(cur, g, dx, dy, dz);
+    decode_hit(c0, g, dx, dy, dz);
     R w;
     if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
-    else w = hit_weight_f(dx, dy, dz, off32[s], kp);
-    if (wsamp) w *= wsamp[s];
+    else w = hit_weight_f(dx, dy, dz, f0, kp);
+    w *= m0;
     if (e + q >= e1) w = 0;
-#define N3D_SSTEP(J)                                                                       \
-    if constexpr (J < LPS) {                                                               \
-      const R wj = grp_bcast<LPS, J>(w);                                                   \
-      const int sj = grp_bcast_i<LPS, J>(s);                                               \
-      acc_coils<R, CPL, FULL>(ar, ai, wj, ks + (long long)sj * g.nc + g.c0, cl0, g.cg);    \
+    // this step's sample values
+    R wj[LPS];
+    cplx<R> val[LPS][CPL];
+#define N3D_SLOAD(J)                                                                 \
+    if constexpr (J < LPS) {                                                         \
+      wj[J] = grp_bcast<LPS, J>(w);                                                  \
+      const int sj = grp_bcast_i<LPS, J>(s);                                         \
+      load_coils<R, CPL, LPS, FULL>(val[J], ks + (long long)sj * g.nc + g.c0, q, g.cg); \
     }
-    N3D_SSTEP(0) N3D_SSTEP(1) N3D_SSTEP(2) N3D_SSTEP(3)
+    N3D_SLOAD(0) N3D_SLOAD(1) N3D_SLOAD(2) N3D_SLOAD(3)
+#undef N3D_SLOAD
+    // advance the pipeline while they are in flight; the loads are issued
+    // on every path (indices clamped into the list) so the wait counts do not
+    // depend on the branch taken
+    {
+      const long long en = e + LPS < e1 ? e + LPS : e;
+      const long long enn = e + 2 * LPS < e1 ? e + 2 * LPS : en;
+      s0 = s1;
+      c0 = c1;
+      if constexpr (sizeof(R) == 4) f0 = off32[s0];
+      m0 = wsamp ? wsamp[s0] : (R)1;
+      s1 = hit_s[ent(enn)];
+      c1 = acode[ent(enn)];
+    }
+#pragma unroll
+    for (int j = 0; j < LPS; ++j) fma_coils<R, CPL>(ar, ai, wj[j], val[j]);
   }
 }
 #undef N3D_STEP
-#undef N3D_SSTEP
 
 // ---------------------------------------------------------------------------
 // Adjoint, output driven: an LPS-lane group per grid point walks its hit list
@@ -290,15 +340,16 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   }
   const long long e0 = rowptr[gpt], e1 = rowptr[gpt + 1];
   if (e1 - e0 > rmax) return;
-  const int cl0 = q * CPL;
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, cl0, hit_s, acode, off, off32, ks, wsamp, g, kp, ar, ai);
-  cplx<R>* gp = grid + gpt * g.nc + g.c0 + cl0;
+  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hit_s, acode, off, off32, ks, wsamp, g, kp, ar, ai);
+  cplx<R>* gp = grid + gpt * g.nc + g.c0;
 #pragma unroll
-  for (int c = 0; c < CPL; ++c)
-    if (cl0 + c < g.cg) { cplx<R> o; o.x = (R)ar[c]; o.y = (R)ai[c]; gp[c] = o; }
+  for (int c = 0; c < CPL; ++c) {
+    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+    if (k < g.cg) { cplx<R> o; o.x = (R)ar[c]; o.y = (R)ai[c]; gp[k] = o; }
+  }
 }
 
 // One segment of a dense point's hit list -> partial[slot][cg]
@@ -318,17 +369,18 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
   const int q = (int)(gt % LPS);
   if (si >= nseg) return;
   const HeavySeg sg = segs[si];
-  const int cl0 = q * CPL;
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, cl0, hit_s, acode, off, off32, ks, wsamp, g, kp,
-                           ar, ai);
+  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hit_s, acode, off, off32, ks, wsamp, g, kp,
+                                 ar, ai);
   // partial sums keep the accumulator precision until the final reduction
-  A* pp = reinterpret_cast<A*>(partial) + ((long long)sg.slot * g.cg + cl0) * 2;
+  A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
 #pragma unroll
-  for (int c = 0; c < CPL; ++c)
-    if (cl0 + c < g.cg) { pp[2 * c] = ar[c]; pp[2 * c + 1] = ai[c]; }
+  for (int c = 0; c < CPL; ++c) {
+    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+    if (k < g.cg) { pp[2 * k] = ar[c]; pp[2 * k + 1] = ai[c]; }
+  }
 }
 
 template <typename R, bool COMP>

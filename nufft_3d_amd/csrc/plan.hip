@@ -294,8 +294,8 @@ __device__ __forceinline__ bool hit_test(double tx, double ty, double tz, const 
 __device__ __forceinline__ int wrapg(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
 
 __global__ void k_hits_count(const int4* __restrict__ base, const double* __restrict__ off,
-                             long long nnz, HitGeom g, int* __restrict__ cnt_s,
-                             int* __restrict__ cnt_g) {
+                             long long nnz, HitGeom g, long long* __restrict__ cnt_s,
+                             unsigned long long* __restrict__ cnt_g) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
   const int4 b = base[s];
@@ -307,7 +307,7 @@ __global__ void k_hits_count(const int4* __restrict__ base, const double* __rest
         if (!hit_test((double)dx - fx, (double)dy - fy, (double)dz - fz, g)) continue;
         const long long gi = ((long long)wrapg(b.z + dz, g.K3) * g.K2 + wrapg(b.y + dy, g.K2)) * g.K1 +
                              wrapg(b.x + dx, g.K1);
-        atomicAdd(&cnt_g[gi], 1);
+        atomicAdd(&cnt_g[gi], 1ULL);
         ++n;
       }
   cnt_s[s] = n;
@@ -584,13 +584,16 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   hg.J2 = p->J * p->J;
   hg.radial = p->radial;
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
-  int *cnt_s = nullptr, *cnt_g = nullptr;
-  N3D_HIP(hipMalloc(&cnt_s, sizeof(int) * (nnz + 1)));
-  N3D_HIP(hipMalloc(&cnt_g, sizeof(int) * (ngrid + 1)));
+  // 64-bit counts: the scans below run in the input type, and the total hit
+  // count exceeds 2^31 for large J=6 plans
+  long long* cnt_s = nullptr;
+  unsigned long long* cnt_g = nullptr;
+  N3D_HIP(hipMalloc(&cnt_s, sizeof(long long) * (nnz + 1)));
+  N3D_HIP(hipMalloc(&cnt_g, sizeof(unsigned long long) * (ngrid + 1)));
   N3D_HIP(hipMalloc(&p->d_spos, sizeof(long long) * (nnz + 1)));
   N3D_HIP(hipMalloc(&p->d_rowptr, sizeof(long long) * (ngrid + 1)));
-  N3D_HIP(hipMemsetAsync(cnt_s, 0, sizeof(int) * (nnz + 1), p->stream));
-  N3D_HIP(hipMemsetAsync(cnt_g, 0, sizeof(int) * (ngrid + 1), p->stream));
+  N3D_HIP(hipMemsetAsync(cnt_s, 0, sizeof(long long) * (nnz + 1), p->stream));
+  N3D_HIP(hipMemsetAsync(cnt_g, 0, sizeof(unsigned long long) * (ngrid + 1), p->stream));
   if (nnz > 0)
     hipLaunchKernelGGL(k_hits_count, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
                        (const double*)p->d_off, nnz, hg, cnt_s, cnt_g);
@@ -598,18 +601,19 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   size_t tb2 = 0;
   N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt_s, p->d_spos, (long long)(nnz + 1),
                                            p->stream));
-  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, cnt_g, p->d_rowptr, (long long)(ngrid + 1),
-                                           p->stream));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, (const long long*)cnt_g, p->d_rowptr,
+                                           (long long)(ngrid + 1), p->stream));
   N3D_HIP(hipMalloc(&tmp, std::max(tb, tb2) + 16));
   N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt_s, p->d_spos, (long long)(nnz + 1),
                                            p->stream));
-  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, cnt_g, p->d_rowptr, (long long)(ngrid + 1),
-                                           p->stream));
+  N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, (const long long*)cnt_g, p->d_rowptr,
+                                           (long long)(ngrid + 1), p->stream));
   long long nhits = 0;
   N3D_HIP(hipMemcpyAsync(&nhits, p->d_spos + nnz, sizeof(long long), hipMemcpyDeviceToHost,
                          p->stream));
-  std::vector<int> hcnt(ngrid);
-  N3D_HIP(hipMemcpyAsync(hcnt.data(), cnt_g, sizeof(int) * ngrid, hipMemcpyDeviceToHost, p->stream));
+  std::vector<long long> hcnt(ngrid);
+  N3D_HIP(hipMemcpyAsync(hcnt.data(), cnt_g, sizeof(long long) * ngrid, hipMemcpyDeviceToHost,
+                         p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(tmp);
   p->nhits = nhits;
@@ -651,7 +655,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     long long pos = 0;
     int maxc = 0;
     for (long long gi = 0; gi < ngrid; ++gi) {
-      const int c = hcnt[gi];
+      const int c = (int)hcnt[gi];
       maxc = std::max(maxc, c);
       if (c > p->rmax) {
         const int nseg = (c + p->chunk_len - 1) / p->chunk_len;

@@ -68,11 +68,13 @@ def make_inputs(w, seed=2):
     return om, x
 
 
-def precision_study(p32c, p64, om, N, w, x, data, iters):
-    """Relative L2 error against the FP64 path, per stage, of FP32 with
-    compensated (double) and plain (float) gridding accumulation: fNUFT,
-    aNUFT, the normal operator A'DA, and an iNUFT of `iters` MINRES
-    iterations (damp 0) on the noisy data."""
+def precision_study(p32, p64, om, N, w, x, data, iters):
+    """Relative L2 error against the FP64 path, per stage (fNUFT, aNUFT, the
+    normal operator A'DA, an iNUFT of `iters` MINRES iterations at damp 0 on
+    the noisy data) of three fp32-array plans: fp32 arithmetic with plain
+    float gridding sums, fp32 arithmetic with compensated (double) gridding
+    sums, and the default policy (`p32`: fp64 arithmetic behind the fp32
+    arrays when the deapodization range exceeds 256)."""
     import torch
     from nufft_3d_amd import nufft_3d
 
@@ -85,13 +87,19 @@ def precision_study(p32c, p64, om, N, w, x, data, iters):
     t0 = time.perf_counter()
     ref["iNUFT"], info64 = p64.iNUFT(d64, iters, 0.0, return_info=True)
     torch.cuda.synchronize()
-    log("fp64 iNUFT(%d) %.2f s" % (iters, time.perf_counter() - t0))
     out = {"iters": iters, "reference": "FP64 HIP path (pinned to the FP64 oracle at 1e-12 by the "
                                         "parity tests)",
+           "fp64_iNUFT_s": round(time.perf_counter() - t0, 4),
            "fp64_relres": info64["relres"][:w["nc"]]}
-    p32p = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1, compensated=False)
-    for name, pl in (("compensated", p32c), ("plain", p32p)):
-        r = {"fNUFT": rel(pl.fNUFT(x), ref["fNUFT"]), "aNUFT": rel(pl.aNUFT(data), ref["aNUFT"]),
+    U = np.abs(p32.U)
+    out["deapodization_range"] = float(U.max() / U.min())
+    arms = [("fp32_plain", dict(compensated=False, fp64_compute=False)),
+            ("fp32_compensated", dict(compensated=True, fp64_compute=False)),
+            ("default_policy", None)]
+    for name, kw in arms:
+        pl = p32 if kw is None else nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1, **kw)
+        r = {"compute_precision": pl.compute_precision,
+             "fNUFT": rel(pl.fNUFT(x), ref["fNUFT"]), "aNUFT": rel(pl.aNUFT(data), ref["aNUFT"]),
              "normal": rel(pl.normal(x), ref["normal"])}
         t0 = time.perf_counter()
         xi, info = pl.iNUFT(data, iters, 0.0, return_info=True)
@@ -101,7 +109,9 @@ def precision_study(p32c, p64, om, N, w, x, data, iters):
         r["relres"] = info["relres"][:w["nc"]]
         out[name] = r
         log("precision", name, r)
-    p32p.close()
+        if kw is not None:
+            pl.close()
+        torch.cuda.empty_cache()
     return out
 
 
@@ -350,6 +360,7 @@ def main():
         "dtype": "c64",
         "data": "synthetic (seeded phantom x coil maps, koosh-ball trajectory)",
         "config": {"workload": args.workload, "N": N, "K": K, "coils": nc, "samples": M,
+                   "compute_precision": plan.compute_precision,
                    "in_bounds": plan.nnz, "J": w["J"], "u": w["u"], "trajectory": w["traj"],
                    "step": "fNUFT+aNUFT, all coils in one call, device-resident arrays"},
         "latency_ms": {"p50": round(lat[len(lat) // 2], 4),

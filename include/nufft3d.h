@@ -51,6 +51,11 @@ typedef struct nufft3d_opts {
   int32_t verbose;   /* print the reference's plan report to stdout */
   int32_t compensated; /* fp32 plans: 1 = compensated (TwoSum/TwoProduct) gridding
                           accumulation (default), 0 = plain fp32 sums */
+  int32_t fp64_compute; /* precision-1 plans: compute in fp64 behind fp32 arrays
+                           (-1 auto: when the deapodization range max|U|/min|U|
+                           exceeds 256, where fp32 rounding anywhere before the
+                           final deapodization is amplified past 1e-4; 0 never;
+                           1 always). No counterpart in the reference. */
 } nufft3d_opts;
 
 /* Per-right-hand-side result of nufft3d_inverse (MATLAB minres flag/relres). */
@@ -102,6 +107,11 @@ int nufft3d_deap(nufft3d_plan* plan, double* U);
 int64_t nufft3d_nnz(const nufft3d_plan* plan);
 int nufft3d_get_params(const nufft3d_plan* plan, int32_t K[3], int32_t N[3], double* J,
                        double* u, double* alpha, int32_t* radial, int32_t* precision);
+
+/* Arithmetic precision of the plan's kernels (1 fp32, 2 fp64): differs from
+ * the array precision of nufft3d_get_params when fp64_compute applies. No
+ * counterpart in the reference. */
+int32_t nufft3d_compute_precision(const nufft3d_plan* plan);
 
 /* Device memory held by a plan: the resident plan arrays (sorted samples,
  * hit lists, density, U, twiddles) and the workspaces grown by calls so far

@@ -23,7 +23,8 @@ class Opts(C.Structure):
                 ("radial", C.c_int32), ("precision", C.c_int32),
                 ("d", C.POINTER(C.c_double)), ("nd", C.c_int64),
                 ("tile", C.c_int32), ("chunk", C.c_int32), ("device", C.c_int32),
-                ("verbose", C.c_int32), ("compensated", C.c_int32)]
+                ("verbose", C.c_int32), ("compensated", C.c_int32),
+                ("fp64_compute", C.c_int32)]
 
 
 class PrunoOpts(C.Structure):
@@ -57,6 +58,7 @@ SIGNATURES = {
                                      C.POINTER(C.c_int32)]),
     "nufft3d_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nufft3d_device_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "nufft3d_compute_precision": (C.c_int32, [C.c_void_p]),
     "nufft3d_release_workspaces": (C.c_int, [C.c_void_p]),
     "nufft3d_profile_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "nufft3d_profile_read": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),

@@ -84,7 +84,9 @@ struct Profiler {
 }  // namespace nufft3d
 
 struct nufft3d_plan {
-  int prec = 1;             // 1 fp32, 2 fp64
+  int prec = 1;             // arithmetic: 1 fp32, 2 fp64
+  int io_prec = 1;          // caller arrays: 1 complex64, 2 complex128
+  int fp64_mode = -1;       // precision-1 plans: -1 auto, 0 never, 1 always fp64 arithmetic
   double J = 4, u = 2, alpha = 0;
   int radial = 1;
   int N[3] = {0, 0, 0};

@@ -504,6 +504,21 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     }
   p->M = M;
 
+  // precision policy of fp32 plans: the adjoint ends in the deapodization
+  // U, which multiplies the image edge by up to max|U|/min|U| relative to its
+  // centre; fp32 rounding anywhere before it (grid, FFT, weights, density, the
+  // forward half of A'WA) is amplified by that range. Above 256 (e.g. u=1.25,
+  // J=6: 2.5e4..3e5) the arithmetic runs in fp64 behind the fp32 arrays.
+  if (p->io_prec == 1 && p->fp64_mode != 0) {
+    deap_host(p);
+    double umax = 0, umin = 1e300;
+    for (double v : p->h_U) { umax = std::max(umax, std::fabs(v)); umin = std::min(umin, std::fabs(v)); }
+    const double range = umin > 0 ? umax / umin : 1e300;
+    if (p->fp64_mode == 1 || range > 256.0) p->prec = 2;
+    if (p->verbose)
+      printf("  deapodization range %.3g: %s arithmetic\n", range, p->prec == 2 ? "fp64" : "fp32");
+  }
+
   // tiles
   if (p->N[2] == 1) { p->T[0] = p->T[1] = 16; p->T[2] = 1; }
   for (int a = 0; a < 3; ++a) {
@@ -744,6 +759,14 @@ static void destroy_plan(nufft3d_plan* p) {
 using namespace nufft3d;
 
 // ================================================================ C ABI
+// elementwise precision conversion of caller arrays (fp32 arrays of plans
+// that compute in fp64)
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (B)in[i];
+}
+
 extern "C" {
 
 const char* nufft3d_last_error(void) { return g_last_error.c_str(); }
@@ -757,6 +780,7 @@ int nufft3d_opts_default(nufft3d_opts* o) {
   o->radial = 1;
   o->precision = 1;
   o->compensated = 1;
+  o->fp64_compute = -1;
   return NUFFT3D_OK;
 }
 
@@ -796,6 +820,8 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   }
   nufft3d_plan* p = new nufft3d_plan();
   p->prec = o.precision;
+  p->io_prec = o.precision;
+  p->fp64_mode = o.fp64_compute < 0 ? -1 : (o.fp64_compute ? 1 : 0);
   p->J = o.J;
   p->u = o.u;
   p->radial = o.radial ? 1 : 0;
@@ -926,9 +952,11 @@ int nufft3d_get_params(const nufft3d_plan* p, int32_t K[3], int32_t N[3], double
   if (u) *u = p->u;
   if (alpha) *alpha = p->alpha;
   if (radial) *radial = p->radial;
-  if (precision) *precision = p->prec;
+  if (precision) *precision = p->io_prec;
   return NUFFT3D_OK;
 }
+
+int32_t nufft3d_compute_precision(const nufft3d_plan* p) { return p ? p->prec : -1; }
 
 int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
   if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
@@ -1017,6 +1045,62 @@ static int finish_out(nufft3d_plan* p, void* dst, size_t bytes, const Staged& s)
   return NUFFT3D_OK;
 }
 
+// ---- caller arrays in the array precision (io_prec), device arrays of the
+// arithmetic precision (prec): fp32 arrays are widened on the way in and
+// rounded on the way out when an fp32 plan computes in fp64.
+static size_t cp_esz(const nufft3d_plan* p) { return p->prec == 2 ? sizeof(double2) : sizeof(float2); }
+
+// n complex values (nreal = 2n) or n reals (cplx = false)
+static int stage_in_c(nufft3d_plan* p, const void* src, size_t n, Workspace& ws, Workspace& wtmp,
+                      Staged& s, bool cplx = true) {
+  const size_t k = cplx ? 2 : 1;
+  if (p->io_prec == p->prec) return stage_in(p, src, n * k * (p->prec == 2 ? 8 : 4), ws, s);
+  const void* d32 = src;
+  s.host = !is_device_ptr(src);
+  if (s.host) {
+    N3D_TRY(ensure(wtmp, n * k * sizeof(float)));
+    N3D_HIP(hipMemcpyAsync(wtmp.p, src, n * k * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    d32 = wtmp.p;
+  }
+  N3D_TRY(ensure(ws, n * k * sizeof(double)));
+  if (n > 0)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks((long long)(n * k))), dim3(256), 0,
+                       p->stream, (const float*)d32, (double*)ws.p, (long long)(n * k));
+  N3D_HIP(hipGetLastError());
+  s.dev = ws.p;
+  return NUFFT3D_OK;
+}
+
+static int stage_out_c(nufft3d_plan* p, void* dst, size_t n, Workspace& ws, Staged& s,
+                       bool cplx = true) {
+  const size_t k = cplx ? 2 : 1;
+  if (p->io_prec == p->prec) return stage_out(p, dst, n * k * (p->prec == 2 ? 8 : 4), ws, s);
+  N3D_TRY(ensure(ws, n * k * sizeof(double)));
+  s.dev = ws.p;
+  s.host = !is_device_ptr(dst);
+  return NUFFT3D_OK;
+}
+
+static int finish_out_c(nufft3d_plan* p, void* dst, size_t n, const Staged& s, Workspace& wtmp,
+                        bool cplx = true) {
+  const size_t k = cplx ? 2 : 1;
+  if (p->io_prec == p->prec) return finish_out(p, dst, n * k * (p->prec == 2 ? 8 : 4), s);
+  void* d32 = dst;
+  if (s.host) {
+    N3D_TRY(ensure(wtmp, n * k * sizeof(float)));
+    d32 = wtmp.p;
+  }
+  if (n > 0)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks((long long)(n * k))), dim3(256), 0,
+                       p->stream, (const double*)s.dev, (float*)d32, (long long)(n * k));
+  N3D_HIP(hipGetLastError());
+  if (s.host) {
+    N3D_HIP(hipMemcpyAsync(dst, d32, n * k * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+  }
+  return NUFFT3D_OK;
+}
+
 #define N3D_PLAN_GUARD(p)                                          \
   if (!(p)) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; } \
   std::lock_guard<std::mutex> lk_((p)->mu);                        \
@@ -1026,21 +1110,22 @@ static int finish_out(nufft3d_plan* p, void* dst, size_t bytes, const Staged& s)
 int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
   N3D_PLAN_GUARD(p);
   if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
-  Workspace tmp_out;
+  Workspace tmp_out, tmp_io;
   Staged xi, ko;
-  N3D_TRY(stage_in(p, x, nimg * esz, p->ws_img, xi));
-  int st = stage_out(p, k, nk * esz, tmp_out, ko);
+  N3D_TRY(stage_in_c(p, x, nimg, p->ws_img, tmp_io, xi));
+  int st = stage_out_c(p, k, nk, tmp_out, ko);
   if (st) return st;
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
   N3D_TRY(ensure(p->ws_grid, ngrid * esz));
   N3D_TRY(fft_forward(p, xi.dev, nc, p->ws_grid.p));                                // fNUFT.m:5-6
   N3D_TRY(interp(p, p->ws_grid.p, nc, ko.dev, 1, nullptr, 1.0 / sqrt((double)p->nnz)));  // :8-9
   N3D_TRY(zero_oob(p, ko.dev, nc));
-  st = finish_out(p, k, nk * esz, ko);
+  st = finish_out_c(p, k, nk, ko, tmp_io);
   release(tmp_out);
+  release(tmp_io);
   if (xi.host) N3D_HIP(hipStreamSynchronize(p->stream));
   return st;
 }
@@ -1048,20 +1133,21 @@ int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
 int nufft3d_adjoint(nufft3d_plan* p, const void* k, int32_t nc, void* x) {
   N3D_PLAN_GUARD(p);
   if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
-  Workspace tmp_out;
+  Workspace tmp_out, tmp_io;
   Staged ki, xo;
-  N3D_TRY(stage_in(p, k, nk * esz, p->ws_img, ki));
-  int st = stage_out(p, x, nimg * esz, tmp_out, xo);
+  N3D_TRY(stage_in_c(p, k, nk, p->ws_img, tmp_io, ki));
+  int st = stage_out_c(p, x, nimg, tmp_out, xo);
   if (st) return st;
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
   N3D_TRY(ensure(p->ws_grid, ngrid * esz));
   N3D_TRY(spread(p, ki.dev, nc, p->ws_grid.p, 1, nullptr));                          // aNUFT.m:5
   N3D_TRY(fft_adjoint(p, p->ws_grid.p, nc, xo.dev, 1.0 / sqrt((double)p->nnz), nullptr, 0.0));
-  st = finish_out(p, x, nimg * esz, xo);
+  st = finish_out_c(p, x, nimg, xo, tmp_io);
   release(tmp_out);
+  release(tmp_io);
   if (ki.host) N3D_HIP(hipStreamSynchronize(p->stream));
   return st;
 }
@@ -1112,15 +1198,15 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   if (!(damp >= 0) || !std::isfinite(damp)) { set_error("damp must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
   if (!(tol > 0)) tol = 1e-4;                                                  // iNUFT.m:114
   auto t0 = std::chrono::steady_clock::now();
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
   N3D_TRY(make_weights(p, W, nW, &wsamp));
-  Workspace tmp_out, tmp_b;
+  Workspace tmp_out, tmp_b, tmp_io;
   Staged ri, xo;
-  int st = stage_in(p, raw, nk * esz, p->ws_img, ri);
-  if (!st) st = stage_out(p, im, nimg * esz, tmp_out, xo);
+  int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
+  if (!st) st = stage_out_c(p, im, nimg, tmp_out, xo);
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
   if (!st) st = ensure(p->ws_grid, ngrid * esz);
   if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
@@ -1139,7 +1225,7 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
     info->nrhs = nc;
     for (int c = 0; c < nc && c < 64; ++c) { info->iters[c] = 0; info->relres[c] = NAN; }
   }
-  if (!st) st = finish_out(p, im, nimg * esz, xo);
+  if (!st) st = finish_out_c(p, im, nimg, xo, tmp_io);
   if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
     (void)hipGetLastError();
     set_error("device error in iNUFT");
@@ -1147,6 +1233,7 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   }
   release(tmp_out);
   release(tmp_b);
+  release(tmp_io);
   if (wsamp != p->d_dens) (void)hipFree(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
@@ -1155,15 +1242,15 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
 int nufft3d_normal(nufft3d_plan* p, const void* x, int32_t nc, double damp, void* y) {
   N3D_PLAN_GUARD(p);
   if (!x || !y || nc <= 0) { set_error("invalid x/y/nc"); return NUFFT3D_ERR_INVALID; }
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
-  Workspace tmp_out;
+  Workspace tmp_out, tmp_io;
   Staged xi, yo;
-  N3D_TRY(stage_in(p, x, nimg * esz, p->ws_img, xi));
-  int st = stage_out(p, y, nimg * esz, tmp_out, yo);
+  N3D_TRY(stage_in_c(p, x, nimg, p->ws_img, tmp_io, xi));
+  int st = stage_out_c(p, y, nimg, tmp_out, yo);
   if (!st) st = apply_normal(p, xi.dev, nc, damp, p->d_dens, yo.dev);
-  if (!st) st = finish_out(p, y, nimg * esz, yo);
+  if (!st) st = finish_out_c(p, y, nimg, yo, tmp_io);
   release(tmp_out);
+  release(tmp_io);
   if (xi.host) N3D_HIP(hipStreamSynchronize(p->stream));
   return st;
 }
@@ -1171,10 +1258,9 @@ int nufft3d_normal(nufft3d_plan* p, const void* x, int32_t nc, double damp, void
 int nufft3d_density(nufft3d_plan* p, void* d) {
   N3D_PLAN_GUARD(p);
   if (!d) { set_error("d is NULL"); return NUFFT3D_ERR_INVALID; }
-  const size_t esz = p->prec == 2 ? sizeof(double) : sizeof(float);
-  Workspace tmp_out;
+  Workspace tmp_out, tmp_io;
   Staged o;
-  N3D_TRY(stage_out(p, d, esz * p->M, tmp_out, o));
+  N3D_TRY(stage_out_c(p, d, (size_t)p->M, tmp_out, o, false));
   if (p->prec == 2)
     hipLaunchKernelGGL((k_scatter_dens<double>), dim3(nblocks(p->M)), dim3(256), 0, p->stream,
                        (const double*)p->d_dens, p->d_orig, p->d_oob, (long long)p->nnz,
@@ -1184,9 +1270,10 @@ int nufft3d_density(nufft3d_plan* p, void* d) {
                        (const float*)p->d_dens, p->d_orig, p->d_oob, (long long)p->nnz,
                        (long long)p->M, (float*)o.dev);
   N3D_HIP(hipGetLastError());
-  int st = finish_out(p, d, esz * p->M, o);
+  int st = finish_out_c(p, d, (size_t)p->M, o, tmp_io, false);
   N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_out);
+  release(tmp_io);
   return st;
 }
 
@@ -1212,17 +1299,17 @@ int nufft3d_gram(nufft3d_plan* p, const void* data, const int32_t dims[4], int32
   const int dd[4] = {dims[0], dims[1], dims[2], dims[3]};
   const long long n = (long long)taps.size() * dims[3];
   if (nout) *nout = (int32_t)n;
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
-  const size_t bytes = esz * (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  const size_t nd = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
   Staged di, ao;
-  Workspace tmp_in, tmp_out;
-  N3D_TRY(stage_in(p, data, bytes, tmp_in, di));
+  Workspace tmp_in, tmp_out, tmp_io;
+  N3D_TRY(stage_in_c(p, data, nd, tmp_in, tmp_io, di));
   int st = stage_out(p, AA, sizeof(double2) * n * n, tmp_out, ao);
   if (!st) st = gram_device(p, di.dev, dd, taps, (double2*)ao.dev);
   if (!st) st = finish_out(p, AA, sizeof(double2) * n * n, ao);
   N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_in);
   release(tmp_out);
+  release(tmp_io);
   return st;
 }
 
@@ -1232,11 +1319,9 @@ int nufft3d_pruno(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_p
   if (!data || nc < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
   nufft3d_pruno_opts o;
   if (opts) o = *opts; else nufft3d_pruno_opts_default(&o);
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
-  const size_t bytes = esz * (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   Staged di;
-  Workspace tmp_in;
-  N3D_TRY(stage_in(p, data, bytes, tmp_in, di));
+  Workspace tmp_in, tmp_io;
+  N3D_TRY(stage_in_c(p, data, (size_t)p->N[0] * p->N[1] * p->N[2] * nc, tmp_in, tmp_io, di));
   std::vector<double> svd;
   int nvv = 0;
   double nz = 0;
@@ -1246,7 +1331,9 @@ int nufft3d_pruno(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_p
     if (noise) *noise = nz;
     if (sv) std::memcpy(sv, svd.data(), sizeof(double) * svd.size());
   }
+  N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_in);
+  release(tmp_io);
   return st;
 }
 
@@ -1267,16 +1354,16 @@ int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   if (opts) o = *opts; else nufft3d_pruno_opts_default(&o);
   const double tol = 1e-4;                                                 // iNUFT.m:114
   auto t0 = std::chrono::steady_clock::now();
-  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
   N3D_TRY(make_weights(p, W, nW, &wsamp));
-  Workspace tmp_out, tmp_b, tmp_x0, tmp_k;
+  Workspace tmp_out, tmp_b, tmp_x0, tmp_k, tmp_io;
   Staged ri, xo;
   void* dB = nullptr;
-  int st = stage_in(p, raw, nk * esz, p->ws_img, ri);
-  if (!st) st = stage_out(p, im, nimg * esz, tmp_out, xo);
+  int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
+  if (!st) st = stage_out_c(p, im, nimg, tmp_out, xo);
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
   if (!st) st = ensure(p->ws_grid, ngrid * esz);
   if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
@@ -1298,14 +1385,14 @@ int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   if (!st) st = pruno_build(p, tmp_k.p, nc, o, lambda, &dB, &nv, nullptr, nullptr);
   // coupled least squares (iNUFT.m:210-218)
   if (!st) st = minres_coupled(p, tmp_b.p, nc, iters, damp, wsamp, dB, tol, xo.dev, info);
-  if (!st) st = finish_out(p, im, nimg * esz, xo);
+  if (!st) st = finish_out_c(p, im, nimg, xo, tmp_io);
   if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
     (void)hipGetLastError();
     set_error("device error in iNUFT (pruno)");
     st = NUFFT3D_ERR_DEVICE;
   }
   if (dB) (void)hipFree(dB);
-  release(tmp_out); release(tmp_b); release(tmp_x0); release(tmp_k);
+  release(tmp_out); release(tmp_b); release(tmp_x0); release(tmp_k); release(tmp_io);
   if (wsamp != p->d_dens) (void)hipFree(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;

@@ -41,7 +41,7 @@ class nufft_3d:
     """Non-uniform 3-D FFT operator (Fessler-style, sparse formulation)."""
 
     def __init__(self, om, N=None, J=4, u=2, alpha=0, radial=1, gpu=1, d=None,
-                 tile=0, chunk=0, device=0, verbose=False, compensated=True):
+                 tile=0, chunk=0, device=0, verbose=False, compensated=True, fp64_compute=None):
         if gpu not in (1, 2):
             raise ValueError("gpu must be 1 (single) or 2 (double): the HIP library has no "
                              "CPU path")
@@ -70,6 +70,9 @@ class nufft_3d:
         opts.device = int(device)
         opts.verbose = int(bool(verbose))
         opts.compensated = int(bool(compensated))
+        # gpu=1 plans: None = automatic (fp64 arithmetic when the deapodization
+        # range exceeds 256), False = always fp32, True = always fp64
+        opts.fp64_compute = -1 if fp64_compute is None else int(bool(fp64_compute))
         self._dkeep = None
         if d is not None:
             dd = np.ascontiguousarray(np.reshape(np.asarray(d, dtype=np.float64), (-1,)))
@@ -94,6 +97,7 @@ class nufft_3d:
         self.gpu = prec.value
         self.M = M
         self.nnz = int(lib.nufft3d_nnz(h))
+        self.compute_precision = int(lib.nufft3d_compute_precision(h))
         self._U = None
         self._d = None
 

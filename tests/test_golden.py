@@ -48,7 +48,7 @@ def test_hip_matches_golden(f, gpu, tol):
     assert rel(p.aNUFT(g["k_in"]), g["x_adj"]) < tol
     assert rel(p.iNUFT(g["raw"], 0), g["im_regrid"]) < tol
     im, info = p.iNUFT(g["raw"], 5, float(g["damp"]), return_info=True)
-    # u=1.25 (|U| up to ~1.6e3 at N=8): fp32 (I)FFT rounding compounds over
-    # MINRES iterations to ~1.4e-4; the fp64 path holds 1e-10
-    tol_inv = 1e-10 if gpu == 2 else (3e-4 if float(g["u"]) < 1.5 else tol)
+    # gpu=1 plans with a wide deapodization range (u=1.25 here) compute in
+    # fp64 behind fp32 arrays, so one tolerance holds for every fixture
+    tol_inv = 1e-10 if gpu == 2 else tol
     assert rel(im, g["im_minres5"]) < tol_inv

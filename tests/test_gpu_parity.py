@@ -100,13 +100,6 @@ def test_adjoint(case, nc):
     assert rel(g32.aNUFT(k), want) < TOL32_OP
 
 
-# u=1.25 on a 16^3 image: the deapodization reaches |U| ~ 2e3 at the
-# field-of-view edge, so fp32 rounding of the (I)FFT alone costs ~2e-5 per
-# operator (measured on the oracle with a float32 FFT); MINRES compounds it
-# over 8 iterations to ~2e-4. The FP64 path is exact to 1e-10 there.
-TOL32_INV = {"radial_J6_u125": 3e-4}
-
-
 def test_inverse(case):
     name, om, orc, g32, g64 = case
     x = _rand(orc.N + (2,), 31)
@@ -118,7 +111,7 @@ def test_inverse(case):
         e32 = rel(got32, want)
         print("%s maxit=%d fp32 rel err %.2e" % (name, maxit, e32), flush=True)
         assert rel(got64, want) < 1e-10, (maxit, damp)
-        assert e32 < TOL32_INV.get(name, TOL32), (maxit, damp)
+        assert e32 < TOL32, (maxit, damp)
         if maxit > 0:
             assert info64["iters"] == winfo["iters"]
 
@@ -200,3 +193,32 @@ def test_2d_trajectory():
     k = _rand((orc.M, 2), 92)
     assert rel(g.aNUFT(k), orc.aNUFT(k)) < TOL64
     assert rel(g.d, orc.d) < TOL64
+
+
+def test_precision_policy_selects_fp64_arithmetic_for_wide_deapodization(case):
+    name, om, orc, g32, g64 = case
+    U = np.abs(g32.U)
+    wide = U.max() / U.min() > 256
+    assert g32.gpu == 1 and g64.gpu == 2 and g64.compute_precision == 2
+    assert g32.compute_precision == (2 if wide else 1), (name, U.max() / U.min())
+
+
+# u=1.25 on a 16^3 image: the deapodization range max|U|/min|U| is ~2.5e4, so
+# in fp32 arithmetic the rounding of the grid, the (I)FFT and the weights is
+# amplified at the field-of-view edge: ~2e-5 per adjoint (measured on the
+# oracle with a float32 FFT), compounded by MINRES over 8 iterations to ~2e-4.
+# This is the plan that the automatic policy moves to fp64 arithmetic; forced
+# to fp32 it keeps this documented bound.
+def test_u125_in_fp32_arithmetic():
+    om = synth.golden_means(16, 80, 32)
+    orc = O.NufftOracle(om, 16, J=6, u=1.25, radial=1)
+    g = nufft_3d(om, 16, J=6, u=1.25, radial=1, gpu=1, fp64_compute=False)
+    assert g.compute_precision == 1
+    x = _rand(orc.N + (2,), 51)
+    k = _rand((orc.M, 2), 52)
+    assert rel(g.fNUFT(x), orc.fNUFT(x)) < TOL32_OP
+    assert rel(g.aNUFT(k), orc.aNUFT(k)) < 1e-4
+    raw = orc.fNUFT(x) + 0.01 * _rand((orc.M, 2), 53)
+    want = orc.iNUFT(raw, 8, 0.05)
+    assert rel(g.iNUFT(raw, 8, 0.05), want) < 3e-4
+

@@ -49,8 +49,9 @@ typedef struct nufft3d_opts {
   int32_t chunk;     /* max samples per spread work chunk (0 = auto) */
   int32_t device;    /* HIP device ordinal */
   int32_t verbose;   /* print the reference's plan report to stdout */
-  int32_t compensated; /* fp32 plans: 1 = compensated (TwoSum/TwoProduct) gridding
-                          accumulation (default), 0 = plain fp32 sums */
+  int32_t compensated; /* fp32 plans: 1 = gridding sums of the (exact) fp32
+                          weight x data products kept in double, one rounding
+                          per output (default); 0 = plain fp32 sums */
   int32_t fp64_compute; /* precision-1 plans: compute in fp64 behind fp32 arrays
                            (-1 auto: when the deapodization range max|U|/min|U|
                            exceeds 256, where fp32 rounding anywhere before the

@@ -12,6 +12,10 @@
 //   x*y ~ xh*yh + xh*yl + xl*yh (the lo*lo term, ~2^-16 relative, dropped);
 //   v_mfma_f32_32x32x16_bf16, 128x128 block tiles, upper tiles only, V split
 //   across work-groups; partial tiles summed in a fixed order in double.
+//   The operands are prepared once per call: the split-bf16 Re/Im planes of
+//   the data, pre-shifted for every x-shift of the kernel and padded to rows
+//   of a multiple of 32, so a K step (32 consecutive x of one row) reads one
+//   aligned 64-byte run per column and array.
 // fp64 plans: the same Gram in double with plain FMAs (oracle mode).
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -33,8 +37,22 @@ struct GramGeom {
   long long V;
   int nc, nk, n, n2;      // n = nc*nk, n2 = 2n
   int ntr;                // tile rows (= tile cols) of C
-  int nsplit;             // V chunks
-  long long vchunk;       // voxels per chunk (multiple of 32)
+  int nsplit;             // V chunks (work-groups per tile along V)
+  int h;                  // kernel half width (shifts in [-h, h] per axis)
+  int Lx;                 // padded row length of the prepared planes (multiple of 32)
+  int nxc;                // K steps per row (Lx / 32)
+  long long plane;        // N3 * N2 * Lx
+  long long nsteps;       // N2 * N3 * nxc
+  long long spw;          // K steps per work-group
+};
+
+// A column of X: element offset of its hi plane in the prepared operands
+// (the lo plane is nc*plane further), its y/z shifts, and whether it exists
+// (the last tile is padded past 2n).
+struct GramCol {
+  long long base;
+  int sy, sz;
+  int valid, pad;
 };
 
 constexpr int GT = 128;   // block tile edge
@@ -52,33 +70,49 @@ __device__ __forceinline__ float bf16_to_f(unsigned short b) {
   return __uint_as_float(((unsigned)b) << 16);
 }
 
-// value of X[v, col] (v given as x,y,z)
-__device__ __forceinline__ float xval(const float2* __restrict__ data, const int4* __restrict__ sh,
-                                      const GramGeom& g, int col, int x, int y, int z) {
-  if (col >= g.n2) return 0.0f;
-  const int p = col >= g.n;
-  const int a = p ? col - g.n : col;
-  const int c = a % g.nc, j = a / g.nc;
-  const int4 s = sh[j];
-  int xs = x - s.x, ys = y - s.y, zs = z - s.z;
-  xs = xs < 0 ? xs + g.N1 : (xs >= g.N1 ? xs - g.N1 : xs);
-  ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
-  zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
-  const float2 d = data[(long long)c * g.V + xs + (long long)g.N1 * (ys + (long long)g.N2 * zs)];
-  return p ? d.y : d.x;
+// Q[sxi][p][hl][c][z][y][x], x in [0, Lx): the hi (hl=0) and lo (hl=1) bf16
+// parts of Re (p=0) / Im (p=1) data_c[(x - sx) mod N1, y, z], sx = sxi - h,
+// for x < N1; zero for x >= N1 (so padded voxels add nothing).
+__global__ void k_gram_prep(const float2* __restrict__ data, unsigned short* __restrict__ Q,
+                            GramGeom g, int nsx) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nsx * g.nc * g.plane;
+  if (idx >= total) return;
+  const int x = (int)(idx % g.Lx);
+  const long long yz = (idx / g.Lx) % ((long long)g.N2 * g.N3);
+  const int c = (int)((idx / g.plane) % g.nc);
+  const int sxi = (int)(idx / (g.plane * g.nc));
+  unsigned short rh = 0, rl = 0, ih = 0, il = 0;
+  if (x < g.N1) {
+    int xs = x - (sxi - g.h);
+    xs = xs < 0 ? xs + g.N1 : (xs >= g.N1 ? xs - g.N1 : xs);
+    const float2 d = data[(long long)c * g.V + xs + (long long)g.N1 * yz];
+    rh = bf16_bits(d.x);
+    rl = bf16_bits(d.x - bf16_to_f(rh));
+    ih = bf16_bits(d.y);
+    il = bf16_bits(d.y - bf16_to_f(ih));
+  }
+  const long long pl = (long long)g.nc * g.plane;       // one (sxi, p, hl) block
+  const long long o = (long long)sxi * 4 * pl + (long long)c * g.plane + yz * g.Lx + x;
+  Q[o] = rh;
+  Q[o + pl] = rl;
+  Q[o + 2 * pl] = ih;
+  Q[o + 3 * pl] = il;
 }
 
 // grid: x = upper tile pair, y = V chunk. 256 threads = 4 waves (2x2 of 64x64).
-__global__ void __launch_bounds__(256) k_gram_bf16x3(const float2* __restrict__ data,
-                                                     const int4* __restrict__ shifts,
+// Thread t stages column t/2 of both tiles, half (t%2) of the 32-voxel run,
+// for the next K step in registers while the MFMAs of this one run.
+__global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __restrict__ Q,
+                                                     const GramCol* __restrict__ cols,
                                                      const int2* __restrict__ tiles,
                                                      float* __restrict__ partial, GramGeom g) {
   __shared__ unsigned short sAh[GT][GK + GPAD], sAl[GT][GK + GPAD];
   __shared__ unsigned short sBh[GT][GK + GPAD], sBl[GT][GK + GPAD];
   const int2 tp = tiles[blockIdx.x];
   const int I0 = tp.x * GT, J0 = tp.y * GT;
-  const long long vbeg = (long long)blockIdx.y * g.vchunk;
-  const long long vend = min(vbeg + g.vchunk, g.V);
+  const long long s0 = (long long)blockIdx.y * g.spw;
+  const long long s1 = min(s0 + g.spw, g.nsteps);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -89,31 +123,51 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const float2* __restrict__ 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  const int vl = tid & (GK - 1);      // this thread's voxel within a K step
-  const int cl0 = tid >> 5;           // first column it loads (stride 8)
-
-  for (long long v0 = vbeg; v0 < vend; v0 += GK) {
-    const long long v = v0 + vl;
-    const bool vin = v < vend;
-    int x = 0, y = 0, z = 0;
-    if (vin) {
-      x = (int)(v % g.N1);
-      const long long r = v / g.N1;
-      y = (int)(r % g.N2);
-      z = (int)(r / g.N2);
+  const int cl = tid >> 1, hh = tid & 1;
+  const GramCol ca = cols[I0 + cl], cb = cols[J0 + cl];
+  const long long lo_off = (long long)g.nc * g.plane;
+  uint4 ra[4], rb[4];     // hi[0..1], lo[0..1] chunks of 8 bf16
+  auto fetch = [&](long long step) {
+    const long long row = step / g.nxc;
+    const int xc = (int)(step - row * g.nxc);
+    const int y = (int)(row % g.N2), z = (int)(row / g.N2);
+    const uint4 zero = make_uint4(0, 0, 0, 0);
+    if (ca.valid) {
+      int ys = y - ca.sy, zs = z - ca.sz;
+      ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
+      zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
+      const uint4* q = reinterpret_cast<const uint4*>(
+          Q + ca.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
+      const uint4* ql = q + lo_off / 8;
+      ra[0] = q[0]; ra[1] = q[1]; ra[2] = ql[0]; ra[3] = ql[1];
+    } else {
+      ra[0] = ra[1] = ra[2] = ra[3] = zero;
+    }
+    if (cb.valid) {
+      int ys = y - cb.sy, zs = z - cb.sz;
+      ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
+      zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
+      const uint4* q = reinterpret_cast<const uint4*>(
+          Q + cb.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
+      const uint4* ql = q + lo_off / 8;
+      rb[0] = q[0]; rb[1] = q[1]; rb[2] = ql[0]; rb[3] = ql[1];
+    } else {
+      rb[0] = rb[1] = rb[2] = rb[3] = zero;
+    }
+  };
+  if (s0 < s1) fetch(s0);
+  for (long long step = s0; step < s1; ++step) {
+    __syncthreads();   // the previous step's MFMA operand reads are done
+    {
+      uint4* ah = reinterpret_cast<uint4*>(&sAh[cl][16 * hh]);
+      uint4* al = reinterpret_cast<uint4*>(&sAl[cl][16 * hh]);
+      uint4* bh = reinterpret_cast<uint4*>(&sBh[cl][16 * hh]);
+      uint4* bl = reinterpret_cast<uint4*>(&sBl[cl][16 * hh]);
+      ah[0] = ra[0]; ah[1] = ra[1]; al[0] = ra[2]; al[1] = ra[3];
+      bh[0] = rb[0]; bh[1] = rb[1]; bl[0] = rb[2]; bl[1] = rb[3];
     }
     __syncthreads();
-    for (int cl = cl0; cl < GT; cl += 8) {
-      const float a = vin ? xval(data, shifts, g, I0 + cl, x, y, z) : 0.0f;
-      const unsigned short ah = bf16_bits(a);
-      sAh[cl][vl] = ah;
-      sAl[cl][vl] = bf16_bits(a - bf16_to_f(ah));
-      const float b = vin ? xval(data, shifts, g, J0 + cl, x, y, z) : 0.0f;
-      const unsigned short bh = bf16_bits(b);
-      sBh[cl][vl] = bh;
-      sBl[cl][vl] = bf16_bits(b - bf16_to_f(bh));
-    }
-    __syncthreads();
+    if (step + 1 < s1) fetch(step + 1);
 #pragma unroll
     for (int kk = 0; kk < GK / 16; ++kk) {
       const int koff = kk * 16 + 8 * (lane >> 5);
@@ -358,29 +412,63 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                        (const int4*)dtaps, g, dAA);
   } else {
     g.ntr = (g.n2 + GT - 1) / GT;
+    int hmax = 0;
+    for (const int4& t : taps) hmax = std::max(hmax, std::max(std::abs(t.x), std::max(std::abs(t.y), std::abs(t.z))));
+    g.h = hmax;
+    g.Lx = ((g.N1 + 31) / 32) * 32;
+    g.nxc = g.Lx / 32;
+    g.plane = (long long)g.N3 * g.N2 * g.Lx;
+    g.nsteps = (long long)g.N2 * g.N3 * g.nxc;
+    const int nsx = 2 * g.h + 1;
     std::vector<int2> tl;
     std::vector<int> tid_of(g.ntr * g.ntr, -1);
     for (int i = 0; i < g.ntr; ++i)
       for (int j = i; j < g.ntr; ++j) { tid_of[i * g.ntr + j] = (int)tl.size(); tl.push_back(make_int2(i, j)); }
     const int ntiles = (int)tl.size();
     int nsplit = (2048 + ntiles - 1) / ntiles;
-    long long vchunk = (g.V + nsplit - 1) / nsplit;
-    vchunk = ((vchunk + GK - 1) / GK) * GK;
-    nsplit = (int)((g.V + vchunk - 1) / vchunk);
+    if (nsplit > g.nsteps) nsplit = (int)g.nsteps;
+    g.spw = (g.nsteps + nsplit - 1) / nsplit;
+    nsplit = (int)((g.nsteps + g.spw - 1) / g.spw);
     g.nsplit = nsplit;
-    g.vchunk = vchunk;
+    // columns of X (p, c, j): hi-plane offsets of their pre-shifted operands
+    std::vector<GramCol> cols((size_t)g.ntr * GT);
+    for (int col = 0; col < g.ntr * GT; ++col) {
+      GramCol& cc = cols[col];
+      cc.valid = col < g.n2;
+      cc.base = 0; cc.sy = 0; cc.sz = 0; cc.pad = 0;
+      if (!cc.valid) continue;
+      const int pp = col >= g.n;
+      const int a = pp ? col - g.n : col;
+      const int c = a % g.nc, j = a / g.nc;
+      const int4 t = taps[j];
+      const int sxi = t.x + g.h;
+      cc.base = ((long long)(sxi * 4 + 2 * pp) * g.nc + c) * g.plane;
+      cc.sy = t.y;
+      cc.sz = t.z;
+    }
+    unsigned short* Q = nullptr;
+    GramCol* dcols = nullptr;
     int2* dtl = nullptr;
     int* dtid = nullptr;
     float* part = nullptr;
+    const long long nq = (long long)nsx * 4 * g.nc * g.plane;
+    N3D_HIP(hipMalloc(&Q, sizeof(unsigned short) * nq));
+    N3D_HIP(hipMalloc(&dcols, sizeof(GramCol) * cols.size()));
     N3D_HIP(hipMalloc(&dtl, sizeof(int2) * ntiles));
     N3D_HIP(hipMalloc(&dtid, sizeof(int) * tid_of.size()));
     N3D_HIP(hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT));
+    N3D_HIP(hipMemcpyAsync(dcols, cols.data(), sizeof(GramCol) * cols.size(), hipMemcpyHostToDevice, p->stream));
     N3D_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, p->stream));
     N3D_HIP(hipMemcpyAsync(dtid, tid_of.data(), sizeof(int) * tid_of.size(), hipMemcpyHostToDevice, p->stream));
     {
+      ProfScope ps(p, "k_gram_prep");
+      hipLaunchKernelGGL(k_gram_prep, dim3(nblk((long long)nsx * g.nc * g.plane)), dim3(256), 0,
+                         p->stream, (const float2*)data, Q, g, nsx);
+    }
+    {
       ProfScope ps(p, "k_gram_bf16x3");
       hipLaunchKernelGGL(k_gram_bf16x3, dim3(ntiles, nsplit), dim3(256), 0, p->stream,
-                         (const float2*)data, (const int4*)dtaps, (const int2*)dtl, part, g);
+                         (const unsigned short*)Q, (const GramCol*)dcols, (const int2*)dtl, part, g);
     }
     {
       ProfScope ps(p, "k_gram_assemble");
@@ -388,7 +476,7 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                          (const float*)part, (const int*)dtid, ntiles, g, dAA);
     }
     if (hipStreamSynchronize(p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
-    (void)hipFree(dtl); (void)hipFree(dtid); (void)hipFree(part);
+    (void)hipFree(Q); (void)hipFree(dcols); (void)hipFree(dtl); (void)hipFree(dtid); (void)hipFree(part);
   }
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));

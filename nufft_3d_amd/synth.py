@@ -116,3 +116,30 @@ def coil_maps(N, nc, seed=2, radius=0.5, width=0.35):
 def complex_noise(shape, sigma, seed):
     rng = np.random.default_rng(seed)
     return sigma * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+
+
+def cartesian_jittered(N, centre=24, block=(2, 2, 1), seed=4):
+    """3-D Cartesian sampling on the integer k-space grid [-N/2, N/2)^3
+    (cycles/FOV): the central ``centre``^3 points fully sampled, elsewhere one
+    random point per ``block`` of grid points (R = prod(block) outside the
+    centre; a jittered, blue-noise stand-in for Poisson-disc sampling).
+    Returns om [3, M] double with integer entries."""
+    rng = np.random.default_rng(seed)
+    bx, by, bz = block
+    g = np.arange(N) - N // 2
+    picks = []
+    for z0 in range(0, N, bz):
+        for y0 in range(0, N, by):
+            # one point per block along x for this (y0, z0) slab, vectorised
+            x0 = np.arange(0, N, bx)
+            dx = rng.integers(0, bx, size=x0.size)
+            dy = rng.integers(0, by, size=x0.size)
+            dz = rng.integers(0, bz, size=x0.size)
+            picks.append(np.stack([x0 + dx, y0 + dy, z0 + dz]))
+    p = np.concatenate(picks, axis=1)
+    h = centre // 2
+    c = np.arange(N // 2 - h, N // 2 - h + centre)
+    C = np.stack(np.meshgrid(c, c, c, indexing='ij')).reshape(3, -1)
+    inside = np.all((p >= N // 2 - h) & (p < N // 2 - h + centre), axis=0)
+    idx = np.concatenate([p[:, ~inside], C], axis=1)
+    return g[idx].astype(np.float64)

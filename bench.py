@@ -323,6 +323,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches (streams)")
     ap.add_argument("--max-batch-coils", type=int, default=64)
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: HBM budget in GB")
+    ap.add_argument("--recon-maxit", type=int, default=20, help="iNUFT iterations per reconstruction")
+    ap.add_argument("--recon-reps", type=int, default=5, help="reconstructions timed (0 = none)")
     ap.add_argument("--study", action="store_true",
                     help="per-stage FP32 (plain / compensated) errors against FP64")
     ap.add_argument("--study-iters", type=int, default=30)
@@ -404,6 +406,17 @@ def main():
     ms_per_step = total_ms / args.steps
     value = M * nc * args.steps / (total_ms / 1e3) / 1e6
 
+    # reconstruction latency: iNUFT with maxit MINRES iterations (damp 0) of
+    # all coils on the noisy data, wall time per call (device-resident arrays)
+    recon = []
+    for _ in range(args.recon_reps):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.iNUFT(data, args.recon_maxit, 0.0)
+        torch.cuda.synchronize()
+        recon.append(1e3 * (time.perf_counter() - t1))
+    recon.sort()
+
     # per-kernel durations (HIP events around each launch) over 3 more steps
     lib = plan._lib
     import ctypes as C
@@ -473,6 +486,11 @@ def main():
                    "step": "fNUFT+aNUFT, all coils in one call, device-resident arrays"},
         "latency_ms": {"p50": round(lat[len(lat) // 2], 4),
                        "p99": round(lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)], 4)},
+        "recon_latency_ms": ({"maxit": args.recon_maxit, "reps": len(recon),
+                              "p50": round(recon[len(recon) // 2], 3),
+                              "p99": round(recon[min(len(recon) - 1,
+                                                     int(math.ceil(0.99 * len(recon))) - 1)], 3)}
+                             if recon else None),
         "rel_err_vs_fp64": rel,
         "plan_s": round(plan_s, 4),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),

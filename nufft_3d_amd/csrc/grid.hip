@@ -7,6 +7,8 @@
 // round(u*om)+d, d in -ceil(J/2)..ceil(J/2) per axis (nufft_3d.m:154,168-170),
 // wrapped mod K (nufft_3d.m:178-180), kept when inside the radial or
 // separable support with <= (nufft_3d.m:182-191).
+#include <algorithm>
+
 #include "kb.h"
 #include "nufft3d_internal.h"
 
@@ -309,10 +311,11 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
 // ---------------------------------------------------------------------------
 // Adjoint, output driven: an LPS-lane group per grid point walks its hit list
 // (sorted sample ids, fixed at plan time) and sums in registers. No atomics:
-// the result is bitwise reproducible and independent of the coil batch.
-// Points with more than rmax hits (the dense k-space centre of radial
-// trajectories) are skipped here and summed by k_spread_heavy +
-// k_spread_hreduce.
+// the result is bitwise reproducible. Points with more than rmax hits (the
+// dense k-space centre of radial trajectories) are skipped here and summed
+// by k_spread_heavy + k_spread_hreduce; the plan holds two segmentations
+// (short segments for launches of at most 4 coils), so a coil's sum is
+// grouped the same way within either size class.
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
                                                 const int* __restrict__ hit_s,
@@ -503,6 +506,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
+  const DenseSet& ds = p->dense[cg <= 4 ? 0 : 1];
   {
     // brick order covers whole 8x8x8 bricks (points outside the grid exit)
     const long long npts = (p->tune_flags & 2) ? p->nbrick * 512 : ngrid;
@@ -511,18 +515,18 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
     hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
                        dim3(256), 0, p->stream, p->d_rowptr, p->d_hit_s, p->d_acode,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
-                       (cplx<R>*)grid, npts, p->d_brick, p->rmax, g, kp);
+                       (cplx<R>*)grid, npts, p->d_brick, ds.rmax, g, kp);
   }
-  if (p->nhseg > 0) {
+  if (ds.nhseg > 0) {
     ProfScope ps(p, "k_spread_heavy");
-    const long long nthreads = (long long)p->nhseg * LPS;
+    const long long nthreads = (long long)ds.nhseg * LPS;
     hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
-                       dim3(256), 0, p->stream, p->d_hseg, p->nhseg, p->d_hit_s, p->d_acode,
+                       dim3(256), 0, p->stream, ds.d_hseg, ds.nhseg, p->d_hit_s, p->d_acode,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
                        (cplx<R>*)partial, g, kp);
     hipLaunchKernelGGL((k_spread_hreduce<R, COMP>),
-                       dim3((unsigned)(((long long)p->nhpt * cg + 255) / 256)), dim3(256), 0,
-                       p->stream, p->d_hpt, p->nhpt, (const cplx<R>*)partial, (cplx<R>*)grid, g);
+                       dim3((unsigned)(((long long)ds.nhpt * cg + 255) / 256)), dim3(256), 0,
+                       p->stream, ds.d_hpt, ds.nhpt, (const cplx<R>*)partial, (cplx<R>*)grid, g);
   }
 }
 
@@ -594,8 +598,9 @@ template <typename R>
 static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const void* wsamp) {
   const int cgmax = nc < 16 ? nc : 16;
   void* partial = nullptr;
-  if (p->nhseg > 0) {
-    N3D_TRY(ensure(p->ws_part, (size_t)p->nhseg * cgmax * 2 * sizeof(double)));
+  const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
+  if (nseg > 0) {
+    N3D_TRY(ensure(p->ws_part, (size_t)nseg * cgmax * 2 * sizeof(double)));
     partial = p->ws_part.p;
   }
   if (sizeof(R) == 4 && p->compensated) run_groups<R, true, true>(p, samp, nc, grid, partial, wsamp, 1.0);

@@ -64,6 +64,17 @@ struct HeavyPt {
   int pad;
 };
 
+// Grid points with more than rmax hits are summed as segments of chunk_len
+// hits (k_spread_heavy) whose partials add in segment order (k_spread_hreduce)
+struct DenseSet {
+  int rmax = 512;
+  int chunk_len = 512;
+  HeavySeg* d_hseg = nullptr;
+  int nhseg = 0;
+  HeavyPt* d_hpt = nullptr;
+  int nhpt = 0;
+};
+
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
@@ -131,12 +142,9 @@ struct nufft3d_plan {
   int* d_hit_s = nullptr;           // [nhits] sample ids (grid-major)
   unsigned short* d_acode = nullptr;  // [nhits] codes (grid-major)
   float4* d_off32 = nullptr;        // [nnz] (fx, fy, fz, 0)
-  int rmax = 512;                   // grid points with more hits are split
-  nufft3d::HeavySeg* d_hseg = nullptr;
-  int nhseg = 0;
-  nufft3d::HeavyPt* d_hpt = nullptr;
-  int nhpt = 0;
-  int chunk_len = 512;              // hits per heavy segment
+  // dense-point segmentations of the same hit lists: [0] for launches of at
+  // most 4 coils (one lane per point: short walks), [1] for larger groups
+  nufft3d::DenseSet dense[2];
   int* d_brick = nullptr;           // [nbrick] 8x8x8 spread bricks in traversal order (tx|ty<<10|tz<<20)
   long long nbrick = 0;
   int max_hits = 0;
@@ -144,6 +152,11 @@ struct nufft3d_plan {
   nufft3d::FftAxis ax[3];
   nufft3d::FftAxis axN[3];  // image-size axes (unshifted FFT of the PRUNO calibration)
   nufft3d::Profiler prof;
+
+  // cached HIP graph of one block of MINRES iterations and what it captured
+  int use_graphs = 1;
+  hipGraphExec_t mr_exec = nullptr;
+  std::vector<uint64_t> mr_key;
 
   // scratch, grown on demand
   nufft3d::Workspace ws_grid, ws_t1, ws_t2, ws_samp, ws_part, ws_img, ws_host, ws_solve, ws_red;

@@ -460,6 +460,62 @@ static int user_density_t(nufft3d_plan* p, const double* dh, int64_t nd) {
 }
 
 // ---------------------------------------------------------------- plan build
+// Segment list of one dense-point set: points with more than ds.rmax hits,
+// in segments of ds.chunk_len hits. Processing order: by segment index
+// within the point, then by the point's place in the Z-ordered 8^3 bricks
+// (neighbouring dense points hold mostly the same spokes in ascending sample
+// order, so their k-th segments read similar samples side by side); each
+// segment keeps the partial slot of its place within its point, so the sums
+// are independent of that order.
+static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long long ngrid,
+                       DenseSet& ds) {
+  std::vector<HeavySeg> hseg;
+  std::vector<HeavyPt> hpt;
+  std::vector<long long> seg_k;
+  long long pos = 0;
+  for (long long gi = 0; gi < ngrid; ++gi) {
+    const int c = (int)hcnt[gi];
+    if (c > ds.rmax) {
+      const int nseg = (c + ds.chunk_len - 1) / ds.chunk_len;
+      hpt.push_back({(int)gi, (int)hseg.size(), nseg, 0});
+      for (int j = 0; j < nseg; ++j) {
+        const long long a = pos + (long long)j * ds.chunk_len;
+        const long long e = std::min(pos + c, a + ds.chunk_len);
+        hseg.push_back({a, e, (int)gi, (int)hseg.size()});
+        seg_k.push_back(j);
+      }
+    }
+    pos += c;
+  }
+  if (!hseg.empty()) {
+    int bits = 1;
+    while ((1 << bits) < std::max((p->K[0] + 7) / 8, std::max((p->K[1] + 7) / 8, (p->K[2] + 7) / 8)))
+      ++bits;
+    std::vector<std::pair<unsigned long long, int>> ord(hseg.size());
+    for (size_t i = 0; i < hseg.size(); ++i) {
+      const long long gi = hseg[i].g;
+      const int x = (int)(gi % p->K[0]), y = (int)((gi / p->K[0]) % p->K[1]),
+                z = (int)(gi / ((long long)p->K[0] * p->K[1]));
+      const unsigned long long brick = morton3(x >> 3, y >> 3, z >> 3, bits);
+      const unsigned local = (unsigned)((x & 7) | ((y & 7) << 3) | ((z & 7) << 6));
+      ord[i] = {((unsigned long long)seg_k[i] << 40) | (brick << 9) | local, (int)i};
+    }
+    std::stable_sort(ord.begin(), ord.end());
+    std::vector<HeavySeg> h2(hseg.size());
+    for (size_t i = 0; i < ord.size(); ++i) h2[i] = hseg[ord[i].second];
+    hseg.swap(h2);
+  }
+  ds.nhseg = (int)hseg.size();
+  ds.nhpt = (int)hpt.size();
+  N3D_HIP(hipMalloc(&ds.d_hseg, sizeof(HeavySeg) * std::max<size_t>(1, hseg.size())));
+  N3D_HIP(hipMalloc(&ds.d_hpt, sizeof(HeavyPt) * std::max<size_t>(1, hpt.size())));
+  if (!hseg.empty()) {
+    N3D_HIP(hipMemcpy(ds.d_hseg, hseg.data(), sizeof(HeavySeg) * hseg.size(), hipMemcpyHostToDevice));
+    N3D_HIP(hipMemcpy(ds.d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
+  }
+  return NUFFT3D_OK;
+}
+
 static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   const bool dev = is_device_ptr(om_in);
   double* om = nullptr;
@@ -663,36 +719,14 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   (void)hipFree(cnt_s); (void)hipFree(cnt_g); (void)hipFree(counts);
 
   // dense grid points (the k-space centre of radial trajectories) are split
-  // into segments of chunk_len hits whose partial sums add in segment order
-  std::vector<HeavySeg> hseg;
-  std::vector<HeavyPt> hpt;
+  // into segments of chunk_len hits whose partial sums add in segment order;
+  // two segmentations over the same hit lists (see DenseSet)
   {
-    long long pos = 0;
     int maxc = 0;
-    for (long long gi = 0; gi < ngrid; ++gi) {
-      const int c = (int)hcnt[gi];
-      maxc = std::max(maxc, c);
-      if (c > p->rmax) {
-        const int nseg = (c + p->chunk_len - 1) / p->chunk_len;
-        hpt.push_back({(int)gi, (int)hseg.size(), nseg, 0});
-        for (int j = 0; j < nseg; ++j) {
-          const long long a = pos + (long long)j * p->chunk_len;
-          const long long e = std::min(pos + c, a + p->chunk_len);
-          hseg.push_back({a, e, (int)gi, (int)hseg.size()});
-        }
-      }
-      pos += c;
-    }
+    for (long long gi = 0; gi < ngrid; ++gi) maxc = std::max(maxc, (int)hcnt[gi]);
     p->max_hits = maxc;
   }
-  p->nhseg = (int)hseg.size();
-  p->nhpt = (int)hpt.size();
-  N3D_HIP(hipMalloc(&p->d_hseg, sizeof(HeavySeg) * std::max<size_t>(1, hseg.size())));
-  N3D_HIP(hipMalloc(&p->d_hpt, sizeof(HeavyPt) * std::max<size_t>(1, hpt.size())));
-  if (!hseg.empty()) {
-    N3D_HIP(hipMemcpy(p->d_hseg, hseg.data(), sizeof(HeavySeg) * hseg.size(), hipMemcpyHostToDevice));
-    N3D_HIP(hipMemcpy(p->d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
-  }
+  for (int set = 0; set < 2; ++set) N3D_TRY(build_dense(p, hcnt, ngrid, p->dense[set]));
 
   // spread traversal: 8x8x8 bricks of the grid, Z-order unless linear order
   // is asked for, so consecutive work-groups cover compact regions
@@ -738,13 +772,15 @@ static void destroy_plan(nufft3d_plan* p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit_s, p->d_acode,
-                  p->d_off32, p->d_hseg, p->d_hpt, p->d_brick};
+                  p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
+                  p->dense[1].d_hpt, p->d_brick};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int a = 0; a < 3; ++a) {
     if (p->ax[a].d_tw) (void)hipFree(p->ax[a].d_tw);
     if (p->axN[a].d_tw) (void)hipFree(p->axN[a].d_tw);
   }
+  if (p->mr_exec) (void)hipGraphExecDestroy(p->mr_exec);
   for (auto& r : p->prof.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
   release(p->ws_grid); release(p->ws_t1); release(p->ws_t2); release(p->ws_samp);
@@ -843,8 +879,12 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_TUNE")) p->tune_flags = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FFTB")) p->fft_bundle = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_GRAPH")) p->use_graphs = atoi(tv) != 0;
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
-  if (o.chunk > 0) { p->chunk_len = o.chunk; p->rmax = o.chunk; }
+  // dense-point segments: 64 hits for launches of <= 4 coils, 512 (or the
+  // caller's chunk) for larger ones
+  p->dense[0].chunk_len = p->dense[0].rmax = o.chunk > 0 ? std::min(64, (int)o.chunk) : 64;
+  p->dense[1].chunk_len = p->dense[1].rmax = o.chunk > 0 ? (int)o.chunk : 512;
   if (p->T[0] * p->T[1] * p->T[2] > 512) {
     set_error("tile too large (at most 512 grid points per tile)");
     delete p;
@@ -888,7 +928,8 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
            p->radial);
     printf("\tbins %d (%dx%dx%d) hits %lld (%.1f/sample) max hits/point %d dense points %d "
            "segments %d\n", p->ntiles, p->T[0], p->T[1], p->T[2], (long long)p->nhits,
-           p->nnz ? (double)p->nhits / p->nnz : 0.0, p->max_hits, p->nhpt, p->nhseg);
+           p->nnz ? (double)p->nhits / p->nnz : 0.0, p->max_hits, p->dense[1].nhpt,
+           p->dense[1].nhseg);
   }
   *out = p;
   return NUFFT3D_OK;
@@ -912,7 +953,8 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
   b += (2 + 4 + 2) * std::max<int64_t>(p->nhits, 1);                  // fcode, hit_s, acode
   if (p->d_off32) b += 16 * nnz;
-  b += (int64_t)sizeof(HeavySeg) * std::max(1, p->nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, p->nhpt);
+  for (const DenseSet& ds : p->dense)
+    b += (int64_t)sizeof(HeavySeg) * std::max(1, ds.nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, ds.nhpt);
   b += 4 * p->nbrick;
   for (int a = 0; a < 3; ++a) {
     if (p->ax[a].d_tw) b += 2 * rsz * p->ax[a].L;

@@ -265,11 +265,25 @@ int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void
 }
 
 // MINRES on nc right-hand sides of length n stored [nc][n]; op(v, y) applies
-// the Hermitian operator to all right-hand sides at once.
+// the Hermitian operator to all right-hand sides at once. `opkey` identifies
+// the operator (its scalars and extra arrays) for the cached iteration graph.
+//
+// Iteration 1 runs eagerly (it also grows every workspace the operator
+// uses). After it, blocks of kGraphBlock iterations - the period of the
+// buffer rotation (w/w2 swap, r1/r2/y cycle), so a block leaves the buffers
+// where it found them - are captured once as a HIP graph and replayed; the
+// host checks for convergence between blocks. The graph is cached on the
+// plan while every pointer and scalar it captured is unchanged. Without a
+// capturable stream (the null stream, profiling on, a caller already
+// capturing, NUFFT3D_GRAPH=0) the blocks run eagerly: same kernels, same
+// order, bitwise the same result.
+constexpr int kGraphBlock = 6;
+
 template <typename R>
 static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int maxit, double tol,
                        void* xout, nufft3d_info* info,
-                       const std::function<int(const void*, void*)>& op) {
+                       const std::function<int(const void*, void*)>& op,
+                       const std::vector<uint64_t>& opkey) {
   using C = cplx<R>;
   const size_t vec = (size_t)n * nc * sizeof(C);
   // vectors: v, r1, r2, y, w, w2 ; x is the caller's output
@@ -296,7 +310,8 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
                      st, x, r2, v);
   N3D_HIP(hipMemsetAsync(w, 0, vec, p->stream));
   N3D_HIP(hipMemsetAsync(w2, 0, vec, p->stream));
-  for (int itn = 1; itn <= maxit; ++itn) {
+
+  auto iterate = [&](int itn) -> int {
     N3D_TRY(op(v, y));
     hipLaunchKernelGGL((k_mr_phaseA<R>), grid, dim3(kRedThreads), 0, p->stream, y, (const C*)r1,
                        (const C*)v, n, (const MrState*)st, itn, part);
@@ -320,14 +335,72 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
     r2 = y;
     y = t;
     N3D_HIP(hipGetLastError());
-    if (itn % 10 == 0 && itn < maxit) {   // early exit once every column stopped
-      std::vector<MrState> hs(nc);
-      N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(MrState), hipMemcpyDeviceToHost, p->stream));
-      N3D_HIP(hipStreamSynchronize(p->stream));
-      bool any = false;
-      for (int c = 0; c < nc; ++c) any = any || hs[c].active;
-      if (!any) break;
+    return NUFFT3D_OK;
+  };
+  auto any_active = [&](bool& any) -> int {   // early exit once every column stopped
+    std::vector<MrState> hs(nc);
+    N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(MrState), hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+    any = false;
+    for (int c = 0; c < nc; ++c) any = any || hs[c].active;
+    return NUFFT3D_OK;
+  };
+
+  int itn = 1;
+  if (maxit >= 1) N3D_TRY(iterate(itn++));
+  // the graph's identity: every pointer and scalar its kernels captured
+  std::vector<uint64_t> key(opkey);
+  const void* ptrs[] = {b, xout, p->ws_solve.p, p->ws_red.p, p->ws_grid.p, p->ws_samp.p,
+                        p->ws_t1.p, p->ws_t2.p, p->ws_part.p, p->stream};
+  for (const void* q : ptrs) key.push_back((uint64_t)(uintptr_t)q);
+  key.push_back((uint64_t)n);
+  key.push_back((uint64_t)nc);
+  key.push_back(__builtin_bit_cast(uint64_t, tol));
+  key.push_back((uint64_t)sizeof(R));
+  bool graphs = p->use_graphs && !p->prof.on && p->stream != nullptr;
+  bool stop = false;
+  while (!stop && itn + kGraphBlock - 1 <= maxit) {
+    bool any = true;
+    N3D_TRY(any_active(any));
+    if (!any) { stop = true; break; }
+    bool done = false;
+    if (graphs) {
+      if (!(p->mr_exec && p->mr_key == key)) {
+        if (p->mr_exec) { (void)hipGraphExecDestroy(p->mr_exec); p->mr_exec = nullptr; }
+        hipGraph_t gph = nullptr;
+        bool ok = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        int st_cap = NUFFT3D_OK;
+        if (ok) {
+          for (int k = 0; k < kGraphBlock && st_cap == NUFFT3D_OK; ++k) st_cap = iterate(itn + k);
+          ok = hipStreamEndCapture(p->stream, &gph) == hipSuccess && st_cap == NUFFT3D_OK && gph;
+        }
+        hipGraphExec_t ex = nullptr;
+        if (ok) ok = hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0) == hipSuccess;
+        if (gph) (void)hipGraphDestroy(gph);
+        (void)hipGetLastError();
+        if (ok) {
+          p->mr_exec = ex;
+          p->mr_key = key;
+        } else {
+          // nothing captured ran; the rotation made a full period, so the
+          // buffers are where they were: run this block eagerly from here on
+          graphs = false;
+        }
+      }
+      if (graphs) {
+        N3D_HIP(hipGraphLaunch(p->mr_exec, p->stream));
+        done = true;
+      }
     }
+    if (!done)
+      for (int k = 0; k < kGraphBlock; ++k) N3D_TRY(iterate(itn + k));
+    itn += kGraphBlock;
+  }
+  if (!stop && itn <= maxit) {
+    bool any = true;
+    N3D_TRY(any_active(any));
+    if (any)
+      for (; itn <= maxit; ++itn) N3D_TRY(iterate(itn));
   }
   std::vector<MrState> hs(nc);
   N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(MrState), hipMemcpyDeviceToHost, p->stream));
@@ -346,9 +419,10 @@ template <typename R>
 static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
                     const void* wsamp, double tol, void* xout, nufft3d_info* info) {
   const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
+  const std::vector<uint64_t> key = {1, __builtin_bit_cast(uint64_t, damp), (uint64_t)(uintptr_t)wsamp};
   return minres_core<R>(p, b, n, nc, maxit, tol, xout, info, [&](const void* v, void* y) {
     return apply_normal(p, v, nc, damp, wsamp, y);
-  });
+  }, key);
 }
 
 // Coils coupled through the PRUNO null term: one right-hand side of length
@@ -358,10 +432,12 @@ static int minres_coupled_t(nufft3d_plan* p, const void* b, int nc, int maxit, d
                             const void* wsamp, const void* B, double tol, void* xout,
                             nufft3d_info* info) {
   const long long n = (long long)p->N[0] * p->N[1] * p->N[2] * nc;
+  const std::vector<uint64_t> key = {2, __builtin_bit_cast(uint64_t, damp), (uint64_t)(uintptr_t)wsamp,
+                                      (uint64_t)(uintptr_t)B, (uint64_t)nc};
   return minres_core<R>(p, b, n, 1, maxit, tol, xout, info, [&](const void* v, void* y) {
     N3D_TRY(apply_normal(p, v, nc, damp, wsamp, y));
     return null_apply(p, B, v, y, nc);
-  });
+  }, key);
 }
 
 int minres_coupled(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,

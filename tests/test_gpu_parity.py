@@ -222,3 +222,47 @@ def test_u125_in_fp32_arithmetic():
     want = orc.iNUFT(raw, 8, 0.05)
     assert rel(g.iNUFT(raw, 8, 0.05), want) < 3e-4
 
+
+
+def _plan_with_env(env, *a, **kw):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return nufft_3d(*a, **kw)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("gpu", [1, 2])
+def test_minres_graph_replay_matches_eager(gpu):
+    # 31 iterations: 1 eager + 5 replays of the captured 6-iteration block,
+    # with the host convergence check (a synchronisation) between replays;
+    # a tiny tolerance keeps every column active throughout
+    om = synth.koosh_ball(16, 60, 32)
+    raw = _rand((om.shape[1], 3), 61)
+    eager = _plan_with_env({"NUFFT3D_GRAPH": "0"}, om, 16, gpu=gpu)
+    graph = nufft_3d(om, 16, gpu=gpu)
+    want, wi = eager.iNUFT(raw, 31, 0.02, tol=1e-30, return_info=True)
+    for rep in range(2):     # the second call runs through the cached graph when the buffers match
+        got, gi = graph.iNUFT(raw, 31, 0.02, tol=1e-30, return_info=True)
+        assert np.array_equal(got, want), rep
+        assert gi["iters"] == wi["iters"] and gi["relres"] == wi["relres"]
+    assert wi["iters"][:3] == [31, 31, 31]
+
+
+def test_minres_graph_replay_coupled_pruno():
+    N, nc = 8, 3
+    om = synth.koosh_ball(N, 60, 16)
+    orc = O.NufftOracle(om, N)
+    maps = synth.coil_maps(N, nc, seed=1)
+    raw = orc.fNUFT(synth.phantom_complex(N)[..., None] * maps) + 1e-3 * _rand((orc.M, nc), 62)
+    kw = dict(constraint="parallel-imaging-pruno", lambda_=0.3,
+              pruno_opts=dict(width=3, noise=1e-3), pruno_iters=19)
+    eager = _plan_with_env({"NUFFT3D_GRAPH": "0"}, om, N, gpu=2)
+    graph = nufft_3d(om, N, gpu=2)
+    assert np.array_equal(graph.iNUFT(raw, 4, 0.01, **kw), eager.iNUFT(raw, 4, 0.01, **kw))

@@ -185,6 +185,7 @@ def test_gpu_results_same_alone_and_in_the_mix():
         e = float(torch.linalg.vector_norm(a - b) / max(float(torch.linalg.vector_norm(b)), 1e-30))
         worst = max(worst, e)
     print("worst relative L2 difference alone vs mixed: %.3e" % worst, flush=True)
-    # the kernels treat coils independently in a fixed order: equality is
-    # expected to the bit; the stated tolerance leaves room only for rounding
+    # the kernels treat coils independently in a fixed order; the only
+    # difference is the grouping of dense-point sums (shorter segments for
+    # launches of at most 4 coils), i.e. rounding
     assert worst <= 1e-6

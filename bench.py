@@ -7,7 +7,9 @@ Kaiser-Bessel J=4, u=2 (256^3 grid), complex64, all 16 coils as right-hand
 sides of one launch, additive complex noise at 1% of max|k|.
 
 One step = one fNUFT (image -> k-space, 16 coils) + one aNUFT (k-space ->
-image, 16 coils) through the public operator on device-resident arrays.
+image, 16 coils) through the public operator on device-resident arrays,
+after the library's autotuner has picked the plan's launch settings for 16
+coils (untimed, ~1 s; --no-autotune skips it).
 value = M * nc * steps / seconds / 1e6.
 
 The FP32 results are checked against the FP64 HIP path (itself pinned to the
@@ -323,6 +325,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches (streams)")
     ap.add_argument("--max-batch-coils", type=int, default=64)
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serving: HBM budget in GB")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the library's autotuner (launch settings at their defaults)")
     ap.add_argument("--recon-maxit", type=int, default=20, help="iNUFT iterations per reconstruction")
     ap.add_argument("--recon-reps", type=int, default=5, help="reconstructions timed (0 = none)")
     ap.add_argument("--study", action="store_true",
@@ -359,6 +363,13 @@ def main():
     plan = nufft_3d(om, N, J=w["J"], u=w["u"], gpu=1, chunk=args.chunk, tile=args.tile)
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
+    tuned = None
+    if args.autotune:
+        t1 = time.perf_counter()
+        rep = plan.autotune(nc, reps=3)
+        tuned = dict(rep["candidates"][rep["best"]], tune_s=round(time.perf_counter() - t1, 3),
+                     default_ms=rep["candidates"][0]["ms_per_pair"])
+        log("autotune:", tuned)
     log("plan %.3f s  K=%s nnz=%d" % (plan_s, plan.K, plan.nnz))
     K = plan.K[0]
 
@@ -493,6 +504,7 @@ def main():
                              if recon else None),
         "rel_err_vs_fp64": rel,
         "plan_s": round(plan_s, 4),
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

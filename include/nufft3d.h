@@ -164,6 +164,15 @@ int nufft3d_inverse_pruno(nufft3d_plan* plan, const void* raw, int32_t nc, int32
                           const nufft3d_pruno_opts* opts, int32_t iters, void* im,
                           nufft3d_info* info);
 
+/* Autotuner, no counterpart in the reference: times `reps` fwd+adj operator
+ * pairs of nc coils for each combination of FFT lines per work-group (auto,
+ * 8, 4), coils per gridding launch (16, 8), XCD block remap (off, runs of 32)
+ * and spread brick traversal (on, off), keeps the fastest on the plan, and
+ * writes the timings as JSON to report (may be NULL). Results are unchanged
+ * by every choice except the coil grouping, which can move the dense-point
+ * segmentation class (rounding). */
+int nufft3d_autotune(nufft3d_plan* plan, int32_t nc, int32_t reps, char* report, int64_t len);
+
 /* Measurement hook, no counterpart in the reference (which only prints
  * tic/toc wall times, nufft_3d.m:130,213): while enabled, every kernel launch
  * of the plan is bracketed by HIP events on the plan's stream. */

@@ -530,13 +530,15 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   }
 }
 
-// coil groups of up to 16; fp32 groups of more than 4 coils put 2 or 4 lanes
-// on one sample / grid point (4 coils each), fp64 keeps one lane
+// coil groups of up to coil_group (16 or 8); fp32 groups of more than 4
+// coils put 2 or 4 lanes on one sample / grid point (4 coils each), fp64
+// keeps one lane
 template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
                        const void* wsamp, double scale) {
-  for (int c0 = 0; c0 < nc; c0 += 16) {
-    const int cg = nc - c0 < 16 ? nc - c0 : 16;
+  const int gmax = p->coil_group;
+  for (int c0 = 0; c0 < nc; c0 += gmax) {
+    const int cg = nc - c0 < gmax ? nc - c0 : gmax;
     // full groups with an even coil stride and offset take aligned vector loads
     const bool even = (nc % 2 == 0) && (c0 % 2 == 0);
 #define N3D_GO2(CPL, LPS, FULL)                                                                  \
@@ -596,7 +598,7 @@ int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout
 // samples in the internal layout ks[s*nc + c] (sorted order)
 template <typename R>
 static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const void* wsamp) {
-  const int cgmax = nc < 16 ? nc : 16;
+  const int cgmax = nc < p->coil_group ? nc : p->coil_group;
   void* partial = nullptr;
   const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
   if (nseg > 0) {

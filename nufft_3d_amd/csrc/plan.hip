@@ -803,6 +803,18 @@ __global__ void k_convert(const A* __restrict__ in, B* __restrict__ out, long lo
   if (i < n) out[i] = (B)in[i];
 }
 
+// deterministic, non-constant test data for timing runs
+template <typename R>
+__global__ void k_fill_pattern(R* __restrict__ a, long long n, unsigned seed) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 2654435761u ^ seed;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  a[i] = (R)((double)(h & 0xffffff) / 16777216.0 - 0.5);
+}
+
 extern "C" {
 
 const char* nufft3d_last_error(void) { return g_last_error.c_str(); }
@@ -1316,6 +1328,100 @@ int nufft3d_density(nufft3d_plan* p, void* d) {
   N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_out);
   release(tmp_io);
+  return st;
+}
+
+int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, int64_t len) {
+  N3D_PLAN_GUARD(p);
+  if (nc <= 0 || reps <= 0 || (report && len <= 0)) {
+    set_error("invalid nc/reps/report");
+    return NUFFT3D_ERR_INVALID;
+  }
+  const size_t esz = cp_esz(p);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nks = (size_t)(p->nnz > 0 ? p->nnz : 1) * nc;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  const double sc = 1.0 / sqrt((double)std::max<int64_t>(p->nnz, 1));
+  Workspace xin, xout, ks;
+  int st = ensure(xin, nimg * esz);
+  if (!st) st = ensure(xout, nimg * esz);
+  if (!st) st = ensure(ks, nks * esz);
+  if (!st) st = ensure(p->ws_grid, ngrid * esz);
+  if (st) { release(xin); release(xout); release(ks); return st; }
+  const long long nr = (long long)(nimg * esz / (p->prec == 2 ? 8 : 4));
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_fill_pattern<double>), dim3(nblocks(nr)), dim3(256), 0, p->stream,
+                       (double*)xin.p, nr, 17u);
+  else
+    hipLaunchKernelGGL((k_fill_pattern<float>), dim3(nblocks(nr)), dim3(256), 0, p->stream,
+                       (float*)xin.p, nr, 17u);
+  // one operator pair on device arrays in the internal layouts
+  auto pair = [&]() -> int {
+    N3D_TRY(fft_forward(p, xin.p, nc, p->ws_grid.p));
+    N3D_TRY(interp(p, p->ws_grid.p, nc, ks.p, 0, nullptr, sc));
+    N3D_TRY(spread(p, ks.p, nc, p->ws_grid.p, 0, nullptr));
+    return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
+  };
+  struct Cand { int fftb, group, xcd, brick; float ms; };
+  std::vector<Cand> cands;
+  for (int fftb : {0, 8, 4})
+    for (int group : {16, 8})
+      for (int xcd : {0, 32})
+        for (int brick : {1, 0}) {
+          if (group == 8 && nc <= 8) continue;   // same launches as 16
+          cands.push_back({fftb, group, xcd, brick, 0.f});
+        }
+  const int keep_flags = p->tune_flags;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipGetLastError();
+    st = NUFFT3D_ERR_DEVICE;
+    set_error("hipEventCreate failed");
+  }
+  int best = -1;
+  for (size_t i = 0; i < cands.size() && !st; ++i) {
+    Cand& c = cands[i];
+    p->fft_bundle = c.fftb;
+    p->coil_group = c.group;
+    p->xcd_chunk = c.xcd;
+    p->tune_flags = c.brick ? (keep_flags | 2) : (keep_flags & ~2);
+    st = pair();                                          // warm-up
+    if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+    for (int r = 0; r < reps && !st; ++r) st = pair();
+    if (!st && hipEventRecord(e1, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+    if (!st && hipEventSynchronize(e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+    float ms = 0;
+    if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+    c.ms = ms / reps;
+    if (!st && (best < 0 || c.ms < cands[best].ms)) best = (int)i;
+  }
+  if (st == NUFFT3D_ERR_DEVICE) { (void)hipGetLastError(); set_error("device error while autotuning"); }
+  if (best >= 0) {
+    const Cand& b = cands[best];
+    p->fft_bundle = b.fftb;
+    p->coil_group = b.group;
+    p->xcd_chunk = b.xcd;
+    p->tune_flags = b.brick ? (keep_flags | 2) : (keep_flags & ~2);
+  } else {
+    p->tune_flags = keep_flags;
+  }
+  if (!st && report) {
+    std::string js = "{\"nc\": " + std::to_string(nc) + ", \"candidates\": [";
+    for (size_t i = 0; i < cands.size(); ++i) {
+      char b[160];
+      snprintf(b, sizeof b, "%s{\"fft_lines\": %d, \"coil_group\": %d, \"xcd_chunk\": %d, "
+               "\"brick_order\": %d, \"ms_per_pair\": %.5f}", i ? ", " : "", cands[i].fftb,
+               cands[i].group, cands[i].xcd, cands[i].brick, cands[i].ms);
+      js += b;
+    }
+    js += "], \"best\": " + std::to_string(best) + "}";
+    if ((int64_t)js.size() + 1 > len) { set_error("report buffer too small"); st = NUFFT3D_ERR_INVALID; }
+    else std::memcpy(report, js.c_str(), js.size() + 1);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(p->stream);
+  release(xin); release(xout); release(ks);
   return st;
 }
 

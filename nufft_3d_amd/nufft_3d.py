@@ -119,6 +119,14 @@ class nufft_3d:
         _lib.check(self._lib.nufft3d_device_bytes(self._h, C.byref(pb), C.byref(wb)))
         return pb.value, wb.value
 
+    def autotune(self, nc=1, reps=3):
+        """Pick the fastest launch settings for nc coils on this plan
+        (nufft3d_autotune); returns the timings as a dict."""
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        _lib.check(self._lib.nufft3d_autotune(self._h, int(nc), int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def release_workspaces(self):
         """Free the per-call workspaces (they regrow on the next call)."""
         _lib.check(self._lib.nufft3d_release_workspaces(self._h))

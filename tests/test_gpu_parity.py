@@ -266,3 +266,19 @@ def test_minres_graph_replay_coupled_pruno():
     eager = _plan_with_env({"NUFFT3D_GRAPH": "0"}, om, N, gpu=2)
     graph = nufft_3d(om, N, gpu=2)
     assert np.array_equal(graph.iNUFT(raw, 4, 0.01, **kw), eager.iNUFT(raw, 4, 0.01, **kw))
+
+
+def test_autotune_picks_a_measured_setting_and_keeps_results():
+    om = synth.koosh_ball(16, 60, 32)
+    x = _rand((16, 16, 16, 12), 71)
+    p = nufft_3d(om, 16, gpu=1)
+    before = p.aNUFT(p.fNUFT(x))
+    rep = p.autotune(nc=12, reps=2)
+    cands = rep["candidates"]
+    assert len(cands) == 24 and 0 <= rep["best"] < len(cands)
+    assert all(c["ms_per_pair"] > 0 for c in cands)
+    assert cands[rep["best"]]["ms_per_pair"] == min(c["ms_per_pair"] for c in cands)
+    after = p.aNUFT(p.fNUFT(x))
+    # every setting computes the same operator; the coil grouping may move
+    # the dense-point segmentation class (rounding only)
+    assert rel(after, before) < 1e-6

@@ -337,23 +337,65 @@ class NufftOracle:
         im = np.reshape(x, (N[0], N[1], N[2], nc), order='F')
         return (im, info) if return_info else im
 
+    # discrep.m:2-69: discrepancy-principle estimate of damp
+    def discrep(self, data, stddev=None, iters=20, return_info=False, maxit=None):
+        data = np.asarray(data, dtype=np.complex128)
+        if data.shape[0] == self.M:
+            data = np.reshape(data, (self.M, -1), order='F')
+        else:
+            nr, ny = data.shape[0], data.shape[1]
+            if nr * ny != self.M:
+                raise ValueError('data leading dimension(s) must be length %d' % self.M)
+            data = np.reshape(data, (self.M, -1), order='F')
+        if stddev is None:                                  # discrep.m:29-33
+            tmp = data[data != 0]
+            tmp = np.concatenate([tmp.real, tmp.imag])
+            stddev = float(np.median(np.abs(tmp - np.median(tmp))))
+        discrepancy = stddev * np.sqrt(data.size)           # discrep.m:37
+        W = self.d
+        N = self.N
+        b = np.reshape(self.aNUFT(W[:, None] * data), (int(np.prod(N)), -1), order='F')
+        if maxit is None:                                   # MATLAB minres default
+            maxit = min(int(np.prod(N)), 20)
+        x = None
+        damp = 0.1
+        hist = []
+        for _ in range(iters):                              # discrep.m:54-68
+            x, _ = minres(lambda v: self.iprojection(v, damp, W), b, 1e-4, maxit, x0=x)
+            k = self.fNUFT(np.reshape(x, (N[0], N[1], N[2], -1), order='F'))
+            resnorm = float(np.linalg.norm(k - data))
+            damp = damp * (discrepancy / resnorm)
+            hist.append(damp)
+        info = {'stddev': stddev, 'discrepancy': discrepancy, 'damp_history': hist}
+        return (damp, info) if return_info else damp
+
 
 # ---------------------------------------------------------------- MINRES
-def minres(A, b, tol=1e-4, maxit=20):
-    """Paige-Saunders MINRES from x0=0 for a Hermitian operator.
+def minres(A, b, tol=1e-4, maxit=20, x0=None):
+    """Paige-Saunders MINRES for a Hermitian operator, from x0 (default 0).
 
     ``b`` is [n, nrhs]; each column is an independent solve with its own
     Lanczos scalars (the matrix-RHS semantics ``iNUFT.m:133-136`` implies).
     ``A`` maps an [n, nrhs] block to an [n, nrhs] block column by column.
-    A column stops updating when its recurrence residual phibar/beta1 drops
-    to ``tol`` (the MATLAB stopping test ``||b-Ax||/||b|| <= tol``).
-    Returns (x, info) with per-column iteration counts and relres.
+    A column stops updating when its recurrence residual drops to
+    ``tol*||b||`` (the MATLAB stopping test ``||b-Ax||/||b|| <= tol``). With
+    an initial guess (as ``minres(A,b,tol,maxit,[],[],x0)`` in
+    ``discrep.m:60``) the recurrence runs on r0 = b - A x0 from x0, and a
+    column whose r0 already meets the test takes no iteration. Returns
+    (x, info) with per-column iteration counts and relres ||b-Ax||/||b||.
     """
     b = np.asarray(b, dtype=np.complex128)
     n, nr = b.shape
-    x = np.zeros_like(b)
+    bnorm = np.sqrt(np.real(np.sum(np.conj(b) * b, axis=0)))
+    if x0 is None:
+        x = np.zeros_like(b)
+    else:
+        x = np.array(x0, dtype=np.complex128).reshape(n, nr)
+        b = b - A(x)
     beta1 = np.sqrt(np.real(np.sum(np.conj(b) * b, axis=0)))
-    active = beta1 > 0
+    bden = np.where(bnorm > 0, bnorm, 1.0)
+    x[:, bnorm == 0] = 0              # zero right-hand side: x = 0, no iteration
+    active = (bnorm > 0) & (beta1 > 0) & (beta1 / bden > tol)
     r1 = b.copy()
     r2 = b.copy()
     y = b.copy()
@@ -367,7 +409,7 @@ def minres(A, b, tol=1e-4, maxit=20):
     cs = -np.ones(nr)
     sn = np.zeros(nr)
     iters = np.zeros(nr, dtype=np.int64)
-    relres = np.where(beta1 > 0, 1.0, 0.0)
+    relres = np.where(bnorm > 0, beta1 / bden, 0.0)
     for itn in range(1, maxit + 1):
         if not np.any(active):
             break
@@ -398,7 +440,7 @@ def minres(A, b, tol=1e-4, maxit=20):
         upd = active.astype(np.float64)
         x = x + (upd * phi)[None, :] * w
         iters = iters + active
-        rr = np.where(beta1 > 0, phibar / np.where(beta1 > 0, beta1, 1.0), 0.0)
+        rr = np.where(bnorm > 0, phibar / bden, 0.0)
         relres = np.where(active, rr, relres)
         active = active & (rr > tol)
     return x, {'iters': iters.tolist(), 'relres': relres.tolist()}

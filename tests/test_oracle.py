@@ -314,3 +314,59 @@ def test_koosh_ball_geometry():
     assert np.allclose(om[:, 1], [0, 0, -24])
     r = np.linalg.norm(om, axis=0)
     assert np.allclose(r.reshape(8, 5, order='F')[:, 0], np.abs(-32 + 8 * np.arange(8)))
+
+
+def test_minres_warm_start():
+    rng = np.random.default_rng(8)
+    n = 12
+    B = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A = B @ B.conj().T / n + 0.5 * np.eye(n)
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    xs = np.linalg.solve(A, b)
+    # the exact solution as initial guess: no iteration, returned unchanged
+    x, info = O.minres(lambda v: A @ v, b[:, None], tol=1e-8, maxit=5, x0=xs[:, None])
+    assert info['iters'] == [0] and np.array_equal(x[:, 0], xs)
+    # otherwise x0 + the Krylov least-squares minimiser over K_k(A, b - A x0)
+    x0 = rng.normal(size=n) + 1j * rng.normal(size=n)
+    r0 = b - A @ x0
+    for k in (1, 3, 6):
+        x, info = O.minres(lambda v: A @ v, b[:, None], tol=0.0, maxit=k, x0=x0[:, None])
+        want = x0 + _krylov_min(A, r0, k)
+        assert np.linalg.norm(x[:, 0] - want) < 1e-8 * np.linalg.norm(want)
+        # relres is measured against ||b|| (the MATLAB convention)
+        assert abs(info['relres'][0] - np.linalg.norm(b - A @ x[:, 0]) / np.linalg.norm(b)) < 1e-8
+
+
+def test_discrep_stddev_estimate_is_the_mad_of_nonzeros():
+    o, om = _small_plan(N=8, nspokes=30, nr=16)
+    rng = np.random.default_rng(9)
+    data = rng.normal(size=(o.M, 2)) + 1j * rng.normal(size=(o.M, 2))
+    data[::7] = 0
+    _, info = o.discrep(data, iters=1, return_info=True)
+    v = data[data != 0]
+    t = np.concatenate([v.real, v.imag])
+    assert info['stddev'] == pytest.approx(np.median(np.abs(t - np.median(t))), rel=1e-15)
+    assert info['discrepancy'] == pytest.approx(info['stddev'] * np.sqrt(data.size), rel=1e-15)
+
+
+def test_discrep_matches_a_dense_restatement():
+    # the same fixed point damp <- damp * discrepancy / ||A x(damp) - data||
+    # with x(damp) from a dense solve of (A'WA + damp^2 I) x = A'W data
+    o, om = _small_plan(N=6, nspokes=24, nr=12)
+    rng = np.random.default_rng(10)
+    img = synth.phantom_complex(6)
+    data = o.fNUFT(img) + 0.05 * (rng.normal(size=(o.M, 1)) + 1j * rng.normal(size=(o.M, 1)))
+    n = 6 ** 3
+    E = np.eye(n, dtype=np.complex128)
+    Acols = np.stack([o.fNUFT(E[:, j].reshape(6, 6, 6, order='F'))[:, 0] for j in range(n)], axis=1)
+    W = o.d
+    AWA = Acols.conj().T @ (W[:, None] * Acols)
+    b = Acols.conj().T @ (W * data[:, 0])
+    damp = 0.1
+    stddev = 0.05
+    disc = stddev * np.sqrt(data.size)
+    for _ in range(8):
+        x = np.linalg.solve(AWA + damp ** 2 * np.eye(n), b)
+        damp = damp * disc / np.linalg.norm(Acols @ x - data[:, 0])
+    got = o.discrep(data, stddev=stddev, iters=8, maxit=4 * n)
+    assert got == pytest.approx(damp, rel=2e-3)

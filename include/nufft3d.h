@@ -94,6 +94,18 @@ int nufft3d_inverse(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t max
                     double damp, const double* W, int64_t nW, double tol, void* im,
                     nufft3d_info* info);
 
+/* damp = obj.discrep(data,stddev) (discrep.m:2-69): discrepancy-principle
+ * estimate of the iNUFT damping. data [M nc]; stddev < 0 = not supplied:
+ * guessed as median(|t - median(t)|) over t = [real; imag] of the nonzero
+ * entries (discrep.m:29-33; an error if every entry is zero). Starting from
+ * damp = 0.1, `iters` rounds (<= 0: the reference's 20) of MINRES on
+ * A'dA + damp^2 I (tol 1e-4, maxit min(prod(N),20), warm-started from the
+ * previous round) each followed by damp *= stddev*sqrt(M*nc)/||fNUFT(x)-data||.
+ * stddev_out (may be NULL) receives the stddev used, history (may be NULL,
+ * else [iters]) damp after each round. */
+int nufft3d_discrep(nufft3d_plan* plan, const void* data, int32_t nc, double stddev,
+                    int32_t iters, double* damp, double* stddev_out, double* history);
+
 /* y = iprojection(x,damp,W) with W = density (nufft_3d.m:305-313):
  * A'WA x + damp^2 x on [N1 N2 N3 nc]. */
 int nufft3d_normal(nufft3d_plan* plan, const void* x, int32_t nc, double damp, void* y);

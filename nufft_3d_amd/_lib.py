@@ -49,6 +49,8 @@ SIGNATURES = {
                                   C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
                                   C.POINTER(Info)]),
     "nufft3d_normal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
+    "nufft3d_discrep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),
     "nufft3d_density": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nufft3d_deap": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nufft3d_nnz": (C.c_int64, [C.c_void_p]),

@@ -161,6 +161,7 @@ struct nufft3d_plan {
 
   // scratch, grown on demand
   nufft3d::Workspace ws_grid, ws_t1, ws_t2, ws_samp, ws_part, ws_img, ws_host, ws_solve, ws_red;
+  nufft3d::Workspace ws_b;  // iNUFT right-hand side A'W raw
   std::mutex mu;
 };
 
@@ -195,7 +196,8 @@ int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out);       // unsh
 
 // solve.hip
 int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
-                 const void* wsamp, double tol, void* x, nufft3d_info* info);
+                 const void* wsamp, double tol, void* x, nufft3d_info* info, bool warm = false);
+int diff_norm2(nufft3d_plan* p, const void* a, const void* c, int64_t n, double* out);
 int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void* wsamp,
                  void* y);
 int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out);

@@ -26,6 +26,10 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+// Grow a workspace to at least `bytes`. hipFree waits for the whole device,
+// stalling every other stream (the serving engine runs several), so growth
+// over-allocates by a quarter: a stream of slightly larger batches does not
+// free and reallocate each time.
 int ensure(Workspace& w, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (w.bytes >= bytes) return NUFFT3D_OK;
@@ -33,6 +37,7 @@ int ensure(Workspace& w, size_t bytes) {
     if (hipFree(w.p) != hipSuccess) { (void)hipGetLastError(); }
     w.p = nullptr;
     w.bytes = 0;
+    bytes += bytes / 4;
   }
   if (hipMalloc(&w.p, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -785,6 +790,7 @@ static void destroy_plan(nufft3d_plan* p) {
   for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
   release(p->ws_grid); release(p->ws_t1); release(p->ws_t2); release(p->ws_samp);
   release(p->ws_part); release(p->ws_img); release(p->ws_solve); release(p->ws_red);
+  release(p->ws_b);
   release(p->ws_host, true);
   if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
@@ -813,6 +819,108 @@ __global__ void k_fill_pattern(R* __restrict__ a, long long n, unsigned seed) {
   h *= 2246822519u;
   h ^= h >> 13;
   a[i] = (R)((double)(h & 0xffffff) / 16777216.0 - 0.5);
+}
+
+// ---------------------------------------------------------------- discrep
+// Sort keys of the stddev guess of discrep.m:30-32: the real and imaginary
+// parts of the nonzero entries; both parts of a zero entry become +inf, which
+// sorts after every finite key. cnt counts the nonzero entries.
+template <typename R>
+__global__ void k_mad_keys(const cplx<R>* __restrict__ d, long long n, R* __restrict__ keys,
+                           unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long c = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cplx<R> v = d[i];
+    const bool nz = v.x != (R)0 || v.y != (R)0;
+    keys[2 * i] = nz ? v.x : (R)INFINITY;
+    keys[2 * i + 1] = nz ? v.y : (R)INFINITY;
+    c += nz;
+  }
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sh[0]) atomicAdd(cnt, sh[0]);
+}
+
+// keys <- |keys - med| for the first m (finite) sorted keys, +inf after them
+template <typename R>
+__global__ void k_mad_dev(const R* __restrict__ sorted, long long n, long long m, R med,
+                          R* __restrict__ keys) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    keys[i] = i < m ? (R)fabs(sorted[i] - med) : (R)INFINITY;
+}
+
+// median of the first m entries of a sorted device array (MATLAB median:
+// mean of the two middle entries when m is even)
+template <typename R>
+static int sorted_median(nufft3d_plan* p, const R* sorted, long long m, double* med) {
+  R v[2];
+  const long long lo = (m - 1) / 2;
+  N3D_HIP(hipMemcpyAsync(v, sorted + lo, 2 * sizeof(R), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  *med = (m % 2) ? (double)v[0] : ((double)v[0] + (double)v[1]) / 2.0;
+  return NUFFT3D_OK;
+}
+
+// stddev = median(|t - median(t)|), t = [real(nonzeros(data)); imag(...)]
+// (discrep.m:30-32), with two device radix sorts of the keys.
+template <typename R>
+static int mad_stddev(nufft3d_plan* p, const void* data, long long n, double* out) {
+  Workspace a, b, tmp;
+  const long long nk = 2 * n;
+  int st = ensure(a, (size_t)nk * sizeof(R) + 64);
+  if (!st) st = ensure(b, (size_t)nk * sizeof(R));
+  R* keys = (R*)a.p;
+  unsigned long long* cnt = (unsigned long long*)((char*)a.p + (size_t)nk * sizeof(R));
+  R* sorted = (R*)b.p;
+  size_t tb = 0;
+  if (!st && nk > 0x7fffffffLL) {
+    set_error("discrep: stddev estimate over more than 2^31-1 values is not supported");
+    st = NUFFT3D_ERR_UNSUPPORTED;
+  }
+  if (!st && hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys, sorted, (int)nk, 0,
+                                               (int)(8 * sizeof(R)), p->stream) != hipSuccess) {
+    set_error("discrep: radix sort set-up failed");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  if (!st) st = ensure(tmp, tb);
+  unsigned long long m2 = 0;
+  if (!st) {
+    const unsigned nbk = std::min<unsigned>(nblocks(n), 4096);
+    N3D_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), p->stream));
+    hipLaunchKernelGGL((k_mad_keys<R>), dim3(nbk), dim3(256), 0, p->stream,
+                       (const cplx<R>*)data, n, keys, cnt);
+    N3D_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys, sorted, (int)nk, 0,
+                                              (int)(8 * sizeof(R)), p->stream));
+    N3D_HIP(hipMemcpyAsync(&m2, cnt, sizeof(m2), hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+    if (m2 == 0) {
+      set_error("discrep: stddev not supplied and data has no nonzero entries");
+      st = NUFFT3D_ERR_INVALID;
+    }
+  }
+  double med = 0, mad = 0;
+  const long long m = 2 * (long long)m2;
+  if (!st) st = sorted_median<R>(p, sorted, m, &med);
+  if (!st) {
+    const unsigned nbk = std::min<unsigned>(nblocks(nk), 4096);
+    hipLaunchKernelGGL((k_mad_dev<R>), dim3(nbk), dim3(256), 0, p->stream, (const R*)sorted, nk, m,
+                       (R)med, keys);
+    N3D_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys, sorted, (int)nk, 0,
+                                              (int)(8 * sizeof(R)), p->stream));
+    st = sorted_median<R>(p, sorted, m, &mad);
+  }
+  release(a);
+  release(b);
+  release(tmp);
+  *out = mad;
+  return st;
 }
 
 extern "C" {
@@ -973,7 +1081,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
     if (p->axN[a].d_tw) b += 2 * rsz * p->axN[a].L;
   }
   const Workspace* ws[] = {&p->ws_grid, &p->ws_t1, &p->ws_t2, &p->ws_samp, &p->ws_part,
-                           &p->ws_img, &p->ws_solve, &p->ws_red};
+                           &p->ws_img, &p->ws_solve, &p->ws_red, &p->ws_b};
   int64_t w = 0;
   for (const Workspace* x : ws) w += (int64_t)x->bytes;
   if (plan_bytes) *plan_bytes = b;
@@ -990,7 +1098,7 @@ int nufft3d_release_workspaces(nufft3d_plan* p) {
     return NUFFT3D_ERR_DEVICE;
   }
   Workspace* ws[] = {&p->ws_grid, &p->ws_t1, &p->ws_t2, &p->ws_samp, &p->ws_part,
-                     &p->ws_img, &p->ws_solve, &p->ws_red};
+                     &p->ws_img, &p->ws_solve, &p->ws_red, &p->ws_b};
   for (Workspace* x : ws) release(*x);
   return NUFFT3D_OK;
 }
@@ -1257,7 +1365,7 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
   N3D_TRY(make_weights(p, W, nW, &wsamp));
-  Workspace tmp_out, tmp_b, tmp_io;
+  Workspace tmp_out, tmp_io;
   Staged ri, xo;
   int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
   if (!st) st = stage_out_c(p, im, nimg, tmp_out, xo);
@@ -1269,8 +1377,8 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr);
   void* bdst = xo.dev;
   if (!st && maxit > 0) {
-    st = ensure(tmp_b, nimg * esz);
-    bdst = tmp_b.p;
+    st = ensure(p->ws_b, nimg * esz);    // kept on the plan: no free per call
+    bdst = p->ws_b.p;
   }
   if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, bdst, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
   if (!st && maxit > 0)                                                        // iNUFT.m:127-136
@@ -1286,10 +1394,69 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
     st = NUFFT3D_ERR_DEVICE;
   }
   release(tmp_out);
-  release(tmp_b);
   release(tmp_io);
   if (wsamp != p->d_dens) (void)hipFree(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+// discrep.m:2-69 (helpers above the C ABI block)
+int nufft3d_discrep(nufft3d_plan* p, const void* data, int32_t nc, double stddev, int32_t iters,
+                    double* damp_out, double* stddev_out, double* history) {
+  N3D_PLAN_GUARD(p);
+  if (!data || !damp_out || nc <= 0) { set_error("invalid data/damp/nc"); return NUFFT3D_ERR_INVALID; }
+  if (!std::isfinite(stddev)) {                                                // discrep.m:28
+    set_error("Expected input stddev to be finite.");
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (iters <= 0) iters = 20;                                                  // discrep.m:54
+  const size_t esz = cp_esz(p);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  const int maxit = (int)std::min<long long>(20, (long long)p->N[0] * p->N[1] * p->N[2]);
+  Workspace tmp_io, xw, kw;
+  Staged di;
+  int st = stage_in_c(p, data, nk, p->ws_img, tmp_io, di);
+  if (!st && stddev < 0)                                                       // discrep.m:29-33
+    st = p->prec == 2 ? mad_stddev<double>(p, di.dev, (long long)nk, &stddev)
+                      : mad_stddev<float>(p, di.dev, (long long)nk, &stddev);
+  const double discrepancy = stddev * std::sqrt((double)nk);                   // discrep.m:37
+  const double sc = 1.0 / std::sqrt((double)p->nnz);
+  if (!st) st = ensure(p->ws_grid, ngrid * esz);
+  if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
+  if (!st) st = ensure(p->ws_b, nimg * esz);
+  if (!st) st = ensure(xw, nimg * esz);
+  if (!st) st = ensure(kw, nk * esz);
+  // b = aNUFT(d.*data) (discrep.m:40-46)
+  if (!st) st = permute_in(p, di.dev, nc, p->ws_samp.p, p->d_dens);
+  if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr);
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, p->ws_b.p, sc, nullptr, 0.0);
+  double damp = 0.1;                                                           // discrep.m:50
+  for (int it = 0; it < iters && !st; ++it) {                                  // discrep.m:54-68
+    st = minres_solve(p, p->ws_b.p, nc, maxit, damp, p->d_dens, 1e-4, xw.p, nullptr, it > 0);
+    if (!st) st = fft_forward(p, xw.p, nc, p->ws_grid.p);
+    if (!st) st = interp(p, p->ws_grid.p, nc, kw.p, 1, nullptr, sc);
+    if (!st) st = zero_oob(p, kw.p, nc);
+    double r2 = 0;
+    if (!st) st = diff_norm2(p, kw.p, di.dev, (int64_t)nk, &r2);
+    if (!st) {
+      damp = damp * (discrepancy / std::sqrt(r2));
+      if (history) history[it] = damp;
+    }
+  }
+  if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error in discrep");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  release(xw);
+  release(kw);
+  release(tmp_io);
+  if (!st) {
+    *damp_out = damp;
+    if (stddev_out) *stddev_out = stddev;
+  }
   return st;
 }
 

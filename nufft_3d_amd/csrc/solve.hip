@@ -22,6 +22,7 @@ namespace nufft3d {
 
 struct MrState {
   double beta1, beta, oldb, alfa, dbar, epsln, oldeps, phibar, cs, sn, gamma, delta, phi, relres;
+  double bnorm;   // ||b||: relres and the stopping test are relative to it
   int active, upd, iters, pad;
 };
 
@@ -83,30 +84,70 @@ __global__ void __launch_bounds__(kRedThreads) k_sum_real_partial(const R* __res
 }
 
 // ---------------------------------------------------------------- MINRES
+// r0 = b - A x0 (warm start) or b; beta1 = ||r0||. A column starts active
+// unless ||r0|| <= tol ||b|| already; b = 0 gives x = 0 and no iteration
+// (MATLAB minres returns the zero vector for a zero right-hand side).
 template <typename R>
-__global__ void k_mr_init(const cplx<R>* __restrict__ b, long long n, const double* __restrict__ bn2,
-                          MrState* __restrict__ st, cplx<R>* __restrict__ x, cplx<R>* __restrict__ r2,
-                          cplx<R>* __restrict__ v) {
+__global__ void k_mr_init(const cplx<R>* __restrict__ r0, long long n, const double* __restrict__ rn2,
+                          const double* __restrict__ bn2, MrState* __restrict__ st,
+                          cplx<R>* __restrict__ x, cplx<R>* __restrict__ r2, cplx<R>* __restrict__ v,
+                          double tol, int warm) {
   const int c = blockIdx.y;
-  const double beta1 = sqrt(bn2[c]);
+  const double beta1 = sqrt(rn2[c]), bnorm = sqrt(bn2[c]);
+  const bool zero_x = !warm || !(bnorm > 0);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MrState s = {};
     s.beta1 = beta1; s.beta = beta1; s.oldb = 0; s.dbar = 0; s.epsln = 0;
-    s.phibar = beta1; s.cs = -1; s.sn = 0; s.relres = beta1 > 0 ? 1.0 : 0.0;
-    s.active = beta1 > 0; s.upd = 0; s.iters = 0;
+    s.phibar = beta1; s.cs = -1; s.sn = 0; s.bnorm = bnorm;
+    s.relres = bnorm > 0 ? beta1 / bnorm : 0.0;
+    s.active = bnorm > 0 && beta1 > 0 && beta1 / bnorm > tol;
+    s.upd = 0; s.iters = 0;
     st[c] = s;
   }
   const R inv = beta1 > 0 ? (R)(1.0 / beta1) : (R)0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const long long o = (long long)c * n + i;
-    const cplx<R> bv = b[o];
-    cplx<R> z; z.x = 0; z.y = 0;
-    x[o] = z;
+    const cplx<R> bv = r0[o];
+    if (zero_x) {
+      cplx<R> z; z.x = 0; z.y = 0;
+      x[o] = z;
+    }
     r2[o] = bv;
     cplx<R> vv; vv.x = bv.x * inv; vv.y = bv.y * inv;
     v[o] = vv;
   }
+}
+
+// y = b - y over all columns
+template <typename R>
+__global__ void k_sub_from(const cplx<R>* __restrict__ b, cplx<R>* __restrict__ y, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cplx<R> bv = b[i];
+    cplx<R> yv = y[i];
+    yv.x = bv.x - yv.x;
+    yv.y = bv.y - yv.y;
+    y[i] = yv;
+  }
+}
+
+// partial[b] = sum over block b of |a - c|^2
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_diff_norm_partial(const cplx<R>* __restrict__ a,
+                                                                   const cplx<R>* __restrict__ c,
+                                                                   long long n,
+                                                                   double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const cplx<R> x = a[i], y = c[i];
+    const double dx = (double)x.x - (double)y.x, dy = (double)x.y - (double)y.y;
+    acc += dx * dx + dy * dy;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // y -= (beta/oldb) r1 (itn>=2); partial <v, y>
@@ -191,7 +232,7 @@ __global__ void k_mr_scalarB(MrState* __restrict__ st, const double* __restrict_
   s.phi = s.cs * s.phibar;
   s.phibar = s.sn * s.phibar;
   s.iters += 1;
-  s.relres = s.beta1 > 0 ? s.phibar / s.beta1 : 0.0;
+  s.relres = s.bnorm > 0 ? s.phibar / s.bnorm : 0.0;
   s.active = s.relres > tol;
   st[c] = s;
 }
@@ -250,6 +291,23 @@ int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out) {
   return NUFFT3D_OK;
 }
 
+// ||a - c||^2 over n complex entries, accumulated in double in a fixed order
+int diff_norm2(nufft3d_plan* p, const void* a, const void* c, int64_t n, double* out) {
+  const int nb = red_blocks(n);
+  N3D_TRY(ensure(p->ws_red, (size_t)(nb + 8) * sizeof(double)));
+  double* part = (double*)p->ws_red.p;
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_diff_norm_partial<double>), dim3(nb), dim3(kRedThreads), 0, p->stream,
+                       (const cplx<double>*)a, (const cplx<double>*)c, (long long)n, part);
+  else
+    hipLaunchKernelGGL((k_diff_norm_partial<float>), dim3(nb), dim3(kRedThreads), 0, p->stream,
+                       (const cplx<float>*)a, (const cplx<float>*)c, (long long)n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, p->stream, part, nb, part + nb);
+  N3D_HIP(hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  return NUFFT3D_OK;
+}
+
 // y = A'WA x + damp^2 x (iprojection with coils independent)
 int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void* wsamp, void* y) {
   const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
@@ -267,6 +325,8 @@ int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void
 // MINRES on nc right-hand sides of length n stored [nc][n]; op(v, y) applies
 // the Hermitian operator to all right-hand sides at once. `opkey` identifies
 // the operator (its scalars and extra arrays) for the cached iteration graph.
+// With `warm` the solve starts from the x0 held in xout (MATLAB's
+// minres(A,b,tol,maxit,[],[],x0)): the recurrence runs on r0 = b - A x0.
 //
 // Iteration 1 runs eagerly (it also grows every workspace the operator
 // uses). After it, blocks of kGraphBlock iterations - the period of the
@@ -283,7 +343,7 @@ template <typename R>
 static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int maxit, double tol,
                        void* xout, nufft3d_info* info,
                        const std::function<int(const void*, void*)>& op,
-                       const std::vector<uint64_t>& opkey) {
+                       const std::vector<uint64_t>& opkey, bool warm = false) {
   using C = cplx<R>;
   const size_t vec = (size_t)n * nc * sizeof(C);
   // vectors: v, r1, r2, y, w, w2 ; x is the caller's output
@@ -297,17 +357,30 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
   C* w2 = (C*)(base + 5 * vec);
   C* x = (C*)xout;
   const int nb = red_blocks(n);
-  N3D_TRY(ensure(p->ws_red, (size_t)nc * (nb + 1) * sizeof(double) + nc * sizeof(MrState) + 64));
+  N3D_TRY(ensure(p->ws_red, (size_t)nc * (nb + 2) * sizeof(double) + nc * sizeof(MrState) + 64));
   double* part = (double*)p->ws_red.p;
   double* sums = part + (size_t)nc * nb;
-  MrState* st = (MrState*)(sums + nc);
+  double* bn2 = sums + nc;
+  MrState* st = (MrState*)(bn2 + nc);
   const dim3 grid(nb, nc);
 
   hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b,
                      (const C*)b, n, part);
-  hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
-  hipLaunchKernelGGL((k_mr_init<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b, n, sums,
-                     st, x, r2, v);
+  hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, bn2);
+  const C* r0 = (const C*)b;
+  const double* rn2 = bn2;
+  if (warm) {   // r0 = b - A x0, held in y until iteration 1 overwrites it
+    N3D_TRY(op(x, y));
+    hipLaunchKernelGGL((k_sub_from<R>), dim3(red_blocks(n * nc)), dim3(kRedThreads), 0, p->stream,
+                       (const C*)b, y, n * nc);
+    hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)y,
+                       (const C*)y, n, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    r0 = y;
+    rn2 = sums;
+  }
+  hipLaunchKernelGGL((k_mr_init<R>), grid, dim3(kRedThreads), 0, p->stream, r0, n, rn2,
+                     (const double*)bn2, st, x, r2, v, tol, warm ? 1 : 0);
   N3D_HIP(hipMemsetAsync(w, 0, vec, p->stream));
   N3D_HIP(hipMemsetAsync(w2, 0, vec, p->stream));
 
@@ -347,7 +420,12 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
   };
 
   int itn = 1;
-  if (maxit >= 1) N3D_TRY(iterate(itn++));
+  if (maxit >= 1) {
+    bool any = true;
+    if (warm) N3D_TRY(any_active(any));   // x0 may already meet the test
+    if (any) N3D_TRY(iterate(itn++));
+    else maxit = 0;
+  }
   // the graph's identity: every pointer and scalar its kernels captured
   std::vector<uint64_t> key(opkey);
   const void* ptrs[] = {b, xout, p->ws_solve.p, p->ws_red.p, p->ws_grid.p, p->ws_samp.p,
@@ -417,12 +495,12 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
 
 template <typename R>
 static int minres_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
-                    const void* wsamp, double tol, void* xout, nufft3d_info* info) {
+                    const void* wsamp, double tol, void* xout, nufft3d_info* info, bool warm) {
   const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
   const std::vector<uint64_t> key = {1, __builtin_bit_cast(uint64_t, damp), (uint64_t)(uintptr_t)wsamp};
   return minres_core<R>(p, b, n, nc, maxit, tol, xout, info, [&](const void* v, void* y) {
     return apply_normal(p, v, nc, damp, wsamp, y);
-  }, key);
+  }, key, warm);
 }
 
 // Coils coupled through the PRUNO null term: one right-hand side of length
@@ -447,9 +525,9 @@ int minres_coupled(nufft3d_plan* p, const void* b, int nc, int maxit, double dam
 }
 
 int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, const void* wsamp,
-                 double tol, void* x, nufft3d_info* info) {
-  return p->prec == 2 ? minres_t<double>(p, b, nc, maxit, damp, wsamp, tol, x, info)
-                      : minres_t<float>(p, b, nc, maxit, damp, wsamp, tol, x, info);
+                 double tol, void* x, nufft3d_info* info, bool warm) {
+  return p->prec == 2 ? minres_t<double>(p, b, nc, maxit, damp, wsamp, tol, x, info, warm)
+                      : minres_t<float>(p, b, nc, maxit, damp, wsamp, tol, x, info, warm);
 }
 
 }  // namespace nufft3d

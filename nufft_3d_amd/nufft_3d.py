@@ -278,6 +278,43 @@ class nufft_3d:
                        "seconds": info.seconds}
         return x
 
+    def discrep(self, data, stddev=None, iters=20, return_info=False):
+        """damp by the discrepancy principle (discrep.m:2-69).
+
+        data is [M nc] or [nr ny nc] with nr*ny = M. Without stddev it is
+        guessed from the data (median absolute deviation of the real and
+        imaginary parts of the nonzero entries) with the reference's warning.
+        """
+        if stddev is not None:
+            if not (np.isscalar(stddev) and np.isfinite(stddev) and stddev >= 0):
+                raise ValueError("stddev must be a finite nonnegative scalar")   # discrep.m:28
+        iters = int(iters)
+        hist = np.zeros(max(iters, 1), dtype=np.float64)
+        damp, sd = C.c_double(), C.c_double()
+        sarg = -1.0 if stddev is None else float(stddev)
+        if _is_torch(data):
+            dt = self._torch_in(data, self.M, "data")
+            nc = dt.numel() // self.M
+            self._bind_stream(dt)
+            _lib.check(self._lib.nufft3d_discrep(self._h, C.c_void_p(dt.data_ptr()), nc, sarg,
+                                                 iters, C.byref(damp), C.byref(sd),
+                                                 hist.ctypes.data))
+        else:
+            da = np.asarray(data)
+            if da.shape[0] != self.M and (da.ndim < 2 or da.shape[0] * da.shape[1] != self.M):
+                raise ValueError("data leading dimension(s) must be length %d" % self.M)
+            if da.size % self.M:
+                raise ValueError("data size %s seems to have too many dimensions" % (da.shape,))
+            nc = da.size // self.M
+            db = np.asfortranarray(np.reshape(da, (self.M, nc), order="F"), dtype=self.cdtype)
+            _lib.check(self._lib.nufft3d_discrep(self._h, db.ctypes.data, nc, sarg, iters,
+                                                 C.byref(damp), C.byref(sd), hist.ctypes.data))
+        if stddev is None:
+            warnings.warn("stddev not supplied. Guessing %.2e." % sd.value)     # discrep.m:33
+        if return_info:
+            return damp.value, {"stddev": sd.value, "damp_history": hist[:iters].tolist()}
+        return damp.value
+
     # ------------------------------------------------------------ SAKE/PRUNO
     def _popts(self, opts):
         o = _lib.PrunoOpts()

@@ -50,6 +50,7 @@ def algorithmic_bytes(N, K, M, nc):
     return {
         "k_interp": nc * (8 * k3 + 8 * M) + 12 * M,
         "k_spread": nc * (8 * M + 8 * k3) + 12 * M,
+        "k_spread_tile": nc * (8 * M + 8 * k3) + 12 * M,
         "fft_fwd": nc * (8 * n3 + 8 * k3) + 4 * n3,
         "fft_adj": nc * (8 * k3 + 8 * n3) + 4 * n3,
         "pair": nc * (16 * n3 + 32 * k3 + 16 * M) + 24 * M + 8 * n3,
@@ -70,7 +71,10 @@ def measured_traffic(workload, kernel, nc):
             tab = json.load(f)["kernels"]
     except (OSError, ValueError, KeyError):
         return None
-    full = "<float, 4, 4," if nc >= 16 else "<float,"
+    if kernel == "k_spread_tile":
+        full = "<16," if nc >= 16 else "<"
+    else:
+        full = "<float, 4, 4," if nc >= 16 else "<float,"
     for name, v in tab.items():
         if name.startswith(kernel + "<") and full in name:
             return round(v["read_bytes"] + v["write_bytes"])

@@ -225,29 +225,43 @@ __device__ __forceinline__ void fft_stage(cplx<R>* buf, int L, int B, int Ns,
   }
 }
 
-// Mixed-radix Stockham FFT of B lines of length L (L*B <= blockDim.x*MaxE).
-// FG: the first stage reads ld(line, pos); otherwise the input is in buf,
-// written and synchronised by the caller. LG: the last stage writes
-// sto(line, pos, v); otherwise the output is left in buf (the caller
-// synchronises before reading it). The LDS twiddles tw must be visible.
+// Mixed-radix Stockham FFT of B lines of length L (L*B <= blockDim.x*MaxE)
+// for any stage sequence factorize() emits: 16s, one 8/4/2 (or 8 then 4),
+// 3s, 5s, 7s. FG: the lines are first copied from ld(line, pos) into buf;
+// otherwise the input is in buf, written and synchronised by the caller.
+// LG: the result is finally copied out through sto(line, pos, v); otherwise
+// it is left in buf (the caller synchronises before reading it). The LDS
+// twiddles tw must be visible. Every stage works LDS to LDS in one loop per
+// radix, so no code region holds more than one radix's registers (a single
+// loop over a switch of radices, or global loads fused into the first and
+// last stages, make the register allocation cover all radices at once).
 template <typename R, bool FG, bool LG, typename LD, typename ST>
 __device__ void fft_lines(cplx<R>* buf, int B, const Stages& st, const cplx<R>* __restrict__ tw,
                           int sign, const LD& ld, const ST& sto) {
   const int L = st.L;
-  int Ns = 1;
-  for (int k = 0; k < st.n; ++k) {
-    const int rad = st.r[k];
-    const bool first = k == 0, last = k == st.n - 1;
-    switch (rad) {
-      case 16: fft_stage<R, 16, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      case 8: fft_stage<R, 8, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      case 4: fft_stage<R, 4, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      case 2: fft_stage<R, 2, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      case 3: fft_stage<R, 3, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      case 5: fft_stage<R, 5, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-      default: fft_stage<R, 7, FG, LG>(buf, L, B, Ns, tw, sign, first, last, ld, sto); break;
-    }
-    Ns *= rad;
+  const int n = st.n;
+  const int total = L * B;
+  if constexpr (FG) {
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) buf[idx] = ld(idx % B, idx / B);
+  }
+  int Ns = 1, k = 0;
+#define N3D_RUN(RAD, LOOP)                                                           \
+  LOOP (k < n && st.r[k] == RAD) {                                                 \
+    fft_stage<R, RAD, false, false>(buf, L, B, Ns, tw, sign, false, false, ld, sto); \
+    Ns *= RAD;                                                                      \
+    ++k;                                                                            \
+  }
+  N3D_RUN(16, while)
+  N3D_RUN(8, if)
+  N3D_RUN(4, if)
+  N3D_RUN(2, if)
+  N3D_RUN(3, while)
+  N3D_RUN(5, while)
+  N3D_RUN(7, while)
+#undef N3D_RUN
+  if constexpr (LG) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) sto(idx % B, idx / B, buf[idx]);
   }
 }
 

@@ -355,6 +355,106 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   }
 }
 
+// Adjoint, output driven with the samples staged in LDS (fp32 arithmetic,
+// CG coils per launch in the aligned layout). One 512-lane work-group per
+// 8x8x8 brick, one lane per grid point. The samples that touch the brick's
+// points (its BrickLists list, ascending ids) pass through LDS in chunks of
+// S: each chunk's coil rows and offsets are read once with coalesced 16-byte
+// loads, then every lane walks its point's hits whose slot falls in the
+// chunk. A point's hits are in ascending sample order, so it meets them
+// chunk by chunk in list order and sums them in the same order as k_spread:
+// the two kernels give bitwise the same grid. Points with more than rmax
+// hits are left to k_spread_heavy.
+template <int CG> struct TileSlots { static constexpr int S = CG == 16 ? 384 : 512; };
+
+template <int CG, bool COMP>
+__global__ void __launch_bounds__(512) k_spread_tile(const long long* __restrict__ rowptr,
+                                                     const unsigned* __restrict__ ent,
+                                                     const long long* __restrict__ bl_ptr,
+                                                     const int* __restrict__ bl_samp,
+                                                     const float4* __restrict__ off32,
+                                                     const float2* __restrict__ ks,
+                                                     float2* __restrict__ grid,
+                                                     const int* __restrict__ bricks, int rmax,
+                                                     GridGeom g, KBParams kp) {
+  using A = typename AccT<float, COMP>::type;
+  constexpr int S = TileSlots<CG>::S;
+  constexpr int V4 = CG / 2;        // 16-byte pieces of a staged row
+  constexpr int ROW4 = V4 + 1;      // row stride with 16 bytes of padding (LDS banks)
+  __shared__ float4 s_val[S * ROW4];
+  __shared__ float4 s_off[S];
+  const long long bi = xcd_block(g.xchunk);
+  const int t = bricks[bi];
+  const int l = threadIdx.x;
+  const int x = (t & 1023) * 8 + (l & 7), y = ((t >> 10) & 1023) * 8 + ((l >> 3) & 7),
+            z = (t >> 20) * 8 + (l >> 6);
+  const bool inside = x < g.K1 && y < g.K2 && z < g.K3;
+  const long long gpt = ((long long)z * g.K2 + y) * g.K1 + x;
+  long long e = 0, e1 = 0;
+  bool store = false;
+  if (inside) {
+    e = rowptr[gpt];
+    e1 = rowptr[gpt + 1];
+    store = e1 - e <= rmax;
+    if (!store) e1 = e;
+  }
+  A ar[CG], ai[CG];
+#pragma unroll
+  for (int c = 0; c < CG; ++c) { ar[c] = 0; ai[c] = 0; }
+  // entries are loaded four ahead; indices past the list are clamped to its
+  // last entry (or entry 0 for an empty list), so every load is in bounds
+  const long long ecl = e1 > e ? e1 - 1 : 0;
+  unsigned pf0 = ent[e < ecl ? e : ecl], pf1 = ent[e + 1 < ecl ? e + 1 : ecl];
+  unsigned pf2 = ent[e + 2 < ecl ? e + 2 : ecl], pf3 = ent[e + 3 < ecl ? e + 3 : ecl];
+  const long long base = bl_ptr[bi];
+  const int nsl = (int)(bl_ptr[bi + 1] - base);
+  const float4* ks4 = reinterpret_cast<const float4*>(ks);
+  const long long rs4 = g.nc / 2;
+  const int co4 = g.c0 / 2;
+  for (int c0 = 0; c0 < nsl; c0 += S) {
+    const int cn = nsl - c0 < S ? nsl - c0 : S;
+    __syncthreads();   // the previous chunk is consumed
+    for (int i = l; i < cn * ROW4; i += blockDim.x) {
+      const int r = i / ROW4, k = i - r * ROW4;
+      const int sm = bl_samp[base + c0 + r];
+      if (k < V4) s_val[r * ROW4 + k] = ks4[(long long)sm * rs4 + co4 + k];
+      else s_off[r] = off32[sm];
+    }
+    __syncthreads();
+    const int cend = c0 + cn;
+    while (e < e1) {
+      const unsigned cur = pf0;
+      const int sl = (int)(cur >> 15);
+      if (sl >= cend) break;
+      pf0 = pf1;
+      pf1 = pf2;
+      pf2 = pf3;
+      pf3 = ent[e + 4 < ecl ? e + 4 : ecl];
+      ++e;
+      int dx, dy, dz;
+      decode_hit((unsigned short)(cur & 0x7fffu), g, dx, dy, dz);
+      const int r = sl - c0;
+      const float w = hit_weight_f(dx, dy, dz, s_off[r], kp);
+      const float4* row = s_val + r * ROW4;
+#pragma unroll
+      for (int k = 0; k < V4; ++k) {
+        const float4 v = row[k];
+        ar[2 * k] += (A)w * (A)v.x;
+        ai[2 * k] += (A)w * (A)v.y;
+        ar[2 * k + 1] += (A)w * (A)v.z;
+        ai[2 * k + 1] += (A)w * (A)v.w;
+      }
+    }
+  }
+  if (store) {
+    float4* gp = reinterpret_cast<float4*>(grid + gpt * g.nc + g.c0);
+#pragma unroll
+    for (int k = 0; k < V4; ++k)
+      gp[k] = make_float4((float)ar[2 * k], (float)ai[2 * k], (float)ar[2 * k + 1],
+                          (float)ai[2 * k + 1]);
+  }
+}
+
 // One segment of a dense point's hit list -> partial[slot][cg]
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
@@ -507,7 +607,15 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   KBParams kp = make_kb(p);
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
   const DenseSet& ds = p->dense[cg <= 4 ? 0 : 1];
-  {
+  constexpr bool kTileable = sizeof(R) == 4 && FULL && LPS * CPL >= 8;
+  if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
+      p->bl.rmax == ds.rmax) {
+    ProfScope ps(p, "k_spread_tile");
+    hipLaunchKernelGGL((k_spread_tile<(kTileable ? CPL * LPS : 8), COMP>), dim3((unsigned)p->nbrick),
+                       dim3(512), 0, p->stream, p->d_rowptr, p->bl.d_ent, p->bl.d_ptr,
+                       p->bl.d_samp, p->d_off32, (const float2*)samp, (float2*)grid, p->d_brick,
+                       ds.rmax, g, kp);
+  } else {
     // brick order covers whole 8x8x8 bricks (points outside the grid exit)
     const long long npts = (p->tune_flags & 2) ? p->nbrick * 512 : ngrid;
     const long long nthreads = npts * LPS;

@@ -75,6 +75,21 @@ struct DenseSet {
   int nhpt = 0;
 };
 
+// LDS-staged spread of fp32-arithmetic 3-D plans: for each 8x8x8 brick of
+// the traversal order, the ascending ids of the samples that touch its
+// points with at most rmax hits, and for each hit of those points one word
+// (slot of the sample in its brick's list << 15 | candidate code). Built
+// for the large-coil-group dense set (its rmax); absent (d_ent null) when a
+// brick would need more than 2^17 slots or the plan has 2^31 hits or more.
+struct BrickLists {
+  int rmax = 0;
+  long long* d_ptr = nullptr;   // [nbrick + 1] offsets into d_samp
+  int* d_samp = nullptr;        // [total] sample ids, ascending per brick
+  unsigned* d_ent = nullptr;    // [nhits] packed slot/code (entries of dense points unused)
+  long long total = 0;
+  int max_len = 0;
+};
+
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
@@ -110,7 +125,8 @@ struct nufft3d_plan {
   bool own_stream = false;
   int verbose = 0;
   int compensated = 1;      // fp32 plans: products accumulated in double
-  int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order)
+  int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order,
+                            // 8 = LDS-staged spread where the plan has brick lists)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
@@ -148,6 +164,7 @@ struct nufft3d_plan {
   nufft3d::DenseSet dense[2];
   int* d_brick = nullptr;           // [nbrick] 8x8x8 spread bricks in traversal order (tx|ty<<10|tz<<20)
   long long nbrick = 0;
+  nufft3d::BrickLists bl;           // LDS-staged spread lists over the same bricks
   int max_hits = 0;
 
   nufft3d::FftAxis ax[3];

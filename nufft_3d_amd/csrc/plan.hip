@@ -353,6 +353,67 @@ __global__ void k_split_vals(const unsigned long long* __restrict__ vals, long l
   acode[i] = (unsigned short)(v & 0xffffu);
 }
 
+// brick lists (BrickLists): key (brick << 32 | sample) per hit of a point
+// with at most rmax hits, all ones for the hits of denser points
+__global__ void k_bl_keys(const long long* __restrict__ rowptr, const int* __restrict__ hit_s,
+                          long long ngrid, int K1, int K2, const int* __restrict__ brick_of,
+                          int bx, int by, int rmax, unsigned long long* __restrict__ keys) {
+  const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ngrid) return;
+  const long long e0 = rowptr[gi], e1 = rowptr[gi + 1];
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const unsigned long long b =
+      (unsigned long long)brick_of[((long long)(z >> 3) * by + (y >> 3)) * bx + (x >> 3)];
+  const bool dense = e1 - e0 > rmax;
+  for (long long e = e0; e < e1; ++e)
+    keys[e] = dense ? ~0ull : (b << 32) | (unsigned)hit_s[e];
+}
+
+__global__ void k_bl_samp(const unsigned long long* __restrict__ ukeys, long long n,
+                          int* __restrict__ samp) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) samp[i] = (int)(ukeys[i] & 0xffffffffull);
+}
+
+__device__ __forceinline__ long long lower_bound_u64(const unsigned long long* a, long long n,
+                                                     unsigned long long v) {
+  long long lo = 0, hi = n;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// ptr[b] = first unique key of brick b (b = 0..nbrick)
+__global__ void k_bl_ptr(const unsigned long long* __restrict__ ukeys, long long n, long long nbrick,
+                         long long* __restrict__ ptr) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbrick) return;
+  ptr[b] = lower_bound_u64(ukeys, n, (unsigned long long)b << 32);
+}
+
+// per hit of a light point: (slot of its sample in the brick list) << 15 | code
+__global__ void k_bl_ent(const long long* __restrict__ rowptr, const int* __restrict__ hit_s,
+                         const unsigned short* __restrict__ acode, long long ngrid, int K1, int K2,
+                         const int* __restrict__ brick_of, int bx, int by, int rmax,
+                         const unsigned long long* __restrict__ ukeys,
+                         const long long* __restrict__ ptr, unsigned* __restrict__ ent) {
+  const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ngrid) return;
+  const long long e0 = rowptr[gi], e1 = rowptr[gi + 1];
+  if (e1 - e0 > rmax) return;
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const int b = brick_of[((long long)(z >> 3) * by + (y >> 3)) * bx + (x >> 3)];
+  const long long base = ptr[b], len = ptr[b + 1] - base;
+  long long lo = 0;   // hits are in ascending sample order: search from the last slot
+  for (long long e = e0; e < e1; ++e) {
+    const unsigned long long key = ((unsigned long long)b << 32) | (unsigned)hit_s[e];
+    lo += lower_bound_u64(ukeys + base + lo, len - lo, key);
+    ent[e] = ((unsigned)lo << 15) | (unsigned)acode[e];
+  }
+}
+
 __global__ void k_off32(const double* __restrict__ off, long long nnz, float4* __restrict__ o32) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
@@ -518,6 +579,92 @@ static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long
     N3D_HIP(hipMemcpy(ds.d_hseg, hseg.data(), sizeof(HeavySeg) * hseg.size(), hipMemcpyHostToDevice));
     N3D_HIP(hipMemcpy(ds.d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
   }
+  return NUFFT3D_OK;
+}
+
+// BrickLists of the plan for points with at most rmax hits: sort the
+// (brick, sample) pairs of all their hits, keep the unique ones, and code
+// every hit with its sample's slot in the brick's list.
+static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, int bx, int by,
+                             int rmax) {
+  BrickLists& bl = p->bl;
+  const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
+  const long long nh = p->nhits;
+  if (nh <= 0 || nh >= 0x7fffffffLL || p->prec != 1 || p->K[2] == 1) return NUFFT3D_OK;
+  int bbits = 1;
+  while ((1LL << bbits) <= p->nbrick) ++bbits;
+  int* d_bof = nullptr;
+  unsigned long long *keys = nullptr, *skeys = nullptr;
+  int* d_nsel = nullptr;
+  void* tmp = nullptr;
+  auto cleanup = [&]() {
+    if (d_bof) (void)hipFree(d_bof);
+    if (keys) (void)hipFree(keys);
+    if (skeys) (void)hipFree(skeys);
+    if (d_nsel) (void)hipFree(d_nsel);
+    if (tmp) (void)hipFree(tmp);
+  };
+  auto fail = [&](const char* what) {
+    cleanup();
+    set_error(std::string("brick list construction: ") + what);
+    return NUFFT3D_ERR_NOMEM;
+  };
+  if (hipMalloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
+      hipMalloc(&keys, sizeof(unsigned long long) * nh) != hipSuccess ||
+      hipMalloc(&skeys, sizeof(unsigned long long) * nh) != hipSuccess ||
+      hipMalloc(&d_nsel, sizeof(int)) != hipSuccess)
+    return fail("allocation failed");
+  N3D_HIP(hipMemcpy(d_bof, brick_of.data(), sizeof(int) * brick_of.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_bl_keys, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
+                     p->d_hit_s, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax, keys);
+  size_t tb1 = 0, tb2 = 0;
+  N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nh, 0, 32 + bbits,
+                                            p->stream));
+  N3D_HIP(hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nh, p->stream));
+  if (hipMalloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
+  // the dense points' all-ones keys sort last within the sorted bits too
+  // (every brick index is below 2^bbits - 1)
+  N3D_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, (int)nh, 0, 32 + bbits,
+                                            p->stream));
+  N3D_HIP(hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, (int)nh, p->stream));
+  int nsel = 0;
+  N3D_HIP(hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  unsigned long long last = 0;
+  if (nsel > 0)
+    N3D_HIP(hipMemcpy(&last, keys + nsel - 1, sizeof(last), hipMemcpyDeviceToHost));
+  const long long total = (nsel > 0 && (last >> 32) >= (unsigned long long)p->nbrick) ? nsel - 1 : nsel;
+  std::vector<long long> hptr(p->nbrick + 1);
+  if (hipMalloc(&bl.d_ptr, sizeof(long long) * (p->nbrick + 1)) != hipSuccess ||
+      hipMalloc(&bl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess)
+    return fail("allocation failed");
+  hipLaunchKernelGGL(k_bl_ptr, dim3(nblocks(p->nbrick + 1)), dim3(256), 0, p->stream,
+                     (const unsigned long long*)keys, total, p->nbrick, bl.d_ptr);
+  if (total > 0)
+    hipLaunchKernelGGL(k_bl_samp, dim3(nblocks(total)), dim3(256), 0, p->stream,
+                       (const unsigned long long*)keys, total, bl.d_samp);
+  N3D_HIP(hipMemcpyAsync(hptr.data(), bl.d_ptr, sizeof(long long) * (p->nbrick + 1),
+                         hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  long long mx = 0;
+  for (long long b = 0; b < p->nbrick; ++b) mx = std::max(mx, hptr[b + 1] - hptr[b]);
+  if (mx >= (1LL << 17)) {   // slots would not fit the 17 bits of an entry
+    cleanup();
+    (void)hipFree(bl.d_ptr); (void)hipFree(bl.d_samp);
+    bl.d_ptr = nullptr; bl.d_samp = nullptr;
+    return NUFFT3D_OK;
+  }
+  if (hipMalloc(&bl.d_ent, sizeof(unsigned) * nh) != hipSuccess) return fail("allocation failed");
+  N3D_HIP(hipMemsetAsync(bl.d_ent, 0, sizeof(unsigned) * nh, p->stream));
+  hipLaunchKernelGGL(k_bl_ent, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
+                     p->d_hit_s, p->d_acode, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax,
+                     (const unsigned long long*)keys, (const long long*)bl.d_ptr, bl.d_ent);
+  N3D_HIP(hipGetLastError());
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  cleanup();
+  bl.rmax = rmax;
+  bl.total = total;
+  bl.max_len = (int)mx;
   return NUFFT3D_OK;
 }
 
@@ -748,11 +895,16 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
           order.push_back({key, x | (y << 10) | (z << 20)});
         }
     std::sort(order.begin(), order.end());
-    std::vector<int> tab(order.size());
-    for (size_t i = 0; i < order.size(); ++i) tab[i] = order[i].second;
+    std::vector<int> tab(order.size()), brick_of(order.size());
+    for (size_t i = 0; i < order.size(); ++i) {
+      tab[i] = order[i].second;
+      const int x = tab[i] & 1023, y = (tab[i] >> 10) & 1023, z = tab[i] >> 20;
+      brick_of[((size_t)z * by + y) * bx + x] = (int)i;
+    }
     p->nbrick = (long long)tab.size();
     N3D_HIP(hipMalloc(&p->d_brick, sizeof(int) * tab.size()));
     N3D_HIP(hipMemcpy(p->d_brick, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+    N3D_TRY(build_brick_lists(p, brick_of, bx, by, p->dense[1].rmax));
   }
 
   // deapodization (deap.m) and FFT twiddles
@@ -778,7 +930,7 @@ static void destroy_plan(nufft3d_plan* p) {
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit_s, p->d_acode,
                   p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
-                  p->dense[1].d_hpt, p->d_brick};
+                  p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int a = 0; a < 3; ++a) {
@@ -1050,6 +1202,9 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
            "segments %d\n", p->ntiles, p->T[0], p->T[1], p->T[2], (long long)p->nhits,
            p->nnz ? (double)p->nhits / p->nnz : 0.0, p->max_hits, p->dense[1].nhpt,
            p->dense[1].nhseg);
+    if (p->bl.d_ent)
+      printf("\tspread bricks %lld: %lld staged samples (%.2f/sample), longest list %d\n",
+             p->nbrick, p->bl.total, p->nnz ? (double)p->bl.total / p->nnz : 0.0, p->bl.max_len);
   }
   *out = p;
   return NUFFT3D_OK;
@@ -1076,6 +1231,8 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   for (const DenseSet& ds : p->dense)
     b += (int64_t)sizeof(HeavySeg) * std::max(1, ds.nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, ds.nhpt);
   b += 4 * p->nbrick;
+  if (p->bl.d_ptr) b += 8 * (p->nbrick + 1) + 4 * std::max<int64_t>(1, p->bl.total);
+  if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
   for (int a = 0; a < 3; ++a) {
     if (p->ax[a].d_tw) b += 2 * rsz * p->ax[a].L;
     if (p->axN[a].d_tw) b += 2 * rsz * p->axN[a].L;
@@ -1529,15 +1686,22 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     N3D_TRY(spread(p, ks.p, nc, p->ws_grid.p, 0, nullptr));
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
+  // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
+  // spread (plans with brick lists)
   struct Cand { int fftb, group, xcd, brick; float ms; };
   std::vector<Cand> cands;
   for (int fftb : {0, 8, 4})
     for (int group : {16, 8})
       for (int xcd : {0, 32})
-        for (int brick : {1, 0}) {
+        for (int brick : {1, 0, 2}) {
           if (group == 8 && nc <= 8) continue;   // same launches as 16
+          if (brick == 2 && !p->bl.d_ent) continue;
           cands.push_back({fftb, group, xcd, brick, 0.f});
         }
+  auto flags_for = [&](int brick, int keep) {
+    keep &= ~(2 | 8);
+    return brick == 0 ? keep : brick == 1 ? (keep | 2) : (keep | 2 | 8);
+  };
   const int keep_flags = p->tune_flags;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1551,7 +1715,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     p->fft_bundle = c.fftb;
     p->coil_group = c.group;
     p->xcd_chunk = c.xcd;
-    p->tune_flags = c.brick ? (keep_flags | 2) : (keep_flags & ~2);
+    p->tune_flags = flags_for(c.brick, keep_flags);
     st = pair();                                          // warm-up
     if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
     for (int r = 0; r < reps && !st; ++r) st = pair();
@@ -1568,17 +1732,19 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     p->fft_bundle = b.fftb;
     p->coil_group = b.group;
     p->xcd_chunk = b.xcd;
-    p->tune_flags = b.brick ? (keep_flags | 2) : (keep_flags & ~2);
+    p->tune_flags = flags_for(b.brick, keep_flags);
   } else {
     p->tune_flags = keep_flags;
   }
   if (!st && report) {
     std::string js = "{\"nc\": " + std::to_string(nc) + ", \"candidates\": [";
     for (size_t i = 0; i < cands.size(); ++i) {
-      char b[160];
+      static const char* spread_kind[3] = {"gather", "gather-bricks", "lds-bricks"};
+      char b[200];
       snprintf(b, sizeof b, "%s{\"fft_lines\": %d, \"coil_group\": %d, \"xcd_chunk\": %d, "
-               "\"brick_order\": %d, \"ms_per_pair\": %.5f}", i ? ", " : "", cands[i].fftb,
-               cands[i].group, cands[i].xcd, cands[i].brick, cands[i].ms);
+               "\"brick_order\": %d, \"spread\": \"%s\", \"ms_per_pair\": %.5f}", i ? ", " : "",
+               cands[i].fftb, cands[i].group, cands[i].xcd, cands[i].brick != 0,
+               spread_kind[cands[i].brick], cands[i].ms);
       js += b;
     }
     js += "], \"best\": " + std::to_string(best) + "}";

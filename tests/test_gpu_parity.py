@@ -275,10 +275,43 @@ def test_autotune_picks_a_measured_setting_and_keeps_results():
     before = p.aNUFT(p.fNUFT(x))
     rep = p.autotune(nc=12, reps=2)
     cands = rep["candidates"]
-    assert len(cands) == 24 and 0 <= rep["best"] < len(cands)
+    assert len(cands) == 36 and 0 <= rep["best"] < len(cands)
+    assert {c["spread"] for c in cands} == {"gather", "gather-bricks", "lds-bricks"}
     assert all(c["ms_per_pair"] > 0 for c in cands)
     assert cands[rep["best"]]["ms_per_pair"] == min(c["ms_per_pair"] for c in cands)
     after = p.aNUFT(p.fNUFT(x))
     # every setting computes the same operator; the coil grouping may move
     # the dense-point segmentation class (rounding only)
     assert rel(after, before) < 1e-6
+
+
+def _kernels_run(p, fn):
+    import ctypes as C
+    import json
+    p._lib.nufft3d_profile_enable(p._h, 1)
+    fn()
+    buf = C.create_string_buffer(1 << 16)
+    p._lib.nufft3d_profile_read(p._h, buf, len(buf))
+    p._lib.nufft3d_profile_enable(p._h, 0)
+    return set(json.loads(buf.value.decode()))
+
+
+@pytest.mark.parametrize("nc,comp", [(16, True), (8, True), (24, True), (32, True), (16, False)])
+def test_lds_tiled_spread_equals_gather_spread(nc, comp):
+    # the LDS-staged spread walks each point's hits in the same order as the
+    # gather kernel: bitwise the same grid (coil groups of 16 and 8, both
+    # accumulator kinds); the k-space centre has points past the dense limit
+    # (300 spokes through it), which both leave to the dense-point kernels
+    N = 24
+    om = synth.koosh_ball(N, 300, 2 * N)
+    k = _rand((om.shape[1], nc), 81)
+    tiled = _plan_with_env({"NUFFT3D_TUNE": "10"}, om, N, gpu=1, compensated=comp)
+    gather = nufft_3d(om, N, gpu=1, compensated=comp)
+    got = {}
+    ran = _kernels_run(tiled, lambda: got.setdefault("x", tiled.aNUFT(k)))
+    assert "k_spread_tile" in ran and "k_spread_heavy" in ran
+    ran_g = _kernels_run(gather, lambda: got.setdefault("y", gather.aNUFT(k)))
+    assert "k_spread_tile" not in ran_g
+    assert np.array_equal(got["x"], got["y"])
+    orc = O.NufftOracle(om, N)
+    assert rel(got["x"], orc.aNUFT(k)) < (TOL32_OP if comp else 1e-4)

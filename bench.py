@@ -372,7 +372,8 @@ def main():
         t1 = time.perf_counter()
         rep = plan.autotune(nc, reps=3)
         tuned = dict(rep["candidates"][rep["best"]], tune_s=round(time.perf_counter() - t1, 3),
-                     default_ms=rep["candidates"][0]["ms_per_pair"])
+                     default_ms=rep["candidates"][0]["ms_per_pair"],
+                     z_threads=rep.get("z_threads"), z_threads_ms=rep.get("z_threads_ms"))
         log("autotune:", tuned)
     log("plan %.3f s  K=%s nnz=%d" % (plan_s, plan.K, plan.nnz))
     K = plan.K[0]

@@ -17,6 +17,7 @@
 // (16, 8, 4, 2, 3, 5, 7) Stockham autosort whose stages exchange data in LDS
 // (element-major layout, line fastest); the first stage reads global memory
 // and the last writes it wherever the access pattern stays coalesced.
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -412,7 +413,7 @@ struct SGeom {
 };
 
 template <typename R, int PL>
-__global__ void __launch_bounds__(256) k_fft_strided(const cplx<R>* __restrict__ in,
+__global__ void __launch_bounds__(512) k_fft_strided(const cplx<R>* __restrict__ in,
                                                      cplx<R>* __restrict__ out, SGeom g,
                                                      Stages st, const cplx<R>* __restrict__ tw,
                                                      int sign) {
@@ -527,20 +528,43 @@ static Stages stages_of(const FftAxis& a) {
 // lines per LDS bundle: L*B elements staged in registers by 256 threads
 // (at most MaxE each) and held in one LDS buffer of at most 64 KiB
 // `want` = requested lines per work-group (0 = automatic), from the plan
-static int bundle_for(int L, size_t esz, int want) {
-  int B = 256 * (esz == sizeof(double2) ? MaxE<double>::v : MaxE<float>::v) / L;
-  const int bl = (int)(65536 / ((size_t)L * esz));
+static int bundle_for(int L, size_t esz, int want, int threads = 256, size_t lds = 65536) {
+  int B = threads * (esz == sizeof(double2) ? MaxE<double>::v : MaxE<float>::v) / L;
+  const int bl = (int)(lds / ((size_t)L * esz)) - 1;   // the L twiddles share the buffer
   if (bl < B) B = bl;
   if (want > 0 && want <= B) {
     int b = 1;
     while (b * 2 <= want) b *= 2;
     return b;
   }
+  if (B >= 32 && threads > 256) return 32;
   if (B >= 16) return 16;
   if (B >= 8) return 8;
   if (B >= 4) return 4;
   if (B >= 2) return 2;
   return 1;
+}
+
+// threads per z-pass work-group: 512 (bundles of 32 lines, 256-byte row
+// pieces) for axes with a compiled stage plan, where it measured faster;
+// 256 for the generic stage loop
+static int z_threads(const nufft3d_plan* p) {
+  if (p->fft_zthreads == 256 || p->fft_zthreads == 512) return p->fft_zthreads;
+  return p->ax[2].plan_id != 0 ? 512 : 256;
+}
+
+// dynamic LDS allowed to the z-pass work-groups (above the 64 KiB default)
+constexpr size_t kStridedLds = 80 * 1024;
+
+// raise a kernel's dynamic LDS limit once when a launch needs more than 64 KiB
+template <typename K>
+static void allow_lds(K* kernel, size_t bytes) {
+  static std::atomic<bool> granted{false};
+  if (bytes <= 65536 || granted.load()) return;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStridedLds) == hipSuccess)
+    granted.store(true);
+  (void)hipGetLastError();
 }
 
 static int bundle_x(int L, size_t esz, int nc, int want) {
@@ -597,12 +621,14 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   if (K3 > 1) {  // z-pass: [N3][K2*K1*nc] -> [K3][K2*K1*nc]
     SGeom g;
     g.Lin = N3; g.Lout = K3; g.n_img = N3; g.mode = 0;
-    g.B = bundle_for(K3, esz, p->fft_bundle);
+    const int zt = z_threads(p);
+    g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_fwd");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(256), \
+#define N3D_L(PL) allow_lds(k_fft_strided<R, PL>, sh);                                           \
+                  hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(zt), \
                                     sh, p->stream, (const cplx<R>*)T2, (cplx<R>*)grid, g,        \
                                     stages_of(p->ax[2]), (const cplx<R>*)p->ax[2].d_tw, -1)
     N3D_PLAN_SWITCH(p->ax[2].plan_id, N3D_L);
@@ -626,12 +652,14 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     N3D_TRY(ensure(p->ws_t2, t2 * esz));
     SGeom g;
     g.Lin = K3; g.Lout = N3; g.n_img = N3; g.mode = 1;
-    g.B = bundle_for(K3, esz, p->fft_bundle);
+    const int zt = z_threads(p);
+    g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K3 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_z_adj");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(256), \
+#define N3D_L(PL) allow_lds(k_fft_strided<R, PL>, sh);                                           \
+                  hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)g.nbundle), dim3(zt), \
                                     sh, p->stream, (const cplx<R>*)grid, (cplx<R>*)p->ws_t2.p,   \
                                     g, stages_of(p->ax[2]), (const cplx<R>*)p->ax[2].d_tw, +1)
     N3D_PLAN_SWITCH(p->ax[2].plan_id, N3D_L);

@@ -173,10 +173,11 @@ __device__ __forceinline__ void fma_coils(A* ar, A* ai, R w, const cplx<R>* val)
 }
 
 // ---------------------------------------------------------------------------
-// Forward: k[s,c] = scale * w_s * sum_{hits g of s} H(s,g) G[g,c] with the
-// samples in the sorted coil-interleaved layout. Hits come in z-y-x candidate
-// order.
-template <typename R, int CPL, int LPS, bool COMP, bool FULL>
+// Forward: k[s,c] = scale * w_s * sum_{hits g of s} H(s,g) G[g,c]. Hits come
+// in z-y-x candidate order. The result goes to the sorted coil-interleaved
+// layout out[s*nc + c], or with USER straight to the caller's [M nc] layout
+// out[orig[s] + c*M] (no separate permutation pass).
+template <typename R, int CPL, int LPS, bool COMP, bool FULL, bool USER>
 __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ spos,
                                                 const unsigned short* __restrict__ fcode,
                                                 const int4* __restrict__ base,
@@ -185,6 +186,7 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
                                                 const cplx<R>* __restrict__ grid,
                                                 cplx<R>* __restrict__ out,
                                                 const R* __restrict__ wsamp, R scale,
+                                                const int* __restrict__ orig,
                                                 GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
   const long long gt = xcd_block(g.xchunk) * blockDim.x + threadIdx.x;
@@ -233,13 +235,25 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
   }
   R sc = scale;
   if (wsamp) sc *= wsamp[s];
-  cplx<R>* op = out + (long long)s * g.nc + g.c0;
+  if constexpr (USER) {
+    cplx<R>* op = out + orig[s] + (long long)g.c0 * g.M;
 #pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
-    if (k < g.cg) {
-      cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
-      op[k] = v;
+    for (int c = 0; c < CPL; ++c) {
+      const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+      if (k < g.cg) {
+        cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
+        op[(long long)k * g.M] = v;
+      }
+    }
+  } else {
+    cplx<R>* op = out + (long long)s * g.nc + g.c0;
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+      const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+      if (k < g.cg) {
+        cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
+        op[k] = v;
+      }
     }
   }
 }
@@ -589,15 +603,22 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
 
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
-                          const void* wsamp, double scale) {
+                          const void* wsamp, double scale, bool user) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
   const long long nthreads = p->nnz * (long long)LPS;
   const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
-  hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL>), dim3(nb), dim3(256), 0, p->stream, p->d_spos,
-                     p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
-                     (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, g, kp);
+  if (user)
+    hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, true>), dim3(nb), dim3(256), 0, p->stream,
+                       p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
+                       (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, p->d_orig,
+                       g, kp);
+  else
+    hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, false>), dim3(nb), dim3(256), 0, p->stream,
+                       p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
+                       (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, p->d_orig,
+                       g, kp);
 }
 
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
@@ -643,7 +664,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
 // keeps one lane
 template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
-                       const void* wsamp, double scale) {
+                       const void* wsamp, double scale, bool user = false) {
   const int gmax = p->coil_group;
   for (int c0 = 0; c0 < nc; c0 += gmax) {
     const int cg = nc - c0 < gmax ? nc - c0 : gmax;
@@ -652,7 +673,7 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
 #define N3D_GO2(CPL, LPS, FULL)                                                                  \
     do {                                                                                         \
       if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, partial, wsamp); \
-      else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale);        \
+      else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale, user);  \
     } while (0)
 #define N3D_GO(CPL, LPS)                                                                         \
     do {                                                                                         \
@@ -679,28 +700,32 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
 
 template <typename R>
 static int interp_t(nufft3d_plan* p, const void* grid, int nc, void* out, const void* wsamp,
-                    double scale) {
+                    double scale, bool user) {
   if (p->nnz == 0) return NUFFT3D_OK;
-  if (sizeof(R) == 4 && p->compensated) run_groups<R, true, false>(p, grid, nc, out, nullptr, wsamp, scale);
-  else run_groups<R, false, false>(p, grid, nc, out, nullptr, wsamp, scale);
+  if (sizeof(R) == 4 && p->compensated)
+    run_groups<R, true, false>(p, grid, nc, out, nullptr, wsamp, scale, user);
+  else
+    run_groups<R, false, false>(p, grid, nc, out, nullptr, wsamp, scale, user);
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
 
 int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user);
 
+// user_layout: the caller's [M nc] array (rows of out-of-bounds samples are
+// left to zero_oob); otherwise the sorted coil-interleaved layout
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
            const void* wsamp, double scale) {
-  if (user_layout) {
-    // interpolate into the sorted coil-interleaved layout, then scatter rows
+  if (user_layout && (p->tune_flags & 16)) {
+    // interpolate into the sorted layout, then scatter rows (separate pass)
     const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
     N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz));
-    N3D_TRY(p->prec == 2 ? interp_t<double>(p, grid, nc, p->ws_samp.p, wsamp, scale)
-                         : interp_t<float>(p, grid, nc, p->ws_samp.p, wsamp, scale));
+    N3D_TRY(p->prec == 2 ? interp_t<double>(p, grid, nc, p->ws_samp.p, wsamp, scale, false)
+                         : interp_t<float>(p, grid, nc, p->ws_samp.p, wsamp, scale, false));
     return permute_out(p, p->ws_samp.p, nc, out);
   }
-  return p->prec == 2 ? interp_t<double>(p, grid, nc, out, wsamp, scale)
-                      : interp_t<float>(p, grid, nc, out, wsamp, scale);
+  return p->prec == 2 ? interp_t<double>(p, grid, nc, out, wsamp, scale, user_layout != 0)
+                      : interp_t<float>(p, grid, nc, out, wsamp, scale, user_layout != 0);
 }
 
 // samples in the internal layout ks[s*nc + c] (sorted order)

@@ -126,10 +126,12 @@ struct nufft3d_plan {
   int verbose = 0;
   int compensated = 1;      // fp32 plans: products accumulated in double
   int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order,
-                            // 8 = LDS-staged spread where the plan has brick lists)
+                            // 8 = LDS-staged spread where the plan has brick lists,
+                            // 16 = forward result through a separate permutation pass)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
+  int fft_zthreads = 0;     // threads per z-pass FFT work-group (256, 512; 0 = automatic)
 
   // sorted, in-bounds samples
   int* d_orig = nullptr;    // [nnz] original row of sorted sample s

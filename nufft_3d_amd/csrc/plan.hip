@@ -1150,6 +1150,7 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   p->compensated = o.compensated ? 1 : 0;
   if (const char* tv = getenv("NUFFT3D_TUNE")) p->tune_flags = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FFTB")) p->fft_bundle = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_FFTZT")) p->fft_zthreads = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_GRAPH")) p->use_graphs = atoi(tv) != 0;
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
@@ -1726,7 +1727,6 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     c.ms = ms / reps;
     if (!st && (best < 0 || c.ms < cands[best].ms)) best = (int)i;
   }
-  if (st == NUFFT3D_ERR_DEVICE) { (void)hipGetLastError(); set_error("device error while autotuning"); }
   if (best >= 0) {
     const Cand& b = cands[best];
     p->fft_bundle = b.fftb;
@@ -1736,6 +1736,26 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   } else {
     p->tune_flags = keep_flags;
   }
+  // z-pass work-group size, timed on the best setting: 256 and 512 threads
+  float zt_ms[2] = {0.f, 0.f};
+  int zt_best = p->fft_zthreads;
+  if (!st && best >= 0 && p->K[2] > 1) {
+    const int keep_zt = p->fft_zthreads;
+    for (int zi = 0; zi < 2 && !st; ++zi) {
+      p->fft_zthreads = zi ? 512 : 256;
+      st = pair();
+      if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      for (int r = 0; r < reps && !st; ++r) st = pair();
+      if (!st && hipEventRecord(e1, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      if (!st && hipEventSynchronize(e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      float ms = 0;
+      if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      zt_ms[zi] = ms / reps;
+    }
+    zt_best = st ? keep_zt : (zt_ms[1] < zt_ms[0] ? 512 : 256);
+    p->fft_zthreads = zt_best;
+  }
+  if (st == NUFFT3D_ERR_DEVICE) { (void)hipGetLastError(); set_error("device error while autotuning"); }
   if (!st && report) {
     std::string js = "{\"nc\": " + std::to_string(nc) + ", \"candidates\": [";
     for (size_t i = 0; i < cands.size(); ++i) {
@@ -1747,7 +1767,10 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
                spread_kind[cands[i].brick], cands[i].ms);
       js += b;
     }
-    js += "], \"best\": " + std::to_string(best) + "}";
+    char zb[128];
+    snprintf(zb, sizeof zb, ", \"z_threads\": %d, \"z_threads_ms\": [%.5f, %.5f]", zt_best,
+             zt_ms[0], zt_ms[1]);
+    js += "], \"best\": " + std::to_string(best) + zb + "}";
     if ((int64_t)js.size() + 1 > len) { set_error("report buffer too small"); st = NUFFT3D_ERR_INVALID; }
     else std::memcpy(report, js.c_str(), js.size() + 1);
   }

@@ -277,6 +277,7 @@ def test_autotune_picks_a_measured_setting_and_keeps_results():
     cands = rep["candidates"]
     assert len(cands) == 36 and 0 <= rep["best"] < len(cands)
     assert {c["spread"] for c in cands} == {"gather", "gather-bricks", "lds-bricks"}
+    assert rep["z_threads"] in (256, 512) and min(rep["z_threads_ms"]) > 0
     assert all(c["ms_per_pair"] > 0 for c in cands)
     assert cands[rep["best"]]["ms_per_pair"] == min(c["ms_per_pair"] for c in cands)
     after = p.aNUFT(p.fNUFT(x))
@@ -315,3 +316,27 @@ def test_lds_tiled_spread_equals_gather_spread(nc, comp):
     assert np.array_equal(got["x"], got["y"])
     orc = O.NufftOracle(om, N)
     assert rel(got["x"], orc.aNUFT(k)) < (TOL32_OP if comp else 1e-4)
+
+
+@pytest.mark.parametrize("gpu", [1, 2])
+def test_z_pass_work_group_sizes_agree(gpu):
+    # 256- and 512-thread z-pass work-groups (bundles of 16 and 32 lines)
+    # run the same butterflies on every line: bitwise the same result
+    om = synth.koosh_ball(32, 200, 64)
+    x = _rand((32, 32, 32, 5), 91)
+    k = _rand((om.shape[1], 5), 92)
+    a = _plan_with_env({"NUFFT3D_FFTZT": "256"}, om, 32, gpu=gpu)
+    b = _plan_with_env({"NUFFT3D_FFTZT": "512"}, om, 32, gpu=gpu)
+    assert np.array_equal(a.fNUFT(x), b.fNUFT(x))
+    assert np.array_equal(a.aNUFT(k), b.aNUFT(k))
+
+
+@pytest.mark.parametrize("gpu,nc", [(1, 16), (1, 3), (2, 5)])
+def test_forward_written_in_caller_layout_equals_permuted(gpu, nc):
+    # fNUFT writes each sample's coils straight to rows orig[s] of the [M nc]
+    # result; the separate sorted-layout + permutation pass gives the same bits
+    om = np.random.default_rng(5).uniform(-9, 9, size=(3, 4000))   # some samples out of bounds
+    x = _rand((16, 16, 16, nc), 93)
+    direct = nufft_3d(om, 16, gpu=gpu)
+    perm = _plan_with_env({"NUFFT3D_TUNE": "18"}, om, 16, gpu=gpu)
+    assert np.array_equal(direct.fNUFT(x), perm.fNUFT(x))

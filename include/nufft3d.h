@@ -160,6 +160,15 @@ int nufft3d_pruno_opts_default(nufft3d_pruno_opts* opts);
 int nufft3d_gram(nufft3d_plan* plan, const void* data, const int32_t dims[4], int32_t width,
                  int32_t radial, void* AA, int32_t* n);
 
+/* make_data_matrix(data,opts) with opts.loraks = 1 (sake.m:176-224): the
+ * 2n x 2n matrix [AA BB; conj(BB) conj(AA)], BB(:,j,:,k) = row_j' *
+ * conj(col_k(flip.x,flip.y,flip.z,:)) with the conjugate reflection about the
+ * 1-based k-space centre (sake.m:79-88; NULL = [1 1 1]). Same data, width,
+ * radial and precision rules as nufft3d_gram; AA is complex128 [2n 2n]
+ * column-major and *n receives 2n. */
+int nufft3d_gram_loraks(nufft3d_plan* plan, const void* data, const int32_t dims[4], int32_t width,
+                        int32_t radial, const int32_t center[3], void* AA, int32_t* n);
+
 /* K = pruno(obj,data) (pruno.m:2-143) on k-space data [N1 N2 N3 nc] of the
  * plan's image size: returns nv, the singular values sv [n] (descending,
  * may be NULL) and the noise std used. */

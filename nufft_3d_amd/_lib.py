@@ -68,6 +68,8 @@ SIGNATURES = {
     "nufft3d_pruno_opts_default": (C.c_int, [C.POINTER(PrunoOpts)]),
     "nufft3d_gram": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
                                C.c_int32, C.c_void_p, C.POINTER(C.c_int32)]),
+    "nufft3d_gram_loraks": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
+                                      C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "nufft3d_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(PrunoOpts),
                                 C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_double)]),
     "nufft3d_inverse_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,

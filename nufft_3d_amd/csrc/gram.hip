@@ -100,11 +100,16 @@ __global__ void k_gram_prep(const float2* __restrict__ data, unsigned short* __r
   Q[o + 3 * pl] = il;
 }
 
-// grid: x = upper tile pair, y = V chunk. 256 threads = 4 waves (2x2 of 64x64).
-// Thread t stages column t/2 of both tiles, half (t%2) of the 32-voxel run,
-// for the next K step in registers while the MFMAs of this one run.
-__global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __restrict__ Q,
-                                                     const GramCol* __restrict__ cols,
+// C = X^T Y tiles: X columns from the prepared operand Qa (column table
+// colsA), Y columns from Qb (colsB); Y = X for AA, the conjugate reflection
+// for the LORAKS block. grid: x = tile pair, y = V chunk. 256 threads = 4
+// waves (2x2 of 64x64). Thread t stages column t/2 of both tiles, half (t%2)
+// of the 32-voxel run, for the next K step in registers while the MFMAs of
+// this one run.
+__global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __restrict__ Qa,
+                                                     const GramCol* __restrict__ colsA,
+                                                     const unsigned short* __restrict__ Qb,
+                                                     const GramCol* __restrict__ colsB,
                                                      const int2* __restrict__ tiles,
                                                      float* __restrict__ partial, GramGeom g) {
   __shared__ unsigned short sAh[GT][GK + GPAD], sAl[GT][GK + GPAD];
@@ -124,7 +129,7 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   const int cl = tid >> 1, hh = tid & 1;
-  const GramCol ca = cols[I0 + cl], cb = cols[J0 + cl];
+  const GramCol ca = colsA[I0 + cl], cb = colsB[J0 + cl];
   const long long lo_off = (long long)g.nc * g.plane;
   uint4 ra[4], rb[4];     // hi[0..1], lo[0..1] chunks of 8 bf16
   auto fetch = [&](long long step) {
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
       ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
       zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
       const uint4* q = reinterpret_cast<const uint4*>(
-          Q + ca.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
+          Qa + ca.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
       const uint4* ql = q + lo_off / 8;
       ra[0] = q[0]; ra[1] = q[1]; ra[2] = ql[0]; ra[3] = ql[1];
     } else {
@@ -148,7 +153,7 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
       ys = ys < 0 ? ys + g.N2 : (ys >= g.N2 ? ys - g.N2 : ys);
       zs = zs < 0 ? zs + g.N3 : (zs >= g.N3 ? zs - g.N3 : zs);
       const uint4* q = reinterpret_cast<const uint4*>(
-          Q + cb.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
+          Qb + cb.base + ((long long)zs * g.N2 + ys) * g.Lx + xc * 32 + 16 * hh);
       const uint4* ql = q + lo_off / 8;
       rb[0] = q[0]; rb[1] = q[1]; rb[2] = ql[0]; rb[3] = ql[1];
     } else {
@@ -208,41 +213,79 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
       }
 }
 
-// fetch C[r][c] = sum over chunks of the partial tiles (fixed order, double)
+// fetch C[r][c] = sum over chunks of the partial tiles (fixed order, double);
+// sym: only the upper tiles were computed (C symmetric)
 __device__ __forceinline__ double cfetch(const float* __restrict__ partial, const int* __restrict__ tid_of,
-                                         int ntiles, int nsplit, int ntr, int r, int c) {
+                                         int ntiles, int nsplit, int ntr, int r, int c, bool sym) {
   int tr = r / GT, tc = c / GT, lr = r % GT, lc = c % GT;
-  if (tr > tc) { int t = tr; tr = tc; tc = t; t = lr; lr = lc; lc = t; }
+  if (sym && tr > tc) { int t = tr; tr = tc; tc = t; t = lr; lr = lc; lc = t; }
   const int t = tid_of[tr * ntr + tc];
   double s = 0;
   for (int k = 0; k < nsplit; ++k) s += (double)partial[((long long)k * ntiles + t) * (GT * GT) + lr * GT + lc];
   return s;
 }
 
-// AA (column-major complex double n x n) from C
+// complex n x n (column-major, double) from the real Gram C = X^T Y:
+// sum_v conj(x) y = (C11 + C22) + i (C12 - C21)
 __global__ void k_gram_assemble(const float* __restrict__ partial, const int* __restrict__ tid_of,
-                                int ntiles, GramGeom g, double2* __restrict__ AA) {
+                                int ntiles, int nsplit, GramGeom g, int sym,
+                                double2* __restrict__ AA) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)g.n * g.n) return;
   const int a = (int)(idx % g.n), b = (int)(idx / g.n);
   const int n = g.n;
-  const double c11 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, a, b);
-  const double c22 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, n + a, n + b);
-  const double c12 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, a, n + b);
-  const double c21 = cfetch(partial, tid_of, ntiles, g.nsplit, g.ntr, n + a, b);
+  const bool sy = sym != 0;
+  const double c11 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, a, b, sy);
+  const double c22 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, n + a, n + b, sy);
+  const double c12 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, a, n + b, sy);
+  const double c21 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, n + a, b, sy);
   double2 o;
   o.x = c11 + c22;
   o.y = c12 - c21;
   AA[idx] = o;
 }
 
+// g[x,y,z,c] = conj(data[fx(x), fy(y), fz(z), c]), f(i) = (2 c0 - i) mod N per
+// axis: the conjugate reflection about the (0-based) centre c0 (sake.m:86-88)
+template <typename R>
+__global__ void k_flip_conj(const cplx<R>* __restrict__ d, cplx<R>* __restrict__ gout, int N1,
+                            int N2, int N3, int nc, int c1, int c2, int c3) {
+  const long long V = (long long)N1 * N2 * N3;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V * nc) return;
+  const long long v = i % V, c = i / V;
+  const int x = (int)(v % N1), y = (int)((v / N1) % N2), z = (int)(v / ((long long)N1 * N2));
+  auto fl = [](int a, int c0, int n) { int r = (2 * c0 - a) % n; return r < 0 ? r + n : r; };
+  const long long src = c * V + fl(x, c1, N1) + (long long)N1 * (fl(y, c2, N2) + (long long)N2 * fl(z, c3, N3));
+  cplx<R> o = d[src];
+  o.y = -o.y;
+  gout[i] = o;
+}
+
+// [AA BB; conj(BB) conj(AA)] (2n x 2n, column-major) from AA and BB (n x n)
+__global__ void k_loraks_blocks(const double2* __restrict__ AA, const double2* __restrict__ BB,
+                                int n, double2* __restrict__ out) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int a = (int)(idx % n), b = (int)(idx / n);
+  const long long m = 2LL * n;
+  const double2 aa = AA[idx], bb = BB[idx];
+  out[a + m * b] = aa;
+  out[a + m * (n + b)] = bb;
+  out[(n + a) + m * b] = make_double2(bb.x, -bb.y);
+  out[(n + a) + m * (n + b)] = make_double2(aa.x, -aa.y);
+}
+
 // fp64 Gram: one thread per (a,b) upper entry, complex double directly
-__global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ data,
-                                                   const int4* __restrict__ shifts, GramGeom g,
-                                                   double2* __restrict__ AA) {
+// sum_v conj(x_a[v]) y_b[v] with x from dataA shifted by s_j and y from dataB
+// shifted by bsign*s_k; sym: y = x, upper entries computed and mirrored
+__global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ dataA,
+                                                   const double2* __restrict__ dataB,
+                                                   const int4* __restrict__ shifts, int bsign,
+                                                   int sym, GramGeom g, double2* __restrict__ AA) {
   __shared__ double2 sa[16][64], sb[16][64];
   const int ta = blockIdx.x * 16, tb = blockIdx.y * 16;
-  if (ta > tb + 15) return;   // strictly below the diagonal band: filled from the upper part
+  if (sym && ta > tb + 15) return;   // strictly below the diagonal band: filled from the upper part
   const int la = threadIdx.x % 16, lb = threadIdx.x / 16;
   const int a = ta + la, b = tb + lb;
   double re = 0, im = 0;
@@ -257,14 +300,16 @@ __global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ d
       if (v < g.V && aa < g.n) {
         const int c = aa % g.nc, j = aa / g.nc;
         const int4 s = shifts[j];
+        const int sg = which ? bsign : 1;
         int x = (int)(v % g.N1);
         const long long r = v / g.N1;
         int y = (int)(r % g.N2), z = (int)(r / g.N2);
-        x -= s.x; y -= s.y; z -= s.z;
+        x -= sg * s.x; y -= sg * s.y; z -= sg * s.z;
         x = x < 0 ? x + g.N1 : (x >= g.N1 ? x - g.N1 : x);
         y = y < 0 ? y + g.N2 : (y >= g.N2 ? y - g.N2 : y);
         z = z < 0 ? z + g.N3 : (z >= g.N3 ? z - g.N3 : z);
-        val = data[(long long)c * g.V + x + (long long)g.N1 * (y + (long long)g.N2 * z)];
+        const double2* src = which ? dataB : dataA;
+        val = src[(long long)c * g.V + x + (long long)g.N1 * (y + (long long)g.N2 * z)];
       }
       if (which) sb[col][vl] = val; else sa[col][vl] = val;
     }
@@ -276,10 +321,10 @@ __global__ void __launch_bounds__(256) k_gram_fp64(const double2* __restrict__ d
       im += p.x * q.y - p.y * q.x;
     }
   }
-  if (a < g.n && b < g.n && a <= b) {
+  if (a < g.n && b < g.n && (!sym || a <= b)) {
     double2 o; o.x = re; o.y = im;
     AA[a + (long long)g.n * b] = o;
-    if (a != b) { o.y = -im; AA[b + (long long)g.n * a] = o; }
+    if (sym && a != b) { o.y = -im; AA[b + (long long)g.n * a] = o; }
   }
 }
 
@@ -391,9 +436,13 @@ void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int&
 }
 
 // AA = Hankel Gram of data [N1 N2 N3 nc] (device, plan precision) -> dAA
-// (device complex double, column-major n x n)
+// (device complex double, column-major n x n). With loraks, dAA receives
+// the 2n x 2n matrix [AA BB; conj(BB) conj(AA)] (sake.m:186-188,210-214,
+// 221-224), BB the Gram of the data against its conjugate reflection about
+// the 1-based centre[3]: column (d,k) of that operand is
+// conj(data_d[flip(v) - s_k]) = g_d[v + s_k], g = conj(data(flip)).
 int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
-                double2* dAA) {
+                double2* dAA, bool loraks, const int* center) {
   GramGeom g;
   g.N1 = dims[0]; g.N2 = dims[1]; g.N3 = dims[2];
   g.V = (long long)g.N1 * g.N2 * g.N3;
@@ -401,15 +450,37 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
   g.nk = (int)taps.size();
   g.n = g.nc * g.nk;
   g.n2 = 2 * g.n;
+  const size_t nn = (size_t)g.n * g.n;
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
   int4* dtaps = nullptr;
+  void* flip = nullptr;
+  double2 *tmpA = nullptr, *tmpB = nullptr;
   N3D_HIP(hipMalloc(&dtaps, sizeof(int4) * taps.size()));
   N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
+  if (loraks) {
+    N3D_HIP(hipMalloc(&flip, esz * g.V * g.nc));
+    N3D_HIP(hipMalloc(&tmpA, sizeof(double2) * nn));
+    N3D_HIP(hipMalloc(&tmpB, sizeof(double2) * nn));
+    const int c1 = center ? center[0] - 1 : 0, c2 = center ? center[1] - 1 : 0,
+              c3 = center ? center[2] - 1 : 0;
+    ProfScope ps(p, "k_flip_conj");
+    if (p->prec == 2)
+      hipLaunchKernelGGL((k_flip_conj<double>), dim3(nblk(g.V * g.nc)), dim3(256), 0, p->stream,
+                         (const double2*)data, (double2*)flip, g.N1, g.N2, g.N3, g.nc, c1, c2, c3);
+    else
+      hipLaunchKernelGGL((k_flip_conj<float>), dim3(nblk(g.V * g.nc)), dim3(256), 0, p->stream,
+                         (const float2*)data, (float2*)flip, g.N1, g.N2, g.N3, g.nc, c1, c2, c3);
+  }
+  double2* outA = loraks ? tmpA : dAA;
   int st = NUFFT3D_OK;
   if (p->prec == 2) {
     dim3 grid((g.n + 15) / 16, (g.n + 15) / 16);
     ProfScope ps(p, "k_gram_fp64");
     hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
-                       (const int4*)dtaps, g, dAA);
+                       (const double2*)data, (const int4*)dtaps, 1, 1, g, outA);
+    if (loraks)
+      hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
+                         (const double2*)flip, (const int4*)dtaps, -1, 0, g, tmpB);
   } else {
     g.ntr = (g.n2 + GT - 1) / GT;
     int hmax = 0;
@@ -420,68 +491,118 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
     g.plane = (long long)g.N3 * g.N2 * g.Lx;
     g.nsteps = (long long)g.N2 * g.N3 * g.nxc;
     const int nsx = 2 * g.h + 1;
-    std::vector<int2> tl;
-    std::vector<int> tid_of(g.ntr * g.ntr, -1);
-    for (int i = 0; i < g.ntr; ++i)
-      for (int j = i; j < g.ntr; ++j) { tid_of[i * g.ntr + j] = (int)tl.size(); tl.push_back(make_int2(i, j)); }
-    const int ntiles = (int)tl.size();
-    int nsplit = (2048 + ntiles - 1) / ntiles;
-    if (nsplit > g.nsteps) nsplit = (int)g.nsteps;
-    g.spw = (g.nsteps + nsplit - 1) / nsplit;
-    nsplit = (int)((g.nsteps + g.spw - 1) / g.spw);
-    g.nsplit = nsplit;
-    // columns of X (p, c, j): hi-plane offsets of their pre-shifted operands
-    std::vector<GramCol> cols((size_t)g.ntr * GT);
-    for (int col = 0; col < g.ntr * GT; ++col) {
-      GramCol& cc = cols[col];
-      cc.valid = col < g.n2;
-      cc.base = 0; cc.sy = 0; cc.sz = 0; cc.pad = 0;
-      if (!cc.valid) continue;
-      const int pp = col >= g.n;
-      const int a = pp ? col - g.n : col;
-      const int c = a % g.nc, j = a / g.nc;
-      const int4 t = taps[j];
-      const int sxi = t.x + g.h;
-      cc.base = ((long long)(sxi * 4 + 2 * pp) * g.nc + c) * g.plane;
-      cc.sy = t.y;
-      cc.sz = t.z;
-    }
-    unsigned short* Q = nullptr;
-    GramCol* dcols = nullptr;
-    int2* dtl = nullptr;
-    int* dtid = nullptr;
-    float* part = nullptr;
+    // columns (p, c, j) of an operand: hi-plane offsets of the pre-shifted
+    // planes; shift sign -1 for the reflected operand (g_d[v + s_k])
+    auto make_cols = [&](int sgn) {
+      std::vector<GramCol> cols((size_t)g.ntr * GT);
+      for (int col = 0; col < g.ntr * GT; ++col) {
+        GramCol& cc = cols[col];
+        cc.valid = col < g.n2;
+        cc.base = 0; cc.sy = 0; cc.sz = 0; cc.pad = 0;
+        if (!cc.valid) continue;
+        const int pp = col >= g.n;
+        const int a = pp ? col - g.n : col;
+        const int c = a % g.nc, j = a / g.nc;
+        const int4 t = taps[j];
+        const int sxi = sgn * t.x + g.h;
+        cc.base = ((long long)(sxi * 4 + 2 * pp) * g.nc + c) * g.plane;
+        cc.sy = sgn * t.y;
+        cc.sz = sgn * t.z;
+      }
+      return cols;
+    };
     const long long nq = (long long)nsx * 4 * g.nc * g.plane;
+    unsigned short *Q = nullptr, *Q2 = nullptr;
+    GramCol *dcols = nullptr, *dcols2 = nullptr;
     N3D_HIP(hipMalloc(&Q, sizeof(unsigned short) * nq));
-    N3D_HIP(hipMalloc(&dcols, sizeof(GramCol) * cols.size()));
-    N3D_HIP(hipMalloc(&dtl, sizeof(int2) * ntiles));
-    N3D_HIP(hipMalloc(&dtid, sizeof(int) * tid_of.size()));
-    N3D_HIP(hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT));
-    N3D_HIP(hipMemcpyAsync(dcols, cols.data(), sizeof(GramCol) * cols.size(), hipMemcpyHostToDevice, p->stream));
-    N3D_HIP(hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice, p->stream));
-    N3D_HIP(hipMemcpyAsync(dtid, tid_of.data(), sizeof(int) * tid_of.size(), hipMemcpyHostToDevice, p->stream));
+    {
+      const std::vector<GramCol> cols = make_cols(1);
+      N3D_HIP(hipMalloc(&dcols, sizeof(GramCol) * cols.size()));
+      N3D_HIP(hipMemcpy(dcols, cols.data(), sizeof(GramCol) * cols.size(), hipMemcpyHostToDevice));
+    }
     {
       ProfScope ps(p, "k_gram_prep");
       hipLaunchKernelGGL(k_gram_prep, dim3(nblk((long long)nsx * g.nc * g.plane)), dim3(256), 0,
                          p->stream, (const float2*)data, Q, g, nsx);
     }
-    {
-      ProfScope ps(p, "k_gram_bf16x3");
-      hipLaunchKernelGGL(k_gram_bf16x3, dim3(ntiles, nsplit), dim3(256), 0, p->stream,
-                         (const unsigned short*)Q, (const GramCol*)dcols, (const int2*)dtl, part, g);
+    if (loraks) {
+      N3D_HIP(hipMalloc(&Q2, sizeof(unsigned short) * nq));
+      const std::vector<GramCol> cols2 = make_cols(-1);
+      N3D_HIP(hipMalloc(&dcols2, sizeof(GramCol) * cols2.size()));
+      N3D_HIP(hipMemcpy(dcols2, cols2.data(), sizeof(GramCol) * cols2.size(), hipMemcpyHostToDevice));
+      ProfScope ps(p, "k_gram_prep");
+      hipLaunchKernelGGL(k_gram_prep, dim3(nblk((long long)nsx * g.nc * g.plane)), dim3(256), 0,
+                         p->stream, (const float2*)flip, Q2, g, nsx);
     }
-    {
-      ProfScope ps(p, "k_gram_assemble");
-      hipLaunchKernelGGL(k_gram_assemble, dim3(nblk((long long)g.n * g.n)), dim3(256), 0, p->stream,
-                         (const float*)part, (const int*)dtid, ntiles, g, dAA);
-    }
-    if (hipStreamSynchronize(p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
-    (void)hipFree(Q); (void)hipFree(dcols); (void)hipFree(dtl); (void)hipFree(dtid); (void)hipFree(part);
+    // C = X^T Y over the upper tiles (sym, Y = X) or all tiles, then the
+    // complex n x n result
+    auto gram_pair = [&](const unsigned short* Qb, const GramCol* colsB, bool sym,
+                         double2* out) -> int {
+      std::vector<int2> tl;
+      std::vector<int> tid_of(g.ntr * g.ntr, -1);
+      for (int i = 0; i < g.ntr; ++i)
+        for (int j = sym ? i : 0; j < g.ntr; ++j) {
+          tid_of[i * g.ntr + j] = (int)tl.size();
+          tl.push_back(make_int2(i, j));
+        }
+      const int ntiles = (int)tl.size();
+      GramGeom gg = g;
+      int nsplit = (2048 + ntiles - 1) / ntiles;
+      if (nsplit > gg.nsteps) nsplit = (int)gg.nsteps;
+      gg.spw = (gg.nsteps + nsplit - 1) / nsplit;
+      nsplit = (int)((gg.nsteps + gg.spw - 1) / gg.spw);
+      gg.nsplit = nsplit;
+      int2* dtl = nullptr;
+      int* dtid = nullptr;
+      float* part = nullptr;
+      int rs = NUFFT3D_OK;
+      if (hipMalloc(&dtl, sizeof(int2) * ntiles) != hipSuccess ||
+          hipMalloc(&dtid, sizeof(int) * tid_of.size()) != hipSuccess ||
+          hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT) != hipSuccess) {
+        set_error("device allocation for the Gram tiles failed");
+        rs = NUFFT3D_ERR_NOMEM;
+      }
+      if (!rs && (hipMemcpyAsync(dtl, tl.data(), sizeof(int2) * ntiles, hipMemcpyHostToDevice,
+                                 p->stream) != hipSuccess ||
+                  hipMemcpyAsync(dtid, tid_of.data(), sizeof(int) * tid_of.size(),
+                                 hipMemcpyHostToDevice, p->stream) != hipSuccess))
+        rs = NUFFT3D_ERR_DEVICE;
+      if (!rs) {
+        {
+          ProfScope ps(p, "k_gram_bf16x3");
+          hipLaunchKernelGGL(k_gram_bf16x3, dim3(ntiles, nsplit), dim3(256), 0, p->stream,
+                             (const unsigned short*)Q, (const GramCol*)dcols, Qb, colsB,
+                             (const int2*)dtl, part, gg);
+        }
+        ProfScope ps(p, "k_gram_assemble");
+        hipLaunchKernelGGL(k_gram_assemble, dim3(nblk((long long)g.n * g.n)), dim3(256), 0,
+                           p->stream, (const float*)part, (const int*)dtid, ntiles, nsplit, gg,
+                           sym ? 1 : 0, out);
+        if (hipStreamSynchronize(p->stream) != hipSuccess) rs = NUFFT3D_ERR_DEVICE;
+      }
+      if (dtl) (void)hipFree(dtl);
+      if (dtid) (void)hipFree(dtid);
+      if (part) (void)hipFree(part);
+      return rs;
+    };
+    st = gram_pair(Q, dcols, true, outA);
+    if (!st && loraks) st = gram_pair(Q2, dcols2, false, tmpB);
+    (void)hipFree(Q); (void)hipFree(dcols);
+    if (Q2) (void)hipFree(Q2);
+    if (dcols2) (void)hipFree(dcols2);
+  }
+  if (!st && loraks) {
+    ProfScope ps(p, "k_loraks_blocks");
+    hipLaunchKernelGGL(k_loraks_blocks, dim3(nblk((long long)nn)), dim3(256), 0, p->stream,
+                       (const double2*)tmpA, (const double2*)tmpB, g.n, dAA);
   }
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(dtaps);
-  if (st) set_error("device error in the Gram kernel");
+  if (flip) (void)hipFree(flip);
+  if (tmpA) (void)hipFree(tmpA);
+  if (tmpB) (void)hipFree(tmpB);
+  if (st == NUFFT3D_ERR_DEVICE) set_error("device error in the Gram kernel");
   return st;
 }
 

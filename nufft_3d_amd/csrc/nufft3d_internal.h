@@ -228,7 +228,7 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
                 double lambda, void** dB, int* nv_out, std::vector<double>* sv_desc,
                 double* noise_out);
 int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
-                double2* dAA);
+                double2* dAA, bool loraks = false, const int* center = nullptr);
 void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int& w);
 int null_apply(nufft3d_plan* p, const void* B, const void* x, void* y, int nc);
 

@@ -1797,6 +1797,11 @@ int nufft3d_gram(nufft3d_plan* p, const void* data, const int32_t dims[4], int32
   if (!data || !dims || !AA || width < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
   for (int a = 0; a < 4; ++a)
     if (dims[a] < 1) { set_error("dims must be positive"); return NUFFT3D_ERR_INVALID; }
+  for (int a = 0; a < (dims[2] == 1 ? 2 : 3); ++a)
+    if (dims[a] <= width / 2) {   // shifted indices wrap once
+      set_error("data dimensions must exceed the kernel half width");
+      return NUFFT3D_ERR_UNSUPPORTED;
+    }
   std::vector<int4> taps;
   int w = 0;
   kernel_taps(width, radial, dims[2] == 1, taps, w);
@@ -1810,6 +1815,43 @@ int nufft3d_gram(nufft3d_plan* p, const void* data, const int32_t dims[4], int32
   int st = stage_out(p, AA, sizeof(double2) * n * n, tmp_out, ao);
   if (!st) st = gram_device(p, di.dev, dd, taps, (double2*)ao.dev);
   if (!st) st = finish_out(p, AA, sizeof(double2) * n * n, ao);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  release(tmp_in);
+  release(tmp_out);
+  release(tmp_io);
+  return st;
+}
+
+int nufft3d_gram_loraks(nufft3d_plan* p, const void* data, const int32_t dims[4], int32_t width,
+                        int32_t radial, const int32_t center[3], void* AA, int32_t* nout) {
+  N3D_PLAN_GUARD(p);
+  if (!data || !dims || !AA || width < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
+  for (int a = 0; a < 4; ++a)
+    if (dims[a] < 1) { set_error("dims must be positive"); return NUFFT3D_ERR_INVALID; }
+  int cen[3] = {1, 1, 1};                                   // sake.m:79-83
+  if (center)
+    for (int a = 0; a < 3; ++a) {
+      if (center[a] < 1 || center[a] > dims[a]) { set_error("center outside the data"); return NUFFT3D_ERR_INVALID; }
+      cen[a] = center[a];
+    }
+  for (int a = 0; a < (dims[2] == 1 ? 2 : 3); ++a)
+    if (dims[a] <= width / 2) {   // shifted indices wrap once
+      set_error("data dimensions must exceed the kernel half width");
+      return NUFFT3D_ERR_UNSUPPORTED;
+    }
+  std::vector<int4> taps;
+  int w = 0;
+  kernel_taps(width, radial, dims[2] == 1, taps, w);
+  const int dd[4] = {dims[0], dims[1], dims[2], dims[3]};
+  const long long n2 = 2LL * (long long)taps.size() * dims[3];
+  if (nout) *nout = (int32_t)n2;
+  const size_t nd = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+  Staged di, ao;
+  Workspace tmp_in, tmp_out, tmp_io;
+  N3D_TRY(stage_in_c(p, data, nd, tmp_in, tmp_io, di));
+  int st = stage_out(p, AA, sizeof(double2) * n2 * n2, tmp_out, ao);
+  if (!st) st = gram_device(p, di.dev, dd, taps, (double2*)ao.dev, true, cen);
+  if (!st) st = finish_out(p, AA, sizeof(double2) * n2 * n2, ao);
   N3D_HIP(hipStreamSynchronize(p->stream));
   release(tmp_in);
   release(tmp_out);

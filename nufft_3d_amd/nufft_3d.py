@@ -331,8 +331,11 @@ class nufft_3d:
     def _host_c(self, a, shape):
         return np.asfortranarray(np.reshape(np.asarray(a), shape, order="F"), dtype=self.cdtype)
 
-    def gram(self, data, width=5, radial=1):
-        """AA = make_data_matrix(data) (pruno.m:146-179): complex128 [n n]."""
+    def gram(self, data, width=5, radial=1, loraks=False, center=None):
+        """AA = make_data_matrix(data) (pruno.m:146-179, sake.m:176-224):
+        complex128 [n n]; with loraks the [2n 2n] matrix
+        [AA BB; conj(BB) conj(AA)] with the conjugate reflection about the
+        1-based k-space centre (default [1 1 1], sake.m:79-88)."""
         data = np.asarray(data)
         if data.ndim != 4:
             raise ValueError("'data' must be a 4d float array")
@@ -342,9 +345,15 @@ class nufft_3d:
         # size query from the tap count
         from . import taps as _taps
         nk = len(_taps.kernel_taps(width, radial, data.shape[2] == 1))
-        AA = np.empty((nk * data.shape[3], nk * data.shape[3]), dtype=np.complex128, order="F")
-        _lib.check(self._lib.nufft3d_gram(self._h, db.ctypes.data, dims, int(width), int(radial),
-                                          AA.ctypes.data, C.byref(n)))
+        m = nk * data.shape[3] * (2 if loraks else 1)
+        AA = np.empty((m, m), dtype=np.complex128, order="F")
+        if loraks:
+            cen = None if center is None else (C.c_int32 * 3)(*[int(c) for c in center])
+            _lib.check(self._lib.nufft3d_gram_loraks(self._h, db.ctypes.data, dims, int(width),
+                                                     int(radial), cen, AA.ctypes.data, C.byref(n)))
+        else:
+            _lib.check(self._lib.nufft3d_gram(self._h, db.ctypes.data, dims, int(width),
+                                              int(radial), AA.ctypes.data, C.byref(n)))
         assert n.value == AA.shape[0]
         return AA
 

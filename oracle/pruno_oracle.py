@@ -44,19 +44,41 @@ def kernel_taps(width=5, radial=1, is2d=False):
     return taps, k
 
 
-def make_data_matrix(data, taps):
-    """pruno.m:146-179: AA(:,j,:,k) = row'*col with circshift copies."""
+def flip_index(n, center=1):
+    """sake.m:86-88: circshift(n:-1:1, [0 2*center-1]) as 0-based indices
+    (reflection about the 1-based k-space centre)."""
+    return np.roll(np.arange(n, 0, -1), 2 * center - 1) - 1
+
+
+def make_data_matrix(data, taps, loraks=False, center=(1, 1, 1)):
+    """pruno.m:146-179 / sake.m:176-224: AA(:,j,:,k) = row'*col with
+    circshift copies; with loraks the conjugate-reflection block
+    BB(:,j,:,k) = row'*conj(col(flip.x,flip.y,flip.z,:)) (sake.m:210-214)
+    and the 2n x 2n matrix [AA BB; conj(BB) conj(AA)] (sake.m:221-224)."""
     nx, ny, nz, nc = data.shape
     nk = len(taps)
     AA = np.zeros((nc, nk, nc, nk), dtype=np.complex128)
+    BB = np.zeros((nc, nk, nc, nk), dtype=np.complex128) if loraks else None
+    if loraks:
+        fl = np.ix_(flip_index(nx, center[0]), flip_index(ny, center[1]), flip_index(nz, center[2]))
     for j in range(nk):
         row = np.roll(data, tuple(taps[j]), axis=(0, 1, 2)).reshape(-1, nc, order='F')
         for k in range(j, nk):
-            col = np.roll(data, tuple(taps[k]), axis=(0, 1, 2)).reshape(-1, nc, order='F')
+            colv = np.roll(data, tuple(taps[k]), axis=(0, 1, 2))
+            col = colv.reshape(-1, nc, order='F')
             blk = row.conj().T @ col
             AA[:, j, :, k] = blk
             AA[:, k, :, j] = blk.conj().T
-    return AA.reshape(nc * nk, nc * nk, order='F')
+            if loraks:
+                colz = np.conj(colv[fl]).reshape(-1, nc, order='F')
+                blk = row.conj().T @ colz
+                BB[:, j, :, k] = blk
+                BB[:, k, :, j] = blk.T
+    AA = AA.reshape(nc * nk, nc * nk, order='F')
+    if not loraks:
+        return AA
+    BB = BB.reshape(nc * nk, nc * nk, order='F')
+    return np.block([[AA, BB], [BB.conj(), AA.conj()]])
 
 
 def gram_fft(data, taps):
@@ -72,6 +94,27 @@ def gram_fft(data, taps):
                 d = taps[j] - taps       # s_j - s_k for all k
                 AA[c, j, c2, :] = R[d[:, 0] % nx, d[:, 1] % ny, d[:, 2] % nz]
     return AA.reshape(nc * nk, nc * nk, order='F')
+
+
+def gram_fft_loraks(data, taps, center=(1, 1, 1)):
+    """The loraks matrix of make_data_matrix by FFT: with g = conj(data(flip)),
+    BB[(c,j),(d,k)] = sum_u conj(D_c[u]) g_d[u + s_j + s_k]."""
+    nx, ny, nz, nc = data.shape
+    nk = len(taps)
+    fl = np.ix_(flip_index(nx, center[0]), flip_index(ny, center[1]), flip_index(nz, center[2]))
+    g = np.conj(data[fl])
+    F = np.fft.fftn(data, axes=(0, 1, 2))
+    G = np.fft.fftn(g, axes=(0, 1, 2))
+    BB = np.zeros((nc, nk, nc, nk), dtype=np.complex128)
+    for c in range(nc):
+        for d in range(nc):
+            R = np.fft.ifftn(np.conj(F[..., c]) * G[..., d])   # R[e] = sum conj(D_c[u]) g_d[u+e]
+            for j in range(nk):
+                e = taps[j] + taps
+                BB[c, j, d, :] = R[e[:, 0] % nx, e[:, 1] % ny, e[:, 2] % nz]
+    AA = gram_fft(data, taps)
+    BB = BB.reshape(nc * nk, nc * nk, order='F')
+    return np.block([[AA, BB], [BB.conj(), AA.conj()]])
 
 
 def _hist(y, nb):

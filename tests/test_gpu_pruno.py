@@ -79,3 +79,32 @@ def test_inverse_pruno(gpu, tol):
     e = rel(got, want)
     print("pruno inverse gpu=%d rel err %.3e nv=%d" % (gpu, e, pr["nv"]), flush=True)
     assert e < tol
+
+
+@pytest.mark.parametrize("width,radial,center", [(3, 1, None), (4, 1, (2, 3, 1)), (3, 0, (4, 1, 2))])
+def test_gram_loraks_small(plans, width, radial, center):
+    # [AA BB; conj(BB) conj(AA)] of sake.m:210-224 against the literal oracle
+    om, g32, g64 = plans
+    d = _data((8, 6, 5, 2), 5)
+    taps, _ = P.kernel_taps(width, radial)
+    cen = center or (1, 1, 1)
+    want = P.make_data_matrix(d, taps, loraks=True, center=cen)
+    got64 = g64.gram(d, width, radial, loraks=True, center=center)
+    assert got64.shape == want.shape
+    assert rel(got64, want) < 1e-12
+    got = g32.gram(d.astype(np.complex64), width, radial, loraks=True, center=center)
+    assert rel(got, P.make_data_matrix(d.astype(np.complex64).astype(np.complex128), taps,
+                                       loraks=True, center=cen)) < 1e-4
+
+
+def test_gram_loraks_many_tiles(plans):
+    # SAKE's default kernel (width 4, radial: 33 taps) on 4 coils: 2n = 264,
+    # a 528-column real Gram for each block, all BB tiles computed
+    om, g32, g64 = plans
+    d = _data((24, 20, 16, 4), 6).astype(np.complex64)
+    taps, _ = P.kernel_taps(4, 1)
+    want = P.gram_fft_loraks(d.astype(np.complex128), taps, (5, 1, 9))
+    got = g32.gram(d, 4, 1, loraks=True, center=(5, 1, 9))
+    e = rel(got, want)
+    print("bf16x3 loraks gram rel err %.3e" % e, flush=True)
+    assert e < 1e-4

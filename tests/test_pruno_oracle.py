@@ -82,3 +82,51 @@ def test_inufft_pruno_runs_and_reduces_residual():
     x, pr, info = P.inufft_pruno(orc, raw, 3, 0.0, 0.1, width=3, noise=1e-3, iters=10)
     assert x.shape == (N, N, N, 3)
     assert info['relres'][0] < 1.0
+
+
+def test_flip_index_is_reflection_about_centre():
+    for n in (5, 8):
+        assert np.array_equal(P.flip_index(n, 1), (-np.arange(n)) % n)
+        for c in (2, 3):
+            assert np.array_equal(P.flip_index(n, c), (2 * (c - 1) - np.arange(n)) % n)
+
+
+@pytest.mark.parametrize("center", [(1, 1, 1), (2, 3, 1)])
+def test_loraks_block_closed_form_and_gram_structure(center):
+    # BB[(c,j),(d,k)] = sum_v conj(D_c[v - s_j]) conj(D_d[flip(v) - s_k]) by
+    # brute force over v, and the 2n x 2n matrix is the Gram of [A, conj(flip A)]
+    # (Hermitian, positive semi-definite, lower-right block conj(AA))
+    rng = np.random.default_rng(3)
+    D = rng.normal(size=(4, 5, 3, 2)) + 1j * rng.normal(size=(4, 5, 3, 2))
+    taps = P.kernel_taps(3, 1)[0]
+    nk, nc = len(taps), 2
+    n = nk * nc
+    M = P.make_data_matrix(D, taps, loraks=True, center=center)
+    assert M.shape == (2 * n, 2 * n)
+    dims = D.shape[:3]
+    c0 = [ci - 1 for ci in center]
+    BB = np.zeros((n, n), complex)
+    for a in range(n):
+        c, j = a % nc, a // nc
+        for b in range(n):
+            d, k = b % nc, b // nc
+            acc = 0
+            for v in np.ndindex(*dims):
+                p = tuple((v[i] - taps[j][i]) % dims[i] for i in range(3))
+                q = tuple((2 * c0[i] - v[i] - taps[k][i]) % dims[i] for i in range(3))
+                acc += np.conj(D[p + (c,)]) * np.conj(D[q + (d,)])
+            BB[a, b] = acc
+    assert np.allclose(M[:n, n:], BB, atol=1e-12)
+    assert np.allclose(M, M.conj().T, atol=1e-12)
+    assert np.allclose(M[n:, n:], M[:n, :n].conj(), atol=1e-12)
+    assert np.linalg.eigvalsh(M).min() > -1e-9 * np.abs(M).max()
+    assert np.allclose(M[:n, :n], P.make_data_matrix(D, taps), atol=1e-12)
+
+
+@pytest.mark.parametrize("center", [(1, 1, 1), (3, 2, 4)])
+def test_loraks_fft_form_equals_literal_loop(center):
+    rng = np.random.default_rng(4)
+    D = rng.normal(size=(6, 5, 4, 2)) + 1j * rng.normal(size=(6, 5, 4, 2))
+    taps = P.kernel_taps(4, 1)[0]
+    want = P.make_data_matrix(D, taps, loraks=True, center=center)
+    assert np.allclose(P.gram_fft_loraks(D, taps, center), want, atol=1e-10)

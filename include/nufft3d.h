@@ -187,7 +187,8 @@ int nufft3d_inverse_pruno(nufft3d_plan* plan, const void* raw, int32_t nc, int32
 
 /* Autotuner, no counterpart in the reference: times `reps` fwd+adj operator
  * pairs of nc coils for each combination of FFT lines per work-group (auto,
- * 8, 4), coils per gridding launch (16, 8), XCD block remap (off, runs of 32)
+ * 8, 4), coils per gridding launch (16, 8), XCD block remap (off, runs of 32,
+ * one contiguous eighth of the launch per XCD)
  * and spread kernel (gather in linear or brick order, or the LDS-staged brick
  * spread where the plan has brick lists), keeps the fastest on the plan, and
  * writes the timings as JSON to report (may be NULL). Results are unchanged

@@ -32,8 +32,10 @@ struct GridGeom {
 // (runs x, x+8, x+16, ...): the blocks an XCD holds at once cover one compact
 // region of samples / grid points, which share grid rows and samples in its
 // L2, while every XCD still gets a share of the dense k-space centre. Blocks
-// past the last whole round of 8C keep their number.
+// past the last whole round of 8C keep their number. C < 0: C = grid / 8,
+// each XCD walks one contiguous eighth of the (Z-ordered) work.
 __device__ __forceinline__ long long xcd_block(int chunk) {
+  if (chunk < 0) chunk = (int)(gridDim.x / 8);   // one contiguous eighth per XCD
   if (chunk <= 0) return blockIdx.x;
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const unsigned span = 8u * (unsigned)chunk;

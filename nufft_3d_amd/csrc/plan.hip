@@ -1693,7 +1693,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   std::vector<Cand> cands;
   for (int fftb : {0, 8, 4})
     for (int group : {16, 8})
-      for (int xcd : {0, 32})
+      for (int xcd : {0, 32, -1})
         for (int brick : {1, 0, 2}) {
           if (group == 8 && nc <= 8) continue;   // same launches as 16
           if (brick == 2 && !p->bl.d_ent) continue;

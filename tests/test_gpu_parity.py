@@ -275,7 +275,8 @@ def test_autotune_picks_a_measured_setting_and_keeps_results():
     before = p.aNUFT(p.fNUFT(x))
     rep = p.autotune(nc=12, reps=2)
     cands = rep["candidates"]
-    assert len(cands) == 36 and 0 <= rep["best"] < len(cands)
+    assert len(cands) == 54 and 0 <= rep["best"] < len(cands)
+    assert {c["xcd_chunk"] for c in cands} == {0, 32, -1}
     assert {c["spread"] for c in cands} == {"gather", "gather-bricks", "lds-bricks"}
     assert rep["z_threads"] in (256, 512) and min(rep["z_threads_ms"]) > 0
     assert all(c["ms_per_pair"] > 0 for c in cands)

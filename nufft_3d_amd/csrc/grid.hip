@@ -502,23 +502,43 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
   }
 }
 
+// Sum of a dense point's segment partials, one work-group per point: the R
+// = 256/cgp threads of coil c (cgp = cg rounded up to a power of two) each
+// add the slots r, r+R, r+2R, ... in order, then a fixed-order tree over the
+// R partial sums. Deterministic, and a coil's value does not depend on the
+// other coils of the launch.
 template <typename R, bool COMP>
-__global__ void k_spread_hreduce(const HeavyPt* __restrict__ pts, int npt,
-                                 const cplx<R>* __restrict__ partial, cplx<R>* __restrict__ grid,
-                                 GridGeom g) {
+__global__ void __launch_bounds__(256) k_spread_hreduce(const HeavyPt* __restrict__ pts,
+                                                        const cplx<R>* __restrict__ partial,
+                                                        cplx<R>* __restrict__ grid, int cgp,
+                                                        GridGeom g) {
   using A = typename AccT<R, COMP>::type;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)npt * g.cg) return;
-  const int pi = (int)(i / g.cg), c = (int)(i % g.cg);
-  const HeavyPt hp = pts[pi];
+  __shared__ A sre[256], sim[256];
+  const HeavyPt hp = pts[blockIdx.x];
+  const int nr = 256 / cgp;
+  const int c = threadIdx.x / nr, r = threadIdx.x % nr;
   const A* pp = reinterpret_cast<const A*>(partial);
   A re = 0, im = 0;
-  for (int j = 0; j < hp.nslots; ++j) {
-    re += pp[((long long)(hp.first_slot + j) * g.cg + c) * 2];
-    im += pp[((long long)(hp.first_slot + j) * g.cg + c) * 2 + 1];
+  if (c < g.cg)
+    for (int j = r; j < hp.nslots; j += nr) {
+      const long long o = ((long long)(hp.first_slot + j) * g.cg + c) * 2;
+      re += pp[o];
+      im += pp[o + 1];
+    }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int w = nr / 2; w > 0; w >>= 1) {
+    if (r < w) {
+      sre[threadIdx.x] += sre[threadIdx.x + w];
+      sim[threadIdx.x] += sim[threadIdx.x + w];
+    }
+    __syncthreads();
   }
-  cplx<R> o; o.x = (R)re; o.y = (R)im;
-  grid[(long long)hp.g * g.nc + g.c0 + c] = o;
+  if (r == 0 && c < g.cg) {
+    cplx<R> o; o.x = (R)sre[threadIdx.x]; o.y = (R)sim[threadIdx.x];
+    grid[(long long)hp.g * g.nc + g.c0 + c] = o;
+  }
 }
 
 template <typename R>
@@ -629,7 +649,9 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
-  const DenseSet& ds = p->dense[cg <= 4 ? 0 : 1];
+  // one lane per point (small fp32 groups, every fp64 group): short dense
+  // segments; lane groups: long ones
+  const DenseSet& ds = p->dense[LPS == 1 ? 0 : 1];
   constexpr bool kTileable = sizeof(R) == 4 && FULL && LPS * CPL >= 8;
   if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
       p->bl.rmax == ds.rmax) {
@@ -649,21 +671,26 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
                        (cplx<R>*)grid, npts, p->d_brick, ds.rmax, g, kp);
   }
   if (ds.nhseg > 0) {
-    ProfScope ps(p, "k_spread_heavy");
-    const long long nthreads = (long long)ds.nhseg * LPS;
-    hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
-                       dim3(256), 0, p->stream, ds.d_hseg, ds.nhseg, p->d_hit_s, p->d_acode,
-                       (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
-                       (cplx<R>*)partial, g, kp);
-    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>),
-                       dim3((unsigned)(((long long)ds.nhpt * cg + 255) / 256)), dim3(256), 0,
-                       p->stream, ds.d_hpt, ds.nhpt, (const cplx<R>*)partial, (cplx<R>*)grid, g);
+    {
+      ProfScope ps(p, "k_spread_heavy");
+      const long long nthreads = (long long)ds.nhseg * LPS;
+      hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>),
+                         dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, p->stream,
+                         ds.d_hseg, ds.nhseg, p->d_hit_s, p->d_acode, (const double*)p->d_off,
+                         p->d_off32, (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)partial, g,
+                         kp);
+    }
+    ProfScope ps(p, "k_spread_hreduce");
+    int cgp = 1;
+    while (cgp < cg) cgp *= 2;
+    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>), dim3((unsigned)ds.nhpt), dim3(256), 0,
+                       p->stream, ds.d_hpt, (const cplx<R>*)partial, (cplx<R>*)grid, cgp, g);
   }
 }
 
 // coil groups of up to coil_group (16 or 8); fp32 groups of more than 4
 // coils put 2 or 4 lanes on one sample / grid point (4 coils each), fp64
-// keeps one lane
+// groups of more than 2 coils put 4 lanes on it (1, 2 or 4 coils each)
 template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
                        const void* wsamp, double scale, bool user = false) {
@@ -685,9 +712,9 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
     if constexpr (sizeof(R) == 8) {
       if (cg <= 1) N3D_GO(1, 1);
       else if (cg <= 2) N3D_GO(2, 1);
-      else if (cg <= 4) N3D_GO(4, 1);
-      else if (cg <= 8) N3D_GO(8, 1);
-      else N3D_GO(16, 1);
+      else if (cg <= 4) N3D_GO(1, 4);
+      else if (cg <= 8) N3D_GO(2, 4);
+      else N3D_GO(4, 4);
     } else {
       if (cg <= 1) N3D_GO(1, 1);
       else if (cg <= 2) N3D_GO(2, 1);

@@ -161,8 +161,9 @@ struct nufft3d_plan {
   int* d_hit_s = nullptr;           // [nhits] sample ids (grid-major)
   unsigned short* d_acode = nullptr;  // [nhits] codes (grid-major)
   float4* d_off32 = nullptr;        // [nnz] (fx, fy, fz, 0)
-  // dense-point segmentations of the same hit lists: [0] for launches of at
-  // most 4 coils (one lane per point: short walks), [1] for larger groups
+  // dense-point segmentations of the same hit lists: [0] for launches with
+  // one lane per point (fp32 groups of at most 4 coils, every fp64 group:
+  // short walks), [1] for lane groups (fp32 groups of more than 4 coils)
   nufft3d::DenseSet dense[2];
   int* d_brick = nullptr;           // [nbrick] 8x8x8 spread bricks in traversal order (tx|ty<<10|tz<<20)
   long long nbrick = 0;

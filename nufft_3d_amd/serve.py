@@ -18,7 +18,7 @@ of mixed matrix size, coil count and trajectory arrive over time. The engine
   MINRES recurrence per right-hand side with its own stopping test), so a
   request's result is the same alone and in a batch up to rounding (the
   spread groups the sums of dense k-space points in shorter segments for
-  launches of at most 4 coils);
+  fp32 launches of at most 4 coils);
 * overlaps: up to ``slots`` batches on different plans run at once, each on
   its own HIP stream driven by its own host thread (library calls release
   the GIL); a plan serves one batch at a time.

@@ -289,7 +289,9 @@ class nufft_3d:
             if not (np.isscalar(stddev) and np.isfinite(stddev) and stddev >= 0):
                 raise ValueError("stddev must be a finite nonnegative scalar")   # discrep.m:28
         iters = int(iters)
-        hist = np.zeros(max(iters, 1), dtype=np.float64)
+        if iters <= 0:
+            iters = 20                                                          # discrep.m:54
+        hist = np.zeros(iters, dtype=np.float64)
         damp, sd = C.c_double(), C.c_double()
         sarg = -1.0 if stddev is None else float(stddev)
         if _is_torch(data):

@@ -528,7 +528,7 @@ static Stages stages_of(const FftAxis& a) {
 // lines per LDS bundle: L*B elements staged in registers by 256 threads
 // (at most MaxE each) and held in one LDS buffer of at most 64 KiB
 // `want` = requested lines per work-group (0 = automatic), from the plan
-static int bundle_for(int L, size_t esz, int want, int threads = 256, size_t lds = 65536) {
+static int bundle_for(int L, size_t esz, int want, int threads, size_t lds) {
   int B = threads * (esz == sizeof(double2) ? MaxE<double>::v : MaxE<float>::v) / L;
   const int bl = (int)(lds / ((size_t)L * esz)) - 1;   // the L twiddles share the buffer
   if (bl < B) B = bl;
@@ -545,16 +545,18 @@ static int bundle_for(int L, size_t esz, int want, int threads = 256, size_t lds
   return 1;
 }
 
-// threads per z-pass work-group: 512 (bundles of 32 lines, 256-byte row
-// pieces) for axes with a compiled stage plan, where it measured faster;
-// 256 for the generic stage loop
-static int z_threads(const nufft3d_plan* p) {
-  if (p->fft_zthreads == 256 || p->fft_zthreads == 512) return p->fft_zthreads;
-  return p->ax[2].plan_id != 0 ? 512 : 256;
-}
-
 // dynamic LDS allowed to the z-pass work-groups (above the 64 KiB default)
 constexpr size_t kStridedLds = 80 * 1024;
+
+// threads per z-pass work-group: 512 where that doubles the bundle to 32
+// lines (256-byte row pieces) on an axis with a compiled stage plan, where
+// it measured faster (K=256: 3.3 -> 4.2 TB/s); 256 otherwise (generic stage
+// loop, or lines too long for a 32-line bundle)
+static int z_threads(const nufft3d_plan* p, size_t esz) {
+  if (p->fft_zthreads == 256 || p->fft_zthreads == 512) return p->fft_zthreads;
+  if (p->ax[2].plan_id == 0) return 256;
+  return bundle_for(p->K[2], esz, p->fft_bundle, 512, kStridedLds) >= 32 ? 512 : 256;
+}
 
 // raise a kernel's dynamic LDS limit once when a launch needs more than 64 KiB
 template <typename K>
@@ -568,7 +570,7 @@ static void allow_lds(K* kernel, size_t bytes) {
 }
 
 static int bundle_x(int L, size_t esz, int nc, int want) {
-  const int Bmax = bundle_for(L, esz, want);
+  const int Bmax = bundle_for(L, esz, want, 256, 65536);
   if (nc <= Bmax) return nc * (Bmax / nc);
   for (int B = Bmax; B >= 1; --B)
     if (nc % B == 0) return B;
@@ -607,7 +609,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   {  // y-pass: [N3][N2][K1*nc] -> [N3][K2][K1*nc]
     SGeom g;
     g.Lin = N2; g.Lout = K2; g.n_img = N2; g.mode = 0;
-    g.B = bundle_for(K2, esz, p->fft_bundle);
+    g.B = bundle_for(K2, esz, p->fft_bundle, 256, 65536);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
@@ -621,7 +623,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   if (K3 > 1) {  // z-pass: [N3][K2*K1*nc] -> [K3][K2*K1*nc]
     SGeom g;
     g.Lin = N3; g.Lout = K3; g.n_img = N3; g.mode = 0;
-    const int zt = z_threads(p);
+    const int zt = z_threads(p, esz);
     g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
@@ -652,7 +654,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     N3D_TRY(ensure(p->ws_t2, t2 * esz));
     SGeom g;
     g.Lin = K3; g.Lout = N3; g.n_img = N3; g.mode = 1;
-    const int zt = z_threads(p);
+    const int zt = z_threads(p, esz);
     g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
@@ -669,7 +671,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   {  // y-pass: [N3][K2][K1*nc] -> [N3][N2][K1*nc]
     SGeom g;
     g.Lin = K2; g.Lout = N2; g.n_img = N2; g.mode = 1;
-    g.B = bundle_for(K2, esz, p->fft_bundle);
+    g.B = bundle_for(K2, esz, p->fft_bundle, 256, 65536);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
@@ -735,7 +737,7 @@ static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
     if (ps.n <= 1) continue;
     SGeom g;
     g.Lin = ps.n; g.Lout = ps.n; g.n_img = ps.n; g.mode = 2;
-    g.B = ps.inner >= 16 ? bundle_for(ps.n, esz, p->fft_bundle) : 1;
+    g.B = ps.inner >= 16 ? bundle_for(ps.n, esz, p->fft_bundle, 256, 65536) : 1;
     g.inner = ps.inner;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)ps.n * (g.B + 1) * esz;

@@ -503,10 +503,10 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
 }
 
 // Sum of a dense point's segment partials, one work-group per point: the R
-// = 256/cgp threads of coil c (cgp = cg rounded up to a power of two) each
-// add the slots r, r+R, r+2R, ... in order, then a fixed-order tree over the
-// R partial sums. Deterministic, and a coil's value does not depend on the
-// other coils of the launch.
+// = blockDim/cgp threads of coil c (cgp = cg rounded up to a power of two)
+// each add the slots r, r+R, r+2R, ... in order, then a fixed-order tree over
+// the R partial sums. Deterministic, and a coil's value does not depend on
+// the other coils of the launch (R = 32, or 16 for 16-coil groups).
 template <typename R, bool COMP>
 __global__ void __launch_bounds__(256) k_spread_hreduce(const HeavyPt* __restrict__ pts,
                                                         const cplx<R>* __restrict__ partial,
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256) k_spread_hreduce(const HeavyPt* __restric
   using A = typename AccT<R, COMP>::type;
   __shared__ A sre[256], sim[256];
   const HeavyPt hp = pts[blockIdx.x];
-  const int nr = 256 / cgp;
+  const int nr = blockDim.x / cgp;
   const int c = threadIdx.x / nr, r = threadIdx.x % nr;
   const A* pp = reinterpret_cast<const A*>(partial);
   A re = 0, im = 0;
@@ -683,7 +683,8 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
     ProfScope ps(p, "k_spread_hreduce");
     int cgp = 1;
     while (cgp < cg) cgp *= 2;
-    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>), dim3((unsigned)ds.nhpt), dim3(256), 0,
+    const int nthr = cgp * 32 < 256 ? cgp * 32 : 256;
+    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>), dim3((unsigned)ds.nhpt), dim3(nthr), 0,
                        p->stream, ds.d_hpt, (const cplx<R>*)partial, (cplx<R>*)grid, cgp, g);
   }
 }

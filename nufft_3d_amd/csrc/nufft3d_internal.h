@@ -168,6 +168,8 @@ struct nufft3d_plan {
   int* d_brick = nullptr;           // [nbrick] 8x8x8 spread bricks in traversal order (tx|ty<<10|tz<<20)
   long long nbrick = 0;
   nufft3d::BrickLists bl;           // LDS-staged spread lists over the same bricks
+  std::vector<int> h_brick_of;      // brick (x,y,z) -> traversal index, for building bl
+  bool bl_tried = false;
   int max_hits = 0;
 
   nufft3d::FftAxis ax[3];

@@ -668,6 +668,15 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
   return NUFFT3D_OK;
 }
 
+// BrickLists on first need (their sort is ~15 ms and 4 B per hit at
+// 3M samples)
+static int ensure_brick_lists(nufft3d_plan* p) {
+  if (p->bl.d_ent || p->bl_tried || p->h_brick_of.empty()) return NUFFT3D_OK;
+  p->bl_tried = true;
+  const int bx = (p->K[0] + 7) / 8, by = (p->K[1] + 7) / 8;
+  return build_brick_lists(p, p->h_brick_of, bx, by, p->dense[1].rmax);
+}
+
 static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   const bool dev = is_device_ptr(om_in);
   double* om = nullptr;
@@ -904,7 +913,10 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     p->nbrick = (long long)tab.size();
     N3D_HIP(hipMalloc(&p->d_brick, sizeof(int) * tab.size()));
     N3D_HIP(hipMemcpy(p->d_brick, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
-    N3D_TRY(build_brick_lists(p, brick_of, bx, by, p->dense[1].rmax));
+    p->h_brick_of.swap(brick_of);
+    // the LDS-staged spread's lists are built when that spread is asked for
+    // (tune flag 8) or timed by the autotuner
+    if (p->tune_flags & 8) N3D_TRY(ensure_brick_lists(p));
   }
 
   // deapodization (deap.m) and FFT twiddles
@@ -1687,6 +1699,8 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     N3D_TRY(spread(p, ks.p, nc, p->ws_grid.p, 0, nullptr));
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
+  if (!st) st = ensure_brick_lists(p);
+  if (st) { release(xin); release(xout); release(ks); return st; }
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
   // spread (plans with brick lists)
   struct Cand { int fftb, group, xcd, brick; float ms; };

@@ -576,6 +576,45 @@ __global__ void k_permute_in(const int* __restrict__ orig, int nnz, long long M,
   ks[i] = v;
 }
 
+// The same gather through LDS, one work-group per 64 sorted samples: coil by
+// coil, 64 lanes read the rows orig[s] of one [M] plane (runs of consecutive
+// rows within a bin share cache lines), then the 64 x nc tile is written as
+// 64 contiguous sample rows. nc is taken in chunks of at most 32 coils.
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_in_tile(const int* __restrict__ orig, int nnz,
+                                                         long long M, int nc,
+                                                         const cplx<R>* __restrict__ ku,
+                                                         const R* __restrict__ wsamp,
+                                                         cplx<R>* __restrict__ ks) {
+  constexpr int TS = 64, CC = 32;
+  __shared__ cplx<R> t[TS][CC + 1];
+  __shared__ int so[TS];
+  __shared__ R sw[TS];
+  const long long s0 = (long long)blockIdx.x * TS;
+  const int ns = nnz - s0 < TS ? (int)(nnz - s0) : TS;
+  if (threadIdx.x < ns) {
+    so[threadIdx.x] = orig[s0 + threadIdx.x];
+    sw[threadIdx.x] = wsamp ? wsamp[s0 + threadIdx.x] : (R)1;
+  }
+  for (int c0 = 0; c0 < nc; c0 += CC) {
+    const int cc = nc - c0 < CC ? nc - c0 : CC;
+    __syncthreads();   // so/sw written; the previous chunk's tile is consumed
+    for (int i = threadIdx.x; i < TS * cc; i += blockDim.x) {
+      const int c = i / TS, j = i % TS;
+      if (j < ns) {
+        cplx<R> v = ku[so[j] + (long long)(c0 + c) * M];
+        if (wsamp) { v.x *= sw[j]; v.y *= sw[j]; }
+        t[j][c] = v;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ns * cc; i += blockDim.x) {
+      const int j = i / cc, c = i % cc;
+      ks[(s0 + j) * nc + c0 + c] = t[j][c];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 static KBParams make_kb(const nufft3d_plan* p) {
   KBParams k;
@@ -822,8 +861,21 @@ int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user) {
 int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp) {
   const long long n = p->nnz * (long long)nc;
   if (n == 0) return NUFFT3D_OK;
-  const int nb = (int)((n + 255) / 256);
   ProfScope ps(p, "k_permute_in");
+  if (nc >= 4) {   // tiled: coalesced rows on both sides
+    const unsigned nb = (unsigned)((p->nnz + 63) / 64);
+    if (p->prec == 2)
+      hipLaunchKernelGGL((k_permute_in_tile<double>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
+                         (int)p->nnz, (long long)p->M, nc, (const double2*)k_user,
+                         (const double*)wsamp, (double2*)ks);
+    else
+      hipLaunchKernelGGL((k_permute_in_tile<float>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
+                         (int)p->nnz, (long long)p->M, nc, (const float2*)k_user,
+                         (const float*)wsamp, (float2*)ks);
+    N3D_HIP(hipGetLastError());
+    return NUFFT3D_OK;
+  }
+  const int nb = (int)((n + 255) / 256);
   if (p->prec == 2)
     hipLaunchKernelGGL((k_permute_in<double>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
                        (int)p->nnz, (long long)p->M, nc, (const double2*)k_user,

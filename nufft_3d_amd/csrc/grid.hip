@@ -267,8 +267,7 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
 // round trip (its sample values) instead of three dependent ones.
 template <typename R, int CPL, int LPS, bool FULL, typename A>
 __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
-                                            const int* __restrict__ hit_s,
-                                            const unsigned short* __restrict__ acode,
+                                            const uint2* __restrict__ hits,
                                             const double* __restrict__ off,
                                             const float4* __restrict__ off32,
                                             const cplx<R>* __restrict__ ks,
@@ -277,14 +276,14 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
   if (e0 >= e1) return;
   // lane q's entry of the step starting at e (the step's first entry past the end)
   auto ent = [&](long long e) { return e + q < e1 ? e + q : e; };
-  int s0 = hit_s[ent(e0)];
-  unsigned short c0 = acode[ent(e0)];
+  uint2 h0 = hits[ent(e0)];
+  int s0 = (int)h0.x;
+  unsigned short c0 = (unsigned short)h0.y;
   float4 f0;
   if constexpr (sizeof(R) == 4) f0 = off32[s0];
   R m0 = wsamp ? wsamp[s0] : (R)1;
   const long long e1st = e0 + LPS < e1 ? e0 + LPS : e0;
-  int s1 = hit_s[ent(e1st)];
-  unsigned short c1 = acode[ent(e1st)];
+  uint2 h1 = hits[ent(e1st)];
   for (long long e = e0; e < e1; e += LPS) {
     const int s = s0;
     int dx, dy, dz;
@@ -311,12 +310,11 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
     {
       const long long en = e + LPS < e1 ? e + LPS : e;
       const long long enn = e + 2 * LPS < e1 ? e + 2 * LPS : en;
-      s0 = s1;
-      c0 = c1;
+      s0 = (int)h1.x;
+      c0 = (unsigned short)h1.y;
       if constexpr (sizeof(R) == 4) f0 = off32[s0];
       m0 = wsamp ? wsamp[s0] : (R)1;
-      s1 = hit_s[ent(enn)];
-      c1 = acode[ent(enn)];
+      h1 = hits[ent(enn)];
     }
 #pragma unroll
     for (int j = 0; j < LPS; ++j) fma_coils<R, CPL>(ar, ai, wj[j], val[j]);
@@ -334,8 +332,7 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
 // grouped the same way within either size class.
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
-                                                const int* __restrict__ hit_s,
-                                                const unsigned short* __restrict__ acode,
+                                                const uint2* __restrict__ hits,
                                                 const double* __restrict__ off,
                                                 const float4* __restrict__ off32,
                                                 const cplx<R>* __restrict__ ks,
@@ -362,7 +359,7 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hit_s, acode, off, off32, ks, wsamp, g, kp, ar, ai);
+  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hits, off, off32, ks, wsamp, g, kp, ar, ai);
   cplx<R>* gp = grid + gpt * g.nc + g.c0;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
@@ -474,8 +471,7 @@ __global__ void __launch_bounds__(512) k_spread_tile(const long long* __restrict
 // One segment of a dense point's hit list -> partial[slot][cg]
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
-                                                      const int* __restrict__ hit_s,
-                                                      const unsigned short* __restrict__ acode,
+                                                      const uint2* __restrict__ hits,
                                                       const double* __restrict__ off,
                                                       const float4* __restrict__ off32,
                                                       const cplx<R>* __restrict__ ks,
@@ -491,7 +487,7 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hit_s, acode, off, off32, ks, wsamp, g, kp,
+  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, off, off32, ks, wsamp, g, kp,
                                  ar, ai);
   // partial sums keep the accumulator precision until the final reduction
   A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
@@ -705,7 +701,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
     const long long nthreads = npts * LPS;
     ProfScope ps(p, "k_spread");
     hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
-                       dim3(256), 0, p->stream, p->d_rowptr, p->d_hit_s, p->d_acode,
+                       dim3(256), 0, p->stream, p->d_rowptr, p->d_hit,
                        (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
                        (cplx<R>*)grid, npts, p->d_brick, ds.rmax, g, kp);
   }
@@ -715,7 +711,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
       const long long nthreads = (long long)ds.nhseg * LPS;
       hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>),
                          dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, p->stream,
-                         ds.d_hseg, ds.nhseg, p->d_hit_s, p->d_acode, (const double*)p->d_off,
+                         ds.d_hseg, ds.nhseg, p->d_hit, (const double*)p->d_off,
                          p->d_off32, (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)partial, g,
                          kp);
     }

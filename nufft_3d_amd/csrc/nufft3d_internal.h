@@ -158,8 +158,8 @@ struct nufft3d_plan {
   long long* d_spos = nullptr;      // [nnz + 1]
   unsigned short* d_fcode = nullptr;  // [nhits] forward codes (sample-major)
   long long* d_rowptr = nullptr;    // [K1*K2*K3 + 1]
-  int* d_hit_s = nullptr;           // [nhits] sample ids (grid-major)
-  unsigned short* d_acode = nullptr;  // [nhits] codes (grid-major)
+  uint2* d_hit = nullptr;           // [nhits] (sample id, code) of each hit (grid-major):
+                                    // one 8-byte load per hit in the spread
   float4* d_off32 = nullptr;        // [nnz] (fx, fy, fz, 0)
   // dense-point segmentations of the same hit lists: [0] for launches with
   // one lane per point (fp32 groups of at most 4 coils, every fp64 group:

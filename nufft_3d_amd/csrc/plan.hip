@@ -345,17 +345,16 @@ __global__ void k_hits_fill(const int4* __restrict__ base, const double* __restr
 }
 
 __global__ void k_split_vals(const unsigned long long* __restrict__ vals, long long n,
-                             int* __restrict__ hit_s, unsigned short* __restrict__ acode) {
+                             uint2* __restrict__ hits) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned long long v = vals[i];
-  hit_s[i] = (int)(v >> 16);
-  acode[i] = (unsigned short)(v & 0xffffu);
+  hits[i] = make_uint2((unsigned)(v >> 16), (unsigned)(v & 0xffffu));
 }
 
 // brick lists (BrickLists): key (brick << 32 | sample) per hit of a point
 // with at most rmax hits, all ones for the hits of denser points
-__global__ void k_bl_keys(const long long* __restrict__ rowptr, const int* __restrict__ hit_s,
+__global__ void k_bl_keys(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
                           long long ngrid, int K1, int K2, const int* __restrict__ brick_of,
                           int bx, int by, int rmax, unsigned long long* __restrict__ keys) {
   const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,7 +365,7 @@ __global__ void k_bl_keys(const long long* __restrict__ rowptr, const int* __res
       (unsigned long long)brick_of[((long long)(z >> 3) * by + (y >> 3)) * bx + (x >> 3)];
   const bool dense = e1 - e0 > rmax;
   for (long long e = e0; e < e1; ++e)
-    keys[e] = dense ? ~0ull : (b << 32) | (unsigned)hit_s[e];
+    keys[e] = dense ? ~0ull : (b << 32) | hits[e].x;
 }
 
 __global__ void k_bl_samp(const unsigned long long* __restrict__ ukeys, long long n,
@@ -394,8 +393,8 @@ __global__ void k_bl_ptr(const unsigned long long* __restrict__ ukeys, long long
 }
 
 // per hit of a light point: (slot of its sample in the brick list) << 15 | code
-__global__ void k_bl_ent(const long long* __restrict__ rowptr, const int* __restrict__ hit_s,
-                         const unsigned short* __restrict__ acode, long long ngrid, int K1, int K2,
+__global__ void k_bl_ent(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
+                         long long ngrid, int K1, int K2,
                          const int* __restrict__ brick_of, int bx, int by, int rmax,
                          const unsigned long long* __restrict__ ukeys,
                          const long long* __restrict__ ptr, unsigned* __restrict__ ent) {
@@ -408,9 +407,10 @@ __global__ void k_bl_ent(const long long* __restrict__ rowptr, const int* __rest
   const long long base = ptr[b], len = ptr[b + 1] - base;
   long long lo = 0;   // hits are in ascending sample order: search from the last slot
   for (long long e = e0; e < e1; ++e) {
-    const unsigned long long key = ((unsigned long long)b << 32) | (unsigned)hit_s[e];
+    const uint2 h = hits[e];
+    const unsigned long long key = ((unsigned long long)b << 32) | h.x;
     lo += lower_bound_u64(ukeys + base + lo, len - lo, key);
-    ent[e] = ((unsigned)lo << 15) | (unsigned)acode[e];
+    ent[e] = ((unsigned)lo << 15) | h.y;
   }
 }
 
@@ -616,7 +616,7 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
     return fail("allocation failed");
   N3D_HIP(hipMemcpy(d_bof, brick_of.data(), sizeof(int) * brick_of.size(), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_bl_keys, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
-                     p->d_hit_s, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax, keys);
+                     p->d_hit, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax, keys);
   size_t tb1 = 0, tb2 = 0;
   N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nh, 0, 32 + bbits,
                                             p->stream));
@@ -657,7 +657,7 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
   if (hipMalloc(&bl.d_ent, sizeof(unsigned) * nh) != hipSuccess) return fail("allocation failed");
   N3D_HIP(hipMemsetAsync(bl.d_ent, 0, sizeof(unsigned) * nh, p->stream));
   hipLaunchKernelGGL(k_bl_ent, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
-                     p->d_hit_s, p->d_acode, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax,
+                     p->d_hit, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax,
                      (const unsigned long long*)keys, (const long long*)bl.d_ptr, bl.d_ent);
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));
@@ -853,8 +853,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   unsigned *hk = nullptr, *hks = nullptr;
   unsigned long long *hv = nullptr, *hvs = nullptr;
   N3D_HIP(hipMalloc(&p->d_fcode, sizeof(unsigned short) * nh));
-  N3D_HIP(hipMalloc(&p->d_hit_s, sizeof(int) * nh));
-  N3D_HIP(hipMalloc(&p->d_acode, sizeof(unsigned short) * nh));
+  N3D_HIP(hipMalloc(&p->d_hit, sizeof(uint2) * nh));
   N3D_HIP(hipMalloc(&hk, sizeof(unsigned) * nh));
   N3D_HIP(hipMalloc(&hks, sizeof(unsigned) * nh));
   N3D_HIP(hipMalloc(&hv, sizeof(unsigned long long) * nh));
@@ -873,7 +872,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hk, hks, hv, hvs, nhits, 0, key_bits,
                                                p->stream));
     hipLaunchKernelGGL(k_split_vals, dim3(nblocks(nhits)), dim3(256), 0, p->stream,
-                       (const unsigned long long*)hvs, nhits, p->d_hit_s, p->d_acode);
+                       (const unsigned long long*)hvs, nhits, p->d_hit);
   }
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(tmp); (void)hipFree(hk); (void)hipFree(hks); (void)hipFree(hv); (void)hipFree(hvs);
@@ -940,7 +939,7 @@ static void destroy_plan(nufft3d_plan* p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
-                  p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit_s, p->d_acode,
+                  p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit,
                   p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
   for (void* q : ptrs)
@@ -1239,7 +1238,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   int64_t b = 4 * nnz + 4 * noob + 16 * nnz + 24 * nnz;              // orig, oob, base, off
   b += rsz * nimg + rsz * nnz;                                        // U, density
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
-  b += (2 + 4 + 2) * std::max<int64_t>(p->nhits, 1);                  // fcode, hit_s, acode
+  b += (2 + 8) * std::max<int64_t>(p->nhits, 1);                      // fcode, (sample, code) hits
   if (p->d_off32) b += 16 * nnz;
   for (const DenseSet& ds : p->dense)
     b += (int64_t)sizeof(HeavySeg) * std::max(1, ds.nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, ds.nhpt);

@@ -190,7 +190,9 @@ int nufft3d_inverse_pruno(nufft3d_plan* plan, const void* raw, int32_t nc, int32
  * 8, 4), coils per gridding launch (16, 8), XCD block remap (off, runs of 32,
  * one contiguous eighth of the launch per XCD)
  * and spread kernel (gather in linear or brick order, or the LDS-staged brick
- * spread where the plan has brick lists), keeps the fastest on the plan, and
+ * spread where the plan has brick lists), then on the fastest of those the
+ * z-pass work-group size (256 threads, or the automatic 1024 forward / 512
+ * adjoint for planned axes), keeps the fastest on the plan, and
  * writes the timings as JSON to report (may be NULL). Results are unchanged
  * by every choice except the coil grouping, which can move the dense-point
  * segmentation class (rounding). */

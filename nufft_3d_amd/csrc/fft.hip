@@ -413,7 +413,7 @@ struct SGeom {
 };
 
 template <typename R, int PL>
-__global__ void __launch_bounds__(512) k_fft_strided(const cplx<R>* __restrict__ in,
+__global__ void __launch_bounds__(1024) k_fft_strided(const cplx<R>* __restrict__ in,
                                                      cplx<R>* __restrict__ out, SGeom g,
                                                      Stages st, const cplx<R>* __restrict__ tw,
                                                      int sign) {
@@ -537,6 +537,7 @@ static int bundle_for(int L, size_t esz, int want, int threads, size_t lds) {
     while (b * 2 <= want) b *= 2;
     return b;
   }
+  if (B >= 64 && threads > 512) return 64;
   if (B >= 32 && threads > 256) return 32;
   if (B >= 16) return 16;
   if (B >= 8) return 8;
@@ -546,15 +547,19 @@ static int bundle_for(int L, size_t esz, int want, int threads, size_t lds) {
 }
 
 // dynamic LDS allowed to the z-pass work-groups (above the 64 KiB default)
-constexpr size_t kStridedLds = 80 * 1024;
+constexpr size_t kStridedLds = 160 * 1024;
 
-// threads per z-pass work-group: 512 where that doubles the bundle to 32
-// lines (256-byte row pieces) on an axis with a compiled stage plan, where
-// it measured faster (K=256: 3.3 -> 4.2 TB/s); 256 otherwise (generic stage
-// loop, or lines too long for a 32-line bundle)
-static int z_threads(const nufft3d_plan* p, size_t esz) {
-  if (p->fft_zthreads == 256 || p->fft_zthreads == 512) return p->fft_zthreads;
+// threads per z-pass work-group (planned axes, measured at K=256, 16
+// coils): the forward pass reads half the rows it writes and runs best with
+// 1024 threads and 64-line bundles (512-byte row pieces: 4.1 -> 4.8 TB/s),
+// the adjoint with 512 threads and 32-line bundles (4.3 TB/s; 1024: 3.9).
+// Fewer where the lines are too long for those bundles; 256 for the generic
+// stage loop. fft_zthreads overrides both (256, 512 or 1024).
+static int z_threads(const nufft3d_plan* p, size_t esz, bool fwd) {
+  if (p->fft_zthreads == 256 || p->fft_zthreads == 512 || p->fft_zthreads == 1024)
+    return p->fft_zthreads;
   if (p->ax[2].plan_id == 0) return 256;
+  if (fwd && bundle_for(p->K[2], esz, p->fft_bundle, 1024, kStridedLds) >= 64) return 1024;
   return bundle_for(p->K[2], esz, p->fft_bundle, 512, kStridedLds) >= 32 ? 512 : 256;
 }
 
@@ -623,7 +628,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   if (K3 > 1) {  // z-pass: [N3][K2*K1*nc] -> [K3][K2*K1*nc]
     SGeom g;
     g.Lin = N3; g.Lout = K3; g.n_img = N3; g.mode = 0;
-    const int zt = z_threads(p, esz);
+    const int zt = z_threads(p, esz, true);
     g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
@@ -654,7 +659,7 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     N3D_TRY(ensure(p->ws_t2, t2 * esz));
     SGeom g;
     g.Lin = K3; g.Lout = N3; g.n_img = N3; g.mode = 1;
-    const int zt = z_threads(p, esz);
+    const int zt = z_threads(p, esz, false);
     g.B = bundle_for(K3, esz, p->fft_bundle, zt, kStridedLds);
     g.inner = (long long)K2 * K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;

@@ -1749,13 +1749,14 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   } else {
     p->tune_flags = keep_flags;
   }
-  // z-pass work-group size, timed on the best setting: 256 and 512 threads
+  // z-pass work-group size, timed on the best setting: 256 threads in both
+  // directions, and the automatic per-direction choice (0)
   float zt_ms[2] = {0.f, 0.f};
   int zt_best = p->fft_zthreads;
   if (!st && best >= 0 && p->K[2] > 1) {
     const int keep_zt = p->fft_zthreads;
     for (int zi = 0; zi < 2 && !st; ++zi) {
-      p->fft_zthreads = zi ? 512 : 256;
+      p->fft_zthreads = zi ? 0 : 256;
       st = pair();
       if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
       for (int r = 0; r < reps && !st; ++r) st = pair();
@@ -1765,7 +1766,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
       if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
       zt_ms[zi] = ms / reps;
     }
-    zt_best = st ? keep_zt : (zt_ms[1] < zt_ms[0] ? 512 : 256);
+    zt_best = st ? keep_zt : (zt_ms[1] < zt_ms[0] ? 0 : 256);
     p->fft_zthreads = zt_best;
   }
   if (st == NUFFT3D_ERR_DEVICE) { (void)hipGetLastError(); set_error("device error while autotuning"); }

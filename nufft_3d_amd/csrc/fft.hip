@@ -325,7 +325,7 @@ struct XGeom {
 // Forward: image rows (x fastest) are staged through LDS with coalesced
 // reads; the last stage writes [yz][kx][c] directly (lines = coils fastest).
 template <typename R, int PL>
-__global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ img,
+__global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<R>* __restrict__ img,
                                                    const R* __restrict__ U,
                                                    cplx<R>* __restrict__ out, XGeom g, Stages st,
                                                    const cplx<R>* __restrict__ tw) {
@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const cplx<R>* __restrict__ i
 // out of LDS with coalesced image-row writes (x fastest), with U, the scale
 // and the optional add term.
 template <typename R, int PL>
-__global__ void __launch_bounds__(256) k_fft_x_adj(const cplx<R>* __restrict__ in,
+__global__ void __launch_bounds__(512) k_fft_x_adj(const cplx<R>* __restrict__ in,
                                                    const R* __restrict__ U,
                                                    cplx<R>* __restrict__ img, XGeom g, Stages st,
                                                    const cplx<R>* __restrict__ tw, R scale,
@@ -574,8 +574,17 @@ static void allow_lds(K* kernel, size_t bytes) {
   (void)hipGetLastError();
 }
 
-static int bundle_x(int L, size_t esz, int nc, int want) {
-  const int Bmax = bundle_for(L, esz, want, 256, 65536);
+// threads per x-pass work-group: 512 on axes with a compiled stage plan
+// (K=256, 16 coils: 3.3 -> 3.6 TB/s forward, 3.5 -> 3.9 adjoint, the same
+// bundles served by twice the threads); 256 for the generic stage loop
+// (slower with 512). fft_xthreads overrides (256 or 512).
+static int x_threads(const nufft3d_plan* p) {
+  if (p->fft_xthreads == 256 || p->fft_xthreads == 512) return p->fft_xthreads;
+  return p->ax[0].plan_id != 0 ? 512 : 256;
+}
+
+static int bundle_x(int L, size_t esz, int nc, int want, int threads = 256) {
+  const int Bmax = bundle_for(L, esz, want, threads, 65536);
   if (nc <= Bmax) return nc * (Bmax / nc);
   for (int B = Bmax; B >= 1; --B)
     if (nc % B == 0) return B;
@@ -598,13 +607,14 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   {  // x-pass
     XGeom g;
     g.N1 = N1; g.K1 = K1; g.nc = nc;
-    g.B = bundle_x(K1, esz, nc, p->fft_bundle);
+    const int xt = x_threads(p);
+    g.B = bundle_x(K1, esz, nc, p->fft_bundle, xt);
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_fwd");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_fwd<R, PL>), dim3((unsigned)nb), dim3(256), sh,   \
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_fwd<R, PL>), dim3((unsigned)nb), dim3(xt), sh,    \
                                     p->stream, (const cplx<R>*)img, (const R*)p->d_U,           \
                                     (cplx<R>*)T1, g, stages_of(p->ax[0]),                       \
                                     (const cplx<R>*)p->ax[0].d_tw)
@@ -614,13 +624,14 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
   {  // y-pass: [N3][N2][K1*nc] -> [N3][K2][K1*nc]
     SGeom g;
     g.Lin = N2; g.Lout = K2; g.n_img = N2; g.mode = 0;
-    g.B = bundle_for(K2, esz, p->fft_bundle, 256, 65536);
+    const int yt = p->fft_ythreads == 512 ? 512 : 256;
+    g.B = bundle_for(K2, esz, p->fft_bundle, yt, 65536);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_fwd");
 #define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * N3)),  \
-                                    dim3(256), sh, p->stream, (const cplx<R>*)T1, (cplx<R>*)T2, \
+                                    dim3(yt), sh, p->stream, (const cplx<R>*)T1, (cplx<R>*)T2,  \
                                     g, stages_of(p->ax[1]), (const cplx<R>*)p->ax[1].d_tw, -1)
     N3D_PLAN_SWITCH(p->ax[1].plan_id, N3D_L);
 #undef N3D_L
@@ -676,13 +687,14 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   {  // y-pass: [N3][K2][K1*nc] -> [N3][N2][K1*nc]
     SGeom g;
     g.Lin = K2; g.Lout = N2; g.n_img = N2; g.mode = 1;
-    g.B = bundle_for(K2, esz, p->fft_bundle, 256, 65536);
+    const int yt = p->fft_ythreads == 512 ? 512 : 256;
+    g.B = bundle_for(K2, esz, p->fft_bundle, yt, 65536);
     g.inner = (long long)K1 * nc;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)K2 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_y_adj");
 #define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * N3)),  \
-                                    dim3(256), sh, p->stream, (const cplx<R>*)T2, (cplx<R>*)T1, \
+                                    dim3(yt), sh, p->stream, (const cplx<R>*)T2, (cplx<R>*)T1,  \
                                     g, stages_of(p->ax[1]), (const cplx<R>*)p->ax[1].d_tw, +1)
     N3D_PLAN_SWITCH(p->ax[1].plan_id, N3D_L);
 #undef N3D_L
@@ -690,13 +702,14 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
   {  // x-pass with crop, U, scale and the optional add term
     XGeom g;
     g.N1 = N1; g.K1 = K1; g.nc = nc;
-    g.B = bundle_x(K1, esz, nc, p->fft_bundle);
+    const int xt = x_threads(p);
+    g.B = bundle_x(K1, esz, nc, p->fft_bundle, xt);
     g.nyz = (long long)N2 * N3;
     g.nlines = g.nyz * nc;
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_adj");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_adj<R, PL>), dim3((unsigned)nb), dim3(256), sh,   \
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_adj<R, PL>), dim3((unsigned)nb), dim3(xt), sh,    \
                                     p->stream, (const cplx<R>*)T1, (const R*)p->d_U,           \
                                     (cplx<R>*)img, g, stages_of(p->ax[0]),                     \
                                     (const cplx<R>*)p->ax[0].d_tw, (R)scale,                   \

@@ -131,6 +131,8 @@ struct nufft3d_plan {
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
+  int fft_xthreads = 0;     // threads per x-pass FFT work-group (256, 512; 0 = automatic)
+  int fft_ythreads = 256;   // threads per y-pass FFT work-group (256 or 512)
   int fft_zthreads = 0;     // threads per z-pass FFT work-group (256, 512; 0 = automatic)
 
   // sorted, in-bounds samples

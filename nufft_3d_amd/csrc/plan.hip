@@ -1162,6 +1162,8 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_TUNE")) p->tune_flags = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FFTB")) p->fft_bundle = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FFTZT")) p->fft_zthreads = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_FFTXT")) p->fft_xthreads = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_FFTYT")) p->fft_ythreads = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_GRAPH")) p->use_graphs = atoi(tv) != 0;
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;

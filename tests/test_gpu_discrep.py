@@ -84,3 +84,16 @@ def test_discrep_all_zero_data_without_stddev_is_an_error(setup):
     g = nufft_3d(om, N, J=4, u=2, radial=1, gpu=1)
     with pytest.raises(Exception, match="no nonzero entries"):
         g.discrep(np.zeros_like(data), iters=1)
+
+
+def test_discrep_accepts_readout_by_spoke_layout(setup):
+    # data as [nr ny nc] with nr*ny = M (discrep.m:14-20) gives the same damp
+    om, orc, data = setup
+    g = nufft_3d(om, N, J=4, u=2, radial=1, gpu=2)
+    nr = 2 * N
+    d3 = np.reshape(data, (nr, om.shape[1] // nr, data.shape[1]), order="F")
+    a = g.discrep(data, stddev=0.02, iters=3)
+    b = g.discrep(d3, stddev=0.02, iters=3)
+    assert a == b
+    with pytest.raises(ValueError):
+        g.discrep(data[:-1], stddev=0.02, iters=1)

@@ -108,3 +108,10 @@ def test_gram_loraks_many_tiles(plans):
     e = rel(got, want)
     print("bf16x3 loraks gram rel err %.3e" % e, flush=True)
     assert e < 1e-4
+
+
+def test_gram_loraks_rejects_a_centre_outside_the_data(plans):
+    om, g32, g64 = plans
+    d = _data((8, 6, 5, 2), 7)
+    with pytest.raises(Exception, match="center outside"):
+        g64.gram(d, 3, 1, loraks=True, center=(9, 1, 1))

@@ -290,6 +290,25 @@ def run_lowrank(args):
     rec_s = time.perf_counter() - t0
     log("pruno reconstruction %.2f s, coupled MINRES iters %s relres %s"
         % (rec_s, info["iters"], info["relres"]))
+    # SAKE Cadzow iterations on the same data and kernel (per-kernel times with
+    # HIP events on a second, profiled run), error against the FP64 plan
+    sake_it = max(1, args.sake_iters)
+    sopts = dict(maxit=sake_it, tol=0.0, width=width, radial=radial)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ksp, sinfo = plan.sake(raw, return_info=True, **sopts)
+    sake_s = time.perf_counter() - t0
+    lib.nufft3d_profile_enable(plan._h, 1)
+    plan.sake(raw, **sopts)
+    lib.nufft3d_profile_read(plan._h, buf, len(buf))
+    lib.nufft3d_profile_enable(plan._h, 0)
+    skern = {k: round(v[1] / sake_it, 3) for k, v in json.loads(buf.value.decode()).items()}
+    ksp64 = p64.sake(raw.astype(np.complex128), **sopts)
+    serr = float(np.linalg.norm(ksp - ksp64) / np.linalg.norm(ksp64))
+    log("sake %d iterations %.2f s (%.3f s/iteration), rank %d, rel err vs FP64 %.2e"
+        % (sake_it, sake_s, sake_s / sake_it, sinfo["rank"], serr))
+    log("sake per-iteration kernel ms: %s" % json.dumps(
+        dict(sorted(skern.items(), key=lambda kv: -kv[1])[:12])))
     out = {
         "metric": "Hankel Gram TFLOP/s (split-BF16 MFMA) and PRUNO reconstruction time",
         "value": round(flops / g_ms / 1e9, 3),
@@ -310,6 +329,9 @@ def run_lowrank(args):
                      "frac": round(mfma_ops / (g_ms / 1e3) / MFMA_BF16_PEAK, 4), "traffic": None},
         "pruno_reconstruction_s": round(rec_s, 3),
         "pruno_info": {"iters": info["iters"], "relres": info["relres"]},
+        "sake": {"iterations": sake_it, "seconds": round(sake_s, 3),
+                 "s_per_iteration": round(sake_s / sake_it, 4), "rank": sinfo["rank"],
+                 "noise": sinfo["noise"], "rel_err_vs_fp64": serr, "kernel_ms_per_iteration": skern},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -336,6 +358,7 @@ def main():
     ap.add_argument("--study", action="store_true",
                     help="per-stage FP32 (plain / compensated) errors against FP64")
     ap.add_argument("--study-iters", type=int, default=30)
+    ap.add_argument("--sake-iters", type=int, default=3, help="lowrank: SAKE Cadzow iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)

@@ -185,6 +185,52 @@ int nufft3d_inverse_pruno(nufft3d_plan* plan, const void* raw, int32_t nc, int32
                           const nufft3d_pruno_opts* opts, int32_t iters, void* im,
                           nufft3d_info* info);
 
+/* Options of sake(obj,data,varargin) (sake.m:25-35). */
+typedef struct nufft3d_sake_opts {
+  int32_t width;           /* kernel width, default 4 (sake.m:26) */
+  int32_t radial;          /* 1 = spherical kernel support (sake.m:27,63-67) */
+  int32_t loraks;          /* 1 = add the conjugate-reflection (phase) block (sake.m:28) */
+  double tol;              /* stop when the largest singular value changes by a
+                              fraction below tol, default 1e-6 (sake.m:29,155-161) */
+  int32_t maxit;           /* maximum Cadzow iterations, default 1000 (sake.m:30) */
+  double noise;            /* noise std; < 0 = estimate on iteration 1 (sake.m:31,128-138) */
+  int32_t center[3];       /* 1-based k-space centre, default [1 1 1] (sake.m:32,79-88) */
+  double damp;             /* Tikhonov damping of the data-consistency solve (sake.m:34,109) */
+  double sparsity;         /* wavelet thresholding (sake.m:35,91-93,114-118): needs the
+                              external DWT, so > 0 returns NUFFT3D_ERR_UNSUPPORTED */
+  int32_t dc_iters;        /* MINRES iterations of the data-consistency iNUFT, 10 (sake.m:109) */
+  int32_t output;          /* 0 = k-space ksp (sake.m:1); 1 = the image ifft3(ksp), the
+                              result iNUFT's 'parallel-imaging-sake' branch means to
+                              return (iNUFT.m:195 returns k-space as the image) */
+} nufft3d_sake_opts;
+
+/* Summary of a sake call. */
+typedef struct nufft3d_sake_info {
+  int32_t iters;           /* Cadzow iterations run */
+  int32_t rank;            /* nnz(f) of the last filter (sake.m:290) */
+  double noise;            /* noise std used (given or estimated) */
+  double sigma;            /* noise floor in singular-value units (sake.m:139) */
+  double seconds;
+} nufft3d_sake_info;
+
+int nufft3d_sake_opts_default(nufft3d_sake_opts* opts);
+
+/* ksp = sake(obj,data,varargin) (sake.m:1-171,176-284): Cadzow matrix
+ * completion from data [M nc] (nc >= 2, sake.m:57-59). Each iteration:
+ * data consistency ksp -= fft3(iNUFT(fNUFT(ifft3(ksp)) - data, dc_iters, damp))
+ * with W = density; the Hankel Gram (the split-BF16 MFMA Gram for fp32 plans)
+ * and its eigen-decomposition (= svd of the PSD Gram); noise floor and the
+ * minimum-variance filter f = max(0, 1 - sigma^2/W^2); F = V diag(f) V';
+ * anti-diagonal averaging of F applied to ksp. fft3/ifft3 are the unshifted
+ * unnormalised 3-D DFT and its inverse; the reference's matrix_density =
+ * obj.sd (a property the class lacks, sake.m:96) is min(1, nnz/prod(N)).
+ * out [N1 N2 N3 nc] (k-space or image, per opts->output; same precision as
+ * data). history (may be NULL, else [maxit][5]) receives per iteration the
+ * largest, sum and 2-norm of the singular values (sake.m:152-154), the
+ * convergence measure (sake.m:155-160) and nnz(f). */
+int nufft3d_sake(nufft3d_plan* plan, const void* data, int32_t nc, const nufft3d_sake_opts* opts,
+                 void* out, double* history, nufft3d_sake_info* info);
+
 /* Autotuner, no counterpart in the reference: times `reps` fwd+adj operator
  * pairs of nc coils for each combination of FFT lines per work-group (auto,
  * 8, 4), coils per gridding launch (16, 8), XCD block remap (off, runs of 32,

@@ -32,6 +32,18 @@ class PrunoOpts(C.Structure):
                 ("tol", C.c_double), ("literal_first_nv", C.c_int32)]
 
 
+class SakeOpts(C.Structure):
+    _fields_ = [("width", C.c_int32), ("radial", C.c_int32), ("loraks", C.c_int32),
+                ("tol", C.c_double), ("maxit", C.c_int32), ("noise", C.c_double),
+                ("center", C.c_int32 * 3), ("damp", C.c_double), ("sparsity", C.c_double),
+                ("dc_iters", C.c_int32), ("output", C.c_int32)]
+
+
+class SakeInfo(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("rank", C.c_int32), ("noise", C.c_double),
+                ("sigma", C.c_double), ("seconds", C.c_double)]
+
+
 class Info(C.Structure):
     _fields_ = [("nrhs", C.c_int32), ("iters", C.c_int32 * 64),
                 ("relres", C.c_double * 64), ("seconds", C.c_double)]
@@ -75,6 +87,9 @@ SIGNATURES = {
     "nufft3d_inverse_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
                                         C.c_void_p, C.c_int64, C.c_double, C.POINTER(PrunoOpts),
                                         C.c_int32, C.c_void_p, C.POINTER(Info)]),
+    "nufft3d_sake_opts_default": (C.c_int, [C.POINTER(SakeOpts)]),
+    "nufft3d_sake": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(SakeOpts),
+                               C.c_void_p, C.c_void_p, C.POINTER(SakeInfo)]),
     "nufft3d_last_error": (C.c_char_p, []),
 }
 

@@ -15,7 +15,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          # packed FP32 ops are kept out of the generated code on purpose
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result"]
-SOURCES = ["plan.hip", "grid.hip", "fft.hip", "solve.hip", "gram.hip"]
+SOURCES = ["plan.hip", "grid.hip", "fft.hip", "solve.hip", "gram.hip", "sake.hip"]
 
 
 def _compile(name: str) -> str:

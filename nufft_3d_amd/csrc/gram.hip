@@ -382,15 +382,15 @@ __global__ void k_null_axis(const double2* __restrict__ in, double2* __restrict_
   out[(o * Nax + xx) * inner + q] = r;
 }
 
-// B stored per voxel [V][nc][nc] in the plan precision, scaled by lambda^2
+// B stored per voxel [V][cc] in the plan precision, scaled
 template <typename R>
-__global__ void k_null_store(const double2* __restrict__ Bd, long long V, int nc, double lam2,
+__global__ void k_null_store(const double2* __restrict__ Bd, long long n, double scale,
                              cplx<R>* __restrict__ B) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= V * nc * nc) return;
+  if (idx >= n) return;
   // Bd layout after the three stages: [z][y][x][cc] with x fastest of voxels
   const double2 v = Bd[idx];
-  cplx<R> o; o.x = (R)(lam2 * v.x); o.y = (R)(lam2 * v.y);
+  cplx<R> o; o.x = (R)(scale * v.x); o.y = (R)(scale * v.y);
   B[idx] = o;
 }
 
@@ -700,7 +700,7 @@ int pruno_select(const std::vector<double>& evals_desc, double noise_in, double 
 int null_operator(nufft3d_plan* p, const double2* dKt, int nk, int nc, int nv, int w,
                   const std::vector<int4>& taps, const int N[3], double lambda, void* dB) {
   const int n = nk * nc;
-  double2 *P = nullptr, *G = nullptr, *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;
+  double2 *P = nullptr, *G = nullptr;
   int4* dtaps = nullptr;
   N3D_HIP(hipMalloc(&P, sizeof(double2) * (size_t)n * n));
   rocblas_handle h;
@@ -728,31 +728,41 @@ int null_operator(nufft3d_plan* p, const double2* dKt, int nk, int nc, int nv, i
   N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
   hipLaunchKernelGGL(k_null_G, dim3(nblk(nG)), dim3(256), 0, p->stream, (const double2*)P,
                      (const int4*)dtaps, nk, nc, w, G);
-  // separable inverse DFT: G [dz][dy][dx][cc] -> B [z][y][x][cc]
-  const long long cc = (long long)nc * nc;
+  const int st = lag_field(p, G, w, (long long)nc * nc, N, lambda * lambda, dB);
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(P); (void)hipFree(G); (void)hipFree(dtaps);
+  return st;
+}
+
+// Separable inverse DFT of lag-domain matrices: G [dz][dy][dx][cc] (offsets
+// d - (w-1), D = 2w-1 per axis) -> B [z][y][x][cc] = scale * sum_d G(d)
+// e^{+2 pi i d.x/N}, stored in the plan precision. Temporaries in double.
+int lag_field(nufft3d_plan* p, const double2* G, int w, long long cc, const int N[3], double scale,
+              void* dB) {
+  const int D = 2 * w - 1;
   const int N1 = N[0], N2 = N[1], N3 = N[2];
+  double2 *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;
   N3D_HIP(hipMalloc(&T1, sizeof(double2) * (size_t)D * D * N1 * cc));
   N3D_HIP(hipMalloc(&T2, sizeof(double2) * (size_t)D * N2 * N1 * cc));
   N3D_HIP(hipMalloc(&T3, sizeof(double2) * (size_t)N3 * N2 * N1 * cc));
   // x: in [dz*dy][dx][cc] -> [dz*dy][x][cc]
   hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)D * D * N1 * cc)), dim3(256), 0, p->stream,
-                     (const double2*)G, T1, D, w, N1, cc, (long long)D * D);
+                     G, T1, D, w, N1, cc, (long long)D * D);
   // y: in [dz][dy][x*cc] -> [dz][y][x*cc]
   hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)D * N2 * N1 * cc)), dim3(256), 0, p->stream,
                      (const double2*)T1, T2, D, w, N2, (long long)N1 * cc, (long long)D);
   // z: in [dz][y*x*cc] -> [z][y*x*cc]
   hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)N3 * N2 * N1 * cc)), dim3(256), 0, p->stream,
                      (const double2*)T2, T3, D, w, N3, (long long)N2 * N1 * cc, 1LL);
-  const long long V = (long long)N1 * N2 * N3;
+  const long long n = (long long)N1 * N2 * N3 * cc;
   if (p->prec == 2)
-    hipLaunchKernelGGL((k_null_store<double>), dim3(nblk(V * cc)), dim3(256), 0, p->stream,
-                       (const double2*)T3, V, nc, lambda * lambda, (double2*)dB);
+    hipLaunchKernelGGL((k_null_store<double>), dim3(nblk(n)), dim3(256), 0, p->stream,
+                       (const double2*)T3, n, scale, (double2*)dB);
   else
-    hipLaunchKernelGGL((k_null_store<float>), dim3(nblk(V * cc)), dim3(256), 0, p->stream,
-                       (const double2*)T3, V, nc, lambda * lambda, (float2*)dB);
+    hipLaunchKernelGGL((k_null_store<float>), dim3(nblk(n)), dim3(256), 0, p->stream,
+                       (const double2*)T3, n, scale, (float2*)dB);
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(P); (void)hipFree(G); (void)hipFree(dtaps);
   (void)hipFree(T1); (void)hipFree(T2); (void)hipFree(T3);
   return NUFFT3D_OK;
 }

@@ -236,5 +236,15 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                 double2* dAA, bool loraks = false, const int* center = nullptr);
 void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int& w);
 int null_apply(nufft3d_plan* p, const void* B, const void* x, void* y, int nc);
+int heev_device(nufft3d_plan* p, double2* dA, int n, double* dW);   // ascending, vectors in dA
+int pruno_select(const std::vector<double>& evals_desc, double noise_in, double density,
+                 int nx, int ny, double tol, double& noise_out, std::vector<double>& f, int& nv);
+int lag_field(nufft3d_plan* p, const double2* G, int w, long long cc, const int N[3], double scale,
+              void* dB);
+
+// sake.hip: the Cadzow loop on device-resident data [M nc] (caller layout,
+// plan precision) with sample weights wsamp; out [nc][N3][N2][N1].
+int sake_run(nufft3d_plan* p, const void* data, int nc, const nufft3d_sake_opts& o,
+             const void* wsamp, void* out, double* history, nufft3d_sake_info* info);
 
 }  // namespace nufft3d

@@ -1960,4 +1960,65 @@ int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   return st;
 }
 
+int nufft3d_sake_opts_default(nufft3d_sake_opts* o) {
+  if (!o) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
+  o->width = 4;            // sake.m:26
+  o->radial = 1;           // sake.m:27
+  o->loraks = 0;           // sake.m:28
+  o->tol = 1e-6;           // sake.m:29
+  o->maxit = 1000;         // sake.m:30
+  o->noise = -1.0;         // sake.m:31 (empty: estimate)
+  o->center[0] = o->center[1] = o->center[2] = 1;   // sake.m:79-83
+  o->damp = 0.0;           // sake.m:34
+  o->sparsity = 0.0;       // sake.m:35
+  o->dc_iters = 10;        // sake.m:109
+  o->output = 0;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_sake(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_sake_opts* opts,
+                 void* out, double* history, nufft3d_sake_info* info) {
+  N3D_PLAN_GUARD(p);
+  if (!data || !out || nc < 2) {
+    set_error("'data' must be a float array [npts nc].");                 // sake.m:57-59
+    return NUFFT3D_ERR_INVALID;
+  }
+  nufft3d_sake_opts o;
+  if (opts) o = *opts; else nufft3d_sake_opts_default(&o);
+  if (o.width < 1 || o.maxit < 0 || o.dc_iters < 0 || !(o.damp >= 0) || !std::isfinite(o.damp) ||
+      !(o.tol >= 0) || (o.output != 0 && o.output != 1)) {
+    set_error("invalid sake option (width, maxit, dc_iters, damp, tol or output)");
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (o.sparsity != 0) {
+    set_error("sparsity needs the DWT wavelet transform (sake.m:91-93), which the reference does not ship");
+    return NUFFT3D_ERR_UNSUPPORTED;
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (o.center[a] < 1 || o.center[a] > p->N[a]) { set_error("center outside the data"); return NUFFT3D_ERR_INVALID; }
+    if (p->N[a] <= o.width / 2) {   // shifted indices wrap at most once
+      set_error("image dimensions must exceed the kernel half width");
+      return NUFFT3D_ERR_UNSUPPORTED;
+    }
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  void* wsamp = p->d_dens;                                                 // iNUFT default W = d
+  Workspace tmp_out, tmp_io, tmp_in;
+  Staged di, xo;
+  int st = stage_in_c(p, data, nk, tmp_in, tmp_io, di);
+  if (!st) st = stage_out_c(p, out, nimg, tmp_out, xo);
+  if (!st) st = sake_run(p, di.dev, nc, o, wsamp, xo.dev, history, info);
+  if (!st) st = finish_out_c(p, out, nimg, xo, tmp_io);
+  if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error in sake");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  release(tmp_out); release(tmp_io); release(tmp_in);
+  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
 }  // extern "C"

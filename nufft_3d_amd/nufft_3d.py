@@ -214,8 +214,14 @@ class nufft_3d:
         return y
 
     def iNUFT(self, raw, maxit=20, damp=0.0, W=None, constraint=None, lambda_=None,
-              tol=1e-4, return_info=False, pruno_opts=None, pruno_iters=100):
-        """Inverse by MINRES on A'WA + damp^2 I (iNUFT.m:2-138,225)."""
+              tol=1e-4, return_info=False, pruno_opts=None, pruno_iters=100, sake_opts=None):
+        """Inverse by MINRES on A'WA + damp^2 I (iNUFT.m:2-138,225).
+
+        constraint 'parallel-imaging-pruno' runs iNUFT.m:200-218;
+        'parallel-imaging-sake' runs sake with damp and loraks = lambda_ > 0
+        (iNUFT.m:191-196) and returns the image ifft3(ksp) (the reference
+        returns the k-space itself as the image, iNUFT.m:195,225);
+        sake_opts passes further sake options (e.g. maxit)."""
         if maxit is None:
             maxit = 20
         if damp is None:
@@ -224,7 +230,7 @@ class nufft_3d:
             raise ValueError("maxit must be a nonnegative integer")
         if not (np.isfinite(damp) and damp >= 0):
             raise ValueError("damp must be finite and nonnegative")
-        pruno_mode = False
+        pruno_mode = sake_mode = False
         if constraint:                                          # iNUFT.m:75-91
             if maxit <= 1:
                 warnings.warn("phase constraint is only effective when maxit>1")
@@ -236,6 +242,8 @@ class nufft_3d:
                 raise ValueError("lambda must be supplied with %s" % constraint)
             elif constraint == "parallel-imaging-pruno":
                 pruno_mode = True
+            elif constraint == "parallel-imaging-sake":
+                sake_mode = True
             else:
                 raise NotImplementedError(
                     "constraint %r is not implemented on the HIP path" % constraint)
@@ -247,6 +255,21 @@ class nufft_3d:
             Wp, nW = Wa, Wa.size
         info = _lib.Info()
         wptr = Wp.ctypes.data if Wp is not None else None
+        if sake_mode:
+            if _is_torch(raw):
+                rs = self._torch_in(raw, self.M, "raw").reshape(-1, self.M)
+            else:
+                ra = np.asarray(raw)
+                if ra.size % self.M or (ra.shape[0] != self.M and (
+                        ra.ndim < 2 or ra.shape[0] * ra.shape[1] != self.M)):
+                    raise ValueError("raw data leading dimension(s) must be length %d" % self.M)
+                rs = np.reshape(ra, (self.M, ra.size // self.M), order="F")
+            if (rs.shape[0] if _is_torch(raw) else rs.shape[1]) == 1:
+                raise ValueError("sake-low-rank requires multiple coils")     # iNUFT.m:84
+            so = dict(sake_opts or {})
+            so.update(damp=float(damp), loraks=lambda_ > 0, output="image")
+            res = self.sake(rs, return_info=return_info, **so)
+            return res
         if pruno_mode:
             return self._inufft_pruno(raw, maxit, damp, wptr, nW, lambda_, pruno_opts or {},
                                       pruno_iters, return_info)
@@ -373,6 +396,68 @@ class nufft_3d:
         _lib.check(self._lib.nufft3d_pruno(self._h, db.ctypes.data, nc, C.byref(o), C.byref(nv),
                                            sv.ctypes.data, C.byref(noise)))
         return {"nv": nv.value, "sv": sv, "noise": noise.value}
+
+    _SAKE_KEYS = ("width", "radial", "loraks", "tol", "maxit", "noise", "center", "W", "damp",
+                  "sparsity", "dc_iters", "output")
+
+    def sake(self, data, return_info=False, **opts):
+        """ksp = sake(obj,data,...) (sake.m:1-171): Cadzow matrix completion.
+
+        data [M nc] (nc >= 2). Options as sake.m:26-35 (width 4, radial 1,
+        loraks 0, tol 1e-6, maxit 1000, noise None = estimate, center [1 1 1],
+        damp 0, sparsity 0) plus dc_iters (10, the data-consistency MINRES
+        iterations, sake.m:109) and output ("kspace" or "image"). ``W`` is
+        accepted and, as in the reference, unused. Returns k-space
+        [N1 N2 N3 nc] (or the image), in the array kind of the input.
+        """
+        o = _lib.SakeOpts()
+        _lib.check(self._lib.nufft3d_sake_opts_default(C.byref(o)))
+        for k, v in opts.items():
+            if k not in self._SAKE_KEYS:
+                warnings.warn("'%s' is not a valid option." % k)              # sake.m:42-44
+                continue
+            if k == "W":
+                continue                                                      # never read by sake.m
+            if k == "noise":
+                v = -1.0 if v is None else float(v)
+            elif k == "center":
+                if v is None:
+                    continue
+                o.center = (C.c_int32 * 3)(*[int(c) for c in v])
+                continue
+            elif k == "output":
+                v = {"kspace": 0, "image": 1}.get(v, v)
+            elif k == "loraks":
+                v = int(bool(v))
+            setattr(o, k, v)
+        hist = np.zeros((max(int(o.maxit), 1), 5), dtype=np.float64)
+        info = _lib.SakeInfo()
+        if _is_torch(data):
+            import torch
+            dt = self._torch_in(data, self.M, "data")
+            nc = dt.numel() // self.M
+            if nc < 2:
+                raise ValueError("'data' must be a float array [npts nc].")
+            out = torch.empty((nc, self.N[2], self.N[1], self.N[0]), dtype=dt.dtype, device=dt.device)
+            self._bind_stream(dt)
+            _lib.check(self._lib.nufft3d_sake(self._h, C.c_void_p(dt.data_ptr()), nc, C.byref(o),
+                                              C.c_void_p(out.data_ptr()), hist.ctypes.data,
+                                              C.byref(info)))
+        else:
+            da = np.asarray(data)
+            if da.ndim != 2 or da.shape[0] != self.M or da.shape[1] < 2:
+                raise ValueError("'data' must be a float array [npts nc].")   # sake.m:57-59
+            nc = da.shape[1]
+            db = self._host_c(da, (self.M, nc))
+            out = np.empty(self.N + (nc,), dtype=self.cdtype, order="F")
+            _lib.check(self._lib.nufft3d_sake(self._h, db.ctypes.data, nc, C.byref(o),
+                                              out.ctypes.data, hist.ctypes.data, C.byref(info)))
+        if return_info:
+            h = [dict(norm_inf=r[0], norm_1=r[1], norm_2=r[2], tol=r[3], rank=int(r[4]))
+                 for r in hist[:info.iters]]
+            return out, {"iters": info.iters, "rank": info.rank, "noise": info.noise,
+                         "sigma": info.sigma, "seconds": info.seconds, "history": h}
+        return out
 
     def _inufft_pruno(self, raw, maxit, damp, wptr, nW, lam, opts, iters, return_info):
         ra = np.asarray(raw)

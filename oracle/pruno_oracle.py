@@ -129,6 +129,24 @@ def _hist(y, nb):
     return np.bincount(k, minlength=nb)
 
 
+def estimate_noise(W, density, nx, ny):
+    """pruno.m:97-106 = sake.m:128-137: noise floor from the squared singular
+    values W (descending); skips true zeros, then the leading values while the
+    histogram's first bin holds the maximum, and takes the median."""
+    n = len(W)
+    thr = np.spacing(n * W[0])
+    hi = int(np.count_nonzero(W > thr))
+    lo = 1
+    while lo <= hi:
+        h = _hist(W[lo - 1:hi], round(math.sqrt(hi - lo)))
+        if int(np.argmax(h)) + 1 > 1:
+            break
+        lo += 1
+    lo = min(lo, hi)
+    sigma = math.sqrt(float(np.median(W[lo - 1:max(hi, lo)])))
+    return sigma / math.sqrt(density * nx * ny)
+
+
 def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=False, AA=None):
     """pruno.m:43-143. Returns dict(K, Kt, nv, sv, f, noise, taps, mask)."""
     nx, ny, nz, nc = data.shape
@@ -144,18 +162,7 @@ def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=Fal
     W = w[order]
     V = V[:, order]
     if noise is None:
-        n = len(W)
-        thr = np.spacing(n * W[0])
-        hi = int(np.count_nonzero(W > thr))
-        lo = 1
-        while lo <= hi:
-            h = _hist(W[lo - 1:hi], round(math.sqrt(hi - lo)))
-            if int(np.argmax(h)) + 1 > 1:
-                break
-            lo += 1
-        lo = min(lo, hi)
-        sigma = math.sqrt(float(np.median(W[lo - 1:max(hi, lo)])))
-        noise = sigma / math.sqrt(density * nx * ny)
+        noise = estimate_noise(W, density, nx, ny)
     sigma = noise * math.sqrt(density * nx * ny)              # pruno.m:108
     Ws = np.sqrt(np.maximum(W, 0))                            # pruno.m:111
     with np.errstate(divide='ignore'):

@@ -94,6 +94,20 @@ int nufft3d_inverse(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t max
                     double damp, const double* W, int64_t nW, double tol, void* im,
                     nufft3d_info* info);
 
+/* im = obj.iNUFT(raw,maxit,damp,W,'phase-constraint',lambda) (iNUFT.m:144-166,
+ * pprojection nufft_3d.m:325-334): the plain MINRES solution x (as
+ * nufft3d_inverse), its smoothed phase P = exp(i angle(convn(x,ones(3),'same'))),
+ * then maxit iterations (tol) of conjugate gradients with real inner products
+ * (the method of the external pcgpc) on conj(P) (A'WA + damp^2 I)(P x') +
+ * i lambda^2 Im(x') = conj(P) A'W raw, and im = P x'. One independent solve
+ * per coil, each coil with the phase of its own solution (the reference
+ * reshapes all coils into one image, which MATLAB rejects for nc > 1).
+ * maxit <= 1 runs nufft3d_inverse (iNUFT.m:78-79). info as nufft3d_inverse,
+ * for the CG stage. */
+int nufft3d_inverse_phase(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t maxit,
+                          double damp, const double* W, int64_t nW, double lambda, double tol,
+                          void* im, nufft3d_info* info);
+
 /* damp = obj.discrep(data,stddev) (discrep.m:2-69): discrepancy-principle
  * estimate of the iNUFT damping. data [M nc]; stddev < 0 = not supplied:
  * guessed as median(|t - median(t)|) over t = [real; imag] of the nonzero

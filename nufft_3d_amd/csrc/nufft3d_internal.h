@@ -227,6 +227,12 @@ int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void
 int sum_real(nufft3d_plan* p, const void* v, int64_t n, double* out);
 int minres_coupled(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
                    const void* wsamp, const void* B, double tol, void* x, nufft3d_info* info);
+// phase constraint (iNUFT.m:144-166): smoothed phase P of x, y = P x or conj(P) x,
+// and CG with real inner products on conj(P)(A'WA + damp^2)P + i lambda^2 Im
+int phase_estimate(nufft3d_plan* p, const void* x, int nc, void* P);
+int phase_mul(nufft3d_plan* p, const void* P, const void* x, void* y, int nc, bool conj);
+int phase_cg(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double lambda,
+             const void* wsamp, const void* P, double tol, void* x, nufft3d_info* info);
 
 // gram.hip
 int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,

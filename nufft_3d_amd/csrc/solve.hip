@@ -530,4 +530,305 @@ int minres_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp,
                       : minres_t<float>(p, b, nc, maxit, damp, wsamp, tol, x, info, warm);
 }
 
+// ---------------------------------------------------------------- phase constraint
+// iNUFT.m:144-166 with pprojection (nufft_3d.m:325-334): conjugate gradients
+// on the real-linear operator  y = conj(P) (A'WA + damp^2)(P x) + i lam^2 Im(x)
+// with the real inner product Re<a,b> (the method of the external pcgpc,
+// Bydder & Robson MRM 2005;53:1393), zero start, one solve per coil, every
+// scalar on the device in double (same reductions as MINRES).
+
+struct CgState {
+  double rho, rho_old, beta, alpha, bnorm, relres;
+  int active, upd, iters, pad;
+};
+
+template <typename R>
+__global__ void k_cg_init(const cplx<R>* __restrict__ b, long long n, const double* __restrict__ bn2,
+                          CgState* __restrict__ st, cplx<R>* __restrict__ x, cplx<R>* __restrict__ r,
+                          cplx<R>* __restrict__ pv, double tol) {
+  const int c = blockIdx.y;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CgState s = {};
+    s.bnorm = sqrt(bn2[c]);
+    s.relres = s.bnorm > 0 ? 1.0 : 0.0;
+    s.active = s.bnorm > 0 && 1.0 > tol;
+    st[c] = s;
+  }
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    cplx<R> z; z.x = 0; z.y = 0;
+    x[o] = z;
+    pv[o] = z;
+    r[o] = b[o];
+  }
+}
+
+// rho = Re<r,r>; beta = rho/rho_old (0 on the first iteration)
+__global__ void k_cg_scalar_rho(CgState* __restrict__ st, const double* __restrict__ sums, int itn) {
+  const int c = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  CgState s = st[c];
+  s.upd = s.active;
+  if (s.active) {
+    s.rho = sums[c];
+    s.beta = (itn > 1 && s.rho_old > 0) ? s.rho / s.rho_old : 0.0;
+  }
+  st[c] = s;
+}
+
+// p = r + beta p
+template <typename R>
+__global__ void k_cg_dir(cplx<R>* __restrict__ pv, const cplx<R>* __restrict__ r, long long n,
+                         const CgState* __restrict__ st) {
+  const int c = blockIdx.y;
+  const CgState s = st[c];
+  if (!s.upd) return;
+  const R be = (R)s.beta;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    const cplx<R> rv = r[o];
+    cplx<R> pp = pv[o];
+    pp.x = rv.x + be * pp.x;
+    pp.y = rv.y + be * pp.y;
+    pv[o] = pp;
+  }
+}
+
+// alpha = rho / Re<p,q>; a non-positive curvature stops the column
+__global__ void k_cg_scalar_alpha(CgState* __restrict__ st, const double* __restrict__ sums) {
+  const int c = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  CgState s = st[c];
+  if (s.upd) {
+    const double pq = sums[c];
+    if (pq > 0 && isfinite(pq)) {
+      s.alpha = s.rho / pq;
+    } else {
+      s.alpha = 0;
+      s.upd = 0;
+      s.active = 0;
+    }
+  }
+  st[c] = s;
+}
+
+// x += alpha p; r -= alpha q; partial ||r||^2
+template <typename R>
+__global__ void __launch_bounds__(kRedThreads) k_cg_step(cplx<R>* __restrict__ x, cplx<R>* __restrict__ r,
+                                                         const cplx<R>* __restrict__ pv,
+                                                         const cplx<R>* __restrict__ q, long long n,
+                                                         const CgState* __restrict__ st,
+                                                         double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads];
+  const int c = blockIdx.y;
+  const int nblk = gridDim.x;
+  const CgState s = st[c];
+  const R al = (R)s.alpha;
+  double acc = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)nblk * blockDim.x) {
+    const long long o = (long long)c * n + i;
+    cplx<R> rv = r[o];
+    if (s.upd) {
+      const cplx<R> pp = pv[o], qq = q[o];
+      cplx<R> xv = x[o];
+      xv.x += al * pp.x;
+      xv.y += al * pp.y;
+      x[o] = xv;
+      rv.x -= al * qq.x;
+      rv.y -= al * qq.y;
+      r[o] = rv;
+    }
+    acc += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+  }
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[(long long)c * nblk + blockIdx.x] = t;
+}
+
+__global__ void k_cg_scalar_res(CgState* __restrict__ st, const double* __restrict__ sums, double tol) {
+  const int c = blockIdx.x;
+  if (threadIdx.x != 0) return;
+  CgState s = st[c];
+  if (s.upd) {
+    s.relres = sqrt(sums[c]) / s.bnorm;
+    s.rho_old = s.rho;
+    s.iters += 1;
+    s.active = s.relres > tol;
+  }
+  st[c] = s;
+}
+
+// P = exp(i angle(sum of the in-bounds 3x3x3 neighbours)) per coil
+// (convn(x, ones(3), 'same'), iNUFT.m:147-149; angle(0) = 0)
+template <typename R>
+__global__ void k_phase_est(const cplx<R>* __restrict__ x, cplx<R>* __restrict__ P, int N1, int N2,
+                            int N3, int nc) {
+  const long long V = (long long)N1 * N2 * N3;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= V * nc) return;
+  const int c = (int)(idx / V);
+  const long long v = idx % V;
+  const int ix = (int)(v % N1), iy = (int)((v / N1) % N2), iz = (int)(v / ((long long)N1 * N2));
+  const cplx<R>* xc = x + (long long)c * V;
+  double re = 0, im = 0;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = iz + dz;
+    if (z < 0 || z >= N3) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = iy + dy;
+      if (y < 0 || y >= N2) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = ix + dx;
+        if (xx < 0 || xx >= N1) continue;
+        const cplx<R> a = xc[xx + (long long)N1 * (y + (long long)N2 * z)];
+        re += (double)a.x;
+        im += (double)a.y;
+      }
+    }
+  }
+  const double th = atan2(im, re);
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  cplx<R> o; o.x = (R)cs; o.y = (R)sn;
+  P[idx] = o;
+}
+
+// y = P x (conj = 0) or conj(P) x (conj = 1)
+template <typename R>
+__global__ void k_phase_mul(const cplx<R>* __restrict__ P, const cplx<R>* __restrict__ x,
+                            cplx<R>* __restrict__ y, long long n, int conj) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<R> a = P[i], b = x[i];
+  const R ay = conj ? -a.y : a.y;
+  cplx<R> o;
+  o.x = a.x * b.x - ay * b.y;
+  o.y = a.x * b.y + ay * b.x;
+  y[i] = o;
+}
+
+// y = conj(P) y + i lam2 Im(x)
+template <typename R>
+__global__ void k_phase_finish(const cplx<R>* __restrict__ P, const cplx<R>* __restrict__ x,
+                               cplx<R>* __restrict__ y, long long n, R lam2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<R> a = P[i], b = y[i];
+  cplx<R> o;
+  o.x = a.x * b.x + a.y * b.y;
+  o.y = a.x * b.y - a.y * b.x + lam2 * x[i].y;
+  y[i] = o;
+}
+
+template <typename R>
+static int phase_cg_t(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double lambda,
+                      const void* wsamp, const void* P, double tol, void* xout, nufft3d_info* info) {
+  using C = cplx<R>;
+  const long long n = (long long)p->N[0] * p->N[1] * p->N[2];
+  const long long tot = n * nc;
+  const size_t vec = (size_t)tot * sizeof(C);
+  // vectors: r, p, q, u (P p) ; x is the caller's output
+  N3D_TRY(ensure(p->ws_solve, 4 * vec));
+  unsigned char* base = (unsigned char*)p->ws_solve.p;
+  C* r = (C*)(base + 0 * vec);
+  C* pv = (C*)(base + 1 * vec);
+  C* q = (C*)(base + 2 * vec);
+  C* u = (C*)(base + 3 * vec);
+  C* x = (C*)xout;
+  const int nb = red_blocks(n);
+  N3D_TRY(ensure(p->ws_red, (size_t)nc * (nb + 2) * sizeof(double) + nc * sizeof(CgState) + 64));
+  double* part = (double*)p->ws_red.p;
+  double* sums = part + (size_t)nc * nb;
+  double* bn2 = sums + nc;
+  CgState* st = (CgState*)(bn2 + nc);
+  const dim3 grid(nb, nc);
+  const R lam2 = (R)(lambda * lambda);
+  hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b,
+                     (const C*)b, n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, bn2);
+  hipLaunchKernelGGL((k_cg_init<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)b, n,
+                     (const double*)bn2, st, x, r, pv, tol);
+  std::vector<CgState> hs(nc);
+  auto fetch = [&]() -> int {
+    N3D_HIP(hipMemcpyAsync(hs.data(), st, nc * sizeof(CgState), hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+    return NUFFT3D_OK;
+  };
+  N3D_TRY(fetch());
+  for (int itn = 1; itn <= maxit; ++itn) {
+    bool any = false;
+    for (int c = 0; c < nc; ++c) any = any || hs[c].active;
+    if (!any) break;
+    hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)r,
+                       (const C*)r, n, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    hipLaunchKernelGGL(k_cg_scalar_rho, dim3(nc), dim3(1), 0, p->stream, st, (const double*)sums, itn);
+    hipLaunchKernelGGL((k_cg_dir<R>), grid, dim3(kRedThreads), 0, p->stream, pv, (const C*)r, n,
+                       (const CgState*)st);
+    {
+      ProfScope ps(p, "k_phase_mul");
+      hipLaunchKernelGGL((k_phase_mul<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, p->stream,
+                         (const C*)P, (const C*)pv, u, tot, 0);
+    }
+    N3D_TRY(apply_normal(p, u, nc, damp, wsamp, q));
+    {
+      ProfScope ps(p, "k_phase_finish");
+      hipLaunchKernelGGL((k_phase_finish<R>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         p->stream, (const C*)P, (const C*)pv, q, tot, lam2);
+    }
+    hipLaunchKernelGGL((k_dot_partial<R>), grid, dim3(kRedThreads), 0, p->stream, (const C*)pv,
+                       (const C*)q, n, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    hipLaunchKernelGGL(k_cg_scalar_alpha, dim3(nc), dim3(1), 0, p->stream, st, (const double*)sums);
+    hipLaunchKernelGGL((k_cg_step<R>), grid, dim3(kRedThreads), 0, p->stream, x, r, (const C*)pv,
+                       (const C*)q, n, (const CgState*)st, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(nc), dim3(kRedThreads), 0, p->stream, part, nb, sums);
+    hipLaunchKernelGGL(k_cg_scalar_res, dim3(nc), dim3(1), 0, p->stream, st, (const double*)sums, tol);
+    N3D_HIP(hipGetLastError());
+    N3D_TRY(fetch());
+  }
+  N3D_TRY(fetch());
+  if (info) {
+    info->nrhs = nc;
+    for (int c = 0; c < nc && c < 64; ++c) {
+      info->iters[c] = hs[c].iters;
+      info->relres[c] = hs[c].relres;
+    }
+  }
+  return NUFFT3D_OK;
+}
+
+int phase_estimate(nufft3d_plan* p, const void* x, int nc, void* P) {
+  const long long tot = (long long)p->N[0] * p->N[1] * p->N[2] * nc;
+  ProfScope ps(p, "k_phase_est");
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_phase_est<double>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       p->stream, (const double2*)x, (double2*)P, p->N[0], p->N[1], p->N[2], nc);
+  else
+    hipLaunchKernelGGL((k_phase_est<float>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       p->stream, (const float2*)x, (float2*)P, p->N[0], p->N[1], p->N[2], nc);
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+int phase_mul(nufft3d_plan* p, const void* P, const void* x, void* y, int nc, bool conj) {
+  const long long tot = (long long)p->N[0] * p->N[1] * p->N[2] * nc;
+  if (p->prec == 2)
+    hipLaunchKernelGGL((k_phase_mul<double>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       p->stream, (const double2*)P, (const double2*)x, (double2*)y, tot, conj ? 1 : 0);
+  else
+    hipLaunchKernelGGL((k_phase_mul<float>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       p->stream, (const float2*)P, (const float2*)x, (float2*)y, tot, conj ? 1 : 0);
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+int phase_cg(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double lambda,
+             const void* wsamp, const void* P, double tol, void* x, nufft3d_info* info) {
+  return p->prec == 2 ? phase_cg_t<double>(p, b, nc, maxit, damp, lambda, wsamp, P, tol, x, info)
+                      : phase_cg_t<float>(p, b, nc, maxit, damp, lambda, wsamp, P, tol, x, info);
+}
+
 }  // namespace nufft3d

@@ -217,7 +217,8 @@ class nufft_3d:
               tol=1e-4, return_info=False, pruno_opts=None, pruno_iters=100, sake_opts=None):
         """Inverse by MINRES on A'WA + damp^2 I (iNUFT.m:2-138,225).
 
-        constraint 'parallel-imaging-pruno' runs iNUFT.m:200-218;
+        constraint 'phase-constraint' runs iNUFT.m:144-166 (each coil with the
+        phase of its own solution); 'parallel-imaging-pruno' runs iNUFT.m:200-218;
         'parallel-imaging-sake' runs sake with damp and loraks = lambda_ > 0
         (iNUFT.m:191-196) and returns the image ifft3(ksp) (the reference
         returns the k-space itself as the image, iNUFT.m:195,225);
@@ -230,7 +231,7 @@ class nufft_3d:
             raise ValueError("maxit must be a nonnegative integer")
         if not (np.isfinite(damp) and damp >= 0):
             raise ValueError("damp must be finite and nonnegative")
-        pruno_mode = sake_mode = False
+        pruno_mode = sake_mode = phase_mode = False
         if constraint:                                          # iNUFT.m:75-91
             if maxit <= 1:
                 warnings.warn("phase constraint is only effective when maxit>1")
@@ -244,6 +245,10 @@ class nufft_3d:
                 pruno_mode = True
             elif constraint == "parallel-imaging-sake":
                 sake_mode = True
+            elif constraint == "phase-constraint":
+                if not (np.isscalar(lambda_) and np.isfinite(lambda_) and lambda_ >= 0):
+                    raise ValueError("lambda must be a finite nonnegative scalar")   # iNUFT.m:90
+                phase_mode = True
             else:
                 raise NotImplementedError(
                     "constraint %r is not implemented on the HIP path" % constraint)
@@ -273,15 +278,21 @@ class nufft_3d:
         if pruno_mode:
             return self._inufft_pruno(raw, maxit, damp, wptr, nW, lambda_, pruno_opts or {},
                                       pruno_iters, return_info)
+        def solve(rptr, nc, xptr):
+            if phase_mode:                                      # iNUFT.m:144-166
+                return self._lib.nufft3d_inverse_phase(self._h, rptr, nc, int(maxit), float(damp),
+                                                       wptr, nW, float(lambda_), float(tol), xptr,
+                                                       C.byref(info))
+            return self._lib.nufft3d_inverse(self._h, rptr, nc, int(maxit), float(damp), wptr, nW,
+                                             float(tol), xptr, C.byref(info))
+
         if _is_torch(raw):
             import torch
             rt = self._torch_in(raw, self.M, "raw")
             nc = rt.numel() // self.M
             x = torch.empty((nc, self.N[2], self.N[1], self.N[0]), dtype=rt.dtype, device=rt.device)
             self._bind_stream(rt)
-            _lib.check(self._lib.nufft3d_inverse(self._h, C.c_void_p(rt.data_ptr()), nc,
-                                                 int(maxit), float(damp), wptr, nW, float(tol),
-                                                 C.c_void_p(x.data_ptr()), C.byref(info)))
+            _lib.check(solve(C.c_void_p(rt.data_ptr()), nc, C.c_void_p(x.data_ptr())))
         else:
             ra = np.asarray(raw)
             if ra.shape[0] == self.M:
@@ -292,9 +303,7 @@ class nufft_3d:
                 nc = ra.size // self.M
             rb = np.asfortranarray(np.reshape(ra, (self.M, nc), order="F"), dtype=self.cdtype)
             x = np.empty((self.N[0], self.N[1], self.N[2], nc), dtype=self.cdtype, order="F")
-            _lib.check(self._lib.nufft3d_inverse(self._h, rb.ctypes.data, nc, int(maxit),
-                                                 float(damp), wptr, nW, float(tol), x.ctypes.data,
-                                                 C.byref(info)))
+            _lib.check(solve(rb.ctypes.data, nc, x.ctypes.data))
         if return_info:
             n = min(nc, 64)
             return x, {"iters": list(info.iters[:n]), "relres": list(info.relres[:n]),

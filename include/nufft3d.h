@@ -108,6 +108,21 @@ int nufft3d_inverse_phase(nufft3d_plan* plan, const void* raw, int32_t nc, int32
                           double damp, const double* W, int64_t nW, double lambda, double tol,
                           void* im, nufft3d_info* info);
 
+/* im = obj.iNUFT(raw,maxit,damp,W,'compressed-sensing',lambda) (iNUFT.m:170-189):
+ * b = A'W raw, then maxit iterations (stop when ||x_k - x_{k-1}|| <= tol ||x_k||)
+ * of FISTA on (A'WA + damp^2 I) x = b with an orthonormal 3-D Haar sparsity
+ * prox, restating the external fistaN and HWT: gradient step 1/L (L = 1.05 x
+ * 20 power iterations), soft threshold of the Haar coefficients at the
+ * magnitude that zeroes a fraction lambda (0..1) of them (the sparsity
+ * convention of sake.m:35), momentum (t_k - 1)/t_{k+1}, zero start. All
+ * coils form one vector (iNUFT.m:180). Haar levels: while every
+ * non-singleton active length is even, at most 3. maxit <= 1 runs
+ * nufft3d_inverse. info->iters[0] / relres[0]: iterations and the last
+ * relative change. */
+int nufft3d_inverse_cs(nufft3d_plan* plan, const void* raw, int32_t nc, int32_t maxit,
+                       double damp, const double* W, int64_t nW, double lambda, double tol,
+                       void* im, nufft3d_info* info);
+
 /* damp = obj.discrep(data,stddev) (discrep.m:2-69): discrepancy-principle
  * estimate of the iNUFT damping. data [M nc]; stddev < 0 = not supplied:
  * guessed as median(|t - median(t)|) over t = [real; imag] of the nonzero

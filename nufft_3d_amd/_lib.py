@@ -63,6 +63,9 @@ SIGNATURES = {
     "nufft3d_inverse_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
                                         C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_void_p,
                                         C.POINTER(Info)]),
+    "nufft3d_inverse_cs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
+                                     C.c_void_p, C.c_int64, C.c_double, C.c_double, C.c_void_p,
+                                     C.POINTER(Info)]),
     "nufft3d_normal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]),
     "nufft3d_discrep": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_int32,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p]),

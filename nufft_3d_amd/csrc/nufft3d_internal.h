@@ -233,6 +233,10 @@ int phase_estimate(nufft3d_plan* p, const void* x, int nc, void* P);
 int phase_mul(nufft3d_plan* p, const void* P, const void* x, void* y, int nc, bool conj);
 int phase_cg(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double lambda,
              const void* wsamp, const void* P, double tol, void* x, nufft3d_info* info);
+// compressed sensing (iNUFT.m:170-189): FISTA with a Haar sparsity prox that
+// zeroes a fraction `frac` of the coefficients; all coils as one vector
+int fista_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double frac,
+                const void* wsamp, double tol, void* x, nufft3d_info* info);
 
 // gram.hip
 int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,

@@ -1621,6 +1621,51 @@ int nufft3d_inverse_phase(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   return st;
 }
 
+int nufft3d_inverse_cs(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit, double damp,
+                       const double* W, int64_t nW, double lambda, double tol, void* im,
+                       nufft3d_info* info) {
+  N3D_PLAN_GUARD(p);
+  if (!raw || !im || nc <= 0) { set_error("invalid raw/im/nc"); return NUFFT3D_ERR_INVALID; }
+  if (maxit < 0) { set_error("maxit must be a nonnegative integer"); return NUFFT3D_ERR_INVALID; }
+  if (!(damp >= 0) || !std::isfinite(damp)) { set_error("damp must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
+  if (!(lambda >= 0) || !(lambda <= 1)) {
+    set_error("lambda (fraction of zero wavelet coefficients) must be in [0, 1]");
+    return NUFFT3D_ERR_INVALID;
+  }
+  if (maxit <= 1)                       // iNUFT.m:78-79: the constraint needs maxit > 1
+    return nufft3d_inverse(p, raw, nc, maxit, damp, W, nW, tol, im, info);
+  if (!(tol > 0)) tol = 1e-4;                                                  // iNUFT.m:114
+  auto t0 = std::chrono::steady_clock::now();
+  const size_t esz = cp_esz(p);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  void* wsamp = nullptr;
+  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  Workspace tmp_out, tmp_io;
+  Staged ri, xo;
+  int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
+  if (!st) st = stage_out_c(p, im, nimg, tmp_out, xo);
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  if (!st) st = ensure(p->ws_grid, ngrid * esz);
+  if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz);
+  if (!st) st = ensure(p->ws_b, nimg * esz);
+  // b = aNUFT(W.*raw) (iNUFT.m:177), x = fistaN(A'WA + damp^2, b, lambda, tol, maxit, Haar)
+  if (!st) st = permute_in(p, ri.dev, nc, p->ws_samp.p, wsamp);
+  if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr);
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, p->ws_b.p, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
+  if (!st) st = fista_solve(p, p->ws_b.p, nc, maxit, damp, lambda, wsamp, tol, xo.dev, info);
+  if (!st) st = finish_out_c(p, im, nimg, xo, tmp_io);
+  if (!st && hipStreamSynchronize(p->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device error in iNUFT (compressed sensing)");
+    st = NUFFT3D_ERR_DEVICE;
+  }
+  release(tmp_out); release(tmp_io);
+  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
 // discrep.m:2-69 (helpers above the C ABI block)
 int nufft3d_discrep(nufft3d_plan* p, const void* data, int32_t nc, double stddev, int32_t iters,
                     double* damp_out, double* stddev_out, double* history) {

@@ -12,6 +12,8 @@
 //   phase C : w = (v - oldeps w1 - delta w2)/gamma; x += phi w; v = y/beta
 // Inner products and norms accumulate in double in a fixed order, so the
 // solve is bitwise reproducible.
+#include <hipcub/hipcub.hpp>
+
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -829,6 +831,282 @@ int phase_cg(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, dou
              const void* wsamp, const void* P, double tol, void* x, nufft3d_info* info) {
   return p->prec == 2 ? phase_cg_t<double>(p, b, nc, maxit, damp, lambda, wsamp, P, tol, x, info)
                       : phase_cg_t<float>(p, b, nc, maxit, damp, lambda, wsamp, P, tol, x, info);
+}
+
+// ---------------------------------------------------------------- compressed sensing
+// iNUFT.m:170-189: FISTA on the normal equations (A'WA + damp^2) x = A'W raw
+// with a Haar-domain sparsity prox (the method of the external fistaN/HWT):
+//   z = y - (A y - b)/L; x' = Q' soft(Q z, t), t zeroing a fraction lambda
+//   of all coefficients; y = x' + ((t_k - 1)/t_{k+1})(x' - x).
+// All coils form one vector (the reference reshapes b to one column).
+// Q: orthonormal 3-D Haar, per level butterflies (a +- b)/sqrt(2) along x,
+// y, z of the low corner (low half first), levels while every non-singleton
+// active length is even, at most 3.
+
+constexpr int kHaarMaxLevels = 3;
+
+static int haar_levels(const int N[3]) {
+  int a[3] = {N[0], N[1], N[2]};
+  int lv = 0;
+  while (lv < kHaarMaxLevels) {
+    bool ok = false, all_even = true;
+    for (int d = 0; d < 3; ++d)
+      if (N[d] > 1) {
+        ok = true;
+        all_even = all_even && a[d] % 2 == 0;
+      }
+    if (!ok || !all_even) break;
+    for (int d = 0; d < 3; ++d)
+      if (N[d] > 1) a[d] /= 2;
+    ++lv;
+  }
+  return lv;
+}
+
+// one butterfly pass along `axis` over the active block [a1 a2 a3]; every
+// other element is copied through
+template <typename R>
+__global__ void k_haar_axis(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int N1, int N2,
+                            int N3, long long total, int a1, int a2, int a3, int axis, int inverse) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long V = (long long)N1 * N2 * N3;
+  const long long v = i % V;
+  const long long cb = i - v;
+  const int x = (int)(v % N1), y = (int)((v / N1) % N2), z = (int)(v / ((long long)N1 * N2));
+  if (x >= a1 || y >= a2 || z >= a3) {
+    out[i] = in[i];
+    return;
+  }
+  const int len = axis == 0 ? a1 : (axis == 1 ? a2 : a3);
+  const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+  const long long stride = axis == 0 ? 1 : (axis == 1 ? (long long)N1 : (long long)N1 * N2);
+  const long long base = cb + v - (long long)pos * stride;
+  const int h = len / 2;
+  const R r2 = (R)0.70710678118654752440;
+  cplx<R> o;
+  if (!inverse) {
+    const int j = pos < h ? pos : pos - h;
+    const cplx<R> a = in[base + (long long)(2 * j) * stride], b = in[base + (long long)(2 * j + 1) * stride];
+    if (pos < h) { o.x = (a.x + b.x) * r2; o.y = (a.y + b.y) * r2; }
+    else { o.x = (a.x - b.x) * r2; o.y = (a.y - b.y) * r2; }
+  } else {
+    const int j = pos / 2;
+    const cplx<R> lo = in[base + (long long)j * stride], hi = in[base + (long long)(h + j) * stride];
+    if ((pos & 1) == 0) { o.x = (lo.x + hi.x) * r2; o.y = (lo.y + hi.y) * r2; }
+    else { o.x = (lo.x - hi.x) * r2; o.y = (lo.y - hi.y) * r2; }
+  }
+  out[i] = o;
+}
+
+template <typename R>
+static int haar_t(nufft3d_plan* p, cplx<R>* buf, cplx<R>* tmp, int nc, bool inverse) {
+  const int N[3] = {p->N[0], p->N[1], p->N[2]};
+  const int L = haar_levels(N);
+  const long long total = (long long)N[0] * N[1] * N[2] * nc;
+  const unsigned nb = (unsigned)((total + 255) / 256);
+  cplx<R>* src = buf;
+  cplx<R>* dst = tmp;
+  ProfScope ps(p, "k_haar_axis");
+  for (int k = 0; k < L; ++k) {
+    const int lv = inverse ? L - 1 - k : k;
+    const int a[3] = {N[0] > 1 ? N[0] >> lv : 1, N[1] > 1 ? N[1] >> lv : 1, N[2] > 1 ? N[2] >> lv : 1};
+    for (int q = 0; q < 3; ++q) {
+      const int axis = inverse ? 2 - q : q;
+      if (N[axis] <= 1) continue;
+      hipLaunchKernelGGL((k_haar_axis<R>), dim3(nb), dim3(256), 0, p->stream, (const cplx<R>*)src, dst,
+                         N[0], N[1], N[2], total, a[0], a[1], a[2], axis, inverse ? 1 : 0);
+      std::swap(src, dst);
+    }
+  }
+  if (src != buf)
+    N3D_HIP(hipMemcpyAsync(buf, src, (size_t)total * sizeof(cplx<R>), hipMemcpyDeviceToDevice, p->stream));
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+__global__ void k_magnitudes(const cplx<R>* __restrict__ w, long long n, double* __restrict__ m) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<R> a = w[i];
+  m[i] = sqrt((double)a.x * (double)a.x + (double)a.y * (double)a.y);
+}
+
+// w *= max(0, 1 - t/|w|) with t = sorted[k] (zero where |w| <= t)
+template <typename R>
+__global__ void k_soft_thresh(cplx<R>* __restrict__ w, long long n, const double* __restrict__ sorted,
+                              long long k) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = sorted[k];
+  const cplx<R> a = w[i];
+  const double m = sqrt((double)a.x * (double)a.x + (double)a.y * (double)a.y);
+  const double s = m > t ? 1.0 - t / m : 0.0;
+  cplx<R> o;
+  o.x = (R)((double)a.x * s);
+  o.y = (R)((double)a.y * s);
+  w[i] = o;
+}
+
+// z = y - (g - b) / L
+template <typename R>
+__global__ void k_fista_grad(const cplx<R>* __restrict__ y, const cplx<R>* __restrict__ g,
+                             const cplx<R>* __restrict__ b, cplx<R>* __restrict__ z, long long n, double invL) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<R> yv = y[i], gv = g[i], bv = b[i];
+  const R s = (R)invL;
+  cplx<R> o;
+  o.x = yv.x - (gv.x - bv.x) * s;
+  o.y = yv.y - (gv.y - bv.y) * s;
+  z[i] = o;
+}
+
+// y = xn + c (xn - x); x = xn
+template <typename R>
+__global__ void k_fista_momentum(const cplx<R>* __restrict__ xn, cplx<R>* __restrict__ x,
+                                 cplx<R>* __restrict__ y, long long n, double c) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<R> a = xn[i], b = x[i];
+  const R cc = (R)c;
+  cplx<R> o;
+  o.x = a.x + cc * (a.x - b.x);
+  o.y = a.y + cc * (a.y - b.y);
+  y[i] = o;
+  x[i] = a;
+}
+
+template <typename R>
+__global__ void k_scale_vec(cplx<R>* __restrict__ v, long long n, double s) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx<R> a = v[i];
+  a.x = (R)((double)a.x * s);
+  a.y = (R)((double)a.y * s);
+  v[i] = a;
+}
+
+template <typename R>
+__global__ void k_fill_vec(cplx<R>* __restrict__ v, long long n, double re) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx<R> a;
+  a.x = (R)re;
+  a.y = 0;
+  v[i] = a;
+}
+
+template <typename R>
+static int norm2_all(nufft3d_plan* p, const cplx<R>* v, long long n, double* out) {
+  const int nb = red_blocks(n);
+  N3D_TRY(ensure(p->ws_red, (size_t)(nb + 8) * sizeof(double)));
+  double* part = (double*)p->ws_red.p;
+  hipLaunchKernelGGL((k_dot_partial<R>), dim3(nb, 1), dim3(kRedThreads), 0, p->stream, v, v, n, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedThreads), 0, p->stream, part, nb, part + nb);
+  N3D_HIP(hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+static int fista_t(nufft3d_plan* p, const void* bv, int nc, int maxit, double damp, double frac,
+                   const void* wsamp, double tol, void* xout, nufft3d_info* info) {
+  using C = cplx<R>;
+  const long long n = (long long)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t vec = (size_t)n * sizeof(C);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  N3D_TRY(ensure(p->ws_solve, 4 * vec));
+  unsigned char* base = (unsigned char*)p->ws_solve.p;
+  C* y = (C*)(base + 0 * vec);
+  C* z = (C*)(base + 1 * vec);
+  C* g = (C*)(base + 2 * vec);
+  C* t = (C*)(base + 3 * vec);
+  C* x = (C*)xout;
+  const C* b = (const C*)bv;
+  const long long nz = (long long)std::llround(frac * (double)n);
+  double *keys = nullptr, *sorted = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+  struct Free {
+    void** q[3];
+    ~Free() { for (void** a : q) if (*a) (void)hipFree(*a); }
+  } fr{{(void**)&keys, (void**)&sorted, &sort_tmp}};
+  if (nz > 0) {
+    N3D_HIP(hipMalloc(&keys, sizeof(double) * n));
+    N3D_HIP(hipMalloc(&sorted, sizeof(double) * n));
+    N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, sorted, (int)n, 0, 64,
+                                              p->stream));
+    N3D_HIP(hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 16));
+  }
+  if (info) {
+    info->nrhs = 1;
+    info->iters[0] = 0;
+    info->relres[0] = 0;
+  }
+  hipLaunchKernelGGL((k_fill_vec<R>), dim3(nb), dim3(256), 0, p->stream, x, n, 0.0);
+  double bn2 = 0;
+  N3D_TRY(norm2_all<R>(p, b, n, &bn2));
+  if (!(bn2 > 0) || maxit <= 0) return NUFFT3D_OK;
+  // Lipschitz constant: 20 power iterations from the constant vector, x 1.05
+  double L = 0;
+  hipLaunchKernelGGL((k_fill_vec<R>), dim3(nb), dim3(256), 0, p->stream, y, n, 1.0 / std::sqrt((double)n));
+  for (int k = 0; k < 20; ++k) {
+    N3D_TRY(apply_normal(p, y, nc, damp, wsamp, g));
+    double g2 = 0;
+    N3D_TRY(norm2_all<R>(p, g, n, &g2));
+    L = std::sqrt(g2);
+    if (!(L > 0)) break;
+    N3D_HIP(hipMemcpyAsync(y, g, vec, hipMemcpyDeviceToDevice, p->stream));
+    hipLaunchKernelGGL((k_scale_vec<R>), dim3(nb), dim3(256), 0, p->stream, y, n, 1.0 / L);
+  }
+  if (!(L > 0)) { set_error("compressed sensing: the normal operator is zero"); return NUFFT3D_ERR_INVALID; }
+  L *= 1.05;
+  hipLaunchKernelGGL((k_fill_vec<R>), dim3(nb), dim3(256), 0, p->stream, y, n, 0.0);
+  double tk = 1.0, rel = 0;
+  int it = 0;
+  for (it = 1; it <= maxit; ++it) {
+    N3D_TRY(apply_normal(p, y, nc, damp, wsamp, g));
+    hipLaunchKernelGGL((k_fista_grad<R>), dim3(nb), dim3(256), 0, p->stream, (const C*)y, (const C*)g, b,
+                       z, n, 1.0 / L);
+    N3D_TRY(haar_t<R>(p, z, t, nc, false));
+    if (nz > 0) {
+      ProfScope ps(p, "k_soft_thresh");
+      hipLaunchKernelGGL((k_magnitudes<R>), dim3(nb), dim3(256), 0, p->stream, (const C*)z, n, keys);
+      N3D_HIP(hipcub::DeviceRadixSort::SortKeys(sort_tmp, sort_bytes, keys, sorted, (int)n, 0, 64,
+                                                p->stream));
+      hipLaunchKernelGGL((k_soft_thresh<R>), dim3(nb), dim3(256), 0, p->stream, z, n,
+                         (const double*)sorted, std::min(nz, n) - 1);
+    }
+    N3D_TRY(haar_t<R>(p, z, t, nc, true));
+    double d2 = 0, x2 = 0;
+    N3D_TRY(diff_norm2(p, z, x, n, &d2));
+    N3D_TRY(norm2_all<R>(p, z, n, &x2));
+    const double tn = (1.0 + std::sqrt(1.0 + 4.0 * tk * tk)) / 2.0;
+    hipLaunchKernelGGL((k_fista_momentum<R>), dim3(nb), dim3(256), 0, p->stream, (const C*)z, x, y, n,
+                       (tk - 1.0) / tn);
+    N3D_HIP(hipGetLastError());
+    tk = tn;
+    rel = x2 > 0 ? std::sqrt(d2 / x2) : 0.0;
+    if (rel <= tol) break;
+  }
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  if (info) {
+    info->iters[0] = std::min(it, maxit);
+    info->relres[0] = rel;
+  }
+  return NUFFT3D_OK;
+}
+
+int fista_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, double frac,
+                const void* wsamp, double tol, void* x, nufft3d_info* info) {
+  if (p->N[0] * (long long)p->N[1] * p->N[2] * nc > 2147483647LL) {
+    set_error("compressed sensing: more than 2^31 coefficients");
+    return NUFFT3D_ERR_UNSUPPORTED;
+  }
+  return p->prec == 2 ? fista_t<double>(p, b, nc, maxit, damp, frac, wsamp, tol, x, info)
+                      : fista_t<float>(p, b, nc, maxit, damp, frac, wsamp, tol, x, info);
 }
 
 }  // namespace nufft3d

@@ -218,7 +218,9 @@ class nufft_3d:
         """Inverse by MINRES on A'WA + damp^2 I (iNUFT.m:2-138,225).
 
         constraint 'phase-constraint' runs iNUFT.m:144-166 (each coil with the
-        phase of its own solution); 'parallel-imaging-pruno' runs iNUFT.m:200-218;
+        phase of its own solution); 'compressed-sensing' runs iNUFT.m:170-189
+        (FISTA with a Haar prox, lambda = fraction of zero coefficients);
+        'parallel-imaging-pruno' runs iNUFT.m:200-218;
         'parallel-imaging-sake' runs sake with damp and loraks = lambda_ > 0
         (iNUFT.m:191-196) and returns the image ifft3(ksp) (the reference
         returns the k-space itself as the image, iNUFT.m:195,225);
@@ -231,7 +233,7 @@ class nufft_3d:
             raise ValueError("maxit must be a nonnegative integer")
         if not (np.isfinite(damp) and damp >= 0):
             raise ValueError("damp must be finite and nonnegative")
-        pruno_mode = sake_mode = phase_mode = False
+        pruno_mode = sake_mode = phase_mode = cs_mode = False
         if constraint:                                          # iNUFT.m:75-91
             if maxit <= 1:
                 warnings.warn("phase constraint is only effective when maxit>1")
@@ -249,6 +251,11 @@ class nufft_3d:
                 if not (np.isscalar(lambda_) and np.isfinite(lambda_) and lambda_ >= 0):
                     raise ValueError("lambda must be a finite nonnegative scalar")   # iNUFT.m:90
                 phase_mode = True
+            elif constraint == "compressed-sensing":
+                if not (np.isscalar(lambda_) and 0 <= lambda_ <= 1):
+                    raise ValueError("lambda (fraction of zero wavelet coefficients) must be in "
+                                     "[0, 1]")
+                cs_mode = True
             else:
                 raise NotImplementedError(
                     "constraint %r is not implemented on the HIP path" % constraint)
@@ -283,6 +290,10 @@ class nufft_3d:
                 return self._lib.nufft3d_inverse_phase(self._h, rptr, nc, int(maxit), float(damp),
                                                        wptr, nW, float(lambda_), float(tol), xptr,
                                                        C.byref(info))
+            if cs_mode:                                         # iNUFT.m:170-189
+                return self._lib.nufft3d_inverse_cs(self._h, rptr, nc, int(maxit), float(damp),
+                                                    wptr, nW, float(lambda_), float(tol), xptr,
+                                                    C.byref(info))
             return self._lib.nufft3d_inverse(self._h, rptr, nc, int(maxit), float(damp), wptr, nW,
                                              float(tol), xptr, C.byref(info))
 
@@ -305,7 +316,7 @@ class nufft_3d:
             x = np.empty((self.N[0], self.N[1], self.N[2], nc), dtype=self.cdtype, order="F")
             _lib.check(solve(rb.ctypes.data, nc, x.ctypes.data))
         if return_info:
-            n = min(nc, 64)
+            n = min(info.nrhs if info.nrhs > 0 else nc, 64)
             return x, {"iters": list(info.iters[:n]), "relres": list(info.relres[:n]),
                        "seconds": info.seconds}
         return x

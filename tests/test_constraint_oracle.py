@@ -122,3 +122,68 @@ def test_large_lambda_pulls_the_solution_onto_the_phase(small):
     strong = PC.inufft_phase(orc, data, 20, 0.0, 10.0)
     off = lambda im: np.linalg.norm(np.imag(np.conj(P) * im)) / np.linalg.norm(im)   # noqa: E731
     assert off(strong) < 0.1 * off(weak)
+
+
+# ---------------------------------------------------------------- compressed sensing
+
+def _haar_matrix(n):
+    """One-level analysis matrix of length n: low rows (e_2i + e_2i+1)/sqrt2, then high rows."""
+    H = np.zeros((n, n))
+    for i in range(n // 2):
+        H[i, 2 * i] = H[i, 2 * i + 1] = np.sqrt(0.5)
+        H[n // 2 + i, 2 * i] = np.sqrt(0.5)
+        H[n // 2 + i, 2 * i + 1] = -np.sqrt(0.5)
+    return H
+
+
+def test_haar_one_level_is_the_kronecker_butterfly():
+    x = _c((4, 6, 2, 1), 12)
+    w = PC.haar_fwd(x, levels=1)
+    # column-major vec: x fastest, so the 3-D transform is Hz (x) Hy (x) Hx
+    K = np.kron(_haar_matrix(2), np.kron(_haar_matrix(6), _haar_matrix(4)))
+    want = K @ x[..., 0].ravel(order='F')
+    assert np.allclose(w[..., 0].ravel(order='F'), want, atol=1e-13)
+
+
+def test_haar_multilevel_orthonormal_and_nested():
+    x = _c((8, 16, 4, 2), 13)
+    assert PC.haar_levels((8, 16, 4)) == 2 and PC.haar_levels((16, 16, 16)) == 3
+    assert PC.haar_levels((12, 12, 1)) == 2 and PC.haar_levels((7, 8, 8)) == 0
+    w = PC.haar_fwd(x)
+    assert np.linalg.norm(w) == pytest.approx(np.linalg.norm(x), rel=1e-13)
+    assert np.allclose(PC.haar_inv(w), x, atol=1e-13)
+    w1 = PC.haar_fwd(x, levels=1)
+    # level 2 only transforms the low corner of level 1
+    keep = np.ones(x.shape[:3], bool)
+    keep[:4, :8, :2] = False
+    assert np.array_equal(w[keep], w1[keep])
+    assert np.allclose(w[:4, :8, :2], PC.haar_fwd(w1[:4, :8, :2], levels=1), atol=1e-13)
+
+
+def test_sparsity_thresh_zero_count_and_shrinkage():
+    w = _c((6, 5, 4, 2), 14)
+    out, t = PC.sparsity_thresh(w, 0.3)
+    nz = int(round(0.3 * w.size))
+    assert np.count_nonzero(out == 0) == nz
+    m = np.abs(w) > t
+    assert np.allclose(np.abs(out[m]), np.abs(w[m]) - t)
+    assert np.allclose(np.angle(out[m]), np.angle(w[m]))
+    same, t0 = PC.sparsity_thresh(w, 0.0)
+    assert t0 == 0.0 and np.array_equal(same, w)
+
+
+def test_fista_without_sparsity_converges_to_least_squares(small):
+    orc, data = small
+    # damp = 1 makes the problem strongly convex (linear rate)
+    x, info = PC.inufft_cs(orc, data, 400, 1.0, 0.0, tol=1e-12, return_info=True)
+    ls = orc.iNUFT(data, 400, 1.0, tol=1e-13)
+    assert info['iters'] < 400
+    assert np.allclose(x, ls, rtol=1e-9, atol=1e-10 * np.abs(ls).max())
+
+
+def test_fista_sparsity_zeroes_the_requested_fraction(small):
+    orc, data = small
+    x = PC.inufft_cs(orc, data, 30, 0.0, 0.5)
+    w = PC.haar_fwd(x)
+    frac = np.count_nonzero(np.abs(w) < 1e-12 * np.abs(w).max()) / w.size
+    assert 0.35 < frac <= 0.5 + 1e-9

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -112,8 +113,10 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
                                                      const GramCol* __restrict__ colsB,
                                                      const int2* __restrict__ tiles,
                                                      float* __restrict__ partial, GramGeom g) {
-  __shared__ unsigned short sAh[GT][GK + GPAD], sAl[GT][GK + GPAD];
-  __shared__ unsigned short sBh[GT][GK + GPAD], sBl[GT][GK + GPAD];
+  // two stages: the MFMAs read stage (step & 1) while the next step's
+  // operands are written to the other, one barrier per step
+  __shared__ unsigned short sAh[2][GT][GK + GPAD], sAl[2][GT][GK + GPAD];
+  __shared__ unsigned short sBh[2][GT][GK + GPAD], sBl[2][GT][GK + GPAD];
   const int2 tp = tiles[blockIdx.x];
   const int I0 = tp.x * GT, J0 = tp.y * GT;
   const long long s0 = (long long)blockIdx.y * g.spw;
@@ -160,19 +163,23 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
       rb[0] = rb[1] = rb[2] = rb[3] = zero;
     }
   };
-  if (s0 < s1) fetch(s0);
+  auto stage = [&](int b) {
+    uint4* ah = reinterpret_cast<uint4*>(&sAh[b][cl][16 * hh]);
+    uint4* al = reinterpret_cast<uint4*>(&sAl[b][cl][16 * hh]);
+    uint4* bh = reinterpret_cast<uint4*>(&sBh[b][cl][16 * hh]);
+    uint4* bl = reinterpret_cast<uint4*>(&sBl[b][cl][16 * hh]);
+    ah[0] = ra[0]; ah[1] = ra[1]; al[0] = ra[2]; al[1] = ra[3];
+    bh[0] = rb[0]; bh[1] = rb[1]; bl[0] = rb[2]; bl[1] = rb[3];
+  };
+  if (s0 < s1) {
+    fetch(s0);
+    stage(0);
+  }
+  __syncthreads();
   for (long long step = s0; step < s1; ++step) {
-    __syncthreads();   // the previous step's MFMA operand reads are done
-    {
-      uint4* ah = reinterpret_cast<uint4*>(&sAh[cl][16 * hh]);
-      uint4* al = reinterpret_cast<uint4*>(&sAl[cl][16 * hh]);
-      uint4* bh = reinterpret_cast<uint4*>(&sBh[cl][16 * hh]);
-      uint4* bl = reinterpret_cast<uint4*>(&sBl[cl][16 * hh]);
-      ah[0] = ra[0]; ah[1] = ra[1]; al[0] = ra[2]; al[1] = ra[3];
-      bh[0] = rb[0]; bh[1] = rb[1]; bl[0] = rb[2]; bl[1] = rb[3];
-    }
-    __syncthreads();
-    if (step + 1 < s1) fetch(step + 1);
+    const int cur = (int)((step - s0) & 1);
+    const bool more = step + 1 < s1;
+    if (more) fetch(step + 1);   // in flight during this step's MFMAs
 #pragma unroll
     for (int kk = 0; kk < GK / 16; ++kk) {
       const int koff = kk * 16 + 8 * (lane >> 5);
@@ -180,14 +187,14 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
         const int row = wm * 64 + mb * 32 + (lane & 31);
-        ah[mb] = *reinterpret_cast<const bf16x8*>(&sAh[row][koff]);
-        al[mb] = *reinterpret_cast<const bf16x8*>(&sAl[row][koff]);
+        ah[mb] = *reinterpret_cast<const bf16x8*>(&sAh[cur][row][koff]);
+        al[mb] = *reinterpret_cast<const bf16x8*>(&sAl[cur][row][koff]);
       }
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) {
         const int col = wn * 64 + nb * 32 + (lane & 31);
-        bh[nb] = *reinterpret_cast<const bf16x8*>(&sBh[col][koff]);
-        bl[nb] = *reinterpret_cast<const bf16x8*>(&sBl[col][koff]);
+        bh[nb] = *reinterpret_cast<const bf16x8*>(&sBh[cur][col][koff]);
+        bl[nb] = *reinterpret_cast<const bf16x8*>(&sBl[cur][col][koff]);
       }
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb)
@@ -198,6 +205,11 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
           acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bh[nb], acc[mb][nb], 0, 0, 0);
         }
     }
+    // the other stage was last read in the previous step, before the barrier
+    // that ended it; the barrier below publishes it for the next step and
+    // ends this step's reads of the current stage
+    if (more) stage(cur ^ 1);
+    __syncthreads();
   }
   // partial[(chunk*ntiles + tile)][128][128], row i (X_I column), col j
   float* out = partial + ((long long)blockIdx.y * gridDim.x + blockIdx.x) * (GT * GT);
@@ -213,32 +225,40 @@ __global__ void __launch_bounds__(256) k_gram_bf16x3(const unsigned short* __res
       }
 }
 
-// fetch C[r][c] = sum over chunks of the partial tiles (fixed order, double);
-// sym: only the upper tiles were computed (C symmetric)
-__device__ __forceinline__ double cfetch(const float* __restrict__ partial, const int* __restrict__ tid_of,
-                                         int ntiles, int nsplit, int ntr, int r, int c, bool sym) {
+// red[t][e] = sum over the V chunks k of partial[k][t][e] (fixed order, double)
+__global__ void k_gram_reduce(const float* __restrict__ partial, int ntiles, int nsplit,
+                              double* __restrict__ red) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)GT * GT;
+  if (idx >= (long long)ntiles * per) return;
+  double s = 0;
+  for (int k = 0; k < nsplit; ++k) s += (double)partial[(long long)k * ntiles * per + idx];
+  red[idx] = s;
+}
+
+// C[r][c] from the reduced tiles; sym: only the upper tiles were computed
+// (C symmetric)
+__device__ __forceinline__ double cfetch(const double* __restrict__ red, const int* __restrict__ tid_of,
+                                         int ntr, int r, int c, bool sym) {
   int tr = r / GT, tc = c / GT, lr = r % GT, lc = c % GT;
   if (sym && tr > tc) { int t = tr; tr = tc; tc = t; t = lr; lr = lc; lc = t; }
   const int t = tid_of[tr * ntr + tc];
-  double s = 0;
-  for (int k = 0; k < nsplit; ++k) s += (double)partial[((long long)k * ntiles + t) * (GT * GT) + lr * GT + lc];
-  return s;
+  return red[(long long)t * (GT * GT) + lr * GT + lc];
 }
 
 // complex n x n (column-major, double) from the real Gram C = X^T Y:
 // sum_v conj(x) y = (C11 + C22) + i (C12 - C21)
-__global__ void k_gram_assemble(const float* __restrict__ partial, const int* __restrict__ tid_of,
-                                int ntiles, int nsplit, GramGeom g, int sym,
-                                double2* __restrict__ AA) {
+__global__ void k_gram_assemble(const double* __restrict__ red, const int* __restrict__ tid_of,
+                                GramGeom g, int sym, double2* __restrict__ AA) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)g.n * g.n) return;
   const int a = (int)(idx % g.n), b = (int)(idx / g.n);
   const int n = g.n;
   const bool sy = sym != 0;
-  const double c11 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, a, b, sy);
-  const double c22 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, n + a, n + b, sy);
-  const double c12 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, a, n + b, sy);
-  const double c21 = cfetch(partial, tid_of, ntiles, nsplit, g.ntr, n + a, b, sy);
+  const double c11 = cfetch(red, tid_of, g.ntr, a, b, sy);
+  const double c22 = cfetch(red, tid_of, g.ntr, n + a, n + b, sy);
+  const double c12 = cfetch(red, tid_of, g.ntr, a, n + b, sy);
+  const double c21 = cfetch(red, tid_of, g.ntr, n + a, b, sy);
   double2 o;
   o.x = c11 + c22;
   o.y = c12 - c21;
@@ -547,7 +567,11 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
         }
       const int ntiles = (int)tl.size();
       GramGeom gg = g;
-      int nsplit = (2048 + ntiles - 1) / ntiles;
+      // ~24 work-groups per CU (measured at 96^3, n = 1000: 2048 -> 16.0 ms,
+      // 4096 -> 15.3, 6144 -> 14.8, 8192 -> 15.1); NUFFT3D_GRAM_BLOCKS overrides
+      const char* eb = getenv("NUFFT3D_GRAM_BLOCKS");
+      const int target = eb ? std::max(64, atoi(eb)) : 6144;
+      int nsplit = (target + ntiles - 1) / ntiles;
       if (nsplit > gg.nsteps) nsplit = (int)gg.nsteps;
       gg.spw = (gg.nsteps + nsplit - 1) / nsplit;
       nsplit = (int)((gg.nsteps + gg.spw - 1) / gg.spw);
@@ -555,10 +579,12 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
       int2* dtl = nullptr;
       int* dtid = nullptr;
       float* part = nullptr;
+      double* red = nullptr;
       int rs = NUFFT3D_OK;
       if (hipMalloc(&dtl, sizeof(int2) * ntiles) != hipSuccess ||
           hipMalloc(&dtid, sizeof(int) * tid_of.size()) != hipSuccess ||
-          hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT) != hipSuccess) {
+          hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT) != hipSuccess ||
+          hipMalloc(&red, sizeof(double) * (size_t)ntiles * GT * GT) != hipSuccess) {
         set_error("device allocation for the Gram tiles failed");
         rs = NUFFT3D_ERR_NOMEM;
       }
@@ -575,14 +601,16 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                              (const int2*)dtl, part, gg);
         }
         ProfScope ps(p, "k_gram_assemble");
+        hipLaunchKernelGGL(k_gram_reduce, dim3(nblk((long long)ntiles * GT * GT)), dim3(256), 0,
+                           p->stream, (const float*)part, ntiles, nsplit, red);
         hipLaunchKernelGGL(k_gram_assemble, dim3(nblk((long long)g.n * g.n)), dim3(256), 0,
-                           p->stream, (const float*)part, (const int*)dtid, ntiles, nsplit, gg,
-                           sym ? 1 : 0, out);
+                           p->stream, (const double*)red, (const int*)dtid, gg, sym ? 1 : 0, out);
         if (hipStreamSynchronize(p->stream) != hipSuccess) rs = NUFFT3D_ERR_DEVICE;
       }
       if (dtl) (void)hipFree(dtl);
       if (dtid) (void)hipFree(dtid);
       if (part) (void)hipFree(part);
+      if (red) (void)hipFree(red);
       return rs;
     };
     st = gram_pair(Q, dcols, true, outA);

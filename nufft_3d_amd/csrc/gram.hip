@@ -436,6 +436,52 @@ __global__ void k_null_apply(const cplx<R>* __restrict__ B, const cplx<R>* __res
   }
 }
 
+// ---------------------------------------------------------------- fp64 Gram by FFTs
+// The Gram is a set of circular cross-correlations: with F_c = DFT(D_c),
+//   R_cd[e] = sum_u conj(D_c[u]) D_d[u + e] = conj(DFT(F_c conj(F_d)))[e] / V,
+// AA[(c,j),(d,k)] = R_cd[s_j - s_k]; the LORAKS block uses g = conj(D(flip)):
+// BB[(c,j),(d,k)] = R'_cd[s_j + s_k] with D_d replaced by g_d.
+// P[q][v] = F_c[v] conj(G_d[v]) for the coil pairs q = (c + nc d) in [q0, q0 + nq)
+__global__ void k_pair_prod(const double2* __restrict__ F, const double2* __restrict__ G, long long V,
+                            int nc, int q0, int nq, double2* __restrict__ P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V * nq) return;
+  const int q = q0 + (int)(i / V);
+  const long long v = i % V;
+  const int c = q % nc, d = q / nc;
+  const double2 a = F[(long long)c * V + v], b = G[(long long)d * V + v];
+  double2 o;
+  o.x = a.x * b.x + a.y * b.y;
+  o.y = a.y * b.x - a.x * b.y;
+  P[i] = o;
+}
+
+// out[(c + nc j) + n (d + nc k)] = conj(Q[q - q0][e]) / V with the lag
+// e = s_j - s_k (sgn = -1) or s_j + s_k (sgn = +1), wrapped per axis
+__global__ void k_lag_gather(const double2* __restrict__ Q, int N1, int N2, int N3, const int4* __restrict__ taps,
+                             int nk, int nc, int q0, int nq, int sgn, double2* __restrict__ out) {
+  const long long per = (long long)nk * nk;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= per * nq) return;
+  const int ql = (int)(i / per);
+  const int q = q0 + ql;
+  const int j = (int)((i % per) % nk), k = (int)((i % per) / nk);
+  const int c = q % nc, d = q / nc;
+  const int4 a = taps[j], b = taps[k];
+  int ex = (a.x + sgn * b.x) % N1, ey = (a.y + sgn * b.y) % N2, ez = (a.z + sgn * b.z) % N3;
+  ex += ex < 0 ? N1 : 0;
+  ey += ey < 0 ? N2 : 0;
+  ez += ez < 0 ? N3 : 0;
+  const long long V = (long long)N1 * N2 * N3;
+  const double2 r = Q[(long long)ql * V + ex + (long long)N1 * (ey + (long long)N2 * ez)];
+  const double iv = 1.0 / (double)V;
+  const long long n = (long long)nk * nc;
+  double2 o;
+  o.x = r.x * iv;
+  o.y = -r.y * iv;
+  out[(c + (long long)nc * j) + n * (d + (long long)nc * k)] = o;
+}
+
 // ---------------------------------------------------------------- host side
 static inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
@@ -461,6 +507,48 @@ void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int&
 // 221-224), BB the Gram of the data against its conjugate reflection about
 // the 1-based centre[3]: column (d,k) of that operand is
 // conj(data_d[flip(v) - s_k]) = g_d[v + s_k], g = conj(data(flip)).
+// fp64 Gram of image-size data by FFTs (needs dims = the plan's N and planned
+// N-size axes); returns NUFFT3D_ERR_UNSUPPORTED to fall back to k_gram_fp64
+static int gram_fft64(nufft3d_plan* p, const void* data, const void* flip, const int dims[4],
+                      const int4* dtaps, int nk, double2* outA, double2* outB) {
+  if (dims[0] != p->N[0] || dims[1] != p->N[1] || dims[2] != p->N[2]) return NUFFT3D_ERR_UNSUPPORTED;
+  const int nc = dims[3];
+  const long long V = (long long)dims[0] * dims[1] * dims[2];
+  const size_t vb = sizeof(double2) * (size_t)V;
+  double2 *F = nullptr, *G = nullptr, *P = nullptr, *Q = nullptr;
+  struct Free {
+    double2** q[4];
+    ~Free() { for (double2** a : q) if (*a) (void)hipFree(*a); }
+  } fr{{&F, &G, &P, &Q}};
+  const int npair = nc * nc;
+  const int chunk = (int)std::max<long long>(1, std::min<long long>(npair, (1LL << 31) / (long long)vb));
+  if (hipMalloc(&F, vb * nc) != hipSuccess || (flip && hipMalloc(&G, vb * nc) != hipSuccess) ||
+      hipMalloc(&P, vb * chunk) != hipSuccess || hipMalloc(&Q, vb * chunk) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("device allocation for the FFT Gram failed");
+    return NUFFT3D_ERR_NOMEM;
+  }
+  N3D_TRY(fft_plain(p, data, nc, F));
+  if (flip) N3D_TRY(fft_plain(p, flip, nc, G));
+  for (int pass = 0; pass < (flip ? 2 : 1); ++pass) {
+    const double2* Gd = pass == 0 ? F : G;
+    double2* out = pass == 0 ? outA : outB;
+    for (int q0 = 0; q0 < npair; q0 += chunk) {
+      const int nq = std::min(chunk, npair - q0);
+      ProfScope ps(p, "k_gram_fft64");
+      hipLaunchKernelGGL(k_pair_prod, dim3(nblk(V * nq)), dim3(256), 0, p->stream, (const double2*)F, Gd, V,
+                         nc, q0, nq, P);
+      N3D_TRY(fft_plain(p, P, nq, Q));
+      hipLaunchKernelGGL(k_lag_gather, dim3(nblk((long long)nk * nk * nq)), dim3(256), 0, p->stream,
+                         (const double2*)Q, dims[0], dims[1], dims[2], dtaps, nk, nc, q0, nq,
+                         pass == 0 ? -1 : 1, out);
+      N3D_HIP(hipGetLastError());
+    }
+  }
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  return NUFFT3D_OK;
+}
+
 int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
                 double2* dAA, bool loraks, const int* center) {
   GramGeom g;
@@ -494,13 +582,20 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
   double2* outA = loraks ? tmpA : dAA;
   int st = NUFFT3D_OK;
   if (p->prec == 2) {
-    dim3 grid((g.n + 15) / 16, (g.n + 15) / 16);
-    ProfScope ps(p, "k_gram_fp64");
-    hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
-                       (const double2*)data, (const int4*)dtaps, 1, 1, g, outA);
-    if (loraks)
+    st = getenv("NUFFT3D_GRAM64_DIRECT") ? NUFFT3D_ERR_UNSUPPORTED
+                                         : gram_fft64(p, data, loraks ? flip : nullptr, dims, dtaps,
+                                                      g.nk, outA, tmpB);
+    if (st == NUFFT3D_ERR_UNSUPPORTED) {   // other sizes: direct double sums
+      (void)hipGetLastError();
+      st = NUFFT3D_OK;
+      dim3 grid((g.n + 15) / 16, (g.n + 15) / 16);
+      ProfScope ps(p, "k_gram_fp64");
       hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
-                         (const double2*)flip, (const int4*)dtaps, -1, 0, g, tmpB);
+                         (const double2*)data, (const int4*)dtaps, 1, 1, g, outA);
+      if (loraks)
+        hipLaunchKernelGGL(k_gram_fp64, grid, dim3(256), 0, p->stream, (const double2*)data,
+                           (const double2*)flip, (const int4*)dtaps, -1, 0, g, tmpB);
+    }
   } else {
     g.ntr = (g.n2 + GT - 1) / GT;
     int hmax = 0;

@@ -115,3 +115,19 @@ def test_gram_loraks_rejects_a_centre_outside_the_data(plans):
     d = _data((8, 6, 5, 2), 7)
     with pytest.raises(Exception, match="center outside"):
         g64.gram(d, 3, 1, loraks=True, center=(9, 1, 1))
+
+
+@pytest.mark.parametrize("loraks,center", [(False, None), (True, (2, 3, 1))])
+def test_fp64_gram_by_ffts_at_the_plan_size(plans, loraks, center, monkeypatch):
+    # data of the plan's image size takes the cross-correlation (FFT) form;
+    # NUFFT3D_GRAM64_DIRECT forces the direct double sums for comparison
+    om, g32, g64 = plans
+    d = _data((8, 8, 8, 3), 7)
+    taps, _ = P.kernel_taps(4, 1)
+    want = P.make_data_matrix(d, taps, loraks, center or (1, 1, 1))
+    fft = g64.gram(d, 4, 1, loraks=loraks, center=center)
+    assert rel(fft, want) < 1e-12
+    monkeypatch.setenv("NUFFT3D_GRAM64_DIRECT", "1")
+    direct = g64.gram(d, 4, 1, loraks=loraks, center=center)
+    assert rel(direct, want) < 1e-12
+    assert rel(fft, direct) < 1e-12

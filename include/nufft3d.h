@@ -77,6 +77,14 @@ int nufft3d_opts_default(nufft3d_opts* opts);
 int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
                         const nufft3d_opts* opts, nufft3d_plan** plan);
 
+/* The warnings the reference constructor issues with warning(...), in order:
+ * "value of J=... is not recommended" (nufft_3d.m:79-81), "radial kernel not
+ * recommended with J<3 or u<1.5" (nufft_3d.m:93-95), "mean(density) should
+ * be 1 (...)" for a user density (nufft_3d.m:235-237). Writes them to buf
+ * (may be NULL) separated by newlines, truncated to len-1 bytes, and returns
+ * their number. */
+int nufft3d_plan_warnings(nufft3d_plan* plan, char* buf, int64_t len);
+
 /* Frees the plan and its device memory (MATLAB clearing the object). */
 int nufft3d_plan_destroy(nufft3d_plan* plan);
 

@@ -186,6 +186,7 @@ struct nufft3d_plan {
   // scratch, grown on demand
   nufft3d::Workspace ws_grid, ws_t1, ws_t2, ws_samp, ws_part, ws_img, ws_host, ws_solve, ws_red;
   nufft3d::Workspace ws_b;  // iNUFT right-hand side A'W raw
+  std::vector<std::string> warnings;   // the reference's constructor warnings, in order
   std::mutex mu;
 };
 

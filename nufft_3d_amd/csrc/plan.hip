@@ -520,8 +520,12 @@ static int user_density_t(nufft3d_plan* p, const double* dh, int64_t nd) {
   if (n > 0) N3D_HIP(hipMemcpy(orig.data(), p->d_orig, sizeof(int) * n, hipMemcpyDeviceToHost));
   for (long long i = 0; i < n; ++i) okm[orig[i]] = 1;
   for (int64_t i = 0; i < nd; ++i) s += okm[i] ? tmp[i] : 0.0;
-  if (s / (double)nd != 1.0 && p->verbose)
-    printf("Warning: mean(density) should be 1 (%f)\n", s / (double)nd);
+  if (s / (double)nd != 1.0) {
+    char b[96];
+    snprintf(b, sizeof b, "mean(density) should be 1 (%f)", s / (double)nd);
+    p->warnings.push_back(b);
+    if (p->verbose) printf("Warning: %s\n", b);
+  }
   return NUFFT3D_OK;
 }
 
@@ -1090,6 +1094,20 @@ extern "C" {
 
 const char* nufft3d_last_error(void) { return g_last_error.c_str(); }
 
+int nufft3d_plan_warnings(nufft3d_plan* p, char* buf, int64_t len) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  if (buf && len <= 0) { set_error("invalid buffer length"); return NUFFT3D_ERR_INVALID; }
+  std::string all;
+  for (size_t i = 0; i < p->warnings.size(); ++i) all += (i ? "\n" : "") + p->warnings[i];
+  if (buf) {
+    const size_t n = std::min<size_t>(all.size(), (size_t)len - 1);
+    memcpy(buf, all.data(), n);
+    buf[n] = 0;
+  }
+  return (int)p->warnings.size();
+}
+
 int nufft3d_opts_default(nufft3d_opts* o) {
   if (!o) { set_error("opts is NULL"); return NUFFT3D_ERR_INVALID; }
   std::memset(o, 0, sizeof(*o));
@@ -1150,11 +1168,15 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
     if (p->radial)
       p->alpha = M_PI * sqrt(p->J * p->J * (1 - 0.5 / p->u) * (1 - 0.5 / p->u) - 0.6 * p->u);
   }
-  if (o.verbose) {
-    if (p->J < 1 || p->J > 7) printf("Warning: value of J=%f is not recommended\n", p->J);
-    if (p->radial && (p->J < 3 || p->u < 1.5))
-      printf("Warning: radial kernel not recommended with J<3 or u<1.5\n");
+  if (p->J < 1 || p->J > 7) {                                              // nufft_3d.m:79-81
+    char b[96];
+    snprintf(b, sizeof b, "value of J=%f is not recommended", p->J);
+    p->warnings.push_back(b);
   }
+  if (p->radial && (p->J < 3 || p->u < 1.5))                               // nufft_3d.m:93-95
+    p->warnings.push_back("radial kernel not recommended with J<3 or u<1.5");
+  if (o.verbose)
+    for (const std::string& w : p->warnings) printf("Warning: %s\n", w.c_str());
   for (int a = 0; a < 3; ++a) p->N[a] = N[a];
   p->device = o.device;
   p->verbose = o.verbose;

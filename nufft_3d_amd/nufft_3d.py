@@ -84,6 +84,11 @@ class nufft_3d:
         _lib.check(lib.nufft3d_plan_create(om_c.ctypes.data, M, Ni, C.byref(opts), C.byref(h)))
         self._h = h
         self._lib = lib
+        wbuf = C.create_string_buffer(4096)
+        nw = lib.nufft3d_plan_warnings(h, wbuf, len(wbuf))   # nufft_3d.m:79-81,93-95,235-237
+        if nw > 0:
+            for msg in wbuf.value.decode(errors="replace").split("\n"):
+                warnings.warn(msg)
         K = (C.c_int32 * 3)()
         Nout = (C.c_int32 * 3)()
         Jd, ud, ad = C.c_double(), C.c_double(), C.c_double()

@@ -342,3 +342,17 @@ def test_forward_written_in_caller_layout_equals_permuted(gpu, nc):
     direct = nufft_3d(om, 16, gpu=gpu)
     perm = _plan_with_env({"NUFFT3D_TUNE": "18"}, om, 16, gpu=gpu)
     assert np.array_equal(direct.fNUFT(x), perm.fNUFT(x))
+
+
+def test_constructor_warnings_match_the_reference():
+    # nufft_3d.m:93-95 (fires for J=6, u=1.25) and :235-237 (user density
+    # whose mean over all rows is not 1); the defaults warn about nothing
+    import warnings
+    om = synth.koosh_ball(8, 20, 16)
+    with pytest.warns(UserWarning, match="radial kernel not recommended with J<3 or u<1.5"):
+        nufft_3d(om, 8, J=6, u=1.25, radial=1, gpu=1)
+    with pytest.warns(UserWarning, match=r"mean\(density\) should be 1"):
+        nufft_3d(om, 8, gpu=1, d=np.full(om.shape[1], 2.0))
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        nufft_3d(om, 8, gpu=1)

@@ -79,3 +79,17 @@ def test_lambda_out_of_range(setup):
     g = nufft_3d(om, N, J=4, u=2, radial=1, gpu=1)
     with pytest.raises(ValueError):
         g.iNUFT(data, 10, 0.0, None, "compressed-sensing", 1.5)
+
+
+def test_cs_on_a_2d_plan_matches_oracle():
+    # 2-D plan (N3 = 1): the Haar transform runs on x and y only
+    r = np.random.default_rng(17)
+    om = np.vstack([r.uniform(-8, 8, size=(2, 800)), np.zeros((1, 800))])
+    orc = O.NufftOracle(om, 16)
+    g = nufft_3d(om, 16, gpu=2)
+    x = r.normal(size=(16, 16, 1, 2)) + 1j * r.normal(size=(16, 16, 1, 2))
+    data = orc.fNUFT(x)
+    want, wi = PC.inufft_cs(orc, data, 12, 0.05, 0.4, return_info=True)
+    got, gi = g.iNUFT(data, 12, 0.05, None, "compressed-sensing", 0.4, return_info=True)
+    assert gi["iters"] == [wi["iters"]]
+    assert _rel(got, want) < 1e-9

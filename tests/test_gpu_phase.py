@@ -83,3 +83,19 @@ def test_torch_input_matches_host(setup):
     dev = g.iNUFT(dt, 8, 0.05, None, "phase-constraint", 0.5)
     d = dev.cpu().numpy().transpose(3, 2, 1, 0)
     assert _rel(d, host) < 1e-5
+
+
+def test_phase_on_a_2d_plan_matches_oracle():
+    # all om(3,:)==0 selects the 2-D path (nufft_3d.m:104-106); the phase box
+    # sum then covers the 3x3 in-plane neighbours
+    r = np.random.default_rng(16)
+    om = np.vstack([r.uniform(-8, 8, size=(2, 800)), np.zeros((1, 800))])
+    orc = O.NufftOracle(om, 16)
+    g = nufft_3d(om, 16, gpu=2)
+    assert g.N == (16, 16, 1)
+    x = r.normal(size=(16, 16, 1, 2)) + 1j * r.normal(size=(16, 16, 1, 2))
+    data = orc.fNUFT(x)
+    want, wi = PC.inufft_phase(orc, data, 8, 0.05, 0.5, return_info=True)
+    got, gi = g.iNUFT(data, 8, 0.05, None, "phase-constraint", 0.5, return_info=True)
+    assert gi["iters"] == wi["iters"]
+    assert _rel(got, want) < 1e-9

@@ -125,3 +125,14 @@ def test_argument_errors(setup):
         g.sake(data, maxit=1, bogus=3)
     with pytest.raises(ValueError, match="multiple coils"):
         g.iNUFT(data[:, :1], 20, 0.0, None, "parallel-imaging-sake", 0.0)
+
+
+def test_2d_plan_is_reported_unsupported():
+    # sake.m builds 3-D taps always (sake.m:62); a 2-D plan (N3 = 1) would
+    # wrap every z tap onto the same plane: refused with a clear status
+    r = np.random.default_rng(18)
+    om = np.vstack([r.uniform(-8, 8, size=(2, 800)), np.zeros((1, 800))])
+    g = nufft_3d(om, 16, gpu=2)
+    data = r.normal(size=(800, 2)) + 1j * r.normal(size=(800, 2))
+    with pytest.raises(Exception, match="kernel half width"):
+        g.sake(data, maxit=1)

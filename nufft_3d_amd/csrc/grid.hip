@@ -34,14 +34,17 @@ struct GridGeom {
 // L2, while every XCD still gets a share of the dense k-space centre. Blocks
 // past the last whole round of 8C keep their number. C < 0: C = grid / 8,
 // each XCD walks one contiguous eighth of the (Z-ordered) work.
-__device__ __forceinline__ long long xcd_block(int chunk) {
-  if (chunk < 0) chunk = (int)(gridDim.x / 8);   // one contiguous eighth per XCD
-  if (chunk <= 0) return blockIdx.x;
-  const unsigned nb = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ long long xcd_block_of(int chunk, unsigned b, unsigned nb) {
+  if (chunk < 0) chunk = (int)(nb / 8);          // one contiguous eighth per XCD
+  if (chunk <= 0) return b;
   const unsigned span = 8u * (unsigned)chunk;
   if (b >= nb / span * span) return b;
   const unsigned x = b % 8, k = b / 8;
   return (long long)((k / chunk) * 8 + x) * chunk + k % chunk;
+}
+
+__device__ __forceinline__ long long xcd_block(int chunk) {
+  return xcd_block_of(chunk, blockIdx.x, gridDim.x);
 }
 
 __device__ __forceinline__ int wrapk(int g, int K) {
@@ -323,6 +326,37 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
 #undef N3D_STEP
 
 // ---------------------------------------------------------------------------
+// One segment of a dense point's hit list -> partial[slot][cg] (work-group b
+// of the segment blocks)
+template <typename R, int CPL, int LPS, bool COMP, bool FULL>
+__device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* __restrict__ segs, int nseg,
+                                                   const uint2* __restrict__ hits,
+                                                   const double* __restrict__ off,
+                                                   const float4* __restrict__ off32,
+                                                   const cplx<R>* __restrict__ ks,
+                                                   const R* __restrict__ wsamp,
+                                                   cplx<R>* __restrict__ partial, const GridGeom& g,
+                                                   const KBParams& kp) {
+  using A = typename AccT<R, COMP>::type;
+  const long long gt = b * blockDim.x + threadIdx.x;
+  const int si = (int)(gt / LPS);
+  const int q = (int)(gt % LPS);
+  if (si >= nseg) return;
+  const HeavySeg sg = segs[si];
+  A ar[CPL], ai[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
+  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, off, off32, ks, wsamp, g, kp,
+                                 ar, ai);
+  // partial sums keep the accumulator precision until the final reduction
+  A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
+    if (k < g.cg) { pp[2 * k] = ar[c]; pp[2 * k + 1] = ai[c]; }
+  }
+}
+
 // Adjoint, output driven: an LPS-lane group per grid point walks its hit list
 // (sorted sample ids, fixed at plan time) and sums in registers. No atomics:
 // the result is bitwise reproducible. Points with more than rmax hits (the
@@ -339,9 +373,20 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
                                                 const R* __restrict__ wsamp,
                                                 cplx<R>* __restrict__ grid, long long ngrid,
                                                 const int* __restrict__ bricks,
-                                                int rmax, GridGeom g, KBParams kp) {
+                                                int rmax, GridGeom g, KBParams kp,
+                                                const HeavySeg* __restrict__ segs, int nseg,
+                                                int nhb, cplx<R>* __restrict__ partial) {
   using A = typename AccT<R, COMP>::type;
-  const long long gt = xcd_block(g.xchunk) * blockDim.x + threadIdx.x;
+  // the first nhb work-groups (a multiple of 8, so the XCD of every later
+  // work-group is unchanged) sum the dense points' segments; the rest are
+  // the light points, renumbered for the XCDs among themselves
+  if ((int)blockIdx.x < nhb) {
+    spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, off, off32, ks, wsamp,
+                                                partial, g, kp);
+    return;
+  }
+  const long long gt = xcd_block_of(g.xchunk, blockIdx.x - nhb, gridDim.x - nhb) * blockDim.x +
+                       threadIdx.x;
   const long long lin = gt / LPS;
   const int q = (int)(gt % LPS);
   if (lin >= ngrid) return;
@@ -468,7 +513,6 @@ __global__ void __launch_bounds__(512) k_spread_tile(const long long* __restrict
   }
 }
 
-// One segment of a dense point's hit list -> partial[slot][cg]
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const uint2* __restrict__ hits,
@@ -478,24 +522,8 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
                                                       const R* __restrict__ wsamp,
                                                       cplx<R>* __restrict__ partial, GridGeom g,
                                                       KBParams kp) {
-  using A = typename AccT<R, COMP>::type;
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int si = (int)(gt / LPS);
-  const int q = (int)(gt % LPS);
-  if (si >= nseg) return;
-  const HeavySeg sg = segs[si];
-  A ar[CPL], ai[CPL];
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, off, off32, ks, wsamp, g, kp,
-                                 ar, ai);
-  // partial sums keep the accumulator precision until the final reduction
-  A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
-#pragma unroll
-  for (int c = 0; c < CPL; ++c) {
-    const int k = coil_of<R, CPL, LPS, FULL>(q, c);
-    if (k < g.cg) { pp[2 * k] = ar[c]; pp[2 * k + 1] = ai[c]; }
-  }
+  spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, off, off32, ks, wsamp,
+                                              partial, g, kp);
 }
 
 // Sum of a dense point's segment partials, one work-group per point: the R
@@ -688,6 +716,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   // segments; lane groups: long ones
   const DenseSet& ds = p->dense[LPS == 1 ? 0 : 1];
   constexpr bool kTileable = sizeof(R) == 4 && FULL && LPS * CPL >= 8;
+  bool merged = false;   // dense segments summed inside the k_spread launch
   if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
       p->bl.rmax == ds.rmax) {
     ProfScope ps(p, "k_spread_tile");
@@ -696,17 +725,22 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
                        p->bl.d_samp, p->d_off32, (const float2*)samp, (float2*)grid, p->d_brick,
                        ds.rmax, g, kp);
   } else {
-    // brick order covers whole 8x8x8 bricks (points outside the grid exit)
+    // brick order covers whole 8x8x8 bricks (points outside the grid exit);
+    // the dense points' segments run in the same launch, ahead of the light
+    // points, unless tune flag 32 asks for their own launch
     const long long npts = (p->tune_flags & 2) ? p->nbrick * 512 : ngrid;
     const long long nthreads = npts * LPS;
+    merged = ds.nhseg > 0 && !(p->tune_flags & 32);
+    const int nhb = merged ? (int)((((long long)ds.nhseg * LPS + 255) / 256 + 7) / 8 * 8) : 0;
     ProfScope ps(p, "k_spread");
-    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>), dim3((unsigned)((nthreads + 255) / 256)),
-                       dim3(256), 0, p->stream, p->d_rowptr, p->d_hit,
-                       (const double*)p->d_off, p->d_off32, (const cplx<R>*)samp, (const R*)wsamp,
-                       (cplx<R>*)grid, npts, p->d_brick, ds.rmax, g, kp);
+    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>),
+                       dim3((unsigned)(nhb + (nthreads + 255) / 256)), dim3(256), 0, p->stream,
+                       p->d_rowptr, p->d_hit, (const double*)p->d_off, p->d_off32,
+                       (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)grid, npts, p->d_brick,
+                       ds.rmax, g, kp, ds.d_hseg, ds.nhseg, nhb, (cplx<R>*)partial);
   }
   if (ds.nhseg > 0) {
-    {
+    if (!merged) {
       ProfScope ps(p, "k_spread_heavy");
       const long long nthreads = (long long)ds.nhseg * LPS;
       hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>),

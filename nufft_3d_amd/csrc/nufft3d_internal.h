@@ -127,7 +127,8 @@ struct nufft3d_plan {
   int compensated = 1;      // fp32 plans: products accumulated in double
   int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order,
                             // 8 = LDS-staged spread where the plan has brick lists,
-                            // 16 = forward result through a separate permutation pass)
+                            // 16 = forward result through a separate permutation pass,
+                            // 32 = dense-point segments in their own launch)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)

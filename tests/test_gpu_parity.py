@@ -356,3 +356,23 @@ def test_constructor_warnings_match_the_reference():
     with warnings.catch_warnings():
         warnings.simplefilter("error")
         nufft_3d(om, 8, gpu=1)
+
+
+@pytest.mark.parametrize("gpu,nc", [(1, 16), (1, 3), (2, 5)])
+def test_dense_segments_in_the_spread_launch_equal_their_own_launch(gpu, nc):
+    # the dense points' segments run as the first work-groups of the k_spread
+    # launch by default; tune flag 32 gives them their own k_spread_heavy
+    # launch: bitwise the same grid (radial centre with 300 spokes through it)
+    N = 24
+    om = synth.koosh_ball(N, 300, 2 * N)
+    k = _rand((om.shape[1], nc), 82)
+    sep = _plan_with_env({"NUFFT3D_TUNE": "34"}, om, N, gpu=gpu)
+    mrg = nufft_3d(om, N, gpu=gpu)
+    got = {}
+    ran_s = _kernels_run(sep, lambda: got.setdefault("s", sep.aNUFT(k)))
+    ran_m = _kernels_run(mrg, lambda: got.setdefault("m", mrg.aNUFT(k)))
+    assert "k_spread_heavy" in ran_s and "k_spread_heavy" not in ran_m
+    assert "k_spread_hreduce" in ran_m
+    assert np.array_equal(got["s"], got["m"])
+    orc = O.NufftOracle(om, N)
+    assert rel(got["m"], orc.aNUFT(k)) < (TOL64 if gpu == 2 else TOL32_OP)

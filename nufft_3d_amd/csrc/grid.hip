@@ -639,6 +639,46 @@ __global__ void __launch_bounds__(256) k_permute_in_tile(const int* __restrict__
   }
 }
 
+// The same permutation driven by the caller's rows, one work-group per 64
+// consecutive rows: coil by coil, 64 lanes read one coalesced run of a [M]
+// plane, then each in-bounds row's nc values are written as one contiguous
+// sample row at its sorted position inv[row] (out-of-bounds rows have -1).
+// Every sorted position is written once; values and weights as k_permute_in.
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_in_rows(const int* __restrict__ inv, long long M,
+                                                         int nc, const cplx<R>* __restrict__ ku,
+                                                         const R* __restrict__ wsamp,
+                                                         cplx<R>* __restrict__ ks) {
+  constexpr int TS = 64, CC = 32;
+  __shared__ cplx<R> t[TS][CC + 1];
+  __shared__ int si[TS];
+  __shared__ R sw[TS];
+  const long long r0 = (long long)blockIdx.x * TS;
+  const int nr = M - r0 < TS ? (int)(M - r0) : TS;
+  if (threadIdx.x < nr) {
+    const int s = inv[r0 + threadIdx.x];
+    si[threadIdx.x] = s;
+    sw[threadIdx.x] = (wsamp && s >= 0) ? wsamp[s] : (R)1;
+  }
+  for (int c0 = 0; c0 < nc; c0 += CC) {
+    const int cc = nc - c0 < CC ? nc - c0 : CC;
+    __syncthreads();   // si/sw written; the previous chunk's tile is consumed
+    for (int i = threadIdx.x; i < TS * cc; i += blockDim.x) {
+      const int c = i / TS, j = i % TS;
+      if (j < nr) t[j][c] = ku[r0 + j + (long long)(c0 + c) * M];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nr * cc; i += blockDim.x) {
+      const int j = i / cc, c = i % cc;
+      const int s = si[j];
+      if (s < 0) continue;
+      cplx<R> v = t[j][c];
+      if (wsamp) { v.x *= sw[j]; v.y *= sw[j]; }
+      ks[(long long)s * nc + c0 + c] = v;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 static KBParams make_kb(const nufft3d_plan* p) {
   KBParams k;
@@ -892,6 +932,19 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
   const long long n = p->nnz * (long long)nc;
   if (n == 0) return NUFFT3D_OK;
   ProfScope ps(p, "k_permute_in");
+  if (nc >= 4 && p->d_inv && !(p->tune_flags & 64)) {   // caller rows read in order
+    const unsigned nb = (unsigned)((p->M + 63) / 64);
+    if (p->prec == 2)
+      hipLaunchKernelGGL((k_permute_in_rows<double>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
+                         (long long)p->M, nc, (const double2*)k_user, (const double*)wsamp,
+                         (double2*)ks);
+    else
+      hipLaunchKernelGGL((k_permute_in_rows<float>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
+                         (long long)p->M, nc, (const float2*)k_user, (const float*)wsamp,
+                         (float2*)ks);
+    N3D_HIP(hipGetLastError());
+    return NUFFT3D_OK;
+  }
   if (nc >= 4) {   // tiled: coalesced rows on both sides
     const unsigned nb = (unsigned)((p->nnz + 63) / 64);
     if (p->prec == 2)

@@ -128,7 +128,9 @@ struct nufft3d_plan {
   int tune_flags = 2;       // gridding launch layout (see GridGeom::flags; 4 = linear tile order,
                             // 8 = LDS-staged spread where the plan has brick lists,
                             // 16 = forward result through a separate permutation pass,
-                            // 32 = dense-point segments in their own launch)
+                            // 32 = dense-point segments in their own launch,
+                            // 64 = aNUFT input gathered by sorted sample, not scattered
+                            // by caller row)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
@@ -139,6 +141,7 @@ struct nufft3d_plan {
   // sorted, in-bounds samples
   int* d_orig = nullptr;    // [nnz] original row of sorted sample s
   int* d_oob = nullptr;     // [M-nnz] rows that are out of bounds
+  int* d_inv = nullptr;     // [M] sorted position of caller row i (-1: out of bounds)
   int4* d_base = nullptr;   // [nnz] wrapped base (round(u*om) mod K) + tile id
   void* d_off = nullptr;    // double [3*nnz]: u*om - round(u*om) (exact)
 

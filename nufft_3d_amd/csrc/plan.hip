@@ -254,6 +254,12 @@ __global__ void k_keys(const double* __restrict__ om, long long M, BinGeom g,
   vals[m] = (int)m;
 }
 
+// inv[orig[s]] = s
+__global__ void k_invert(const int* __restrict__ orig, long long nnz, int* __restrict__ inv) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnz) inv[orig[s]] = (int)s;
+}
+
 __global__ void k_gather(const double* __restrict__ om, const unsigned* __restrict__ skeys,
                          const int* __restrict__ svals, long long M, long long nnz, BinGeom g,
                          int* __restrict__ orig, int* __restrict__ oob, int4* __restrict__ base,
@@ -801,6 +807,13 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     hipLaunchKernelGGL(k_gather, dim3(nblocks(M)), dim3(256), 0, p->stream, om, skeys, svals,
                        (long long)M, nnz, g, p->d_orig, p->d_oob, p->d_base, (double*)p->d_off);
   N3D_HIP(hipGetLastError());
+  // inverse of orig: caller row -> sorted position (-1 for out-of-bounds rows)
+  N3D_HIP(hipMalloc(&p->d_inv, sizeof(int) * (M > 0 ? M : 1)));
+  N3D_HIP(hipMemsetAsync(p->d_inv, 0xff, sizeof(int) * (M > 0 ? M : 1), p->stream));
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_invert, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
+                       (const int*)p->d_orig, nnz, p->d_inv);
+  N3D_HIP(hipGetLastError());
   (void)hipFree(keys); (void)hipFree(skeys); (void)hipFree(vals); (void)hipFree(svals);
   (void)hipFree(om);
 
@@ -942,7 +955,7 @@ static void destroy_plan(nufft3d_plan* p) {
   if (!p) return;
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  void* ptrs[] = {p->d_orig, p->d_oob, p->d_base, p->d_off, p->d_U, p->d_dens,
+  void* ptrs[] = {p->d_orig, p->d_oob, p->d_inv, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit,
                   p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
@@ -1260,6 +1273,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   const int64_t nimg = (int64_t)p->N[0] * p->N[1] * p->N[2];
   const int64_t rsz = p->prec == 2 ? 8 : 4;
   int64_t b = 4 * nnz + 4 * noob + 16 * nnz + 24 * nnz;              // orig, oob, base, off
+  b += 4 * std::max<int64_t>(p->M, 1);                                // inv
   b += rsz * nimg + rsz * nnz;                                        // U, density
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
   b += (2 + 8) * std::max<int64_t>(p->nhits, 1);                      // fcode, (sample, code) hits

@@ -376,3 +376,22 @@ def test_dense_segments_in_the_spread_launch_equal_their_own_launch(gpu, nc):
     assert np.array_equal(got["s"], got["m"])
     orc = O.NufftOracle(om, N)
     assert rel(got["m"], orc.aNUFT(k)) < (TOL64 if gpu == 2 else TOL32_OP)
+
+
+@pytest.mark.parametrize("gpu,nc", [(1, 16), (1, 4), (2, 5)])
+def test_caller_row_permutation_equals_sorted_gather(gpu, nc):
+    # aNUFT's input permutation reads the caller's rows in order and writes
+    # each to its sorted position (default), or gathers by sorted sample
+    # (tune flag 64): bitwise the same, with out-of-bounds rows (om scaled
+    # past the grid) skipped and, in iNUFT, the density weights applied
+    N = 16
+    om = synth.koosh_ball(N, 60, 2 * N) * 1.3
+    orc = O.NufftOracle(om, N)
+    assert orc.nnz < om.shape[1]
+    k = _rand((om.shape[1], nc), 83)
+    rows = nufft_3d(om, N, gpu=gpu)
+    gath = _plan_with_env({"NUFFT3D_TUNE": "66"}, om, N, gpu=gpu)
+    a, b = rows.aNUFT(k), gath.aNUFT(k)
+    assert np.array_equal(a, b)
+    assert rel(a, orc.aNUFT(k)) < (TOL64 if gpu == 2 else TOL32_OP)
+    assert np.array_equal(rows.iNUFT(k, 3, 0.1), gath.iNUFT(k, 3, 0.1))

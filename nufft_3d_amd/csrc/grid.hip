@@ -679,6 +679,40 @@ __global__ void __launch_bounds__(256) k_permute_in_rows(const int* __restrict__
   }
 }
 
+// The inverse permutation driven by the caller's rows: each work-group reads
+// the sorted sample rows inv[row] of 64 consecutive caller rows (contiguous
+// nc values each; zeros for out-of-bounds rows), then writes every coil
+// plane as one coalesced run of 64 rows.
+template <typename R>
+__global__ void __launch_bounds__(256) k_permute_out_rows(const int* __restrict__ inv, long long M,
+                                                          int nc, const cplx<R>* __restrict__ ks,
+                                                          cplx<R>* __restrict__ ku) {
+  constexpr int TS = 64, CC = 32;
+  __shared__ cplx<R> t[TS][CC + 1];
+  __shared__ int si[TS];
+  const long long r0 = (long long)blockIdx.x * TS;
+  const int nr = M - r0 < TS ? (int)(M - r0) : TS;
+  if (threadIdx.x < nr) si[threadIdx.x] = inv[r0 + threadIdx.x];
+  for (int c0 = 0; c0 < nc; c0 += CC) {
+    const int cc = nc - c0 < CC ? nc - c0 : CC;
+    __syncthreads();   // si written; the previous chunk's tile is consumed
+    for (int i = threadIdx.x; i < nr * cc; i += blockDim.x) {
+      const int j = i / cc, c = i % cc;
+      const int s = si[j];
+      cplx<R> v;
+      v.x = 0;
+      v.y = 0;
+      if (s >= 0) v = ks[(long long)s * nc + c0 + c];
+      t[j][c] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TS * cc; i += blockDim.x) {
+      const int c = i / TS, j = i % TS;
+      if (j < nr) ku[r0 + j + (long long)(c0 + c) * M] = t[j][c];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 static KBParams make_kb(const nufft3d_plan* p) {
   KBParams k;
@@ -916,6 +950,18 @@ int zero_oob(nufft3d_plan* p, void* k_user, int nc) {
 int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user) {
   const long long n = p->nnz * (long long)nc;
   if (n == 0) return NUFFT3D_OK;
+  if (p->d_inv && !(p->tune_flags & 64)) {   // caller rows written in order (zeros out of bounds)
+    const unsigned nb = (unsigned)((p->M + 63) / 64);
+    ProfScope ps(p, "k_permute_out");
+    if (p->prec == 2)
+      hipLaunchKernelGGL((k_permute_out_rows<double>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
+                         (long long)p->M, nc, (const double2*)ks, (double2*)k_user);
+    else
+      hipLaunchKernelGGL((k_permute_out_rows<float>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
+                         (long long)p->M, nc, (const float2*)ks, (float2*)k_user);
+    N3D_HIP(hipGetLastError());
+    return NUFFT3D_OK;
+  }
   const int nb = (int)((n + 255) / 256);
   ProfScope ps(p, "k_permute_out");
   if (p->prec == 2)

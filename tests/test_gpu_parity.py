@@ -395,3 +395,29 @@ def test_caller_row_permutation_equals_sorted_gather(gpu, nc):
     assert np.array_equal(a, b)
     assert rel(a, orc.aNUFT(k)) < (TOL64 if gpu == 2 else TOL32_OP)
     assert np.array_equal(rows.iNUFT(k, 3, 0.1), gath.iNUFT(k, 3, 0.1))
+
+
+def test_solvers_on_an_fp64_compute_plan_with_fp32_arrays():
+    # J=6, u=1.25 makes a precision-1 plan compute in fp64 behind fp32 arrays
+    # (the precision policy); SAKE, the phase and the compressed-sensing
+    # constraints then agree with the fp64 plan to the fp32 rounding of
+    # their inputs and outputs
+    N = 12
+    om = synth.koosh_ball(N, 40, 2 * N)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")             # nufft_3d.m:93-95 fires for u < 1.5
+        p32 = nufft_3d(om, N, J=6, u=1.25, radial=1, gpu=1)
+        p64 = nufft_3d(om, N, J=6, u=1.25, radial=1, gpu=2)
+    assert p32.compute_precision == 2
+    data = _rand((om.shape[1], 2), 84).astype(np.complex64).astype(np.complex128)
+    pairs = [
+        (p32.sake(data, maxit=2), p64.sake(data, maxit=2)),
+        (p32.iNUFT(data, 6, 0.05, None, "phase-constraint", 0.5),
+         p64.iNUFT(data, 6, 0.05, None, "phase-constraint", 0.5)),
+        (p32.iNUFT(data, 6, 0.05, None, "compressed-sensing", 0.3),
+         p64.iNUFT(data, 6, 0.05, None, "compressed-sensing", 0.3)),
+    ]
+    for a, b in pairs:
+        assert a.dtype == np.complex64
+        assert rel(a.astype(np.complex128), b) < 1e-5

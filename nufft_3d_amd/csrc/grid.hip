@@ -264,13 +264,14 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
 }
 
 // Sum over hits [e0, e1) of one grid point's list (sorted sample ids, codes
-// alongside), weights times wsamp when given, into ar/ai (LPS-lane group).
+// alongside, or with pw the fp32 weights of the plan's table), weights times
+// wsamp when given, into ar/ai (LPS-lane group).
 // Software pipeline over steps of LPS hits: the list entries are loaded two
 // steps ahead and the sample offsets one step ahead, so a step waits for one
 // round trip (its sample values) instead of three dependent ones.
 template <typename R, int CPL, int LPS, bool FULL, typename A>
 __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
-                                            const uint2* __restrict__ hits,
+                                            const uint2* __restrict__ hits, bool pw,
                                             const double* __restrict__ off,
                                             const float4* __restrict__ off32,
                                             const cplx<R>* __restrict__ ks,
@@ -281,19 +282,28 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
   auto ent = [&](long long e) { return e + q < e1 ? e + q : e; };
   uint2 h0 = hits[ent(e0)];
   int s0 = (int)h0.x;
-  unsigned short c0 = (unsigned short)h0.y;
+  unsigned c0 = h0.y;                  // code, or weight bits when pw
   float4 f0;
-  if constexpr (sizeof(R) == 4) f0 = off32[s0];
+  if constexpr (sizeof(R) == 4) {
+    if (!pw) f0 = off32[s0];
+  }
   R m0 = wsamp ? wsamp[s0] : (R)1;
   const long long e1st = e0 + LPS < e1 ? e0 + LPS : e0;
   uint2 h1 = hits[ent(e1st)];
   for (long long e = e0; e < e1; e += LPS) {
     const int s = s0;
-    int dx, dy, dz;
-    decode_hit(c0, g, dx, dy, dz);
     R w;
-    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
-    else w = hit_weight_f(dx, dy, dz, f0, kp);
+    if constexpr (sizeof(R) == 8) {
+      int dx, dy, dz;
+      decode_hit((unsigned short)c0, g, dx, dy, dz);
+      w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+    } else if (pw) {
+      w = __uint_as_float(c0);
+    } else {
+      int dx, dy, dz;
+      decode_hit((unsigned short)c0, g, dx, dy, dz);
+      w = hit_weight_f(dx, dy, dz, f0, kp);
+    }
     w *= m0;
     if (e + q >= e1) w = 0;
     // this step's sample values
@@ -314,8 +324,10 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
       const long long en = e + LPS < e1 ? e + LPS : e;
       const long long enn = e + 2 * LPS < e1 ? e + 2 * LPS : en;
       s0 = (int)h1.x;
-      c0 = (unsigned short)h1.y;
-      if constexpr (sizeof(R) == 4) f0 = off32[s0];
+      c0 = h1.y;
+      if constexpr (sizeof(R) == 4) {
+        if (!pw) f0 = off32[s0];
+      }
       m0 = wsamp ? wsamp[s0] : (R)1;
       h1 = hits[ent(enn)];
     }
@@ -330,7 +342,7 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
 // of the segment blocks)
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* __restrict__ segs, int nseg,
-                                                   const uint2* __restrict__ hits,
+                                                   const uint2* __restrict__ hits, bool pw,
                                                    const double* __restrict__ off,
                                                    const float4* __restrict__ off32,
                                                    const cplx<R>* __restrict__ ks,
@@ -346,7 +358,7 @@ __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* 
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, off, off32, ks, wsamp, g, kp,
+  spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, pw, off, off32, ks, wsamp, g, kp,
                                  ar, ai);
   // partial sums keep the accumulator precision until the final reduction
   A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
@@ -366,7 +378,7 @@ __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* 
 // grouped the same way within either size class.
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
-                                                const uint2* __restrict__ hits,
+                                                const uint2* __restrict__ hits, bool pw,
                                                 const double* __restrict__ off,
                                                 const float4* __restrict__ off32,
                                                 const cplx<R>* __restrict__ ks,
@@ -381,8 +393,8 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   // work-group is unchanged) sum the dense points' segments; the rest are
   // the light points, renumbered for the XCDs among themselves
   if ((int)blockIdx.x < nhb) {
-    spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, off, off32, ks, wsamp,
-                                                partial, g, kp);
+    spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, pw, off, off32, ks,
+                                                wsamp, partial, g, kp);
     return;
   }
   const long long gt = xcd_block_of(g.xchunk, blockIdx.x - nhb, gridDim.x - nhb) * blockDim.x +
@@ -404,7 +416,7 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hits, off, off32, ks, wsamp, g, kp, ar, ai);
+  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hits, pw, off, off32, ks, wsamp, g, kp, ar, ai);
   cplx<R>* gp = grid + gpt * g.nc + g.c0;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
@@ -515,15 +527,15 @@ __global__ void __launch_bounds__(512) k_spread_tile(const long long* __restrict
 
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
-                                                      const uint2* __restrict__ hits,
+                                                      const uint2* __restrict__ hits, bool pw,
                                                       const double* __restrict__ off,
                                                       const float4* __restrict__ off32,
                                                       const cplx<R>* __restrict__ ks,
                                                       const R* __restrict__ wsamp,
                                                       cplx<R>* __restrict__ partial, GridGeom g,
                                                       KBParams kp) {
-  spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, off, off32, ks, wsamp,
-                                              partial, g, kp);
+  spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, pw, off, off32, ks,
+                                              wsamp, partial, g, kp);
 }
 
 // Sum of a dense point's segment partials, one work-group per point: the R
@@ -780,6 +792,33 @@ static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int
                        g, kp);
 }
 
+// The fp32 weight of every hit of the grid-major lists, stored beside its
+// sample id: the value hit_weight_f gives in the spread itself, computed once
+// per plan. The fp32 spread then reads one 8-byte entry per hit and neither
+// the sample's 16-byte offsets nor the kernel evaluation (bitwise the same
+// sums).
+__global__ void k_hit_weights(const uint2* __restrict__ hits, long long nhits,
+                              const float4* __restrict__ off32, uint2* __restrict__ hw, GridGeom g,
+                              KBParams kp) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nhits) return;
+  const uint2 h = hits[e];
+  int dx, dy, dz;
+  decode_hit((unsigned short)h.y, g, dx, dy, dz);
+  hw[e] = make_uint2(h.x, __float_as_uint(hit_weight_f(dx, dy, dz, off32[h.x], kp)));
+}
+
+// Built with the plan's hit lists (fp32 arithmetic: 8 bytes per hit).
+int build_hit_weights(nufft3d_plan* p) {
+  if (p->d_hitw || p->prec != 1 || p->nhits <= 0 || !p->d_off32) return NUFFT3D_OK;
+  N3D_HIP(hipMalloc(&p->d_hitw, sizeof(uint2) * p->nhits));
+  const GridGeom g = make_geom(p, 1, 0, 1);
+  hipLaunchKernelGGL(k_hit_weights, dim3((unsigned)((p->nhits + 255) / 256)), dim3(256), 0,
+                     p->stream, p->d_hit, (long long)p->nhits, p->d_off32, p->d_hitw, g, make_kb(p));
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
                           void* partial, const void* wsamp) {
@@ -791,6 +830,10 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   const DenseSet& ds = p->dense[LPS == 1 ? 0 : 1];
   constexpr bool kTileable = sizeof(R) == 4 && FULL && LPS * CPL >= 8;
   bool merged = false;   // dense segments summed inside the k_spread launch
+  // fp32 arithmetic reads the plan's per-hit weights unless tune flag 128
+  const uint2* hw = (sizeof(R) == 4 && !(p->tune_flags & 128)) ? p->d_hitw : nullptr;
+  const uint2* hits = hw ? hw : p->d_hit;
+  const bool pw = hw != nullptr;
   if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
       p->bl.rmax == ds.rmax) {
     ProfScope ps(p, "k_spread_tile");
@@ -809,7 +852,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
     ProfScope ps(p, "k_spread");
     hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>),
                        dim3((unsigned)(nhb + (nthreads + 255) / 256)), dim3(256), 0, p->stream,
-                       p->d_rowptr, p->d_hit, (const double*)p->d_off, p->d_off32,
+                       p->d_rowptr, hits, pw, (const double*)p->d_off, p->d_off32,
                        (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)grid, npts, p->d_brick,
                        ds.rmax, g, kp, ds.d_hseg, ds.nhseg, nhb, (cplx<R>*)partial);
   }
@@ -819,7 +862,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
       const long long nthreads = (long long)ds.nhseg * LPS;
       hipLaunchKernelGGL((k_spread_heavy<R, CPL, LPS, COMP, FULL>),
                          dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, p->stream,
-                         ds.d_hseg, ds.nhseg, p->d_hit, (const double*)p->d_off,
+                         ds.d_hseg, ds.nhseg, hits, pw, (const double*)p->d_off,
                          p->d_off32, (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)partial, g,
                          kp);
     }

@@ -934,6 +934,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     // (tune flag 8) or timed by the autotuner
     if (p->tune_flags & 8) N3D_TRY(ensure_brick_lists(p));
   }
+  N3D_TRY(build_hit_weights(p));
 
   // deapodization (deap.m) and FFT twiddles
   deap_host(p);
@@ -956,7 +957,7 @@ static void destroy_plan(nufft3d_plan* p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_inv, p->d_base, p->d_off, p->d_U, p->d_dens,
-                  p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit,
+                  p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit, p->d_hitw,
                   p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
   for (void* q : ptrs)
@@ -1277,6 +1278,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += rsz * nimg + rsz * nnz;                                        // U, density
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
   b += (2 + 8) * std::max<int64_t>(p->nhits, 1);                      // fcode, (sample, code) hits
+  if (p->d_hitw) b += 8 * p->nhits;                                   // (sample, weight) hits
   if (p->d_off32) b += 16 * nnz;
   for (const DenseSet& ds : p->dense)
     b += (int64_t)sizeof(HeavySeg) * std::max(1, ds.nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, ds.nhpt);

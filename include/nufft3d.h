@@ -176,6 +176,14 @@ int nufft3d_release_workspaces(nufft3d_plan* plan);
 /* Bind subsequent work of this plan to a HIP stream (hipStream_t as void*). */
 int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
 
+/* Run iNUFT's MINRES iterations as captured HIP graphs (enable = 1, the
+ * default) or launch by launch (0). No counterpart in the reference: a
+ * process whose other threads allocate, free or synchronise while this plan
+ * captures (a multi-slot serving loop) turns graphs off, because HIP may
+ * then invalidate the capture or fail the other thread's call. The results
+ * are the same either way. */
+int nufft3d_set_graphs(nufft3d_plan* plan, int32_t enable);
+
 /* Options of pruno(obj,data,varargin) (pruno.m:26-41). */
 typedef struct nufft3d_pruno_opts {
   int32_t width;           /* kernel width, default 5 (pruno.m:27) */

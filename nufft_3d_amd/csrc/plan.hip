@@ -1341,6 +1341,17 @@ int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
   return NUFFT3D_OK;
 }
 
+int nufft3d_set_graphs(nufft3d_plan* p, int32_t enable) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->use_graphs = enable != 0;
+  if (!p->use_graphs && p->mr_exec) {
+    (void)hipGraphExecDestroy(p->mr_exec);
+    p->mr_exec = nullptr;
+  }
+  return NUFFT3D_OK;
+}
+
 int nufft3d_profile_enable(nufft3d_plan* p, int32_t enable) {
   if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
   std::lock_guard<std::mutex> lk(p->mu);

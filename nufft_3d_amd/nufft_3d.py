@@ -136,6 +136,12 @@ class nufft_3d:
         """Free the per-call workspaces (they regrow on the next call)."""
         _lib.check(self._lib.nufft3d_release_workspaces(self._h))
 
+    def set_graphs(self, enable):
+        """Run iNUFT's MINRES iterations as captured HIP graphs (default) or
+        launch by launch; turn graphs off where other threads allocate, free
+        or synchronise while this plan runs. Results are the same."""
+        _lib.check(self._lib.nufft3d_set_graphs(self._h, 1 if enable else 0))
+
     # ------------------------------------------------------------ helpers
     @property
     def cdtype(self):

@@ -80,10 +80,11 @@ def grid_size(N, u=2.0):
 
 def est_plan_bytes(N, M, u=2.0, hits_per_sample=33.5):
     """Resident plan arrays before the plan exists (FP32 plan, J=4 radial):
-    per sample 100 B of coordinates/ids/density plus 8 B per hit, 8 B per
-    grid point of row pointers, U."""
+    per sample 100 B of coordinates/ids/density plus 18 B per hit (2 B
+    forward code, 8 B adjoint entry, 8 B adjoint weight entry), 8 B per grid
+    point of row pointers, U."""
     K = grid_size(N, u)
-    return int(M * (100 + 8 * hits_per_sample) + 8 * K ** 3 + 4 * N ** 3)
+    return int(M * (100 + 18 * hits_per_sample) + 8 * K ** 3 + 4 * N ** 3)
 
 
 def est_workspace_bytes(N, M, nc, kind, u=2.0):
@@ -133,7 +134,12 @@ class GpuBackend:
 
     def make_plan(self, req):
         from .nufft_3d import nufft_3d
-        return nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
+        plan = nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
+        # the slots' threads build, free and synchronise while others run:
+        # HIP does not keep a stream capture on one thread apart from those
+        # calls on another, so MINRES runs launch by launch here
+        plan.set_graphs(False)
+        return plan
 
     def plan_bytes(self, plan):
         return plan.device_bytes()

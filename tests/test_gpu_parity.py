@@ -443,3 +443,17 @@ def test_solvers_on_an_fp64_compute_plan_with_fp32_arrays():
     for a, b in pairs:
         assert a.dtype == np.complex64
         assert rel(a.astype(np.complex128), b) < 1e-5
+
+
+def test_minres_without_graphs_equals_graphs():
+    # nufft3d_set_graphs(plan, 0) runs the MINRES blocks launch by launch:
+    # the same kernels in the same order, so the same bits
+    om = synth.koosh_ball(16, 60, 32)
+    k = _rand((om.shape[1], 3), 89)
+    a = nufft_3d(om, 16, gpu=1)
+    b = nufft_3d(om, 16, gpu=1)
+    b.set_graphs(False)
+    x, ia = a.iNUFT(k, 14, 0.05, return_info=True)
+    y, ib = b.iNUFT(k, 14, 0.05, return_info=True)
+    assert ia["iters"] == ib["iters"]
+    assert np.array_equal(x, y)

@@ -293,12 +293,12 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
   for (long long e = e0; e < e1; e += LPS) {
     const int s = s0;
     R w;
-    if constexpr (sizeof(R) == 8) {
+    if (pw) {
+      w = (R)__uint_as_float(c0);        // exact: fp64 weights are float-rounded too
+    } else if constexpr (sizeof(R) == 8) {
       int dx, dy, dz;
       decode_hit((unsigned short)c0, g, dx, dy, dz);
       w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
-    } else if (pw) {
-      w = __uint_as_float(c0);
     } else {
       int dx, dy, dz;
       decode_hit((unsigned short)c0, g, dx, dy, dz);
@@ -792,29 +792,43 @@ static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int
                        g, kp);
 }
 
-// The fp32 weight of every hit of the grid-major lists, stored beside its
-// sample id: the value hit_weight_f gives in the spread itself, computed once
-// per plan. The fp32 spread then reads one 8-byte entry per hit and neither
-// the sample's 16-byte offsets nor the kernel evaluation (bitwise the same
-// sums).
+// The weight of every hit of the grid-major lists, stored as float bits
+// beside its sample id: the value the spread itself would evaluate
+// (hit_weight_f in fp32 arithmetic; hit_weight_d in fp64, which is the
+// reference's float-rounded weight, so float holds it exactly), computed once
+// per plan. The spread then reads one 8-byte entry per hit and neither the
+// sample's offsets (16 or 24 bytes) nor the kernel evaluation: bitwise the
+// same sums.
+template <bool D>
 __global__ void k_hit_weights(const uint2* __restrict__ hits, long long nhits,
-                              const float4* __restrict__ off32, uint2* __restrict__ hw, GridGeom g,
-                              KBParams kp) {
+                              const float4* __restrict__ off32, const double* __restrict__ off,
+                              uint2* __restrict__ hw, GridGeom g, KBParams kp) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nhits) return;
   const uint2 h = hits[e];
   int dx, dy, dz;
   decode_hit((unsigned short)h.y, g, dx, dy, dz);
-  hw[e] = make_uint2(h.x, __float_as_uint(hit_weight_f(dx, dy, dz, off32[h.x], kp)));
+  float w;
+  if constexpr (D) w = (float)hit_weight_d(dx, dy, dz, off + 3LL * h.x, kp);
+  else w = hit_weight_f(dx, dy, dz, off32[h.x], kp);
+  hw[e] = make_uint2(h.x, __float_as_uint(w));
 }
 
-// Built with the plan's hit lists (fp32 arithmetic: 8 bytes per hit).
+// Built with the plan's hit lists (8 bytes per hit).
 int build_hit_weights(nufft3d_plan* p) {
-  if (p->d_hitw || p->prec != 1 || p->nhits <= 0 || !p->d_off32) return NUFFT3D_OK;
+  if (p->d_hitw || p->nhits <= 0) return NUFFT3D_OK;
+  if (p->prec == 1 ? !p->d_off32 : !p->d_off) return NUFFT3D_OK;
   N3D_HIP(hipMalloc(&p->d_hitw, sizeof(uint2) * p->nhits));
   const GridGeom g = make_geom(p, 1, 0, 1);
-  hipLaunchKernelGGL(k_hit_weights, dim3((unsigned)((p->nhits + 255) / 256)), dim3(256), 0,
-                     p->stream, p->d_hit, (long long)p->nhits, p->d_off32, p->d_hitw, g, make_kb(p));
+  const dim3 grid((unsigned)((p->nhits + 255) / 256));
+  if (p->prec == 1)
+    hipLaunchKernelGGL(k_hit_weights<false>, grid, dim3(256), 0, p->stream, p->d_hit,
+                       (long long)p->nhits, p->d_off32, (const double*)nullptr, p->d_hitw, g,
+                       make_kb(p));
+  else
+    hipLaunchKernelGGL(k_hit_weights<true>, grid, dim3(256), 0, p->stream, p->d_hit,
+                       (long long)p->nhits, (const float4*)nullptr, (const double*)p->d_off,
+                       p->d_hitw, g, make_kb(p));
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
@@ -830,8 +844,8 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   const DenseSet& ds = p->dense[LPS == 1 ? 0 : 1];
   constexpr bool kTileable = sizeof(R) == 4 && FULL && LPS * CPL >= 8;
   bool merged = false;   // dense segments summed inside the k_spread launch
-  // fp32 arithmetic reads the plan's per-hit weights unless tune flag 128
-  const uint2* hw = (sizeof(R) == 4 && !(p->tune_flags & 128)) ? p->d_hitw : nullptr;
+  // the plan's per-hit weights, unless tune flag 128
+  const uint2* hw = !(p->tune_flags & 128) ? p->d_hitw : nullptr;
   const uint2* hits = hw ? hw : p->d_hit;
   const bool pw = hw != nullptr;
   if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&

@@ -130,7 +130,7 @@ struct nufft3d_plan {
                             // 16 = forward result through a separate permutation pass,
                             // 32 = dense-point segments in their own launch,
                             // 64 = aNUFT input gathered by sorted sample, not scattered
-                            // by caller row, 128 = fp32 spread recomputes each hit's
+                            // by caller row, 128 = the spread evaluates each hit's
                             // weight instead of reading the plan's weight table)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
@@ -167,8 +167,8 @@ struct nufft3d_plan {
   long long* d_rowptr = nullptr;    // [K1*K2*K3 + 1]
   uint2* d_hit = nullptr;           // [nhits] (sample id, code) of each hit (grid-major):
                                     // one 8-byte load per hit in the spread
-  uint2* d_hitw = nullptr;          // [nhits] (sample id, fp32 weight bits) of each hit, in
-                                    // d_hit's order (fp32-arithmetic plans)
+  uint2* d_hitw = nullptr;          // [nhits] (sample id, weight as float bits) of each hit,
+                                    // in d_hit's order
   float4* d_off32 = nullptr;        // [nnz] (fx, fy, fz, 0)
   // dense-point segmentations of the same hit lists: [0] for launches with
   // one lane per point (fp32 groups of at most 4 coils, every fp64 group:

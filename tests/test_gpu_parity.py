@@ -397,26 +397,28 @@ def test_caller_row_permutation_equals_sorted_gather(gpu, nc):
     assert np.array_equal(rows.iNUFT(k, 3, 0.1), gath.iNUFT(k, 3, 0.1))
 
 
-@pytest.mark.parametrize("nc,radial,comp", [(16, 1, True), (3, 1, True), (1, 0, True),
-                                            (16, 0, False), (5, 1, False)])
-def test_spread_weight_table_equals_evaluated_weights(nc, radial, comp):
-    # fp32 plans read each hit's weight from a plan-time table (sample id,
-    # weight bits); tune flag 128 evaluates it in the spread as before:
-    # bitwise the same grid, with the dense centre's segments (300 spokes),
-    # lane groups (16 coils) and single lanes (1-3 coils), and in iNUFT,
-    # where the spread also multiplies by the data weights W
+@pytest.mark.parametrize("gpu,nc,radial,comp", [(1, 16, 1, True), (1, 3, 1, True), (1, 1, 0, True),
+                                                (1, 16, 0, False), (1, 5, 1, False),
+                                                (2, 5, 1, True), (2, 1, 0, True), (2, 16, 1, True)])
+def test_spread_weight_table_equals_evaluated_weights(gpu, nc, radial, comp):
+    # the spread reads each hit's weight from a plan-time table (sample id,
+    # float weight bits; fp64 weights are float-rounded, so exact); tune flag
+    # 128 evaluates it in the spread as before: bitwise the same grid, in
+    # fp32 and fp64 arithmetic, with the dense centre's segments (300
+    # spokes), lane groups and single lanes, and in iNUFT, where the spread
+    # also multiplies by the data weights W
     N = 24
     om = synth.koosh_ball(N, 300, 2 * N)
     k = _rand((om.shape[1], nc), 87)
-    tab = nufft_3d(om, N, gpu=1, radial=radial, compensated=comp)
-    ev = _plan_with_env({"NUFFT3D_TUNE": "130"}, om, N, gpu=1, radial=radial, compensated=comp)
-    assert tab.compute_precision == 1
+    tab = nufft_3d(om, N, gpu=gpu, radial=radial, compensated=comp)
+    ev = _plan_with_env({"NUFFT3D_TUNE": "130"}, om, N, gpu=gpu, radial=radial, compensated=comp)
+    assert tab.compute_precision == gpu
     a, b = tab.aNUFT(k), ev.aNUFT(k)
     assert np.array_equal(a, b)
     W = np.random.default_rng(8).uniform(0.2, 1.0, size=300)
     assert np.array_equal(tab.iNUFT(k, 3, 0.05, W), ev.iNUFT(k, 3, 0.05, W))
     orc = O.NufftOracle(om, N, radial=radial)
-    assert rel(a, orc.aNUFT(k)) < (TOL32_OP if comp else 1e-4)
+    assert rel(a, orc.aNUFT(k)) < (TOL64 if gpu == 2 else TOL32_OP if comp else 1e-4)
 
 
 def test_solvers_on_an_fp64_compute_plan_with_fp32_arrays():

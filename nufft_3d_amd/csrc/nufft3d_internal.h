@@ -122,6 +122,7 @@ struct nufft3d_plan {
   int64_t nnz = 0;          // in-bounds samples
   int device = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;     // recorded on the plan's stream as each entry point returns
   bool own_stream = false;
   int verbose = 0;
   int compensated = 1;      // fp32 plans: products accumulated in double

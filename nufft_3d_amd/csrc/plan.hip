@@ -952,10 +952,27 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   return NUFFT3D_OK;
 }
 
+// Waits for the plan's enqueued work. A stream the plan does not own (the
+// caller's, e.g. one shared by several plans of a serving loop) is not
+// synchronised: another thread may be capturing a graph on it, and a
+// synchronisation would invalidate that capture. The event recorded as the
+// plan's last entry point returned marks the end of its work there.
+static int wait_plan_work(nufft3d_plan* p) {
+  hipError_t e = hipSuccess;
+  if (p->own_stream && p->stream) e = hipStreamSynchronize(p->stream);
+  else if (p->done_ev) e = hipEventSynchronize(p->done_ev);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return NUFFT3D_ERR_DEVICE;
+  }
+  return NUFFT3D_OK;
+}
+
 static void destroy_plan(nufft3d_plan* p) {
   if (!p) return;
   (void)hipSetDevice(p->device);
-  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  (void)wait_plan_work(p);
+  if (p->done_ev) (void)hipEventDestroy(p->done_ev);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_inv, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit, p->d_hitw,
                   p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
@@ -1219,6 +1236,13 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
     return NUFFT3D_ERR_DEVICE;
   }
   p->own_stream = true;
+  if (hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    p->done_ev = nullptr;
+    destroy_plan(p);
+    set_error("hipEventCreate failed");
+    return NUFFT3D_ERR_DEVICE;
+  }
   auto t0 = std::chrono::steady_clock::now();
   int st = build_plan(p, om, M);
   if (st == NUFFT3D_OK) {
@@ -1301,8 +1325,8 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
 int nufft3d_release_workspaces(nufft3d_plan* p) {
   if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
   std::lock_guard<std::mutex> lk(p->mu);
-  if (hipStreamSynchronize(p->stream) != hipSuccess) {
-    (void)hipGetLastError();
+  (void)hipSetDevice(p->device);
+  if (wait_plan_work(p) != NUFFT3D_OK) {
     set_error("device error while releasing workspaces");
     return NUFFT3D_ERR_DEVICE;
   }
@@ -1483,11 +1507,22 @@ static int finish_out_c(nufft3d_plan* p, void* dst, size_t n, const Staged& s, W
   return NUFFT3D_OK;
 }
 
+// Records the plan's completion event on its stream when an entry point
+// returns (declared after the lock, so it runs while the lock is held).
+struct DoneMark {
+  nufft3d_plan* p;
+  ~DoneMark() {
+    if (p->done_ev && p->stream && hipEventRecord(p->done_ev, p->stream) != hipSuccess)
+      (void)hipGetLastError();
+  }
+};
+
 #define N3D_PLAN_GUARD(p)                                          \
   if (!(p)) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; } \
   std::lock_guard<std::mutex> lk_((p)->mu);                        \
   g_last_error.clear();                                            \
-  if (hipSetDevice((p)->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; }
+  if (hipSetDevice((p)->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; } \
+  DoneMark dm_{(p)};
 
 int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
   N3D_PLAN_GUARD(p);

@@ -134,12 +134,7 @@ class GpuBackend:
 
     def make_plan(self, req):
         from .nufft_3d import nufft_3d
-        plan = nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
-        # the slots' threads build, free and synchronise while others run:
-        # HIP does not keep a stream capture on one thread apart from those
-        # calls on another, so MINRES runs launch by launch here
-        plan.set_graphs(False)
-        return plan
+        return nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
 
     def plan_bytes(self, plan):
         return plan.device_bytes()

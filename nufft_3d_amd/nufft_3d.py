@@ -138,8 +138,8 @@ class nufft_3d:
 
     def set_graphs(self, enable):
         """Run iNUFT's MINRES iterations as captured HIP graphs (default) or
-        launch by launch; turn graphs off where other threads allocate, free
-        or synchronise while this plan runs. Results are the same."""
+        launch by launch, for callers whose other threads synchronise the
+        stream this plan captures on. Results are the same."""
         _lib.check(self._lib.nufft3d_set_graphs(self._h, 1 if enable else 0))
 
     # ------------------------------------------------------------ helpers

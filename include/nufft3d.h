@@ -178,10 +178,11 @@ int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
 
 /* Run iNUFT's MINRES iterations as captured HIP graphs (enable = 1, the
  * default) or launch by launch (0). No counterpart in the reference: for a
- * caller whose other threads synchronise the stream this plan captures on,
- * which would invalidate the capture. The results are the same either way.
- * (The library's own destroy and release wait on the plan's completion
- * event, not on a stream it does not own.) */
+ * process whose other threads free device memory (or synchronise this
+ * plan's stream) while it may be capturing; HIP then invalidates the
+ * capture. The results are the same either way. (The library's own destroy
+ * and release wait on the plan's completion event, not on a stream it does
+ * not own.) */
 int nufft3d_set_graphs(nufft3d_plan* plan, int32_t enable);
 
 /* Options of pruno(obj,data,varargin) (pruno.m:26-41). */

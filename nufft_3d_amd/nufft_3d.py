@@ -138,8 +138,9 @@ class nufft_3d:
 
     def set_graphs(self, enable):
         """Run iNUFT's MINRES iterations as captured HIP graphs (default) or
-        launch by launch, for callers whose other threads synchronise the
-        stream this plan captures on. Results are the same."""
+        launch by launch, for processes whose other threads free device
+        memory while this plan may be capturing (HIP then invalidates the
+        capture). Results are the same."""
         _lib.check(self._lib.nufft3d_set_graphs(self._h, 1 if enable else 0))
 
     # ------------------------------------------------------------ helpers

@@ -134,7 +134,12 @@ class GpuBackend:
 
     def make_plan(self, req):
         from .nufft_3d import nufft_3d
-        return nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
+        plan = nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
+        # the main thread frees evicted plans and trimmed workspaces while
+        # slots run; HIP invalidates a graph capture in progress on another
+        # thread when memory is freed, so MINRES runs launch by launch here
+        plan.set_graphs(False)
+        return plan
 
     def plan_bytes(self, plan):
         return plan.device_bytes()

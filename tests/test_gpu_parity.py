@@ -459,3 +459,4 @@ def test_minres_without_graphs_equals_graphs():
     y, ib = b.iNUFT(k, 14, 0.05, return_info=True)
     assert ia["iters"] == ib["iters"]
     assert np.array_equal(x, y)
+

@@ -759,17 +759,35 @@ int heev_device(nufft3d_plan* p, double2* dA, int n, double* dW) {
 }
 
 // MATLAB hist(y, nbins) counts with equally spaced bins over [min, max]
-static std::vector<int> hist_counts(const std::vector<double>& y, int nb) {
-  std::vector<int> h(std::max(nb, 1), 0);
-  if (y.empty()) return h;
+// MATLAB hist(y, x) with a scalar bin count x: floor(x) bins (0:x stops at
+// floor(x)) whose edges are miny + k*(maxy-miny)/x, k = 0..floor(x), the last
+// edge replaced by maxy and every edge raised by eps(edge), so a bin is
+// (a, b]; values at or below the first edge count in bin 1. A constant y is
+// spread over [y - floor(x/2) - 0.5, y + ceil(x/2) - 0.5]. floor(x) < 1
+// gives no bins.
+static double eps_of(double v) {
+  const double a = std::fabs(v);
+  return std::nextafter(a, INFINITY) - a;
+}
+
+static std::vector<int> hist_counts(const std::vector<double>& y, double x) {
+  const int n = x >= 1 ? (int)std::floor(x) : 0;
+  std::vector<int> h(n, 0);
+  if (n < 1 || y.empty()) return h;
   double lo = *std::min_element(y.begin(), y.end()), hi = *std::max_element(y.begin(), y.end());
-  if (hi == lo) { h[h.size() / 2] = (int)y.size(); return h; }
-  const double bw = (hi - lo) / h.size();
+  if (lo == hi) {
+    lo = lo - std::floor(x / 2) - 0.5;
+    hi = hi + std::ceil(x / 2) - 0.5;
+  }
+  const double bw = (hi - lo) / x;
+  std::vector<double> e(n + 1);
+  for (int k = 0; k <= n; ++k) e[k] = lo + bw * k;
+  e[n] = hi;
+  for (double& v : e) v += eps_of(v);
   for (double v : y) {
-    int k = (int)std::floor((v - lo) / bw);
-    if (k >= (int)h.size()) k = (int)h.size() - 1;
-    if (k < 0) k = 0;
-    h[k]++;
+    if (!(v <= e[n])) continue;   // above the last edge (or NaN): not counted
+    const int c = (int)(std::upper_bound(e.begin(), e.end(), v) - e.begin());   // edges <= v
+    h[std::min(std::max(c, 1), n) - 1]++;
   }
   return h;
 }
@@ -795,14 +813,17 @@ int pruno_select(const std::vector<double>& evals_desc, double noise_in, double 
     int lo = 1;
     for (lo = 1; lo <= hi; ++lo) {
       std::vector<double> seg(evals_desc.begin() + (lo - 1), evals_desc.begin() + hi);
-      const int nb = std::max(1, (int)std::lround(std::sqrt((double)(hi - lo))));
-      std::vector<int> h = hist_counts(seg, nb);
-      const int k = (int)(std::max_element(h.begin(), h.end()) - h.begin()) + 1;
+      std::vector<int> h = hist_counts(seg, std::sqrt((double)(hi - lo)));
+      // [~,k] = max(h): the first maximum; no bins leaves k empty (no break)
+      const int k = h.empty() ? 0 : (int)(std::max_element(h.begin(), h.end()) - h.begin()) + 1;
       if (k > 1) break;
     }
-    if (lo > hi) lo = hi;
-    std::vector<double> seg(evals_desc.begin() + (lo - 1), evals_desc.begin() + std::max(hi, lo));
-    const double sigma = std::sqrt(median_of(seg));
+    if (lo > hi) lo = hi;   // the for loop's variable keeps its last value
+    double sigma = 0;       // hi = 0: every eigenvalue is zero
+    if (hi > 0) {
+      std::vector<double> seg(evals_desc.begin() + (lo - 1), evals_desc.begin() + hi);
+      sigma = std::sqrt(median_of(seg));
+    }
     noise = sigma / std::sqrt(density * nx * ny);
   }
   noise_out = noise;

@@ -190,6 +190,8 @@ struct nufft3d_plan {
   int use_graphs = 1;
   hipGraphExec_t mr_exec = nullptr;
   std::vector<uint64_t> mr_key;
+  int capture_fail_at = 0;  // test hook: a MINRES block capture fails after this many
+                            // iterations (NUFFT3D_TEST_CAPTURE_FAIL; 0 = never)
 
   // scratch, grown on demand
   nufft3d::Workspace ws_grid, ws_t1, ws_t2, ws_samp, ws_part, ws_img, ws_host, ws_solve, ws_red;

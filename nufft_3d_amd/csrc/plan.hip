@@ -1219,6 +1219,7 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_FFTYT")) p->fft_ythreads = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_GRAPH")) p->use_graphs = atoi(tv) != 0;
+  if (const char* tv = getenv("NUFFT3D_TEST_CAPTURE_FAIL")) p->capture_fail_at = atoi(tv);
   if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
   // dense-point segments: 64 hits for launches of <= 4 coils, 512 (or the
   // caller's chunk) for larger ones
@@ -1356,12 +1357,19 @@ int32_t nufft3d_compute_precision(const nufft3d_plan* p) { return p ? p->prec : 
 int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
   if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
   std::lock_guard<std::mutex> lk(p->mu);
+  hipStream_t ns = (hipStream_t)stream;
+  if (ns == p->stream) return NUFFT3D_OK;
+  if (hipSetDevice(p->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; }
   if (p->own_stream && p->stream) {
     (void)hipStreamSynchronize(p->stream);
     (void)hipStreamDestroy(p->stream);
+  } else if (p->done_ev) {
+    // the plan's workspaces are shared by every stream it runs on: work on
+    // the new stream starts after the last entry point on the old one ended
+    N3D_HIP(hipStreamWaitEvent(ns, p->done_ev, 0));
   }
   p->own_stream = false;
-  p->stream = (hipStream_t)stream;
+  p->stream = ns;
   return NUFFT3D_OK;
 }
 
@@ -1569,12 +1577,20 @@ int nufft3d_adjoint(nufft3d_plan* p, const void* k, int32_t nc, void* x) {
   return st;
 }
 
-// W handling of iNUFT.m:55-74 -> per sorted sample weight W(row)*d
-static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, void** out) {
+// W handling of iNUFT.m:55-74 -> per sorted sample weight W(row)*d. With
+// maxit <= 1 the reference only warns (iNUFT.m:57-58): W stays the caller's
+// length-nrow vector, not multiplied by the density and not validated, and
+// W.*raw needs it to have nrow elements.
+static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, int maxit, void** out) {
   *out = p->d_dens;
   if (!W || nW <= 1) return NUFFT3D_OK;   // absent, scalar or empty: W = d
   const int64_t M = p->M;
+  const bool as_given = maxit <= 1;
   std::vector<double> full;
+  if (as_given && nW != M) {
+    set_error("W must have one element per row of raw when maxit<=1 (W.*raw, iNUFT.m:122)");
+    return NUFFT3D_ERR_INVALID;
+  }
   if (nW == M) {
     full.assign(W, W + M);
   } else {
@@ -1585,8 +1601,9 @@ static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, void** out
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t i = 0; i < nr; ++i) full[i + nr * j] = W[j];
   }
-  for (double v : full)
-    if (!std::isfinite(v) || v < 0) { set_error("W must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
+  if (!as_given)
+    for (double v : full)
+      if (!std::isfinite(v) || v < 0) { set_error("W must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
   double* dW = nullptr;
   N3D_HIP(hipMalloc(&dW, sizeof(double) * M));
   N3D_HIP(hipMemcpyAsync(dW, full.data(), sizeof(double) * M, hipMemcpyHostToDevice, p->stream));
@@ -1596,10 +1613,12 @@ static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, void** out
   if (p->nnz > 0) {
     if (p->prec == 2)
       hipLaunchKernelGGL((k_gather_real<double>), dim3(nblocks(p->nnz)), dim3(256), 0, p->stream,
-                         dW, p->d_orig, (const double*)p->d_dens, (long long)p->nnz, (double*)wd);
+                         dW, p->d_orig, as_given ? nullptr : (const double*)p->d_dens,
+                         (long long)p->nnz, (double*)wd);
     else
       hipLaunchKernelGGL((k_gather_real<float>), dim3(nblocks(p->nnz)), dim3(256), 0, p->stream,
-                         dW, p->d_orig, (const float*)p->d_dens, (long long)p->nnz, (float*)wd);
+                         dW, p->d_orig, as_given ? nullptr : (const float*)p->d_dens,
+                         (long long)p->nnz, (float*)wd);
   }
   N3D_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(dW);
@@ -1619,7 +1638,7 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
-  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  N3D_TRY(make_weights(p, W, nW, maxit, &wsamp));
   Workspace tmp_out, tmp_io;
   Staged ri, xo;
   int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
@@ -1671,7 +1690,7 @@ int nufft3d_inverse_phase(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
-  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  N3D_TRY(make_weights(p, W, nW, maxit, &wsamp));
   Workspace tmp_out, tmp_io, tmp_x, tmp_P, tmp_b;
   Staged ri, xo;
   int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
@@ -1724,7 +1743,7 @@ int nufft3d_inverse_cs(nufft3d_plan* p, const void* raw, int32_t nc, int32_t max
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
-  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  N3D_TRY(make_weights(p, W, nW, maxit, &wsamp));
   Workspace tmp_out, tmp_io;
   Staged ri, xo;
   int st = stage_in_c(p, raw, nk, p->ws_img, tmp_io, ri);
@@ -2099,7 +2118,7 @@ int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   void* wsamp = nullptr;
-  N3D_TRY(make_weights(p, W, nW, &wsamp));
+  N3D_TRY(make_weights(p, W, nW, maxit, &wsamp));
   Workspace tmp_out, tmp_b, tmp_x0, tmp_k, tmp_io;
   Staged ri, xo;
   void* dB = nullptr;

@@ -438,6 +438,7 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
   key.push_back(__builtin_bit_cast(uint64_t, tol));
   key.push_back((uint64_t)sizeof(R));
   bool graphs = p->use_graphs && !p->prof.on && p->stream != nullptr;
+  const int test_capture_fail_at = p->capture_fail_at;
   bool stop = false;
   while (!stop && itn + kGraphBlock - 1 <= maxit) {
     bool any = true;
@@ -448,11 +449,18 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
       if (!(p->mr_exec && p->mr_key == key)) {
         if (p->mr_exec) { (void)hipGraphExecDestroy(p->mr_exec); p->mr_exec = nullptr; }
         hipGraph_t gph = nullptr;
+        // the rotation state at the block's start: a capture that stops
+        // part-way leaves the pointers rotated by fewer than a full period
+        C* const rot[6] = {v, r1, r2, y, w, w2};
         bool ok = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         int st_cap = NUFFT3D_OK;
         if (ok) {
-          for (int k = 0; k < kGraphBlock && st_cap == NUFFT3D_OK; ++k) st_cap = iterate(itn + k);
+          for (int k = 0; k < kGraphBlock && st_cap == NUFFT3D_OK; ++k)
+            st_cap = (test_capture_fail_at == k + 1) ? NUFFT3D_ERR_DEVICE : iterate(itn + k);
           ok = hipStreamEndCapture(p->stream, &gph) == hipSuccess && st_cap == NUFFT3D_OK && gph;
+        }
+        if (!ok) {   // nothing captured ran: back to the block's start
+          v = rot[0]; r1 = rot[1]; r2 = rot[2]; y = rot[3]; w = rot[4]; w2 = rot[5];
         }
         hipGraphExec_t ex = nullptr;
         if (ok) ok = hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0) == hipSuccess;
@@ -462,8 +470,8 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
           p->mr_exec = ex;
           p->mr_key = key;
         } else {
-          // nothing captured ran; the rotation made a full period, so the
-          // buffers are where they were: run this block eagerly from here on
+          // nothing captured ran and the buffers are back at the block's
+          // start: run this block eagerly from here on
           graphs = false;
         }
       }

@@ -317,6 +317,12 @@ class NufftOracle:
         nc = raw.shape[1]
         if W is None or np.isscalar(W) or np.size(W) == 0:
             Wv = self.d
+        elif maxit <= 1:
+            # iNUFT.m:57-58 only warns: W is used as given (no density, no
+            # expansion), and W.*raw needs one weight per row
+            Wv = np.reshape(np.asarray(W, dtype=np.float64), (-1,))
+            if Wv.size != self.M:
+                raise ValueError('W must have one element per row of raw when maxit<=1')
         else:
             Wv = np.reshape(np.asarray(W, dtype=np.float64), (-1,))
             if Wv.size != self.M:

@@ -117,16 +117,37 @@ def gram_fft_loraks(data, taps, center=(1, 1, 1)):
     return np.block([[AA, BB], [BB.conj(), AA.conj()]])
 
 
-def _hist(y, nb):
-    nb = max(1, int(nb))
-    lo, hi = float(np.min(y)), float(np.max(y))
-    if hi == lo:
-        h = np.zeros(nb, int)
-        h[nb // 2] = len(y)
+def _eps(v):
+    a = abs(float(v))
+    return float(np.nextafter(a, np.inf) - a)
+
+
+def _hist(y, x):
+    """MATLAB hist(y, x) with a scalar bin count x (hist.m): floor(x) bins
+    (0:x stops at floor(x)) with edges miny + k*(maxy-miny)/x, k=0..floor(x),
+    the last edge replaced by maxy and every edge raised by eps(edge), so the
+    bins are (a, b]; values at or below the first edge fall in bin 1. A
+    constant y is spread over [y - floor(x/2) - 0.5, y + ceil(x/2) - 0.5].
+    floor(x) < 1 gives no bins."""
+    n = int(math.floor(x)) if x >= 1 else 0
+    h = np.zeros(n, int)
+    y = np.asarray(y, dtype=np.float64)
+    if n < 1 or y.size == 0:
         return h
-    k = np.floor((np.asarray(y) - lo) / ((hi - lo) / nb)).astype(int)
-    k = np.clip(k, 0, nb - 1)
-    return np.bincount(k, minlength=nb)
+    lo, hi = float(np.min(y)), float(np.max(y))
+    if lo == hi:
+        lo = lo - math.floor(x / 2) - 0.5
+        hi = hi + math.ceil(x / 2) - 0.5
+    bw = (hi - lo) / x
+    e = [lo + bw * k for k in range(n + 1)]
+    e[n] = hi
+    e = np.array([v + _eps(v) for v in e])
+    for v in y:
+        if not v <= e[n]:
+            continue
+        c = int(np.searchsorted(e, v, side='right'))     # edges <= v
+        h[min(max(c, 1), n) - 1] += 1
+    return h
 
 
 def estimate_noise(W, density, nx, ny):
@@ -136,14 +157,17 @@ def estimate_noise(W, density, nx, ny):
     n = len(W)
     thr = np.spacing(n * W[0])
     hi = int(np.count_nonzero(W > thr))
+    if hi == 0:
+        return 0.0
     lo = 1
     while lo <= hi:
-        h = _hist(W[lo - 1:hi], round(math.sqrt(hi - lo)))
-        if int(np.argmax(h)) + 1 > 1:
+        h = _hist(W[lo - 1:hi], math.sqrt(hi - lo))
+        # [~,k] = max(h): the first maximum; no bins leaves k empty (no break)
+        if h.size and int(np.argmax(h)) + 1 > 1:
             break
         lo += 1
     lo = min(lo, hi)
-    sigma = math.sqrt(float(np.median(W[lo - 1:max(hi, lo)])))
+    sigma = math.sqrt(float(np.median(W[lo - 1:hi])))
     return sigma / math.sqrt(density * nx * ny)
 
 

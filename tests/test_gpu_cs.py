@@ -3,7 +3,7 @@ prox) on the HIP path vs the FP64 oracle (GPU).
 
 The external fistaN/HWT are restated (oracle/constraint_oracle.py); parity
 here is against that restatement: FP64 plans to 1e-9 relative L2 with the
-same iteration count, FP32 plans to 1e-3.
+same iteration count, FP32 plans to the 1e-4 gate.
 """
 import numpy as np
 import pytest
@@ -54,7 +54,7 @@ def test_cs_fp32_matches_oracle(setup):
     assert got.dtype == np.complex64
     err = _rel(got.astype(np.complex128), want)
     print("cs fp32 rel err vs FP64 oracle", err, flush=True)
-    assert err < 1e-3
+    assert err < 1e-4
 
 
 def test_no_sparsity_converges_to_least_squares(setup):

@@ -9,6 +9,7 @@ rounding the grid alone costs ~7e-6 (measured on the oracle), which is why
 the single-operator bound is not tighter.
 """
 import math
+import warnings
 
 import numpy as np
 import pytest
@@ -460,3 +461,68 @@ def test_minres_without_graphs_equals_graphs():
     assert ia["iters"] == ib["iters"]
     assert np.array_equal(x, y)
 
+
+
+@pytest.mark.parametrize("fail_at", [1, 2, 5])
+def test_minres_capture_failure_mid_block_falls_back_bitwise(fail_at):
+    # a capture that stops after fail_at-1 captured iterations leaves the
+    # buffer rotation part-way through its period; the eager fallback must
+    # start from the block's own buffers (bitwise the graph-free solve)
+    om = synth.koosh_ball(16, 60, 32)
+    raw = _rand((om.shape[1], 2), 63)
+    eager = _plan_with_env({"NUFFT3D_GRAPH": "0"}, om, 16, gpu=1)
+    failing = _plan_with_env({"NUFFT3D_TEST_CAPTURE_FAIL": str(fail_at)}, om, 16, gpu=1)
+    want, wi = eager.iNUFT(raw, 19, 0.02, tol=1e-30, return_info=True)
+    got, gi = failing.iNUFT(raw, 19, 0.02, tol=1e-30, return_info=True)
+    assert gi["iters"] == wi["iters"]
+    assert np.array_equal(got, want)
+
+
+def test_alternating_torch_streams_keep_plan_workspaces_ordered():
+    # fNUFT on one stream and aNUFT on another, never synchronising the
+    # host: the plan's hidden workspaces are shared, so each switch of
+    # stream must order the new stream after the plan's last call
+    import torch
+    om = synth.koosh_ball(24, 200, 48)
+    g = nufft_3d(om, 24, gpu=1)
+    x = torch.from_numpy(np.ascontiguousarray(
+        _rand((4, 24, 24, 24), 91).astype(np.complex64))).cuda()
+    xs = [x * (i + 1) for i in range(6)]
+    want = []
+    for xi in xs:
+        k = g.fNUFT(xi)
+        want.append((k.clone(), g.aNUFT(k).clone()))
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    s1.wait_stream(torch.cuda.current_stream())
+    got = []
+    for xi in xs:
+        with torch.cuda.stream(s1):
+            k = g.fNUFT(xi)
+        s2.wait_stream(s1)          # the data dependency only
+        with torch.cuda.stream(s2):
+            a = g.aNUFT(k)
+        got.append((k, a))
+    torch.cuda.synchronize()
+    for (k, a), (kw, aw) in zip(got, want):
+        assert torch.equal(k, kw)
+        assert torch.equal(a, aw)
+
+
+@pytest.mark.parametrize("maxit", [0, 1])
+def test_weights_with_maxit_at_most_1_are_used_as_given(maxit):
+    # iNUFT.m:55-58: with maxit <= 1 a vector W only warns and is used as
+    # given (not multiplied by the density)
+    om = synth.koosh_ball(12, 40, 24)
+    orc = O.NufftOracle(om, 12)
+    g = nufft_3d(om, 12, gpu=2)
+    raw = _rand((orc.M, 2), 72)
+    W = np.random.default_rng(6).uniform(0.1, 1, size=orc.M)
+    want = orc.iNUFT(raw, maxit, 0.0, W=W)
+    with pytest.warns(UserWarning):
+        got = g.iNUFT(raw, maxit, 0.0, W=W)
+    assert rel(got, want) < 1e-10
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        with pytest.raises(Exception):
+            g.iNUFT(raw, maxit, 0.0, W=W[:40])

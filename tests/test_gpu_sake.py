@@ -2,7 +2,7 @@
 
 FP64 plans agree with the oracle to 1e-9 relative L2 after several
 iterations (MINRES, Gram and eigen-decomposition in a different summation
-order); FP32 plans (fp32 MINRES, split-BF16 MFMA Gram) to 1e-3. The
+order); FP32 plans (fp32 MINRES, split-BF16 MFMA Gram) to the 1e-4 gate. The
 image-domain averaging the device uses is pinned against the literal
 circshift loop of sake.m:227-284 on the CPU (tests/test_sake_oracle.py).
 """
@@ -62,7 +62,7 @@ def test_sake_fp32_matches_oracle(setup, loraks):
     assert got.dtype == np.complex64
     err = _rel(got.astype(np.complex128), want)
     print("fp32 rel err vs FP64 oracle (loraks=%s)" % loraks, err, flush=True)
-    assert err < 1e-3
+    assert err < 1e-4
 
 
 def test_zero_noise_first_iteration_is_data_consistency(setup):

@@ -236,6 +236,48 @@ def test_density_invariants():
     assert np.linalg.norm(d1 - d) / np.linalg.norm(d) < 0.1
 
 
+@pytest.mark.parametrize("radial", [1, 0])
+def test_density_full_cartesian_coverage_is_one(radial):
+    # every integer point of [-N/2, N/2)^3 sampled once: u*om covers every
+    # even point of the periodic K-grid, so H H' 1 is constant by translation
+    # symmetry and the Pipe weights are exactly 1 after any iteration
+    N = 6
+    g = np.arange(-N // 2, N // 2, dtype=float)
+    om = np.stack([a.ravel() for a in np.meshgrid(g, g, g, indexing='ij')])
+    o = O.NufftOracle(om, N, J=4, u=2, radial=radial)
+    assert o.nnz == N ** 3
+    assert np.max(np.abs(o.d - 1)) < 1e-12
+
+
+def test_density_takes_twenty_pipe_iterations():
+    # density.m:5,11-14: 20 steps d = d./max(H H' d, eps), then mean 1. A
+    # small cluster whose iteration is still moving at step 20, with H
+    # assembled by brute force from scipy's I0 and rounded to fp32
+    # (nufft_3d.m:160): 19 or 21 steps differ from 20 by ~3e-3
+    om = np.array([[2.0, -0.5786, -0.1867, -0.8895, -0.4275, -0.1419],
+                   [0.0, 0.2397, -0.2152, 0.4055, 0.2770, -0.1238],
+                   [0.0, 0.2378, 0.5585, -0.2848, 0.0786, -0.5467]])
+    o = O.NufftOracle(om, 8, J=4, u=2, radial=1)
+    bf = _brute_H(o.om_s, o.ok, o.K, o.J, o.alpha, 1)
+    H = np.zeros((om.shape[1], int(np.prod(o.K))))
+    for m, ent in bf.items():
+        for c, v in ent.items():
+            H[m, c] = np.float32(v)
+    A = H @ H.T
+
+    def pipe(k):
+        d = o.ok.astype(float)
+        for _ in range(k):
+            d = d / np.maximum(A @ d, O.EPS)
+        return d / (np.sum(d) / o.nnz)
+
+    def r(a, b):
+        return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+    assert r(o.d, pipe(20)) < 1e-6
+    assert r(pipe(19), pipe(20)) > 1e-3 and r(pipe(21), pipe(20)) > 1e-3
+
+
 # ------------------------------------------------------------------ MINRES
 def _krylov_min(A, b, k):
     n = b.size
@@ -279,6 +321,25 @@ def test_minres_columns_independent():
     for c in range(3):
         x1, _ = O.minres(lambda v: A @ v, b[:, c:c + 1], tol=0.0, maxit=5)
         assert np.allclose(X[:, c], x1[:, 0], rtol=1e-13, atol=1e-14)
+
+
+def test_inufft_weights_with_maxit_at_most_1_are_used_as_given():
+    # iNUFT.m:55-58: maxit <= 1 with a vector W only warns, so W is not
+    # multiplied by the density: regridding is aNUFT(W.*raw)
+    o, om = _small_plan(N=10, nspokes=60, nr=20)
+    raw = np.random.default_rng(3).normal(size=(o.M, 2)) + 0j
+    W = np.random.default_rng(4).uniform(0.2, 1, size=o.M)
+    assert np.allclose(o.iNUFT(raw, 0, W=W), o.aNUFT(W[:, None] * raw), rtol=0, atol=1e-15)
+    # maxit = 1: one MINRES step, x1 = b <Ab,b>/||Ab||^2 per coil, with the
+    # operator A'WA of the same (undensified) W
+    x1 = o.iNUFT(raw, 1, W=W, tol=0.0).reshape(-1, 2, order='F')
+    b = o.aNUFT(W[:, None] * raw).reshape(-1, 2, order='F')
+    Ab = o.iprojection(b, 0.0, W)
+    for c in range(2):
+        coef = np.vdot(Ab[:, c], b[:, c]).real / np.vdot(Ab[:, c], Ab[:, c]).real
+        assert np.linalg.norm(x1[:, c] - coef * b[:, c]) < 1e-12 * np.linalg.norm(x1[:, c])
+    with pytest.raises(ValueError):
+        o.iNUFT(raw, 0, W=W[:20])
 
 
 def test_inufft_regridding_and_iterations():

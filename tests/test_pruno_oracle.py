@@ -130,3 +130,48 @@ def test_loraks_fft_form_equals_literal_loop(center):
     taps = P.kernel_taps(4, 1)[0]
     want = P.make_data_matrix(D, taps, loraks=True, center=center)
     assert np.allclose(P.gram_fft_loraks(D, taps, center), want, atol=1e-10)
+
+
+# ------------------------------------------------ noise floor (pruno.m:97-108)
+@pytest.mark.parametrize("nb", [2, 3, 5, 8, 13])
+def test_hist_integer_bin_counts_match_numpy_histogram(nb):
+    # integer bin counts: equal-width bins over [min, max]; with no value on
+    # an inner edge the (a, b] bins of hist.m and numpy's [a, b) agree
+    y = np.random.default_rng(nb).gamma(2.0, size=400)
+    assert np.array_equal(P._hist(y, nb), np.histogram(y, bins=nb)[0])
+
+
+def test_hist_non_integer_bin_count_convention():
+    # x = 3.7: floor(x) = 3 bins of width range/3.7, the last one up to max
+    # (edges 0, 2.70, 5.41, 10): {0,1,2} {3,4,5} {6..10}
+    y = np.arange(11.0)
+    assert P._hist(y, 3.7).tolist() == [3, 3, 5]
+    # an edge value belongs to the bin below it ((a, b]): x = 2 on 0..10
+    # has its inner edge at 5
+    assert P._hist(y, 2.0).tolist() == [6, 5]
+    # fewer than one bin: no counts
+    assert P._hist(y, 0.8).size == 0
+
+
+def test_hist_constant_input():
+    # y = 5 (x = 3): spread over [3.5, 6.5] in unit bins: 5 is in (4.5, 5.5]
+    assert P._hist(np.full(4, 5.0), 3).tolist() == [0, 4, 0]
+
+
+@pytest.mark.parametrize("nsig,m", [(6, 61), (12, 121), (3, 35)])
+def test_estimate_noise_recovers_a_flat_noise_plateau(nsig, m):
+    # squared singular values: a few signal values well above a plateau of
+    # m values sigma^2 (1 + 1e-3 t), t Gaussian quantiles (a histogram peaked
+    # in its middle, median exactly sigma^2), then true zeros. The loop drops
+    # the signal values one by one (their range puts the whole plateau in the
+    # first bin) and stops at the plateau; the estimate is
+    # sigma / sqrt(density nx ny).
+    from scipy.stats import norm
+    sigma2 = 2.5
+    t = norm.ppf((np.arange(m) + 0.5) / m)[::-1]
+    plateau = sigma2 * (1 + 1e-3 * t)
+    signal = sigma2 * np.geomspace(1e4, 50, nsig)
+    W = np.concatenate([signal, plateau, np.zeros(4)])
+    density, nx, ny = 0.8, 24, 20
+    got = P.estimate_noise(W, density, nx, ny)
+    assert got == pytest.approx(np.sqrt(sigma2) / np.sqrt(density * nx * ny), rel=1e-12)

@@ -91,6 +91,17 @@ int nufft3d_plan_destroy(nufft3d_plan* plan);
 /* k = obj.fNUFT(x) (fNUFT.m:2-9): x [N1 N2 N3 nc] -> k [M nc]. */
 int nufft3d_forward(nufft3d_plan* plan, const void* x, int32_t nc, void* k);
 
+/* k = obj.fNUFT(x) as nufft3d_forward, allocating nothing: no counterpart in
+ * the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x and k must be device arrays. The intermediates live in a fixed 64 MiB
+ * device array per GPU, held until the process ends; coils run in groups
+ * that fit it, and a plan whose single coil needs more (roughly K1*K2*K3 *
+ * 1.75 * 8 bytes for fp32 arithmetic) returns NUFFT3D_ERR_UNSUPPORTED.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * race. Results equal nufft3d_forward's. */
+int nufft3d_forward_noalloc(nufft3d_plan* plan, const void* x, int32_t nc, void* k);
+
 /* x = obj.aNUFT(k) (aNUFT.m:2-9): k [M nc] -> x [N1 N2 N3 nc]. */
 int nufft3d_adjoint(nufft3d_plan* plan, const void* k, int32_t nc, void* x);
 

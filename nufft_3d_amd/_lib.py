@@ -57,6 +57,7 @@ SIGNATURES = {
     "nufft3d_plan_destroy": (C.c_int, [C.c_void_p]),
     "nufft3d_plan_warnings": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "nufft3d_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "nufft3d_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "nufft3d_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "nufft3d_inverse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
                                   C.c_void_p, C.c_int64, C.c_double, C.c_void_p,

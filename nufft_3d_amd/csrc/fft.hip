@@ -592,17 +592,24 @@ static int bundle_x(int L, size_t esz, int nc, int want, int threads = 256) {
 }
 
 template <typename R>
-static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid) {
+static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1buf,
+                         void* t2buf) {
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
   const size_t t1 = (size_t)K1 * N2 * N3 * nc, t2 = (size_t)K1 * K2 * N3 * nc;
-  N3D_TRY(ensure(p->ws_t1, t1 * esz));
-  void* T1 = p->ws_t1.p;
+  void* T1 = t1buf;
+  if (!T1) {
+    N3D_TRY(ensure(p->ws_t1, t1 * esz));
+    T1 = p->ws_t1.p;
+  }
   void* T2 = grid;
   if (K3 > 1) {
-    N3D_TRY(ensure(p->ws_t2, t2 * esz));
-    T2 = p->ws_t2.p;
+    T2 = t2buf;
+    if (!T2) {
+      N3D_TRY(ensure(p->ws_t2, t2 * esz));
+      T2 = p->ws_t2.p;
+    }
   }
   {  // x-pass
     XGeom g;
@@ -779,9 +786,9 @@ int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out) {
   return p->prec == 2 ? fft_plain_t<double>(p, in, nc, out) : fft_plain_t<float>(p, in, nc, out);
 }
 
-int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid) {
-  return p->prec == 2 ? fft_forward_t<double>(p, img, nc, grid)
-                      : fft_forward_t<float>(p, img, nc, grid);
+int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1, void* t2) {
+  return p->prec == 2 ? fft_forward_t<double>(p, img, nc, grid, t1, t2)
+                      : fft_forward_t<float>(p, img, nc, grid, t1, t2);
 }
 
 int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,

@@ -945,8 +945,8 @@ int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user);
 // user_layout: the caller's [M nc] array (rows of out-of-bounds samples are
 // left to zero_oob); otherwise the sorted coil-interleaved layout
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
-           const void* wsamp, double scale) {
-  if (user_layout && (p->tune_flags & 16)) {
+           const void* wsamp, double scale, bool allow_ws) {
+  if (user_layout && (p->tune_flags & 16) && allow_ws) {
     // interpolate into the sorted layout, then scatter rows (separate pass)
     const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
     N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * esz));

@@ -215,8 +215,9 @@ struct ProfScope {
 int ensure_host(Workspace& w, size_t bytes);
 
 // grid.hip
+// allow_ws = false never takes the plan's workspaces (the direct caller-row write)
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
-           const void* wsamp, double scale);
+           const void* wsamp, double scale, bool allow_ws = true);
 int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
            const void* wsamp);
 int zero_oob(nufft3d_plan* p, void* k_user, int nc);
@@ -225,7 +226,10 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
 
 // fft.hip
 int fft_setup_axis(nufft3d_plan* p, int axis);
-int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid);   // deap, pad, FFT
+// deap, pad, FFT; t1/t2 = caller-given intermediates ([K1 N2 N3 nc], [K1 K2 N3 nc]
+// complex of the arithmetic precision) or NULL for the plan's workspaces
+int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1 = nullptr,
+                void* t2 = nullptr);
 int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                 const void* img_add, double add_coef);                    // IFFT, crop, deap
 int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out);       // unshifted N-size FFT

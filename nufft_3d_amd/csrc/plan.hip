@@ -1577,6 +1577,114 @@ int nufft3d_adjoint(nufft3d_plan* p, const void* k, int32_t nc, void* x) {
   return st;
 }
 
+// ---- fNUFT that allocates nothing (no counterpart in the reference).
+// Its intermediates (the padded and transformed image, the oversampled grid
+// and, for fp64 arithmetic behind fp32 arrays, the widened image and the
+// unrounded result) live in one file-scope device array of fixed size; coil
+// groups are sized to fit it. Each GPU the library runs on holds its own copy
+// until the process ends. Calls on one GPU share it: calls that overlap in
+// time on one GPU race.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of this device's scratch array, looked up once per
+// device with that device current (the array's host-side name is not an
+// address the device can use).
+static int scratch_on_device(int dev, void** out) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    set_error("device ordinal out of range for the no-allocation scratch");
+    return NUFFT3D_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    N3D_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) { set_error("hipGetSymbolAddress returned NULL for the scratch array"); return NUFFT3D_ERR_DEVICE; }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NUFFT3D_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// scratch layout of one group of g coils: [widened image][T1][T2][grid][unrounded k]
+struct NoallocLayout {
+  size_t img, t1, t2, grid, kout, total;
+};
+static NoallocLayout noalloc_layout(const nufft3d_plan* p, int g) {
+  const size_t esz = cp_esz(p);
+  const bool conv = p->io_prec != p->prec;
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2];
+  const size_t n1 = (size_t)p->K[0] * p->N[1] * p->N[2];
+  const size_t n2 = p->K[2] > 1 ? (size_t)p->K[0] * p->K[1] * p->N[2] : 0;
+  const size_t ng = (size_t)p->K[0] * p->K[1] * p->K[2];
+  NoallocLayout L;
+  L.img = 0;
+  L.t1 = L.img + (conv ? align256(nimg * g * esz) : 0);
+  L.t2 = L.t1 + align256(n1 * g * esz);
+  L.grid = L.t2 + align256(n2 * g * esz);
+  L.kout = L.grid + align256(ng * g * esz);
+  L.total = L.kout + (conv ? align256((size_t)p->M * g * esz) : 0);
+  return L;
+}
+
+int nufft3d_forward_noalloc(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
+  N3D_PLAN_GUARD(p);
+  if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
+  if (!is_device_ptr(x) || !is_device_ptr(k)) {
+    set_error("nufft3d_forward_noalloc takes device arrays only (a host array needs a staging copy)");
+    return NUFFT3D_ERR_INVALID;
+  }
+  const NoallocLayout one = noalloc_layout(p, 1);
+  if (one.total > kScratchBytes) {
+    char b[200];
+    snprintf(b, sizeof b, "fNUFT of this plan needs %.1f MiB of scratch per coil; the no-allocation "
+             "entry point has %zu MiB", one.total / 1048576.0, kScratchBytes >> 20);
+    set_error(b);
+    return NUFFT3D_ERR_UNSUPPORTED;
+  }
+  int g = nc;
+  while (g > 1 && noalloc_layout(p, g).total > kScratchBytes) --g;
+  const NoallocLayout L = noalloc_layout(p, g);
+  unsigned char* base = nullptr;
+  N3D_TRY(scratch_on_device(p->device, (void**)&base));
+  const bool conv = p->io_prec != p->prec;
+  const size_t io_esz = p->io_prec == 2 ? sizeof(double2) : sizeof(float2);
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2];
+  const size_t M = (size_t)p->M;
+  const bool prof = p->prof.on;   // no profiling events here
+  p->prof.on = false;
+  int st = NUFFT3D_OK;
+  for (int c0 = 0; c0 < nc && st == NUFFT3D_OK; c0 += g) {
+    const int cg = nc - c0 < g ? nc - c0 : g;
+    const unsigned char* xg = (const unsigned char*)x + (size_t)c0 * nimg * io_esz;
+    unsigned char* kg = (unsigned char*)k + (size_t)c0 * M * io_esz;
+    const void* img = xg;
+    if (conv) {
+      const long long n = (long long)(nimg * cg * 2);
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblocks(n)), dim3(256), 0, p->stream,
+                         (const float*)xg, (double*)(base + L.img), n);
+      img = base + L.img;
+    }
+    void* out = conv ? (void*)(base + L.kout) : (void*)kg;
+    st = fft_forward(p, img, cg, base + L.grid, base + L.t1, base + L.t2);             // fNUFT.m:5-6
+    if (!st) st = interp(p, base + L.grid, cg, out, 1, nullptr, 1.0 / sqrt((double)p->nnz), false);
+    if (!st) st = zero_oob(p, out, cg);
+    if (!st && conv && M > 0) {
+      const long long n = (long long)(M * cg * 2);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblocks(n)), dim3(256), 0, p->stream,
+                         (const double*)out, (float*)kg, n);
+    }
+    if (!st && hipGetLastError() != hipSuccess) { set_error("kernel launch failed"); st = NUFFT3D_ERR_DEVICE; }
+  }
+  p->prof.on = prof;
+  return st;
+}
+
 // W handling of iNUFT.m:55-74 -> per sorted sample weight W(row)*d. With
 // maxit <= 1 the reference only warns (iNUFT.m:57-58): W stays the caller's
 // length-nrow vector, not multiplied by the density and not validated, and

@@ -501,9 +501,24 @@ int fft_setup_axis(nufft3d_plan* p, int axis) {
   a.plan_id = plan_id_of(st);
   const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
   std::vector<unsigned char> host(esz * L);
+  // roots of unity in extended precision (x86 long double), rounded once:
+  // the angle 2 pi t / L formed in double is off by up to ~6 ulps, and the
+  // fp64 plans' transforms feed the deapodization, whose range (up to 2e7
+  // at u = 1.25, J = 6) amplifies any rounding at the field-of-view edge
+  const long double two_pi = 6.283185307179586476925286766559005768L;
   for (int t = 0; t < L; ++t) {
-    const double ang = -2.0 * M_PI * (double)t / (double)L;
-    const double c = cos(ang), s = sin(ang);
+    double c, s;
+    if ((8LL * t) % L == 0) {   // multiples of pi/4: exact values
+      static const double c8[8] = {1, 0.70710678118654752440, 0, -0.70710678118654752440, -1,
+                                   -0.70710678118654752440, 0, 0.70710678118654752440};
+      const int o = (int)(8LL * t / L);
+      c = c8[o];
+      s = -c8[(o + 6) % 8];   // sin(-x) = -cos(x - pi/2)
+    } else {
+      const long double ang = -two_pi * (long double)t / (long double)L;
+      c = (double)cosl(ang);
+      s = (double)sinl(ang);
+    }
     if (p->prec == 2) {
       double2 v; v.x = c; v.y = s;
       reinterpret_cast<double2*>(host.data())[t] = v;

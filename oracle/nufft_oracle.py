@@ -222,38 +222,45 @@ class NufftOracle:
             d[~ok] = 0
             self.d = d
 
-    # nufft_3d.m:153-212
+    # nufft_3d.m:153-212. The per-axis squares (x-om)^2 and wrapped indices
+    # are formed once per axis offset and combined per (dx,dy,dz): the same
+    # values and the same sum order dx2 + dy2 + dz2 as the reference's loop.
     def _assemble_H(self):
         J, K = self.J, self.K
         oms, ok = self.om_s, self.ok
-        rng = range(-int(math.ceil(J / 2)), int(math.ceil(J / 2)) + 1)
+        h = int(math.ceil(J / 2))
+        rng = range(-h, h + 1)
         rows, cols, vals = [], [], []
         base = [matlab_round(oms[a]) for a in range(3)]
+        stride = (1, K[0], K[0] * K[1])
+        d2, idx = [], []
+        for a in range(3):
+            d2.append({})
+            idx.append({})
+            for d in rng:
+                x = base[a] + d
+                d2[a][d] = (x - oms[a]) * (x - oms[a])
+                idx[a][d] = np.mod(x, K[a]).astype(np.int64) * stride[a]
+        J2 = J ** 2
         for dx in rng:
             for dy in rng:
+                sxy = d2[0][dx] + d2[1][dy]
+                cxy = idx[0][dx] + idx[1][dy]
                 for dz in rng:
                     if self.N[2] == 1 and dz != 0:
                         continue
-                    x = base[0] + dx
-                    y = base[1] + dy
-                    z = base[2] + dz
-                    dx2 = (x - oms[0]) * (x - oms[0])
-                    dy2 = (y - oms[1]) * (y - oms[1])
-                    dz2 = (z - oms[2]) * (z - oms[2])
-                    x = np.mod(x, K[0])
-                    y = np.mod(y, K[1])
-                    z = np.mod(z, K[2])
                     if self.radial:
-                        dist2 = dx2 + dy2 + dz2
-                        i = ok & (4 * dist2 <= J ** 2)
-                        s = kernel(dist2[i], J, self.alpha)
+                        dist2 = sxy + d2[2][dz]
+                        i = ok & (4 * dist2 <= J2)
+                        r = np.nonzero(i)[0]
+                        s = kernel(dist2[r], J, self.alpha)
                     else:
-                        i = ok & (4 * dx2 <= J ** 2) & (4 * dy2 <= J ** 2) & (4 * dz2 <= J ** 2)
-                        s = (kernel(dx2[i], J, self.alpha) * kernel(dy2[i], J, self.alpha)
-                             * kernel(dz2[i], J, self.alpha))
-                    col = x[i] + K[0] * y[i] + K[0] * K[1] * z[i]
-                    rows.append(np.nonzero(i)[0])
-                    cols.append(col.astype(np.int64))
+                        i = ok & (4 * d2[0][dx] <= J2) & (4 * d2[1][dy] <= J2) & (4 * d2[2][dz] <= J2)
+                        r = np.nonzero(i)[0]
+                        s = (kernel(d2[0][dx][r], J, self.alpha) * kernel(d2[1][dy][r], J, self.alpha)
+                             * kernel(d2[2][dz][r], J, self.alpha))
+                    rows.append(r)
+                    cols.append(cxy[r] + idx[2][dz][r])
                     vals.append(s.astype(np.float32).astype(np.float64))  # nufft_3d.m:160,205
         rows = np.concatenate(rows)
         cols = np.concatenate(cols)

@@ -171,6 +171,20 @@ def estimate_noise(W, density, nx, ny):
     return sigma / math.sqrt(density * nx * ny)
 
 
+def select(W, density, nx, ny, noise=None, tol=0.01):
+    """pruno.m:96-115 on the squared singular values W (descending): the noise
+    std (estimated when None), the singular values sqrt(W), the
+    minimum-variance filter f = max(0, 1 - sigma^2/W) and nv = #(f < 1-tol)."""
+    if noise is None:
+        noise = estimate_noise(W, density, nx, ny)
+    sigma = noise * math.sqrt(density * nx * ny)              # pruno.m:108
+    Ws = np.sqrt(np.maximum(W, 0))                            # pruno.m:111
+    with np.errstate(divide='ignore'):
+        f = np.where(Ws > 0, np.maximum(0, 1 - sigma ** 2 / np.where(Ws > 0, Ws, 1) ** 2), 0.0)
+    nv = int(np.count_nonzero(f < 1 - tol))                   # pruno.m:115
+    return noise, Ws, f, nv
+
+
 def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=False, AA=None):
     """pruno.m:43-143. Returns dict(K, Kt, nv, sv, f, noise, taps, mask)."""
     nx, ny, nz, nc = data.shape
@@ -185,13 +199,7 @@ def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=Fal
     order = np.argsort(w)[::-1]
     W = w[order]
     V = V[:, order]
-    if noise is None:
-        noise = estimate_noise(W, density, nx, ny)
-    sigma = noise * math.sqrt(density * nx * ny)              # pruno.m:108
-    Ws = np.sqrt(np.maximum(W, 0))                            # pruno.m:111
-    with np.errstate(divide='ignore'):
-        f = np.where(Ws > 0, np.maximum(0, 1 - sigma ** 2 / np.where(Ws > 0, Ws, 1) ** 2), 0.0)
-    nv = int(np.count_nonzero(f < 1 - tol))                   # pruno.m:115
+    noise, Ws, f, nv = select(W, density, nx, ny, noise, tol)
     n = nc * nk
     sel = np.arange(nv) if literal_first_nv else np.arange(n - nv, n)
     Kt = V[:, sel] * (1 - f[sel])[None, :]                    # [(c + nc*t), v]

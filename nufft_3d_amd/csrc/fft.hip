@@ -17,7 +17,9 @@
 // (16, 8, 4, 2, 3, 5, 7) Stockham autosort whose stages exchange data in LDS
 // (element-major layout, line fastest); the first stage reads global memory
 // and the last writes it wherever the access pattern stays coalesced.
+#include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <cmath>
 #include <vector>
 
@@ -579,13 +581,18 @@ static int z_threads(const nufft3d_plan* p, size_t esz, bool fwd) {
 }
 
 // raise a kernel's dynamic LDS limit once when a launch needs more than 64 KiB
+// (one flag per kernel instantiation: instantiations share a function type)
 template <typename K>
 static void allow_lds(K* kernel, size_t bytes) {
-  static std::atomic<bool> granted{false};
-  if (bytes <= 65536 || granted.load()) return;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStridedLds) == hipSuccess)
-    granted.store(true);
+  static std::mutex mu;
+  static std::vector<const void*> granted;
+  if (bytes <= 65536) return;
+  const void* f = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(granted.begin(), granted.end(), f) != granted.end()) return;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStridedLds) ==
+      hipSuccess)
+    granted.push_back(f);
   (void)hipGetLastError();
 }
 

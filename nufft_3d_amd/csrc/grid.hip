@@ -8,6 +8,7 @@
 // wrapped mod K (nufft_3d.m:178-180), kept when inside the radial or
 // separable support with <= (nufft_3d.m:182-191).
 #include <algorithm>
+#include <type_traits>
 
 #include "kb.h"
 #include "nufft3d_internal.h"
@@ -79,11 +80,29 @@ __device__ __forceinline__ double hit_weight_d(int dx, int dy, int dz, const dou
   return (double)(float)(kb_d(ex * ex, p) * kb_d(ey * ey, p) * kb_d(ez * ez, p));
 }
 
+// fp64 arithmetic behind fp32 arrays: the tabulated kernel, rounded to float
+__device__ __forceinline__ double hit_weight_tab(int dx, int dy, int dz, double fx, double fy,
+                                                 double fz, const double* tab, const KBParams& p) {
+  const double ex = (double)dx - fx, ey = (double)dy - fy, ez = (double)dz - fz;
+  if (p.radial) return (double)(float)kb_tab_d(ex * ex + ey * ey + ez * ez, tab, p);
+  return (double)(float)(kb_tab_d(ex * ex, tab, p) * kb_tab_d(ey * ey, tab, p) *
+                         kb_tab_d(ez * ez, tab, p));
+}
+
 __device__ __forceinline__ float hit_weight_f(int dx, int dy, int dz, const float4 f,
                                               const KBParams& p) {
   const float ex = (float)dx - f.x, ey = (float)dy - f.y, ez = (float)dz - f.z;
   if (p.radial) return kb_f(ex * ex + ey * ey + ez * ez, p);
   return kb_f(ex * ex, p) * kb_f(ey * ey, p) * kb_f(ez * ez, p);
+}
+
+// Stage the kernel table (kb_tab_d) in LDS; every thread of the work-group
+// must call it (it synchronises). tab == nullptr: nothing to stage.
+__device__ __forceinline__ const double* stage_kb_table(const double* __restrict__ tab, double* lds) {
+  if (!tab) return nullptr;
+  for (int i = threadIdx.x; i < kKbIntervals * kKbCoef; i += blockDim.x) lds[i] = tab[i];
+  __syncthreads();
+  return lds;
 }
 
 // accumulator type: fp64 plans and compensated fp32 plans accumulate in
@@ -192,8 +211,12 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
                                                 cplx<R>* __restrict__ out,
                                                 const R* __restrict__ wsamp, R scale,
                                                 const int* __restrict__ orig,
+                                                const double* __restrict__ kbtab,
                                                 GridGeom g, KBParams kp) {
   using A = typename AccT<R, COMP>::type;
+  __shared__ double stab[sizeof(R) == 8 ? kKbIntervals * kKbCoef : 1];
+  const double* tab = nullptr;
+  if constexpr (sizeof(R) == 8) tab = stage_kb_table(kbtab, stab);
   const long long gt = xcd_block(g.xchunk) * blockDim.x + threadIdx.x;
   const int s = (int)(gt / LPS);
   const int q = (int)(gt % LPS);
@@ -221,8 +244,11 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
     int dx, dy, dz;
     decode_hit(cur, g, dx, dy, dz);
     R w;
-    if constexpr (sizeof(R) == 8) w = hit_weight_d(dx, dy, dz, f64, kp);
-    else w = hit_weight_f(dx, dy, dz, f32, kp);
+    if constexpr (sizeof(R) == 8)
+      w = tab ? hit_weight_tab(dx, dy, dz, f64[0], f64[1], f64[2], tab, kp)
+              : hit_weight_d(dx, dy, dz, f64, kp);
+    else
+      w = hit_weight_f(dx, dy, dz, f32, kp);
     if (e + q >= e1) w = 0;
     const int gx = wrapk(b.x + dx, g.K1), gy = wrapk(b.y + dy, g.K2), gz = wrapk(b.z + dz, g.K3);
     const int gi = (gz * g.K2 + gy) * g.K1 + gx;
@@ -263,6 +289,153 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward from LDS tiles (3-D plans with a 7-point footprint, ceil(J/2) = 3).
+// Samples are sorted by 4x4x4 sub-bins of their rounded position; a
+// work-group takes one piece (up to 128 samples of one sub-bin, one per
+// 4-lane group) whose hits all lie in the sub-bin's 10x10x10 grid
+// neighbourhood (wrapped). That neighbourhood is staged in LDS NL z-layers
+// at a time (coalesced row reads), and each group walks its sample's hit
+// list through the passes: the codes are in z-y-x order, so the hits of a
+// pass are a run of the list, and a pass boundary can fall inside a step of
+// LPS hits without changing the order of the sums. Same hits, weights and
+// summation order as k_interp; a few KB of LDS per work-group, so several
+// work-groups share a CU and one's staging overlaps another's arithmetic.
+constexpr int kTileEdge = 10;
+constexpr int kTileThreads = 512;
+
+// Lane q of a 4-lane group owns CPL coils of the group: UPL 16-byte units of
+// UB coils each. Every staged grid point holds CG*sizeof(C) = 128 bytes and so
+// starts on the same LDS bank; with two units per lane, odd groups own their
+// units in the opposite order (unit_of), so in each read instruction the 8
+// lanes of two neighbouring groups read 8 different units - all 32 banks -
+// whatever points they read.
+template <typename R, int CPL>
+struct TileLane {
+  static constexpr int UB = 16 / (int)sizeof(cplx<R>);   // coils per 16-byte unit
+  static constexpr int UPL = CPL / UB > 0 ? CPL / UB : 1;
+  // the coil (within the group) of the lane's c-th accumulator
+  static __device__ __forceinline__ int coil(int q, int c, int par) {
+    if constexpr (UPL == 2) return q * CPL + ((c / UB) ^ par) * UB + c % UB;
+    else return q * CPL + c;
+  }
+};
+
+template <typename R, int CG, int LPS, bool COMP, bool USER, bool FASTW>
+__global__ void __launch_bounds__(kTileThreads) k_interp_tile(
+    const int4* __restrict__ pieces, const long long* __restrict__ spos,
+    const unsigned short* __restrict__ fcode, const int4* __restrict__ base,
+    const double* __restrict__ off, const float4* __restrict__ off32,
+    const cplx<R>* __restrict__ grid, cplx<R>* __restrict__ out, const R* __restrict__ wsamp,
+    R scale, const int* __restrict__ orig, const double* __restrict__ kbtab, GridGeom g,
+    KBParams kp, int NL) {
+  using A = typename AccT<R, COMP>::type;
+  using C = cplx<R>;
+  using TL = TileLane<R, CG / LPS>;
+  constexpr int CPL = CG / LPS;
+  constexpr int P2 = kTileEdge * kTileEdge;   // points of one z-layer
+  constexpr int TAB = FASTW ? kKbIntervals * kKbCoef : 0;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* stab = reinterpret_cast<double*>(smem_raw);
+  C* tile = reinterpret_cast<C*>(smem_raw + TAB * sizeof(double));
+  if constexpr (FASTW)
+    for (int i = threadIdx.x; i < TAB; i += kTileThreads) stab[i] = kbtab[i];
+  const int4 pc = pieces[xcd_block(g.xchunk)];
+  const int x0 = (pc.x & 1023) * 4 - 3, y0 = ((pc.x >> 10) & 1023) * 4 - 3, z0 = (pc.x >> 20) * 4 - 3;
+  const int q = threadIdx.x % LPS;
+  const int lane = threadIdx.x & 63;
+  const int gsh = lane & ~(LPS - 1);
+  const int par = (lane / LPS) & 1;
+  const int j = threadIdx.x / LPS;
+  const bool act = j < pc.z;
+  const int s = pc.y + (act ? j : 0);
+  const int4 b = base[s];
+  const int lbz = b.z - z0;
+  const int lb = (lbz * kTileEdge + (b.y - y0)) * kTileEdge + (b.x - x0);
+  float4 f32;
+  if constexpr (sizeof(R) == 4) f32 = off32[s];
+  const double fx = off[3LL * s], fy = off[3LL * s + 1], fz = off[3LL * s + 2];
+  A ar[CPL], ai[CPL];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
+  const long long e0 = spos[s];
+  const unsigned short* codes = fcode + e0;          // this sample's hit codes
+  const int nh = act ? (int)(spos[s + 1] - e0) : 0;  // >= 1 for an in-bounds sample
+  const int hl = nh > 0 ? nh - 1 : 0;
+  int e = 0;
+  unsigned short cur = codes[min(e + q, hl)];
+  for (int zl = 0; zl < kTileEdge; zl += NL) {
+    const int nl = kTileEdge - zl < NL ? kTileEdge - zl : NL;
+    __syncthreads();   // the previous layers are consumed (and the table staged)
+    for (int i = threadIdx.x; i < nl * P2 * CG; i += kTileThreads) {
+      const int pt = i / CG, c = i - pt * CG;
+      const int lx = pt % kTileEdge, lyz = pt / kTileEdge;
+      const int ly = lyz % kTileEdge, lz = zl + lyz / kTileEdge;
+      const int gx = wrapk(x0 + lx, g.K1), gy = wrapk(y0 + ly, g.K2), gz = wrapk(z0 + lz, g.K3);
+      C v;
+      v.x = 0;
+      v.y = 0;
+      if (c < g.cg) v = grid[(((long long)gz * g.K2 + gy) * g.K1 + gx) * g.nc + g.c0 + c];
+      tile[pt * CG + c] = v;
+    }
+    __syncthreads();
+    const int zh = zl + nl;
+    while (e < nh) {
+      const unsigned short nxt = codes[min(e + LPS + q, hl)];
+      int dx, dy, dz;
+      decode_hit(cur, g, dx, dy, dz);
+      // hits of this pass: a run at the head of the step (dz never decreases)
+      const bool mine = e + q < nh && lbz + dz < zh;
+      const unsigned long long bal = __ballot(mine);
+      const int n = __popcll((bal >> gsh) & ((1ull << LPS) - 1));
+      R w = 0;
+      if (mine) {
+        if constexpr (sizeof(R) == 8) {
+          if constexpr (FASTW) w = hit_weight_tab(dx, dy, dz, fx, fy, fz, stab, kp);
+          else w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+        } else {
+          w = hit_weight_f(dx, dy, dz, f32, kp);
+        }
+      }
+      const int li = lb + (dz * kTileEdge + dy) * kTileEdge + dx - zl * P2;
+#define N3D_TSTEP(J)                                                        \
+      if constexpr (J < LPS) {                                              \
+        if (J < n) {                                                        \
+          const R wj = grp_bcast<LPS, J>(w);                                \
+          const C* row = tile + grp_bcast_i<LPS, J>(li) * CG;               \
+          _Pragma("unroll") for (int c = 0; c < CPL; ++c) {                 \
+            const C v = row[TL::coil(q, c, par)];                           \
+            ar[c] += (A)wj * (A)v.x;                                        \
+            ai[c] += (A)wj * (A)v.y;                                        \
+          }                                                                 \
+        }                                                                   \
+      }
+      N3D_TSTEP(0) N3D_TSTEP(1) N3D_TSTEP(2) N3D_TSTEP(3)
+#undef N3D_TSTEP
+      e += n;
+      if (n < LPS) {   // the rest of the list belongs to later passes
+        cur = codes[min(e + q, hl)];
+        break;
+      }
+      cur = nxt;
+    }
+  }
+  if (!act) return;
+  R sc = scale;
+  if (wsamp) sc *= wsamp[s];
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    const int k = TL::coil(q, c, par);
+    if (k < g.cg) {
+      C v;
+      v.x = (R)ar[c] * sc;
+      v.y = (R)ai[c] * sc;
+      if constexpr (USER) out[orig[s] + (long long)(g.c0 + k) * g.M] = v;
+      else out[(long long)s * g.nc + g.c0 + k] = v;
+    }
+  }
+}
+
 // Sum over hits [e0, e1) of one grid point's list (sorted sample ids, codes
 // alongside, or with pw the fp32 weights of the plan's table), weights times
 // wsamp when given, into ar/ai (LPS-lane group).
@@ -276,7 +449,8 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
                                             const float4* __restrict__ off32,
                                             const cplx<R>* __restrict__ ks,
                                             const R* __restrict__ wsamp, const GridGeom& g,
-                                            const KBParams& kp, A* ar, A* ai) {
+                                            const KBParams& kp, const double* tab, A* ar,
+                                            A* ai) {
   if (e0 >= e1) return;
   // lane q's entry of the step starting at e (the step's first entry past the end)
   auto ent = [&](long long e) { return e + q < e1 ? e + q : e; };
@@ -298,7 +472,9 @@ __device__ __forceinline__ void spread_hits(long long e0, long long e1, int q,
     } else if constexpr (sizeof(R) == 8) {
       int dx, dy, dz;
       decode_hit((unsigned short)c0, g, dx, dy, dz);
-      w = hit_weight_d(dx, dy, dz, off + 3LL * s, kp);
+      const double* f = off + 3LL * s;
+      w = tab ? hit_weight_tab(dx, dy, dz, f[0], f[1], f[2], tab, kp)
+              : hit_weight_d(dx, dy, dz, f, kp);
     } else {
       int dx, dy, dz;
       decode_hit((unsigned short)c0, g, dx, dy, dz);
@@ -348,7 +524,7 @@ __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* 
                                                    const cplx<R>* __restrict__ ks,
                                                    const R* __restrict__ wsamp,
                                                    cplx<R>* __restrict__ partial, const GridGeom& g,
-                                                   const KBParams& kp) {
+                                                   const KBParams& kp, const double* tab) {
   using A = typename AccT<R, COMP>::type;
   const long long gt = b * blockDim.x + threadIdx.x;
   const int si = (int)(gt / LPS);
@@ -359,7 +535,7 @@ __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* 
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
   spread_hits<R, CPL, LPS, FULL>(sg.start, sg.end, q, hits, pw, off, off32, ks, wsamp, g, kp,
-                                 ar, ai);
+                                 tab, ar, ai);
   // partial sums keep the accumulator precision until the final reduction
   A* pp = reinterpret_cast<A*>(partial) + (long long)sg.slot * g.cg * 2;
 #pragma unroll
@@ -376,25 +552,29 @@ __device__ __forceinline__ void spread_heavy_block(long long b, const HeavySeg* 
 // by k_spread_heavy + k_spread_hreduce; the plan holds two segmentations
 // (short segments for launches of at most 4 coils), so a coil's sum is
 // grouped the same way within either size class.
-template <typename R, int CPL, int LPS, bool COMP, bool FULL>
+template <typename R, int CPL, int LPS, bool COMP, bool FULL, typename GO = cplx<R>>
 __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ rowptr,
                                                 const uint2* __restrict__ hits, bool pw,
                                                 const double* __restrict__ off,
                                                 const float4* __restrict__ off32,
                                                 const cplx<R>* __restrict__ ks,
                                                 const R* __restrict__ wsamp,
-                                                cplx<R>* __restrict__ grid, long long ngrid,
+                                                GO* __restrict__ grid, long long ngrid,
                                                 const int* __restrict__ bricks,
                                                 int rmax, GridGeom g, KBParams kp,
                                                 const HeavySeg* __restrict__ segs, int nseg,
-                                                int nhb, cplx<R>* __restrict__ partial) {
+                                                int nhb, cplx<R>* __restrict__ partial,
+                                                const double* __restrict__ kbtab) {
   using A = typename AccT<R, COMP>::type;
+  __shared__ double stab[sizeof(R) == 8 ? kKbIntervals * kKbCoef : 1];
+  const double* tab = nullptr;
+  if constexpr (sizeof(R) == 8) tab = stage_kb_table(kbtab, stab);
   // the first nhb work-groups (a multiple of 8, so the XCD of every later
   // work-group is unchanged) sum the dense points' segments; the rest are
   // the light points, renumbered for the XCDs among themselves
   if ((int)blockIdx.x < nhb) {
     spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, pw, off, off32, ks,
-                                                wsamp, partial, g, kp);
+                                                wsamp, partial, g, kp, tab);
     return;
   }
   const long long gt = xcd_block_of(g.xchunk, blockIdx.x - nhb, gridDim.x - nhb) * blockDim.x +
@@ -416,12 +596,17 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
   A ar[CPL], ai[CPL];
 #pragma unroll
   for (int c = 0; c < CPL; ++c) { ar[c] = 0; ai[c] = 0; }
-  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hits, pw, off, off32, ks, wsamp, g, kp, ar, ai);
-  cplx<R>* gp = grid + gpt * g.nc + g.c0;
+  spread_hits<R, CPL, LPS, FULL>(e0, e1, q, hits, pw, off, off32, ks, wsamp, g, kp, tab, ar, ai);
+  GO* gp = grid + gpt * g.nc + g.c0;
 #pragma unroll
   for (int c = 0; c < CPL; ++c) {
     const int k = coil_of<R, CPL, LPS, FULL>(q, c);
-    if (k < g.cg) { cplx<R> o; o.x = (R)ar[c]; o.y = (R)ai[c]; gp[k] = o; }
+    if (k < g.cg) {
+      GO o;
+      o.x = (decltype(o.x))ar[c];
+      o.y = (decltype(o.y))ai[c];
+      gp[k] = o;
+    }
   }
 }
 
@@ -533,9 +718,12 @@ __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict
                                                       const cplx<R>* __restrict__ ks,
                                                       const R* __restrict__ wsamp,
                                                       cplx<R>* __restrict__ partial, GridGeom g,
-                                                      KBParams kp) {
+                                                      KBParams kp, const double* __restrict__ kbtab) {
+  __shared__ double stab[sizeof(R) == 8 ? kKbIntervals * kKbCoef : 1];
+  const double* tab = nullptr;
+  if constexpr (sizeof(R) == 8) tab = stage_kb_table(kbtab, stab);
   spread_heavy_block<R, CPL, LPS, COMP, FULL>(blockIdx.x, segs, nseg, hits, pw, off, off32, ks,
-                                              wsamp, partial, g, kp);
+                                              wsamp, partial, g, kp, tab);
 }
 
 // Sum of a dense point's segment partials, one work-group per point: the R
@@ -755,9 +943,23 @@ static KBParams make_kb(const nufft3d_plan* p) {
                       y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
                       y * (-0.1647633e-1 + y * 0.392377e-2)))))));
     k.inv_P0f = (float)(1.0 / P0);
+    k.inv_P0 = 1.0 / P0;
   }
+  k.inv_half2 = 1.0 / k.half2;
+  k.inv_b0 = 1.0 / b0;
+  k.c375a = 3.75 / a;
+  k.qb = a >= 3.75 ? 1.0 - (3.75 / a) * (3.75 / a) : -1.0;
+  k.tscale = a >= 3.75 ? kKbIntervals / k.qb : 0.0;
+  k.cy = (a / 3.75) * (a / 3.75);
   k.radial = p->radial;
   return k;
+}
+
+// The kernel table replaces the literal evaluation in plans computing in
+// fp64 behind fp32 arrays (their accuracy bar is 1e-4); fp64 plans keep
+// kernel.m's arithmetic for the 1e-12 bar.
+static const double* kb_table_of(const nufft3d_plan* p) {
+  return (p->prec == 2 && p->io_prec == 1 && !(p->tune_flags & 256)) ? p->d_kbtab : nullptr;
 }
 
 static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
@@ -784,12 +986,129 @@ static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int
     hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, true>), dim3(nb), dim3(256), 0, p->stream,
                        p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
                        (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, p->d_orig,
-                       g, kp);
+                       kb_table_of(p), g, kp);
   else
     hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, false>), dim3(nb), dim3(256), 0, p->stream,
                        p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
                        (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, p->d_orig,
-                       g, kp);
+                       kb_table_of(p), g, kp);
+}
+
+// The exponential branch of kernel.m (bessi0(ax)/bessi0(alpha) with ax =
+// alpha*sqrt(1-q), both >= 3.75) in extended precision: exp(ax - alpha) *
+// sqrt(alpha/ax) * P(3.75/ax) / P(3.75/alpha), P the polynomial of
+// kernel.m:28-30.
+static long double kb_large_ld(long double q, long double alpha) {
+  auto P = [](long double z) {
+    const long double c[9] = {0.39894228L, 0.1328592e-1L, 0.225319e-2L, -0.157565e-2L,
+                              0.916281e-2L, -0.2057706e-1L, 0.2635537e-1L, -0.1647633e-1L,
+                              0.392377e-2L};
+    long double r = 0;
+    for (int k = 8; k >= 0; --k) r = r * z + c[k];
+    return r;
+  };
+  const long double ax = alpha * sqrtl(1 - q);
+  return expl(ax - alpha) * sqrtl(alpha / ax) * P(3.75L / ax) / P(3.75L / alpha);
+}
+
+// kb_tab_d's table: on each interval the polynomial through the values at
+// kKbCoef Chebyshev nodes (Vandermonde system in t, solved in long double).
+int build_kb_table(nufft3d_plan* p) {
+  if (p->d_kbtab) return NUFFT3D_OK;
+  std::vector<double> tab((size_t)kKbIntervals * kKbCoef, 0.0);
+  const long double a = p->alpha;
+  if (a >= 3.75L) {
+    const long double qb = 1 - (3.75L / a) * (3.75L / a);
+    const long double h = qb / kKbIntervals;
+    for (int i = 0; i < kKbIntervals; ++i) {
+      const long double mid = (i + 0.5L) * h;
+      long double V[kKbCoef][kKbCoef + 1];
+      for (int k = 0; k < kKbCoef; ++k) {
+        const long double t = cosl(3.141592653589793238462643383279502884L * (k + 0.5L) / kKbCoef);
+        long double tp = 1;
+        for (int j = 0; j < kKbCoef; ++j) { V[k][j] = tp; tp *= t; }
+        V[k][kKbCoef] = kb_large_ld(mid + t * h / 2, a);
+      }
+      for (int c = 0; c < kKbCoef; ++c) {   // Gaussian elimination, partial pivoting
+        int piv = c;
+        for (int r = c + 1; r < kKbCoef; ++r)
+          if (fabsl(V[r][c]) > fabsl(V[piv][c])) piv = r;
+        for (int j = 0; j <= kKbCoef; ++j) std::swap(V[c][j], V[piv][j]);
+        for (int r = 0; r < kKbCoef; ++r) {
+          if (r == c) continue;
+          const long double f = V[r][c] / V[c][c];
+          for (int j = c; j <= kKbCoef; ++j) V[r][j] -= f * V[c][j];
+        }
+      }
+      for (int j = 0; j < kKbCoef; ++j) tab[(size_t)i * kKbCoef + j] = (double)(V[j][kKbCoef] / V[j][j]);
+    }
+  }
+  N3D_HIP(hipMalloc(&p->d_kbtab, sizeof(double) * tab.size()));
+  N3D_HIP(hipMemcpy(p->d_kbtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+  return NUFFT3D_OK;
+}
+
+// raise a kernel's dynamic LDS limit once (launches above 64 KiB)
+// (one flag per kernel instantiation: instantiations share a function type)
+template <typename K>
+static void allow_tile_lds(K* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<const void*> granted;
+  if (bytes <= 65536) return;
+  const void* f = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(granted.begin(), granted.end(), f) != granted.end()) return;
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess)
+    granted.push_back(f);
+  (void)hipGetLastError();
+}
+
+// Tiled forward of one coil group (c0, cg): launches of CG staged coils
+// (8 or 4 for fp64 arithmetic, 16 or 8 for fp32; NUFFT3D_TILE_CG picks the
+// smaller), 4 lanes per sample.
+template <typename R, int CG, bool COMP, bool FASTW>
+static void launch_interp_tile_cg(nufft3d_plan* p, const void* grid, int nc, int c0, int cg,
+                                  void* out, const void* wsamp, double scale, bool user) {
+  GridGeom g = make_geom(p, nc, c0, cg);
+  KBParams kp = make_kb(p);
+  const int NL = p->tile_layers > 0 && p->tile_layers <= kTileEdge ? p->tile_layers : 2;
+  const size_t lds = (size_t)NL * kTileEdge * kTileEdge * CG * sizeof(cplx<R>) +
+                     (FASTW ? sizeof(double) * kKbIntervals * kKbCoef : 0);
+  ProfScope ps(p, "k_interp_tile");
+  if (user) {
+    auto* k = k_interp_tile<R, CG, 4, COMP, true, FASTW>;
+    allow_tile_lds(k, lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)p->npiece), dim3(kTileThreads), lds, p->stream,
+                       p->d_piece, p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off,
+                       p->d_off32, (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale,
+                       p->d_orig, (const double*)p->d_kbtab, g, kp, NL);
+  } else {
+    auto* k = k_interp_tile<R, CG, 4, COMP, false, FASTW>;
+    allow_tile_lds(k, lds);
+    hipLaunchKernelGGL(k, dim3((unsigned)p->npiece), dim3(kTileThreads), lds, p->stream,
+                       p->d_piece, p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off,
+                       p->d_off32, (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale,
+                       p->d_orig, (const double*)p->d_kbtab, g, kp, NL);
+  }
+}
+
+template <typename R, bool COMP>
+static void launch_interp_tile(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
+                               const void* wsamp, double scale, bool user) {
+  constexpr int CGMAX = sizeof(R) == 8 ? 8 : 16;
+  const int cgmax = p->tile_cg == CGMAX / 2 ? CGMAX / 2 : CGMAX;
+  // fp64 arithmetic behind fp32 arrays: the tabulated weights
+  const bool fast = sizeof(R) == 8 && kb_table_of(p) != nullptr;
+  for (int a = 0; a < cg; a += cgmax) {
+    const int n = cg - a < cgmax ? cg - a : cgmax;
+    if (n > CGMAX / 2) {
+      if (fast) launch_interp_tile_cg<R, CGMAX, COMP, true>(p, grid, nc, c0 + a, n, out, wsamp, scale, user);
+      else launch_interp_tile_cg<R, CGMAX, COMP, false>(p, grid, nc, c0 + a, n, out, wsamp, scale, user);
+    } else {
+      if (fast) launch_interp_tile_cg<R, CGMAX / 2, COMP, true>(p, grid, nc, c0 + a, n, out, wsamp, scale, user);
+      else launch_interp_tile_cg<R, CGMAX / 2, COMP, false>(p, grid, nc, c0 + a, n, out, wsamp, scale, user);
+    }
+  }
 }
 
 // The weight of every hit of the grid-major lists, stored as float bits
@@ -835,7 +1154,10 @@ int build_hit_weights(nufft3d_plan* p) {
 
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
-                          void* partial, const void* wsamp) {
+                          void* partial, const void* wsamp, bool wide) {
+  // wide: float samples into a double grid (the split adjoint), the double
+  // accumulators stored unrounded
+  using G = typename std::conditional<sizeof(R) == 4, double2, cplx<R>>::type;
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
@@ -848,7 +1170,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   const uint2* hw = !(p->tune_flags & 128) ? p->d_hitw : nullptr;
   const uint2* hits = hw ? hw : p->d_hit;
   const bool pw = hw != nullptr;
-  if (kTileable && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
+  if (kTileable && !wide && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
       p->bl.rmax == ds.rmax) {
     ProfScope ps(p, "k_spread_tile");
     hipLaunchKernelGGL((k_spread_tile<(kTileable ? CPL * LPS : 8), COMP>), dim3((unsigned)p->nbrick),
@@ -864,11 +1186,20 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
     merged = ds.nhseg > 0 && !(p->tune_flags & 32);
     const int nhb = merged ? (int)((((long long)ds.nhseg * LPS + 255) / 256 + 7) / 8 * 8) : 0;
     ProfScope ps(p, "k_spread");
-    hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>),
-                       dim3((unsigned)(nhb + (nthreads + 255) / 256)), dim3(256), 0, p->stream,
-                       p->d_rowptr, hits, pw, (const double*)p->d_off, p->d_off32,
-                       (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)grid, npts, p->d_brick,
-                       ds.rmax, g, kp, ds.d_hseg, ds.nhseg, nhb, (cplx<R>*)partial);
+    if (wide)
+      hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL, G>),
+                         dim3((unsigned)(nhb + (nthreads + 255) / 256)), dim3(256), 0, p->stream,
+                         p->d_rowptr, hits, pw, (const double*)p->d_off, p->d_off32,
+                         (const cplx<R>*)samp, (const R*)wsamp, (G*)grid, npts, p->d_brick,
+                         ds.rmax, g, kp, ds.d_hseg, ds.nhseg, nhb, (cplx<R>*)partial,
+                         kb_table_of(p));
+    else
+      hipLaunchKernelGGL((k_spread<R, CPL, LPS, COMP, FULL>),
+                         dim3((unsigned)(nhb + (nthreads + 255) / 256)), dim3(256), 0, p->stream,
+                         p->d_rowptr, hits, pw, (const double*)p->d_off, p->d_off32,
+                         (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)grid, npts, p->d_brick,
+                         ds.rmax, g, kp, ds.d_hseg, ds.nhseg, nhb, (cplx<R>*)partial,
+                         kb_table_of(p));
   }
   if (ds.nhseg > 0) {
     if (!merged) {
@@ -878,14 +1209,18 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
                          dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0, p->stream,
                          ds.d_hseg, ds.nhseg, hits, pw, (const double*)p->d_off,
                          p->d_off32, (const cplx<R>*)samp, (const R*)wsamp, (cplx<R>*)partial, g,
-                         kp);
+                         kp, kb_table_of(p));
     }
     ProfScope ps(p, "k_spread_hreduce");
     int cgp = 1;
     while (cgp < cg) cgp *= 2;
     const int nthr = cgp * 32 < 256 ? cgp * 32 : 256;
-    hipLaunchKernelGGL((k_spread_hreduce<R, COMP>), dim3((unsigned)ds.nhpt), dim3(nthr), 0,
-                       p->stream, ds.d_hpt, (const cplx<R>*)partial, (cplx<R>*)grid, cgp, g);
+    if (wide)   // double partials (COMP) summed into the double grid
+      hipLaunchKernelGGL((k_spread_hreduce<double, false>), dim3((unsigned)ds.nhpt), dim3(nthr), 0,
+                         p->stream, ds.d_hpt, (const double2*)partial, (double2*)grid, cgp, g);
+    else
+      hipLaunchKernelGGL((k_spread_hreduce<R, COMP>), dim3((unsigned)ds.nhpt), dim3(nthr), 0,
+                         p->stream, ds.d_hpt, (const cplx<R>*)partial, (cplx<R>*)grid, cgp, g);
   }
 }
 
@@ -894,7 +1229,7 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
 // groups of more than 2 coils put 4 lanes on it (1, 2 or 4 coils each)
 template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
-                       const void* wsamp, double scale, bool user = false) {
+                       const void* wsamp, double scale, bool user = false, bool wide = false) {
   const int gmax = p->coil_group;
   for (int c0 = 0; c0 < nc; c0 += gmax) {
     const int cg = nc - c0 < gmax ? nc - c0 : gmax;
@@ -902,7 +1237,7 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
     const bool even = (nc % 2 == 0) && (c0 % 2 == 0);
 #define N3D_GO2(CPL, LPS, FULL)                                                                  \
     do {                                                                                         \
-      if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, partial, wsamp); \
+      if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, partial, wsamp, wide); \
       else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale, user);  \
     } while (0)
 #define N3D_GO(CPL, LPS)                                                                         \
@@ -910,6 +1245,12 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
       if (cg == (CPL) * (LPS) && ((CPL) == 1 || even)) N3D_GO2(CPL, LPS, true);                  \
       else N3D_GO2(CPL, LPS, false);                                                             \
     } while (0)
+    if constexpr (!SPREAD) {
+      if (p->tile_interp && p->d_piece && p->npiece > 0) {
+        launch_interp_tile<R, COMP>(p, in, nc, c0, cg, out, wsamp, scale, user);
+        continue;
+      }
+    }
     if constexpr (sizeof(R) == 8) {
       if (cg <= 1) N3D_GO(1, 1);
       else if (cg <= 2) N3D_GO(2, 1);
@@ -975,7 +1316,29 @@ static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const
 }
 
 int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
-           const void* wsamp) {
+           const void* wsamp, int ep) {
+  if (ep == 1 && p->prec == 2) {   // split adjoint: float samples, double grid
+    if (user_layout) {
+      N3D_TRY(ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * sizeof(float2)));
+      N3D_TRY(permute_in(p, samp, nc, p->ws_samp.p, wsamp, 1));
+      samp = p->ws_samp.p;
+      wsamp = nullptr;
+    }
+    if (!p->d_hitw || (p->tune_flags & 128)) {   // the float samples carry no offsets
+      set_error("the split-precision adjoint needs the plan's weight table");
+      return NUFFT3D_ERR_INVALID;
+    }
+    const int cgmax = nc < p->coil_group ? nc : p->coil_group;
+    void* partial = nullptr;
+    const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
+    if (nseg > 0) {
+      N3D_TRY(ensure(p->ws_part, (size_t)nseg * cgmax * 2 * sizeof(double)));
+      partial = p->ws_part.p;
+    }
+    run_groups<float, true, true>(p, samp, nc, grid, partial, wsamp, 1.0, false, true);
+    N3D_HIP(hipGetLastError());
+    return NUFFT3D_OK;
+  }
   if (user_layout) {
     // gather the caller's [M nc] samples into the sorted coil-interleaved layout
     const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
@@ -988,13 +1351,13 @@ int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layou
                       : spread_t<float>(p, samp, nc, grid, wsamp);
 }
 
-int zero_oob(nufft3d_plan* p, void* k_user, int nc) {
+int zero_oob(nufft3d_plan* p, void* k_user, int nc, int ep) {
   const long long noob = p->M - p->nnz;
   if (noob <= 0) return NUFFT3D_OK;
   const long long n = noob * nc;
   const int nb = (int)((n + 255) / 256);
   ProfScope ps(p, "k_zero_oob");
-  if (p->prec == 2)
+  if ((ep ? ep : p->prec) == 2)
     hipLaunchKernelGGL((k_zero_oob<double>), dim3(nb), dim3(256), 0, p->stream, p->d_oob, noob,
                        (long long)p->M, nc, (double2*)k_user);
   else
@@ -1031,13 +1394,14 @@ int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user) {
   return NUFFT3D_OK;
 }
 
-int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp) {
+int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp, int ep) {
   const long long n = p->nnz * (long long)nc;
   if (n == 0) return NUFFT3D_OK;
   ProfScope ps(p, "k_permute_in");
+  const int pr = ep ? ep : p->prec;
   if (nc >= 4 && p->d_inv && !(p->tune_flags & 64)) {   // caller rows read in order
     const unsigned nb = (unsigned)((p->M + 63) / 64);
-    if (p->prec == 2)
+    if (pr == 2)
       hipLaunchKernelGGL((k_permute_in_rows<double>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
                          (long long)p->M, nc, (const double2*)k_user, (const double*)wsamp,
                          (double2*)ks);
@@ -1050,7 +1414,7 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
   }
   if (nc >= 4) {   // tiled: coalesced rows on both sides
     const unsigned nb = (unsigned)((p->nnz + 63) / 64);
-    if (p->prec == 2)
+    if (pr == 2)
       hipLaunchKernelGGL((k_permute_in_tile<double>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
                          (int)p->nnz, (long long)p->M, nc, (const double2*)k_user,
                          (const double*)wsamp, (double2*)ks);
@@ -1062,7 +1426,7 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
     return NUFFT3D_OK;
   }
   const int nb = (int)((n + 255) / 256);
-  if (p->prec == 2)
+  if (pr == 2)
     hipLaunchKernelGGL((k_permute_in<double>), dim3(nb), dim3(256), 0, p->stream, p->d_orig,
                        (int)p->nnz, (long long)p->M, nc, (const double2*)k_user,
                        (const double*)wsamp, (double2*)ks);

@@ -23,7 +23,24 @@ struct KBParams {
   float inv_P0f;     // 1/P(3.75/alpha), P the large-argument polynomial of bessi0
   int large_b0;      // alpha >= 3.75 (bessi0(alpha) in the exponential branch)
   int radial;
+  double inv_half2;  // 1/(J/2)^2
+  double inv_b0;     // 1/bessi0(alpha)
+  double inv_P0;     // 1/P(3.75/alpha)
+  double c375a;      // 3.75/alpha
+  double qb;         // 1 - (3.75/alpha)^2: bessi0's branch point in q = dist2/(J/2)^2
+                     // (-1 when alpha < 3.75: the polynomial branch throughout)
+  double tscale;     // kKbIntervals / qb
+  double cy;         // (alpha/3.75)^2
 };
+
+// Piecewise-polynomial table of the exponential branch of kernel(q):
+// kKbIntervals equal intervals of q over [0, qb], a degree-kKbDeg polynomial
+// in the local variable t in [-1, 1] on each (interpolated at Chebyshev
+// nodes from an extended-precision evaluation of kernel.m's formula; within
+// ~2e-16 relative for alpha <= 11, ~2e-15 at alpha = 14).
+constexpr int kKbIntervals = 128;
+constexpr int kKbDeg = 7;
+constexpr int kKbCoef = kKbDeg + 1;
 
 __device__ __forceinline__ double bessi0_d(double ax) {
 #pragma clang fp contract(off)
@@ -48,6 +65,30 @@ __device__ __forceinline__ double kb_d(double dist2, const KBParams& p) {
 #pragma clang fp contract(off)
   double arg = p.alpha * sqrt(fmax(1.0 - dist2 / p.half2, 0.0));
   return bessi0_d(arg) / p.b0;
+}
+
+// kernel(dist2) in double from the table (tab: kKbIntervals x kKbCoef
+// coefficients, lowest degree first) for fp64 arithmetic behind fp32 arrays:
+// the exponential branch from the table, the polynomial branch (kernel.m:
+// 19-23) directly, as a polynomial in y = (alpha/3.75)^2 (1 - q). Within a
+// few ulps of kernel.m; rounded to float by the caller, as the reference's
+// single weights are, it differs from kb_d's only where the double values
+// straddle a float rounding boundary (a few parts in 1e6 of hits, by one
+// float ulp).
+__device__ __forceinline__ double kb_tab_d(double dist2, const double* tab, const KBParams& p) {
+  const double q = dist2 * p.inv_half2;
+  const double y = __builtin_fma(-p.cy, q, p.cy);
+  const double small = (1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+                        y * (0.360768e-1 + y * 0.45813e-2)))))) * p.inv_b0;
+  const double x = q * p.tscale;
+  int i = (int)x;
+  i = i < kKbIntervals - 1 ? i : kKbIntervals - 1;
+  const double t = 2.0 * (x - (double)i) - 1.0;
+  const double* c = tab + i * kKbCoef;
+  double g = c[kKbDeg];
+#pragma unroll
+  for (int k = kKbDeg - 1; k >= 0; --k) g = __builtin_fma(g, t, c[k]);
+  return q > p.qb ? small : g;
 }
 
 // fp32 weight. For arguments in the exponential branch the ratio is taken as

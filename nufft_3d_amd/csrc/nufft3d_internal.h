@@ -132,7 +132,10 @@ struct nufft3d_plan {
                             // 32 = dense-point segments in their own launch,
                             // 64 = aNUFT input gathered by sorted sample, not scattered
                             // by caller row, 128 = the spread evaluates each hit's
-                            // weight instead of reading the plan's weight table)
+                            // weight instead of reading the plan's weight table,
+                            // 256 = kernel.m's literal arithmetic for the weights of
+                            // fp64 arithmetic behind fp32 arrays, not the kernel table,
+                            // 512 = no split-precision adjoint (see split_precision))
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
@@ -182,6 +185,19 @@ struct nufft3d_plan {
   bool bl_tried = false;
   int max_hits = 0;
 
+  // LDS-tiled forward interpolation (3-D plans with a 7-point footprint,
+  // ceil(J/2) = 3): samples sorted by 4x4x4 sub-bins; each piece is up to
+  // 128 samples of one sub-bin, whose 10x10x10 grid neighbourhood a
+  // work-group stages in LDS. x = sub-bin origin (tx | ty << 10 | tz << 20, in
+  // sub-bins), y = first sorted sample, z = sample count.
+  int tile_interp = 0;
+  int tile_cg = 0;                  // coils staged per tiled launch (0 = 8 fp64 / 16 fp32)
+  int tile_layers = 0;              // z-layers staged per pass (0 = 2)
+  bool tile_user = false;           // the caller fixed the sample bin size (opts.tile)
+  int4* d_piece = nullptr;
+  long long npiece = 0;
+  double* d_kbtab = nullptr;        // piecewise-polynomial kernel table (fp64 behind fp32 arrays)
+
   nufft3d::FftAxis ax[3];
   nufft3d::FftAxis axN[3];  // image-size axes (unshifted FFT of the PRUNO calibration)
   nufft3d::Profiler prof;
@@ -202,6 +218,19 @@ struct nufft3d_plan {
 
 namespace nufft3d {
 
+// Split precision: a plan with fp32 arrays that computes in fp64 (the
+// deapodization range makes fp32 rounding before the final U too large)
+// runs its standalone aNUFT on the caller's fp32 samples as given: the
+// reference's fp32-rounded weights times fp32 data are exact in double, so
+// the spread accumulates them in double into an fp64 grid and the inverse
+// transform runs in fp64, with no widened copy of the samples. (The forward
+// stays fp64: U multiplies the image before the transform, and with energy
+// near the field-of-view edge an fp32 transform or grid loses up to the
+// deapodization range.) Tune flag 512 turns it off.
+inline bool split_precision(const nufft3d_plan* p) {
+  return p->prec == 2 && p->io_prec == 1 && !(p->tune_flags & (128 | 512));
+}
+
 int ensure(Workspace& w, size_t bytes);
 void prof_begin(nufft3d_plan* p, const char* name);
 void prof_end(nufft3d_plan* p);
@@ -218,11 +247,15 @@ int ensure_host(Workspace& w, size_t bytes);
 // allow_ws = false never takes the plan's workspaces (the direct caller-row write)
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
            const void* wsamp, double scale, bool allow_ws = true);
+// ep = 1 on an fp64-arithmetic plan: float samples, the plan's float
+// weight table, double accumulation into a double grid (the split adjoint)
 int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
-           const void* wsamp);
-int zero_oob(nufft3d_plan* p, void* k_user, int nc);
+           const void* wsamp, int ep = 0);
+int zero_oob(nufft3d_plan* p, void* k_user, int nc, int ep = 0);
 int build_hit_weights(nufft3d_plan* p);   // per-hit fp32 weights of the spread lists
-int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp);
+int build_kb_table(nufft3d_plan* p);      // kb_tab_d's table for the plan's J and alpha
+int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void* wsamp,
+               int ep = 0);
 
 // fft.hip
 int fft_setup_axis(nufft3d_plan* p, int axis);

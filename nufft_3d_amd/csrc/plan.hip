@@ -687,6 +687,44 @@ static int ensure_brick_lists(nufft3d_plan* p) {
   return build_brick_lists(p, p->h_brick_of, bx, by, p->dense[1].rmax);
 }
 
+// Pieces of the tiled interpolation: runs of equal sub-bin key in the sorted
+// samples, cut into at most kPieceMax samples.
+constexpr int kPieceMax = 128;   // one sample per 4-lane group of a 512-thread work-group
+
+static int build_pieces(nufft3d_plan* p, const unsigned* skeys, const BinGeom& g) {
+  const long long nnz = p->nnz;
+  std::vector<unsigned> k(nnz);
+  N3D_HIP(hipMemcpy(k.data(), skeys, sizeof(unsigned) * nnz, hipMemcpyDeviceToHost));
+  std::vector<int4> pc;
+  pc.reserve(nnz / 32 + 16);
+  for (long long s = 0; s < nnz;) {
+    long long e = s + 1;
+    while (e < nnz && k[e] == k[s]) ++e;
+    unsigned tx = 0, ty = 0, tz = 0;
+    if (g.mbits > 0) {
+      for (int b = 0; b < g.mbits; ++b) {
+        tx |= ((k[s] >> (3 * b)) & 1u) << b;
+        ty |= ((k[s] >> (3 * b + 1)) & 1u) << b;
+        tz |= ((k[s] >> (3 * b + 2)) & 1u) << b;
+      }
+    } else {
+      tx = k[s] % g.nt1;
+      ty = (k[s] / g.nt1) % g.nt2;
+      tz = k[s] / (g.nt1 * g.nt2);
+    }
+    for (long long a = s; a < e; a += kPieceMax) {
+      const int n = (int)std::min<long long>(kPieceMax, e - a);
+      pc.push_back(make_int4((int)(tx | (ty << 10) | (tz << 20)), (int)a, n, 0));
+    }
+    s = e;
+  }
+  p->npiece = (long long)pc.size();
+  N3D_HIP(hipMalloc(&p->d_piece, sizeof(int4) * (pc.empty() ? 1 : pc.size())));
+  if (!pc.empty())
+    N3D_HIP(hipMemcpy(p->d_piece, pc.data(), sizeof(int4) * pc.size(), hipMemcpyHostToDevice));
+  return NUFFT3D_OK;
+}
+
 static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   const bool dev = is_device_ptr(om_in);
   double* om = nullptr;
@@ -746,7 +784,17 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
       printf("  deapodization range %.3g: %s arithmetic\n", range, p->prec == 2 ? "fp64" : "fp32");
   }
 
-  // tiles
+  // tiles: 4x4x4 sample bins where the forward interpolation runs from LDS
+  // tiles (3-D, 7-point footprint: the 10^3 neighbourhood of a bin fits in LDS)
+  {
+    // opt-in (NUFFT3D_TILE_INTERP=1): measured slower than the gather
+    // interpolation at 256^3 / J=6 (21.6 against 18.4 ms per 8-coil launch)
+    const char* te = getenv("NUFFT3D_TILE_INTERP");
+    const bool allow = te && atoi(te) != 0;
+    p->tile_interp = allow && !p->tile_user && p->N[2] > 1 && p->W[0] == 3 && p->W[1] == 3 &&
+                     p->W[2] == 3 && p->K[0] >= 16 && p->K[1] >= 16 && p->K[2] >= 16;
+    if (p->tile_interp) p->T[0] = p->T[1] = p->T[2] = 4;
+  }
   if (p->N[2] == 1) { p->T[0] = p->T[1] = 16; p->T[2] = 1; }
   for (int a = 0; a < 3; ++a) {
     if (p->T[a] > p->K[a]) p->T[a] = p->K[a];
@@ -814,6 +862,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     hipLaunchKernelGGL(k_invert, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
                        (const int*)p->d_orig, nnz, p->d_inv);
   N3D_HIP(hipGetLastError());
+  if (p->tile_interp && nnz > 0) N3D_TRY(build_pieces(p, skeys, g));
   (void)hipFree(keys); (void)hipFree(skeys); (void)hipFree(vals); (void)hipFree(svals);
   (void)hipFree(om);
 
@@ -935,6 +984,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     if (p->tune_flags & 8) N3D_TRY(ensure_brick_lists(p));
   }
   N3D_TRY(build_hit_weights(p));
+  N3D_TRY(build_kb_table(p));
 
   // deapodization (deap.m) and FFT twiddles
   deap_host(p);
@@ -975,7 +1025,7 @@ static void destroy_plan(nufft3d_plan* p) {
   if (p->done_ev) (void)hipEventDestroy(p->done_ev);
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_inv, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit, p->d_hitw,
-                  p->d_off32, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
+                  p->d_off32, p->d_piece, p->d_kbtab, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1220,7 +1270,12 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_XCD")) p->xcd_chunk = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_GRAPH")) p->use_graphs = atoi(tv) != 0;
   if (const char* tv = getenv("NUFFT3D_TEST_CAPTURE_FAIL")) p->capture_fail_at = atoi(tv);
-  if (o.tile > 0) p->T[0] = p->T[1] = p->T[2] = o.tile;
+  if (const char* tv = getenv("NUFFT3D_TILE_CG")) p->tile_cg = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_TILE_LAYERS")) p->tile_layers = atoi(tv);
+  if (o.tile > 0) {
+    p->T[0] = p->T[1] = p->T[2] = o.tile;
+    p->tile_user = true;
+  }
   // dense-point segments: 64 hits for launches of <= 4 coils, 512 (or the
   // caller's chunk) for larger ones
   p->dense[0].chunk_len = p->dense[0].rmax = o.chunk > 0 ? std::min(64, (int)o.chunk) : 64;
@@ -1305,6 +1360,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += (2 + 8) * std::max<int64_t>(p->nhits, 1);                      // fcode, (sample, code) hits
   if (p->d_hitw) b += 8 * p->nhits;                                   // (sample, weight) hits
   if (p->d_off32) b += 16 * nnz;
+  if (p->d_piece) b += 16 * p->npiece;
   for (const DenseSet& ds : p->dense)
     b += (int64_t)sizeof(HeavySeg) * std::max(1, ds.nhseg) + (int64_t)sizeof(HeavyPt) * std::max(1, ds.nhpt);
   b += 4 * p->nbrick;
@@ -1532,6 +1588,28 @@ struct DoneMark {
   if (hipSetDevice((p)->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; } \
   DoneMark dm_{(p)};
 
+// Split-precision aNUFT: the caller's complex64 samples, float weights,
+// double sums into a double grid, fp64 inverse transform, rounded once.
+static int adjoint_split(nufft3d_plan* p, const void* k, int nc, void* x) {
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  Workspace tmp_out, tmp_io;
+  Staged ki, xo;
+  N3D_TRY(stage_in(p, k, nk * sizeof(float2), p->ws_img, ki));
+  int st = stage_out_c(p, x, nimg, tmp_out, xo);
+  if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * sizeof(float2));
+  if (!st) st = ensure(p->ws_grid, ngrid * sizeof(double2));
+  if (!st) st = permute_in(p, ki.dev, nc, p->ws_samp.p, nullptr, 1);
+  if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr, 1);              // aNUFT.m:5
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, xo.dev, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
+  if (!st) st = finish_out_c(p, x, nimg, xo, tmp_io);
+  release(tmp_out);
+  release(tmp_io);
+  if (!st && ki.host) N3D_HIP(hipStreamSynchronize(p->stream));
+  return st;
+}
+
 int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
   N3D_PLAN_GUARD(p);
   if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
@@ -1558,6 +1636,7 @@ int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
 int nufft3d_adjoint(nufft3d_plan* p, const void* k, int32_t nc, void* x) {
   N3D_PLAN_GUARD(p);
   if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
+  if (split_precision(p)) return adjoint_split(p, k, nc, x);
   const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
@@ -2000,14 +2079,32 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     hipLaunchKernelGGL((k_fill_pattern<float>), dim3(nblocks(nr)), dim3(256), 0, p->stream,
                        (float*)xin.p, nr, 17u);
   // one operator pair on device arrays in the internal layouts
+  // (split-precision plans: their fNUFT/aNUFT sequence, the adjoint from
+  // float samples)
+  const bool split = split_precision(p);
+  Workspace ks32;
+  if (split) {
+    if (!st) st = ensure(ks32, nks * sizeof(float2));
+    if (!st && hipMemsetAsync(ks32.p, 0, nks * sizeof(float2), p->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("hipMemsetAsync failed");
+      st = NUFFT3D_ERR_DEVICE;
+    }
+  }
   auto pair = [&]() -> int {
+    if (split) {   // the adjoint from float samples (ks32)
+      N3D_TRY(fft_forward(p, xin.p, nc, p->ws_grid.p));
+      N3D_TRY(interp(p, p->ws_grid.p, nc, ks.p, 0, nullptr, sc));
+      N3D_TRY(spread(p, ks32.p, nc, p->ws_grid.p, 0, nullptr, 1));
+      return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
+    }
     N3D_TRY(fft_forward(p, xin.p, nc, p->ws_grid.p));
     N3D_TRY(interp(p, p->ws_grid.p, nc, ks.p, 0, nullptr, sc));
     N3D_TRY(spread(p, ks.p, nc, p->ws_grid.p, 0, nullptr));
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
   if (!st) st = ensure_brick_lists(p);
-  if (st) { release(xin); release(xout); release(ks); return st; }
+  if (st) { release(xin); release(xout); release(ks); release(ks32); return st; }
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
   // spread (plans with brick lists)
   struct Cand { int fftb, group, xcd, brick; float ms; };
@@ -2099,7 +2196,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   (void)hipStreamSynchronize(p->stream);
-  release(xin); release(xout); release(ks);
+  release(xin); release(xout); release(ks); release(ks32);
   return st;
 }
 

@@ -526,3 +526,48 @@ def test_weights_with_maxit_at_most_1_are_used_as_given(maxit):
         warnings.simplefilter("ignore")
         with pytest.raises(Exception):
             g.iNUFT(raw, maxit, 0.0, W=W[:40])
+
+
+@pytest.mark.parametrize("gpu,fp64c,radial,nc,env", [
+    (2, None, 1, 3, {}), (2, None, 1, 11, {}), (2, None, 1, 6, {"NUFFT3D_TILE_CG": "4"}),
+    (1, False, 1, 16, {}), (1, False, 0, 5, {}), (1, False, 1, 9, {"NUFFT3D_TILE_CG": "8"}),
+    (1, True, 1, 8, {})])
+def test_tiled_interp_equals_gather_interp(gpu, fp64c, radial, nc, env):
+    # J=6, NUFFT3D_TILE_INTERP=1: the forward runs from LDS tiles of 4^3
+    # sub-bins; the gather kernel walks the same hits in the same order, so the
+    # results are bitwise equal, except that fp64 arithmetic behind fp32
+    # arrays evaluates the weights without divisions (kb_fast_d), which can
+    # move a float-rounded weight by one ulp
+    om = synth.golden_means(24, 200, 48)
+    kw = dict(J=6, u=1.25, radial=radial, gpu=gpu, fp64_compute=fp64c)
+    tiled = _plan_with_env(dict(env, NUFFT3D_TILE_INTERP="1"), om, 24, **kw)
+    gath = _plan_with_env({"NUFFT3D_TILE_INTERP": "0"}, om, 24, **kw)
+    x = _rand((24, 24, 24, nc), 95 + nc)
+    a, b = tiled.fNUFT(x), gath.fNUFT(x)
+    if gpu == 1 and fp64c:
+        assert rel(a, b) < 1e-9
+    else:
+        assert np.array_equal(a, b)
+    k = _rand((om.shape[1], nc), 96)
+    ia = tiled.iNUFT(k, 5, 0.01)
+    ib = gath.iNUFT(k, 5, 0.01)
+    assert rel(ia, ib) < (1e-9 if gpu == 1 and fp64c else 1e-12 if gpu == 2 else 1e-5)
+
+
+@pytest.mark.parametrize("radial,nc", [(1, 8), (1, 3), (0, 5)])
+def test_split_precision_adjoint_matches_fp64_arithmetic(radial, nc):
+    # fp32 arrays on a plan computing in fp64 (u=1.25, J=6): the standalone
+    # aNUFT takes the fp32 samples as they are (float weight x float sample
+    # is exact in double) into a double grid and an fp64 inverse transform;
+    # tune flag 512 widens the samples first. Same products, different
+    # summation grouping only: 1e-12. fNUFT and iNUFT are fp64 in both.
+    om = synth.golden_means(24, 300, 48)
+    kw = dict(J=6, u=1.25, radial=radial, gpu=1)
+    split = nufft_3d(om, 24, **kw)
+    full = _plan_with_env({"NUFFT3D_TUNE": str(2 | 512)}, om, 24, **kw)
+    assert split.compute_precision == 2
+    k = _rand((om.shape[1], nc), 98).astype(np.complex64)
+    assert rel(split.aNUFT(k), full.aNUFT(k)) < 1e-12
+    x = _rand((24, 24, 24, nc), 97).astype(np.complex64)
+    assert np.array_equal(split.fNUFT(x), full.fNUFT(x))
+    assert np.array_equal(split.iNUFT(k, 4, 0.0), full.iNUFT(k, 4, 0.0))

@@ -261,6 +261,8 @@ __device__ void fft_lines(cplx<R>* buf, int B, const Stages& st, const cplx<R>* 
   N3D_RUN(3, while)
   N3D_RUN(5, while)
   N3D_RUN(7, while)
+  N3D_RUN(11, while)
+  N3D_RUN(13, while)
 #undef N3D_RUN
   if constexpr (LG) {
     __syncthreads();
@@ -444,6 +446,101 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const cplx<R>* __restrict_
   fft_lines_pl<R, true, true, PL>(a, g.B, st, stw, sign, ld, sto);
 }
 
+// Bluestein (chirp-z) transform of B lines of length L with a prime factor
+// above 13, in the geometry of k_fft_strided (pad / crop / plain): with
+// c[n] = exp(sign i pi n^2 / L), X[k] = c[k] sum_n (x[n] c[n]) conj(c[k-n]),
+// a linear convolution evaluated as a circular one of the smooth length L2
+// >= 2L - 1 by two L2-point transforms in LDS, with the conjugate chirp's
+// spectrum (over L2) precomputed per sign.
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_bluestein(const cplx<R>* __restrict__ in,
+                                                       cplx<R>* __restrict__ out, SGeom g,
+                                                       Stages st2, const cplx<R>* __restrict__ tw2,
+                                                       const cplx<R>* __restrict__ chirp,
+                                                       const cplx<R>* __restrict__ bhat, int L,
+                                                       int sign) {
+  extern __shared__ unsigned char smem_raw[];
+  const int L2 = st2.L;
+  const int B = g.B;
+  cplx<R>* a = reinterpret_cast<cplx<R>*>(smem_raw);
+  cplx<R>* stw = a + (size_t)L2 * B;
+  stage_twiddles<R>(stw, tw2, L2);
+  const long long outer = blockIdx.x / g.nbundle;
+  const long long col0 = (blockIdx.x % g.nbundle) * B;
+  const cplx<R>* src = in + outer * g.Lin * g.inner;
+  cplx<R>* dst = out + outer * g.Lout * g.inner;
+  auto chirp_at = [&](int n) {
+    cplx<R> c = chirp[n];
+    if (sign > 0) c.y = -c.y;
+    return c;
+  };
+  for (int idx = threadIdx.x; idx < L2 * B; idx += blockDim.x) {
+    const int li = idx % B, n = idx / B;
+    const long long col = col0 + li;
+    cplx<R> v;
+    v.x = 0;
+    v.y = 0;
+    if (n < L && col < g.inner) {
+      const int i = g.mode == 0 ? unpad(n, g.n_img, L) : n;
+      if (i >= 0) v = cmul<R>(src[(long long)i * g.inner + col], chirp_at(n));
+    }
+    a[idx] = v;
+  }
+  __syncthreads();
+  auto none_ld = [](int, int) { cplx<R> z; z.x = 0; z.y = 0; return z; };
+  auto none_st = [](int, int, cplx<R>) {};
+  fft_lines<R, false, false>(a, B, st2, stw, -1, none_ld, none_st);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < L2 * B; idx += blockDim.x) a[idx] = cmul<R>(a[idx], bhat[idx / B]);
+  __syncthreads();
+  fft_lines<R, false, false>(a, B, st2, stw, +1, none_ld, none_st);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < L * B; idx += blockDim.x) {
+    const int li = idx % B, k = idx / B;
+    const long long col = col0 + li;
+    const int i = g.mode == 1 ? unpad(k, g.n_img, L) : k;
+    if (i >= 0 && col < g.inner) dst[(long long)i * g.inner + col] = cmul<R>(a[idx], chirp_at(k));
+  }
+}
+
+// Layout changes of the transform path for plans with a Bluestein axis:
+// image [nc][N3][N2][N1] times U <-> coil-fastest [N3][N2][N1][nc] (the
+// layout the strided passes then take along x, y and z).
+template <typename R>
+__global__ void k_img_to_cl(const cplx<R>* __restrict__ img, const R* __restrict__ U,
+                            cplx<R>* __restrict__ w, long long nvox, int nc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nvox * nc) return;
+  const long long v = t / nc;
+  const int c = (int)(t - v * nc);
+  const cplx<R> a = img[(long long)c * nvox + v];
+  const R u = U[v];
+  cplx<R> o;
+  o.x = a.x * u;
+  o.y = a.y * u;
+  w[t] = o;
+}
+
+template <typename R>
+__global__ void k_cl_to_img(const cplx<R>* __restrict__ w, const R* __restrict__ U,
+                            cplx<R>* __restrict__ img, long long nvox, int nc, R scale,
+                            const cplx<R>* __restrict__ add, R add_coef) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nvox * nc) return;
+  const int c = (int)(t / nvox);
+  const long long v = t - (long long)c * nvox;
+  const cplx<R> a = w[v * nc + c];
+  const R u = U[v] * scale;
+  cplx<R> o;
+  o.x = a.x * u;
+  o.y = a.y * u;
+  if (add) {
+    o.x += add_coef * add[t].x;
+    o.y += add_coef * add[t].y;
+  }
+  img[t] = o;
+}
+
 // ---------------------------------------------------------------- host side
 #define N3D_PLAN_SWITCH(ID, LAUNCH)     \
   switch (ID) {                         \
@@ -458,7 +555,8 @@ __global__ void __launch_bounds__(1024) k_fft_strided(const cplx<R>* __restrict_
   }
 // Radices 16 first (the fewest stages, hence LDS exchanges), the leftover
 // power of two as one 2, 4 or 8 (2^(4k+1) as 8*4 instead of 16*2), then the
-// odd primes 3, 5, 7.
+// odd primes 3, 5, 7, 11, 13 (11 and 13 as direct DFT butterflies in the
+// runtime stage loop). Other lengths: n = -1 (Bluestein).
 static void factorize(int L, Stages& st) {
   st.L = L;
   st.n = 0;
@@ -470,7 +568,7 @@ static void factorize(int L, Stages& st) {
   for (int k = 0; k < n16; ++k) st.r[st.n++] = 16;
   if (split) { st.r[st.n++] = 8; st.r[st.n++] = 4; }
   else if (rest > 0) st.r[st.n++] = 1 << rest;
-  const int primes[] = {3, 5, 7};
+  const int primes[] = {3, 5, 7, 11, 13};
   for (int q : primes)
     while (m % q == 0) { st.r[st.n++] = q; m /= q; }
   if (m != 1) st.n = -1;
@@ -487,37 +585,103 @@ static int plan_id_of(const Stages& st) {
   return 0;
 }
 
+// exp(sign i pi n^2 / L) in extended precision (n^2 reduced mod 2L exactly)
+static void chirp_ld(int L, int sign, std::vector<long double>& re, std::vector<long double>& im,
+                     int count) {
+  const long double pi = 3.141592653589793238462643383279502884L;
+  re.resize(count);
+  im.resize(count);
+  for (int n = 0; n < count; ++n) {
+    const long long m = ((long long)n * n) % (2LL * L);
+    const long double ang = sign * pi * (long double)m / (long double)L;
+    re[n] = cosl(ang);
+    im[n] = sinl(ang);
+  }
+}
+
+// Bluestein set-up of an axis of length L: the convolution length L2, its
+// roots of unity (d_tw), the chirp and the two conjugate-chirp spectra
+static int setup_bluestein(nufft3d_plan* p, FftAxis& a, int L, Stages& st) {
+  int L2 = 2 * L - 1;
+  for (;; ++L2) {
+    factorize(L2, st);
+    if (st.n > 0) break;
+  }
+  a.bs = 1;
+  a.L2 = L2;
+  const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+  auto put = [&](std::vector<unsigned char>& h, int i, long double x, long double y) {
+    if (p->prec == 2) reinterpret_cast<double2*>(h.data())[i] = make_double2((double)x, (double)y);
+    else reinterpret_cast<float2*>(h.data())[i] = make_float2((float)x, (float)y);
+  };
+  std::vector<long double> cr, ci;
+  chirp_ld(L, -1, cr, ci, L);
+  std::vector<unsigned char> hc(esz * L);
+  for (int n = 0; n < L; ++n) put(hc, n, cr[n], ci[n]);
+  N3D_HIP(hipMalloc(&a.d_chirp, esz * L));
+  N3D_HIP(hipMemcpy(a.d_chirp, hc.data(), esz * L, hipMemcpyHostToDevice));
+  // b[m] = exp(-s i pi m^2 / L) on m in (-L, L), wrapped to length L2; its
+  // L2-point DFT (e^{-2 pi i jk / L2}) over L2, by direct sums
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int si = 0; si < 2; ++si) {
+    const int s = si == 0 ? -1 : +1;
+    std::vector<long double> br(L2, 0), bi(L2, 0), er, ei;
+    chirp_ld(L, -s, er, ei, L);
+    for (int m = 0; m < L; ++m) {
+      br[m] = er[m];
+      bi[m] = ei[m];
+      if (m > 0) { br[L2 - m] = er[m]; bi[L2 - m] = ei[m]; }
+    }
+    std::vector<long double> tc(L2), ts(L2);
+    for (int t = 0; t < L2; ++t) {
+      tc[t] = cosl(2 * pi * t / L2);
+      ts[t] = -sinl(2 * pi * t / L2);
+    }
+    std::vector<unsigned char> hb(esz * L2);
+    for (int k = 0; k < L2; ++k) {
+      long double xr = 0, xi = 0;
+      for (int j = 0; j < L2; ++j) {
+        if (br[j] == 0 && bi[j] == 0) continue;
+        const int t = (int)(((long long)j * k) % L2);
+        xr += br[j] * tc[t] - bi[j] * ts[t];
+        xi += br[j] * ts[t] + bi[j] * tc[t];
+      }
+      put(hb, k, xr / L2, xi / L2);
+    }
+    N3D_HIP(hipMalloc(&a.d_bhat[si], esz * L2));
+    N3D_HIP(hipMemcpy(a.d_bhat[si], hb.data(), esz * L2, hipMemcpyHostToDevice));
+  }
+  return NUFFT3D_OK;
+}
+
 int fft_setup_axis(nufft3d_plan* p, int axis) {
   const int L = p->K[axis];
   FftAxis& a = p->ax[axis];
   a.L = L;
   Stages st;
   factorize(L, st);
-  if (st.n < 0) {
-    set_error("oversampled size K=" + std::to_string(L) +
-              " has a prime factor above 7 (radices 2, 3, 4, 5, 7 supported)");
-    return NUFFT3D_ERR_UNSUPPORTED;
-  }
+  if (st.n < 0) N3D_TRY(setup_bluestein(p, a, L, st));
+  const int Ltw = a.bs ? a.L2 : L;   // length of the staged transforms
   a.nstage = st.n;
   for (int i = 0; i < st.n; ++i) a.radix[i] = st.r[i];
   a.plan_id = plan_id_of(st);
   const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
-  std::vector<unsigned char> host(esz * L);
+  std::vector<unsigned char> host(esz * Ltw);
   // roots of unity in extended precision (x86 long double), rounded once:
   // the angle 2 pi t / L formed in double is off by up to ~6 ulps, and the
   // fp64 plans' transforms feed the deapodization, whose range (up to 2e7
   // at u = 1.25, J = 6) amplifies any rounding at the field-of-view edge
   const long double two_pi = 6.283185307179586476925286766559005768L;
-  for (int t = 0; t < L; ++t) {
+  for (int t = 0; t < Ltw; ++t) {
     double c, s;
-    if ((8LL * t) % L == 0) {   // multiples of pi/4: exact values
+    if ((8LL * t) % Ltw == 0) {   // multiples of pi/4: exact values
       static const double c8[8] = {1, 0.70710678118654752440, 0, -0.70710678118654752440, -1,
                                    -0.70710678118654752440, 0, 0.70710678118654752440};
-      const int o = (int)(8LL * t / L);
+      const int o = (int)(8LL * t / Ltw);
       c = c8[o];
       s = -c8[(o + 6) % 8];   // sin(-x) = -cos(x - pi/2)
     } else {
-      const long double ang = -two_pi * (long double)t / (long double)L;
+      const long double ang = -two_pi * (long double)t / (long double)Ltw;
       c = (double)cosl(ang);
       s = (double)sinl(ang);
     }
@@ -529,14 +693,14 @@ int fft_setup_axis(nufft3d_plan* p, int axis) {
       reinterpret_cast<float2*>(host.data())[t] = v;
     }
   }
-  N3D_HIP(hipMalloc(&a.d_tw, esz * L));
-  N3D_HIP(hipMemcpy(a.d_tw, host.data(), esz * L, hipMemcpyHostToDevice));
+  N3D_HIP(hipMalloc(&a.d_tw, esz * Ltw));
+  N3D_HIP(hipMemcpy(a.d_tw, host.data(), esz * Ltw, hipMemcpyHostToDevice));
   return NUFFT3D_OK;
 }
 
 static Stages stages_of(const FftAxis& a) {
   Stages st;
-  st.L = a.L;
+  st.L = a.bs ? a.L2 : a.L;
   st.n = a.nstage;
   for (int i = 0; i < a.nstage; ++i) st.r[i] = a.radix[i];
   return st;
@@ -613,9 +777,125 @@ static int bundle_x(int L, size_t esz, int nc, int want, int threads = 256) {
   return 1;
 }
 
+// One pass of B-line bundles along an axis for the transform path of plans
+// with a Bluestein axis (and the plain image-size transforms): the compiled
+// or runtime stage loop for a smooth length, k_fft_bluestein otherwise.
+template <typename R>
+static int strided_pass(nufft3d_plan* p, const FftAxis& a, const void* in, void* out, SGeom g,
+                        long long outer, int sign, const char* name) {
+  const size_t esz = sizeof(cplx<R>);
+  ProfScope ps(p, name);
+  if (a.bs) {
+    g.B = bundle_for(a.L2, esz, p->fft_bundle, 256, kStridedLds);
+    if (g.B < 1) {
+      set_error("transform length " + std::to_string(a.L) + " needs a Bluestein length " +
+                std::to_string(a.L2) + " beyond the work-group's LDS");
+      return NUFFT3D_ERR_UNSUPPORTED;
+    }
+    if (g.B > g.inner) g.B = (int)g.inner;
+    g.nbundle = (g.inner + g.B - 1) / g.B;
+    const size_t sh = (size_t)a.L2 * (g.B + 1) * esz;
+    allow_lds(k_fft_bluestein<R>, sh);
+    hipLaunchKernelGGL(k_fft_bluestein<R>, dim3((unsigned)(g.nbundle * outer)), dim3(256), sh,
+                       p->stream, (const cplx<R>*)in, (cplx<R>*)out, g, stages_of(a),
+                       (const cplx<R>*)a.d_tw, (const cplx<R>*)a.d_chirp,
+                       (const cplx<R>*)a.d_bhat[sign < 0 ? 0 : 1], a.L, sign);
+  } else {
+    g.B = bundle_for(a.L, esz, p->fft_bundle, 256, 65536);
+    if (g.B > g.inner) g.B = (int)g.inner;
+    g.nbundle = (g.inner + g.B - 1) / g.B;
+    const size_t sh = (size_t)a.L * (g.B + 1) * esz;
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * outer)), \
+                                    dim3(256), sh, p->stream, (const cplx<R>*)in, (cplx<R>*)out,  \
+                                    g, stages_of(a), (const cplx<R>*)a.d_tw, sign)
+    N3D_PLAN_SWITCH(a.plan_id, N3D_L);
+#undef N3D_L
+  }
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
+static bool has_bluestein(const nufft3d_plan* p) {
+  return p->ax[0].bs || p->ax[1].bs || p->ax[2].bs;
+}
+
+// Transform path of plans with a Bluestein axis: U and the coil-fastest
+// layout first, then pad + transform along x, y, z (forward), or the reverse
+// with crops and U, scale and the add term last (adjoint). T0 (the
+// coil-fastest image) shares t2's buffer, which it precedes (forward) or
+// follows (adjoint).
+template <typename R>
+static int fft_forward_generic(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1buf,
+                               void* t2buf) {
+  const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
+  const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
+  const size_t esz = sizeof(cplx<R>);
+  const long long nvox = (long long)N1 * N2 * N3;
+  const size_t t1 = (size_t)K1 * N2 * N3 * nc;
+  const size_t t2 = std::max((size_t)K1 * K2 * N3, (size_t)nvox) * nc;
+  void* T1 = t1buf;
+  void* T2 = t2buf;
+  if (!T1) { N3D_TRY(ensure(p->ws_t1, t1 * esz)); T1 = p->ws_t1.p; }
+  if (!T2) { N3D_TRY(ensure(p->ws_t2, t2 * esz)); T2 = p->ws_t2.p; }
+  {
+    ProfScope ps(p, "k_img_to_cl");
+    const long long n = nvox * nc;
+    hipLaunchKernelGGL((k_img_to_cl<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                       (const cplx<R>*)img, (const R*)p->d_U, (cplx<R>*)T2, nvox, nc);
+  }
+  SGeom g;
+  g.mode = 0;
+  g.Lin = N1; g.Lout = K1; g.n_img = N1; g.inner = nc;
+  N3D_TRY(strided_pass<R>(p, p->ax[0], T2, T1, g, (long long)N2 * N3, -1, "k_fft_x_fwd"));
+  g.Lin = N2; g.Lout = K2; g.n_img = N2; g.inner = (long long)K1 * nc;
+  N3D_TRY(strided_pass<R>(p, p->ax[1], T1, K3 > 1 ? T2 : grid, g, N3, -1, "k_fft_y_fwd"));
+  if (K3 > 1) {
+    g.Lin = N3; g.Lout = K3; g.n_img = N3; g.inner = (long long)K2 * K1 * nc;
+    N3D_TRY(strided_pass<R>(p, p->ax[2], T2, grid, g, 1, -1, "k_fft_z_fwd"));
+  }
+  return NUFFT3D_OK;
+}
+
+template <typename R>
+static int fft_adjoint_generic(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
+                               const void* add, double add_coef) {
+  const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
+  const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
+  const size_t esz = sizeof(cplx<R>);
+  const long long nvox = (long long)N1 * N2 * N3;
+  const size_t t1 = (size_t)K1 * N2 * N3 * nc;
+  const size_t t2 = std::max((size_t)K1 * K2 * N3, (size_t)nvox) * nc;
+  N3D_TRY(ensure(p->ws_t1, t1 * esz));
+  N3D_TRY(ensure(p->ws_t2, t2 * esz));
+  void* T1 = p->ws_t1.p;
+  void* T2 = p->ws_t2.p;
+  SGeom g;
+  g.mode = 1;
+  const void* yin = grid;
+  if (K3 > 1) {
+    g.Lin = K3; g.Lout = N3; g.n_img = N3; g.inner = (long long)K2 * K1 * nc;
+    N3D_TRY(strided_pass<R>(p, p->ax[2], grid, T2, g, 1, +1, "k_fft_z_adj"));
+    yin = T2;
+  }
+  g.Lin = K2; g.Lout = N2; g.n_img = N2; g.inner = (long long)K1 * nc;
+  N3D_TRY(strided_pass<R>(p, p->ax[1], yin, T1, g, N3, +1, "k_fft_y_adj"));
+  g.Lin = K1; g.Lout = N1; g.n_img = N1; g.inner = nc;
+  N3D_TRY(strided_pass<R>(p, p->ax[0], T1, T2, g, (long long)N2 * N3, +1, "k_fft_x_adj"));
+  {
+    ProfScope ps(p, "k_cl_to_img");
+    const long long n = nvox * nc;
+    hipLaunchKernelGGL((k_cl_to_img<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                       (const cplx<R>*)T2, (const R*)p->d_U, (cplx<R>*)img, nvox, nc, (R)scale,
+                       (const cplx<R>*)add, (R)add_coef);
+  }
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
 template <typename R>
 static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1buf,
                          void* t2buf) {
+  if (has_bluestein(p)) return fft_forward_generic<R>(p, img, nc, grid, t1buf, t2buf);
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -688,6 +968,7 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, v
 template <typename R>
 static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                          const void* add, double add_coef) {
+  if (has_bluestein(p)) return fft_adjoint_generic<R>(p, grid, nc, img, scale, add, add_coef);
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -782,13 +1063,21 @@ static int fft_plain_t(nufft3d_plan* p, const void* in, int nc, void* out) {
       {N3, (long long)nc, (long long)N1 * N2, 2}};
   for (auto& ps : passes) {
     if (ps.n <= 1) continue;
+    void* dst = bufs[which];
+    if (p->axN[ps.a].bs) {   // Bluestein length
+      SGeom gb;
+      gb.Lin = ps.n; gb.Lout = ps.n; gb.n_img = ps.n; gb.mode = 2; gb.inner = ps.inner;
+      N3D_TRY(strided_pass<R>(p, p->axN[ps.a], src, dst, gb, ps.outer, -1, "k_fft_plain"));
+      src = dst;
+      which ^= 1;
+      continue;
+    }
     SGeom g;
     g.Lin = ps.n; g.Lout = ps.n; g.n_img = ps.n; g.mode = 2;
     g.B = ps.inner >= 16 ? bundle_for(ps.n, esz, p->fft_bundle, 256, 65536) : 1;
     g.inner = ps.inner;
     g.nbundle = (g.inner + g.B - 1) / g.B;
     const size_t sh = (size_t)ps.n * (g.B + 1) * esz;
-    void* dst = bufs[which];
     ProfScope prs(p, "k_fft_plain");
 #define N3D_L(PL) hipLaunchKernelGGL((k_fft_strided<R, PL>), dim3((unsigned)(g.nbundle * ps.outer)), \
                                     dim3(256), sh, p->stream, (const cplx<R>*)src, (cplx<R>*)dst,  \

@@ -48,6 +48,15 @@ struct FftAxis {
   int radix[24] = {0};
   int plan_id = 0;        // compiled stage plan (0 = generic stage loop)
   void* d_tw = nullptr;   // L roots of unity exp(-2 pi i t / L), precision of the plan
+  // Bluestein (chirp-z) transform for a length with a prime factor above 13:
+  // the stages above describe the smooth length L2 >= 2L - 1 of the
+  // convolution, d_tw holds L2 roots; d_chirp the L values exp(-i pi n^2/L),
+  // d_bhat[0] / [1] the L2-point DFT of the conjugate chirp, wrapped, over L2
+  // for the forward (-) and inverse (+) transform.
+  int bs = 0;
+  int L2 = 0;
+  void* d_chirp = nullptr;
+  void* d_bhat[2] = {nullptr, nullptr};
 };
 
 // A dense grid point (k-space centre) whose hit list is split into segments;

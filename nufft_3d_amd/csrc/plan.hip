@@ -1029,10 +1029,10 @@ static void destroy_plan(nufft3d_plan* p) {
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  for (int a = 0; a < 3; ++a) {
-    if (p->ax[a].d_tw) (void)hipFree(p->ax[a].d_tw);
-    if (p->axN[a].d_tw) (void)hipFree(p->axN[a].d_tw);
-  }
+  for (int a = 0; a < 3; ++a)
+    for (FftAxis* x : {&p->ax[a], &p->axN[a]})
+      for (void* q : {x->d_tw, x->d_chirp, x->d_bhat[0], x->d_bhat[1]})
+        if (q) (void)hipFree(q);
   if (p->mr_exec) (void)hipGraphExecDestroy(p->mr_exec);
   for (auto& r : p->prof.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
@@ -1366,10 +1366,9 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += 4 * p->nbrick;
   if (p->bl.d_ptr) b += 8 * (p->nbrick + 1) + 4 * std::max<int64_t>(1, p->bl.total);
   if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
-  for (int a = 0; a < 3; ++a) {
-    if (p->ax[a].d_tw) b += 2 * rsz * p->ax[a].L;
-    if (p->axN[a].d_tw) b += 2 * rsz * p->axN[a].L;
-  }
+  for (int a = 0; a < 3; ++a)
+    for (const FftAxis* x : {&p->ax[a], &p->axN[a]})
+      if (x->d_tw) b += 2 * rsz * (x->bs ? x->L + 3 * x->L2 : x->L);   // roots (+ chirp, spectra)
   const Workspace* ws[] = {&p->ws_grid, &p->ws_t1, &p->ws_t2, &p->ws_samp, &p->ws_part,
                            &p->ws_img, &p->ws_solve, &p->ws_red, &p->ws_b};
   int64_t w = 0;

@@ -228,9 +228,17 @@ int nufft3d_gram_loraks(nufft3d_plan* plan, const void* data, const int32_t dims
 
 /* K = pruno(obj,data) (pruno.m:2-143) on k-space data [N1 N2 N3 nc] of the
  * plan's image size: returns nv, the singular values sv [n] (descending,
- * may be NULL) and the noise std used. */
+ * may be NULL), the noise std used and, in Kt (may be NULL; complex128,
+ * column-major, room for n x n), the tap-domain null kernels of
+ * pruno.m:124-130: column v < nv is (1 - f(v)) times the selected right
+ * singular vector, row c + nc*t for coil c and kernel tap t (taps in
+ * column-major order of the width^3 (or width^2) mask, x fastest). The
+ * image-space K [N1 N2 N3 nc nv] of pruno.m:132-143 is the zero-padded
+ * inverse DFT of these taps placed in the mask cube, times N per axis; the
+ * Python and MATLAB bindings build it on the host. */
 int nufft3d_pruno(nufft3d_plan* plan, const void* data, int32_t nc,
-                  const nufft3d_pruno_opts* opts, int32_t* nv, double* sv, double* noise);
+                  const nufft3d_pruno_opts* opts, int32_t* nv, double* sv, double* noise,
+                  void* Kt);
 
 /* iNUFT(raw,maxit,damp,W,'parallel-imaging-pruno',lambda) (iNUFT.m:200-218):
  * MINRES (maxit) or regridding (maxit=0) for calibration, k-space by an

@@ -92,7 +92,8 @@ SIGNATURES = {
     "nufft3d_gram_loraks": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
                                       C.c_int32, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "nufft3d_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(PrunoOpts),
-                                C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_double)]),
+                                C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_double),
+                                C.c_void_p]),
     "nufft3d_inverse_pruno": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_double,
                                         C.c_void_p, C.c_int64, C.c_double, C.POINTER(PrunoOpts),
                                         C.c_int32, C.c_void_p, C.POINTER(Info)]),

@@ -952,7 +952,7 @@ __global__ void k_pick_columns(const double2* __restrict__ E, int n, const int* 
 // fnull = K*lambda (iNUFT.m:207-208). dB receives [V][nc][nc].
 int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,
                 double lambda, void** dB, int* nv_out, std::vector<double>* sv_desc,
-                double* noise_out) {
+                double* noise_out, void* kt_host) {
   const int dims[4] = {p->N[0], p->N[1], p->N[2], nc};
   const long long V = (long long)dims[0] * dims[1] * dims[2];
   // fully sampled check (pruno.m:77-81)
@@ -998,7 +998,7 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
   }
   if (noise_out) *noise_out = noise;
   if (nv_out) *nv_out = nv;
-  if (!st && dB) {
+  if (!st && (dB || kt_host)) {
     // the nv vectors: literal = first nv (largest, pruno.m:128-130),
     // otherwise the nv smallest (those with f < 1 - tol)
     std::vector<int> cols(std::max(nv, 1));
@@ -1019,10 +1019,16 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
     if (nv > 0)
       hipLaunchKernelGGL(k_pick_columns, dim3(nblk((long long)n * nv)), dim3(256), 0, p->stream,
                          (const double2*)dAA, n, (const int*)dcols, (const double*)dscale, nv, dKt);
-    const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
-    N3D_HIP(hipMalloc(dB, esz * (size_t)V * nc * nc));
-    if (nv > 0) st = null_operator(p, dKt, nk, nc, nv, w, taps, p->N, lambda, *dB);
-    else N3D_HIP(hipMemset(*dB, 0, esz * (size_t)V * nc * nc));
+    if (kt_host && nv > 0)
+      N3D_HIP(hipMemcpyAsync(kt_host, dKt, sizeof(double2) * (size_t)n * nv, hipMemcpyDeviceToHost,
+                             p->stream));
+    if (dB) {
+      const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
+      N3D_HIP(hipMalloc(dB, esz * (size_t)V * nc * nc));
+      if (nv > 0) st = null_operator(p, dKt, nk, nc, nv, w, taps, p->N, lambda, *dB);
+      else N3D_HIP(hipMemset(*dB, 0, esz * (size_t)V * nc * nc));
+    }
+    N3D_HIP(hipStreamSynchronize(p->stream));
     (void)hipFree(dcols); (void)hipFree(dscale); (void)hipFree(dKt);
   }
   (void)hipFree(dAA);

@@ -299,7 +299,7 @@ int fista_solve(nufft3d_plan* p, const void* b, int nc, int maxit, double damp, 
 // gram.hip
 int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_opts& o,
                 double lambda, void** dB, int* nv_out, std::vector<double>* sv_desc,
-                double* noise_out);
+                double* noise_out, void* kt_host = nullptr);
 int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std::vector<int4>& taps,
                 double2* dAA, bool loraks = false, const int* center = nullptr);
 void kernel_taps(int width, int radial, bool is2d, std::vector<int4>& taps, int& w);

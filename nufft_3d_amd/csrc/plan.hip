@@ -2278,7 +2278,7 @@ int nufft3d_gram_loraks(nufft3d_plan* p, const void* data, const int32_t dims[4]
 }
 
 int nufft3d_pruno(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_pruno_opts* opts,
-                  int32_t* nv, double* sv, double* noise) {
+                  int32_t* nv, double* sv, double* noise, void* Kt) {
   N3D_PLAN_GUARD(p);
   if (!data || nc < 1) { set_error("invalid argument"); return NUFFT3D_ERR_INVALID; }
   nufft3d_pruno_opts o;
@@ -2289,7 +2289,7 @@ int nufft3d_pruno(nufft3d_plan* p, const void* data, int32_t nc, const nufft3d_p
   std::vector<double> svd;
   int nvv = 0;
   double nz = 0;
-  int st = pruno_build(p, di.dev, nc, o, 1.0, nullptr, &nvv, &svd, &nz);
+  int st = pruno_build(p, di.dev, nc, o, 1.0, nullptr, &nvv, &svd, &nz, Kt);
   if (!st) {
     if (nv) *nv = nvv;
     if (noise) *noise = nz;

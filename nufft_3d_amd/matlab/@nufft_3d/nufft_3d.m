@@ -16,10 +16,9 @@ classdef nufft_3d < handle
     %    single precision, gpu=2 in double (the library's FP64 mode);
     %  - arrays cross as host arrays (MATLAB gpuArray is CUDA only): inputs
     %    are gathered and cast, outputs come back as host arrays;
-    %  - pruno returns the null-space size, the singular values and the
-    %    noise level; the image-space null kernels K [N nc nv] (pruno.m:141-143)
-    %    stay inside the library, which applies them as a per-voxel operator
-    %    in iNUFT's 'parallel-imaging-pruno' branch;
+    %  - pruno returns K [N1 N2 N3 nc nv] as the reference does, built here
+    %    from the library's tap-domain kernels, and also nv, the singular
+    %    values and the noise level as further outputs; no plot is drawn;
     %  - the 'parallel-imaging-sake' branch returns the image ifft3(ksp)
     %    (iNUFT.m:195 returns SAKE's k-space as the image).
 
@@ -176,16 +175,42 @@ classdef nufft_3d < handle
             ksp = nufft3d_mex('sake',obj.h,obj.cast(data),opts);
         end
 
-        %% PRUNO (pruno.m:2-143): null-space size, spectrum and noise
-        function [nv,sv,noise] = pruno(obj,data,varargin)
-            opts = struct();
+        %% PRUNO (pruno.m:2-143): image-space null kernels, null-space
+        %% size, spectrum and noise
+        function [K,nv,sv,noise] = pruno(obj,data,varargin)
+            opts = struct('width',5,'radial',1);
             for k = 1:2:numel(varargin)
                 if k==numel(varargin) || ~ischar(varargin{k})
                     error('''varargin'' must be option/value pairs.');
                 end
                 opts.(varargin{k}) = varargin{k+1};
             end
-            [nv,sv,noise] = nufft3d_mex('pruno',obj.h,obj.cast(data),opts);
+            [nv,sv,noise,Kt] = nufft3d_mex('pruno',obj.h,obj.cast(data),opts);
+            % kernel mask (pruno.m:56-65): x fastest, z = 0 for 2d data
+            h = fix(opts.width/2);
+            if obj.N(3)==1
+                [x,y] = ndgrid(-h:h); z = zeros(size(x));
+            else
+                [x,y,z] = ndgrid(-h:h);
+            end
+            if opts.radial
+                mask = sqrt(x.^2+y.^2+z.^2)<=opts.width/2;
+            else
+                mask = abs(x)<=opts.width/2 & abs(y)<=opts.width/2 & abs(z)<=opts.width/2;
+            end
+            % taps into the mask, then the zero-padded inverse DFT times N
+            % per axis (pruno.m:124-143); Kt rows are coil fastest
+            nc = size(Kt,1)/nnz(mask);
+            K = zeros(numel(mask),nc,nv);
+            K(mask,:,:) = permute(reshape(Kt,nc,[],nv),[2 1 3]);
+            if obj.N(3)==1
+                K = reshape(K,[size(mask) 1 nc nv]);
+            else
+                K = reshape(K,[size(mask) nc nv]);
+            end
+            K = ifft(K,obj.N(1),1) * obj.N(1);
+            K = ifft(K,obj.N(2),2) * obj.N(2);
+            K = ifft(K,obj.N(3),3) * obj.N(3);
         end
 
         %% discrepancy principle (discrep.m:2-69)

@@ -266,7 +266,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     return;
   }
 
-  if (!strcmp(cmd, "pruno")) {                         /* [nv, sv, noise] = pruno(h, data, opts) */
+  if (!strcmp(cmd, "pruno")) {                         /* [nv, sv, noise, Kt] = pruno(h, data, opts) */
     const mxArray* s = nrhs > 3 ? prhs[3] : NULL;
     nufft3d_pruno_opts o;
     check(nufft3d_pruno_opts_default(&o));
@@ -279,10 +279,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     double noise = 0;
     const mwSize n = (mwSize)nc * count_taps(o.width, o.radial, q.N[2] == 1);
     mxArray* sv = mxCreateDoubleMatrix(n, 1, mxREAL);      /* singular values, descending */
-    check(nufft3d_pruno(p, cdata(prhs[2], q.prec, "data"), nc, &o, &nv, mxGetDoubles(sv), &noise));
+    mxArray* kt = mxCreateDoubleMatrix(n, n, mxCOMPLEX);   /* tap-domain kernels, n x nv used */
+    check(nufft3d_pruno(p, cdata(prhs[2], q.prec, "data"), nc, &o, &nv, mxGetDoubles(sv), &noise,
+                        mxGetComplexDoubles(kt)));
+    mxSetN(kt, (mwSize)nv);
     plhs[0] = mxCreateDoubleScalar(nv);
     if (nlhs > 1) plhs[1] = sv; else mxDestroyArray(sv);
     if (nlhs > 2) plhs[2] = mxCreateDoubleScalar(noise);
+    if (nlhs > 3) plhs[3] = kt; else mxDestroyArray(kt);
     return;
   }
 

@@ -414,20 +414,34 @@ class nufft_3d:
         assert n.value == AA.shape[0]
         return AA
 
-    def pruno(self, data, **opts):
-        """pruno(obj,data,...) (pruno.m:2-143): returns dict(nv, sv, noise)."""
+    def pruno(self, data, return_info=False, output="image", **opts):
+        """K = pruno(obj,data,...) (pruno.m:2-143): the image-space null
+        kernels K [N1 N2 N3 nc nv] (complex128). output="taps" returns the
+        tap-domain kernels Kt [n nv] instead (row c + nc*t, taps in
+        ``taps.kernel_taps`` order), which skips the N1*N2*N3*nc*nv
+        expansion. return_info=True also returns dict(nv, sv, noise, Kt,
+        taps): the null-space size, the singular values (descending) and
+        the noise std used."""
+        if output not in ("image", "taps"):
+            raise ValueError("output must be 'image' or 'taps'")
         data = np.asarray(data)
         nc = data.size // (self.N[0] * self.N[1] * self.N[2])
         db = self._host_c(data, self.N + (nc,))
         o = self._popts(opts)
         from . import taps as _taps
-        n = len(_taps.kernel_taps(o.width, o.radial, self.N[2] == 1)) * nc
+        taps = _taps.kernel_taps(o.width, o.radial, self.N[2] == 1)
+        n = len(taps) * nc
         sv = np.empty(n, dtype=np.float64)
+        Kt = np.zeros((n, n), dtype=np.complex128, order="F")
         nv = C.c_int32()
         noise = C.c_double()
         _lib.check(self._lib.nufft3d_pruno(self._h, db.ctypes.data, nc, C.byref(o), C.byref(nv),
-                                           sv.ctypes.data, C.byref(noise)))
-        return {"nv": nv.value, "sv": sv, "noise": noise.value}
+                                           sv.ctypes.data, C.byref(noise), Kt.ctypes.data))
+        Kt = np.ascontiguousarray(Kt[:, :nv.value])
+        K = Kt if output == "taps" else _taps.null_kernels_image(Kt, taps, o.width, self.N, nc)
+        if not return_info:
+            return K
+        return K, {"nv": nv.value, "sv": sv, "noise": noise.value, "Kt": Kt, "taps": taps}
 
     _SAKE_KEYS = ("width", "radial", "loraks", "tol", "maxit", "noise", "center", "W", "damp",
                   "sparsity", "dc_iters", "output")

@@ -203,17 +203,24 @@ def pruno(data, N, width=5, radial=1, noise=None, tol=0.01, literal_first_nv=Fal
     n = nc * nk
     sel = np.arange(nv) if literal_first_nv else np.arange(n - nv, n)
     Kt = V[:, sel] * (1 - f[sel])[None, :]                    # [(c + nc*t), v]
-    # pruno.m:124-143: taps into the w^3 cube, zero-padded ifft to N, x N
+    K = kernels_image(Kt, mask, N, nc)
+    return dict(K=K, Kt=Kt, nv=nv, sv=Ws, f=f, noise=noise, taps=taps, mask=mask)
+
+
+def kernels_image(Kt, mask, N, nc):
+    """pruno.m:124-143: the tap-domain kernels Kt [(c + nc*t), v] into the
+    mask cube (K(mask,:,v) = V(:,:,v) after the [nc nk] -> [nk nc] permute),
+    reshaped to [size(mask) (1) nc nv], then ifft(K, N(a), a) * N(a) per axis."""
+    nv = Kt.shape[1]
+    nk = int(np.count_nonzero(mask))
     wdt = mask.shape[0]
-    Kc = np.zeros((wdt, wdt, mask.shape[2] if mask.ndim == 3 else 1, nc, nv), dtype=np.complex128)
+    shape = (wdt, wdt, mask.shape[2] if mask.ndim == 3 else 1, nc, nv)
     flat = np.zeros((mask.size, nc, nv), dtype=np.complex128)
-    Vr = Kt.reshape(nc, nk, nv, order='F').transpose(1, 0, 2)   # [t, c, v]
-    flat[mask.ravel(order='F')] = Vr
-    Kc = flat.reshape(Kc.shape, order='F')
-    K = Kc
+    flat[mask.ravel(order='F')] = Kt.reshape(nc, nk, nv, order='F').transpose(1, 0, 2)
+    K = flat.reshape(shape, order='F')
     for ax in range(3):
         K = np.fft.ifft(K, n=N[ax], axis=ax) * N[ax]
-    return dict(K=K, Kt=Kt, nv=nv, sv=Ws, f=f, noise=noise, taps=taps, mask=mask)
+    return K
 
 
 def null_term(fnull, x):

@@ -222,7 +222,7 @@ def test_config4_lowrank_96_full_size_gram_and_pruno():
     assert e64 < 1e-12 and e32 < 1e-4
     w = np.linalg.eigvalsh(want)[::-1]
     noise, sv, f, nv = P.select(w, 1.0, N, N)
-    got = g64.pruno(data, width=width, radial=radial)
+    _, got = g64.pruno(data, width=width, radial=radial, output="taps", return_info=True)
     print("config 4 pruno noise %.4e (oracle %.4e) nv %d (oracle %d)"
           % (got["noise"], noise, got["nv"], nv), flush=True)
     # eigenvalues agree to the eigen-solver's backward error (n eps lambda_max,
@@ -244,7 +244,8 @@ def test_config4_lowrank_96_full_size_gram_and_pruno():
     assert abs(got["nv"] - nv) <= 1
     # fp32 plan (split-BF16 Gram): by Weyl's inequality each eigenvalue moves
     # by at most the Gram's error in the 2-norm (<= its Frobenius norm)
-    got32 = g32.pruno(data.astype(np.complex64), width=width, radial=radial, noise=noise)
+    _, got32 = g32.pruno(data.astype(np.complex64), width=width, radial=radial, noise=noise,
+                         output="taps", return_info=True)
     lam32 = np.linalg.eigvalsh(want32)[::-1]
     bound = np.linalg.norm(G32 - want32) + 1e-12 * lam32[0]
     assert np.max(np.abs(got32["sv"] ** 2 - lam32)) <= bound

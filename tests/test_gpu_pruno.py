@@ -54,14 +54,50 @@ def test_pruno_spectrum_and_nv(plans):
     om, g32, g64 = plans
     d = _data((8, 8, 8, 3), 3)
     ref = P.pruno(d, (8, 8, 8), width=3, noise=0.05)
-    got = g64.pruno(d, width=3, noise=0.05)
+    _, got = g64.pruno(d, width=3, noise=0.05, output="taps", return_info=True)
     assert got["nv"] == ref["nv"]
     assert rel(got["sv"], ref["sv"]) < 1e-10
     est_ref = P.pruno(d, (8, 8, 8), width=3)
-    est = g64.pruno(d, width=3)
+    _, est = g64.pruno(d, width=3, output="taps", return_info=True)
     assert est["noise"] == pytest.approx(est_ref["noise"], rel=1e-9)
-    got32 = g32.pruno(d.astype(np.complex64), width=3, noise=0.05)
+    _, got32 = g32.pruno(d.astype(np.complex64), width=3, noise=0.05, output="taps",
+                         return_info=True)
     assert rel(got32["sv"], ref["sv"]) < 1e-4
+
+
+def _voxel_gram(K):
+    # sum_v K_v K_v^H per voxel: invariant to the phase of each singular
+    # vector (and to rotations inside equal-filter subspaces)
+    return np.einsum('xyzcv,xyzdv->xyzcd', K, K.conj())
+
+
+@pytest.mark.parametrize("gpu,literal,radial,N", [(2, 0, 1, (8, 8, 8)), (2, 1, 1, (8, 8, 8)),
+                                                  (2, 0, 0, (10, 8, 6)), (1, 0, 1, (8, 8, 8))])
+def test_pruno_returns_the_image_space_null_kernels(gpu, literal, radial, N):
+    # K = pruno(obj,data) is [N1 N2 N3 nc nv] (pruno.m:2,124-143): compared
+    # with the oracle through the tap-domain projector Kt Kt' and the
+    # per-voxel coil Gram of K, which do not depend on the eigenvectors'
+    # phases
+    om = synth.koosh_ball(8, 40, 16)
+    g = nufft_3d(om, N, gpu=gpu)
+    d = _data(N + (3,), 5)
+    ref = P.pruno(d, N, width=3, radial=radial, noise=0.35, literal_first_nv=bool(literal))
+    if gpu == 1:
+        d = d.astype(np.complex64)
+    K, info = g.pruno(d, width=3, radial=radial, noise=0.35, literal_first_nv=literal,
+                      return_info=True)
+    assert info["nv"] == ref["nv"] and 0 < ref["nv"] < len(ref["sv"])
+    assert K.shape == ref["K"].shape == N + (3, ref["nv"])
+    Kt = info["Kt"]
+    tol = 1e-10 if gpu == 2 else 1e-4
+    e_taps = rel(Kt @ Kt.conj().T, ref["Kt"] @ ref["Kt"].conj().T)
+    e_img = rel(_voxel_gram(K), _voxel_gram(ref["K"]))
+    print("pruno K gpu=%d literal=%d radial=%d nv=%d taps %.2e image %.2e"
+          % (gpu, literal, radial, ref["nv"], e_taps, e_img), flush=True)
+    assert e_taps < tol and e_img < tol
+    # the taps output is the same kernels before the expansion
+    Kt2 = g.pruno(d, width=3, radial=radial, noise=0.35, literal_first_nv=literal, output="taps")
+    assert np.array_equal(Kt2, Kt)
 
 
 @pytest.mark.parametrize("gpu,tol", [(2, 1e-8), (1, 1e-4)])

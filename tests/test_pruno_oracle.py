@@ -175,3 +175,34 @@ def test_estimate_noise_recovers_a_flat_noise_plateau(nsig, m):
     density, nx, ny = 0.8, 24, 20
     got = P.estimate_noise(W, density, nx, ny)
     assert got == pytest.approx(np.sqrt(sigma2) / np.sqrt(density * nx * ny), rel=1e-12)
+
+
+@pytest.mark.parametrize("width,radial,N", [(3, 1, (8, 6, 4)), (4, 0, (6, 6, 6)), (5, 1, (8, 8, 1)),
+                                            (5, 0, (4, 6, 2))])
+def test_image_kernels_by_brute_force_and_host_builder(width, radial, N):
+    # pruno.m:124-143 on random tap-domain kernels: K[x,y,z,c,v] =
+    # sum_t Kt[c + nc t, v] exp(2 pi i (ix x/N1 + iy y/N2 + iz z/N3)), the
+    # tap at cube index i = offset + fix(width/2) (iz = 0 for 2d); a cube
+    # longer than N keeps its first N entries (ifft(K, N) truncates). The
+    # oracle (numpy ifft) and the bindings' host builder both match it.
+    from nufft_3d_amd import taps as T
+    is2d = N[2] == 1
+    taps, mask = P.kernel_taps(width, radial, is2d)
+    nc, nv = 2, 3
+    r = np.random.default_rng(11)
+    Kt = r.normal(size=(len(taps) * nc, nv)) + 1j * r.normal(size=(len(taps) * nc, nv))
+    h = width // 2
+    x, y, z = np.meshgrid(*[np.arange(n) for n in N], indexing='ij')
+    want = np.zeros(N + (nc, nv), dtype=np.complex128)
+    for t, (ox, oy, oz) in enumerate(taps):
+        ix, iy, iz = ox + h, oy + h, 0 if is2d else oz + h
+        if ix >= N[0] or iy >= N[1] or iz >= N[2]:
+            continue                                       # truncated away
+        ph = np.exp(2j * np.pi * (ix * x / N[0] + iy * y / N[1] + iz * z / N[2]))
+        want += ph[..., None, None] * Kt[t * nc:(t + 1) * nc][None, None, None]
+    got_o = P.kernels_image(Kt, mask, N, nc)
+    got_h = T.null_kernels_image(Kt, [tuple(int(a) for a in t) for t in taps], width, N, nc)
+    assert T.kernel_taps(width, radial, is2d) == [tuple(int(a) for a in t) for t in taps]
+    assert got_o.shape == got_h.shape == want.shape
+    assert np.max(np.abs(got_o - want)) < 1e-12 * np.abs(want).max()
+    assert np.max(np.abs(got_h - want)) < 1e-12 * np.abs(want).max()

@@ -328,8 +328,8 @@ struct XGeom {
 
 // Forward: image rows (x fastest) are staged through LDS with coalesced
 // reads; the last stage writes [yz][kx][c] directly (lines = coils fastest).
-template <typename R, int PL>
-__global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<R>* __restrict__ img,
+template <typename R, int PL, typename IO = R>
+__global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<IO>* __restrict__ img,
                                                    const R* __restrict__ U,
                                                    cplx<R>* __restrict__ out, XGeom g, Stages st,
                                                    const cplx<R>* __restrict__ tw) {
@@ -346,9 +346,10 @@ __global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<R>* __restrict__ i
     if (l >= g.nlines) continue;
     const long long yz = l / g.nc;
     const int c = (int)(l % g.nc);
-    cplx<R> v = img[x + (long long)g.N1 * (yz + g.nyz * c)];
+    const cplx<IO> vi = img[x + (long long)g.N1 * (yz + g.nyz * c)];
     const R u = U[x + (long long)g.N1 * yz];
-    v.x *= u; v.y *= u;
+    cplx<R> v;
+    v.x = (R)vi.x * u; v.y = (R)vi.y * u;
     a[padpos(x, g.N1, g.K1) * g.B + li] = v;
   }
   __syncthreads();
@@ -364,10 +365,10 @@ __global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<R>* __restrict__ i
 // Adjoint: the first stage reads [yz][kx][c] directly; the result is cropped
 // out of LDS with coalesced image-row writes (x fastest), with U, the scale
 // and the optional add term.
-template <typename R, int PL>
+template <typename R, int PL, typename IO = R>
 __global__ void __launch_bounds__(512) k_fft_x_adj(const cplx<R>* __restrict__ in,
                                                    const R* __restrict__ U,
-                                                   cplx<R>* __restrict__ img, XGeom g, Stages st,
+                                                   cplx<IO>* __restrict__ img, XGeom g, Stages st,
                                                    const cplx<R>* __restrict__ tw, R scale,
                                                    const cplx<R>* __restrict__ add, R add_coef) {
   extern __shared__ unsigned char smem_raw[];
@@ -397,7 +398,9 @@ __global__ void __launch_bounds__(512) k_fft_x_adj(const cplx<R>* __restrict__ i
     v.x *= u; v.y *= u;
     const long long o = x + (long long)g.N1 * (yz + g.nyz * c);
     if (add) { const cplx<R> q = add[o]; v.x += add_coef * q.x; v.y += add_coef * q.y; }
-    img[o] = v;
+    cplx<IO> vo;
+    vo.x = (IO)v.x; vo.y = (IO)v.y;
+    img[o] = vo;
   }
 }
 
@@ -506,24 +509,24 @@ __global__ void __launch_bounds__(256) k_fft_bluestein(const cplx<R>* __restrict
 // Layout changes of the transform path for plans with a Bluestein axis:
 // image [nc][N3][N2][N1] times U <-> coil-fastest [N3][N2][N1][nc] (the
 // layout the strided passes then take along x, y and z).
-template <typename R>
-__global__ void k_img_to_cl(const cplx<R>* __restrict__ img, const R* __restrict__ U,
+template <typename R, typename IO = R>
+__global__ void k_img_to_cl(const cplx<IO>* __restrict__ img, const R* __restrict__ U,
                             cplx<R>* __restrict__ w, long long nvox, int nc) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nvox * nc) return;
   const long long v = t / nc;
   const int c = (int)(t - v * nc);
-  const cplx<R> a = img[(long long)c * nvox + v];
+  const cplx<IO> a = img[(long long)c * nvox + v];
   const R u = U[v];
   cplx<R> o;
-  o.x = a.x * u;
-  o.y = a.y * u;
+  o.x = (R)a.x * u;
+  o.y = (R)a.y * u;
   w[t] = o;
 }
 
-template <typename R>
+template <typename R, typename IO = R>
 __global__ void k_cl_to_img(const cplx<R>* __restrict__ w, const R* __restrict__ U,
-                            cplx<R>* __restrict__ img, long long nvox, int nc, R scale,
+                            cplx<IO>* __restrict__ img, long long nvox, int nc, R scale,
                             const cplx<R>* __restrict__ add, R add_coef) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nvox * nc) return;
@@ -538,7 +541,9 @@ __global__ void k_cl_to_img(const cplx<R>* __restrict__ w, const R* __restrict__
     o.x += add_coef * add[t].x;
     o.y += add_coef * add[t].y;
   }
-  img[t] = o;
+  cplx<IO> oo;
+  oo.x = (IO)o.x; oo.y = (IO)o.y;
+  img[t] = oo;
 }
 
 // ---------------------------------------------------------------- host side
@@ -824,7 +829,7 @@ static bool has_bluestein(const nufft3d_plan* p) {
 // with crops and U, scale and the add term last (adjoint). T0 (the
 // coil-fastest image) shares t2's buffer, which it precedes (forward) or
 // follows (adjoint).
-template <typename R>
+template <typename R, typename IO>
 static int fft_forward_generic(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1buf,
                                void* t2buf) {
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
@@ -840,8 +845,8 @@ static int fft_forward_generic(nufft3d_plan* p, const void* img, int nc, void* g
   {
     ProfScope ps(p, "k_img_to_cl");
     const long long n = nvox * nc;
-    hipLaunchKernelGGL((k_img_to_cl<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
-                       (const cplx<R>*)img, (const R*)p->d_U, (cplx<R>*)T2, nvox, nc);
+    hipLaunchKernelGGL((k_img_to_cl<R, IO>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       p->stream, (const cplx<IO>*)img, (const R*)p->d_U, (cplx<R>*)T2, nvox, nc);
   }
   SGeom g;
   g.mode = 0;
@@ -856,7 +861,7 @@ static int fft_forward_generic(nufft3d_plan* p, const void* img, int nc, void* g
   return NUFFT3D_OK;
 }
 
-template <typename R>
+template <typename R, typename IO>
 static int fft_adjoint_generic(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                                const void* add, double add_coef) {
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
@@ -884,18 +889,18 @@ static int fft_adjoint_generic(nufft3d_plan* p, const void* grid, int nc, void* 
   {
     ProfScope ps(p, "k_cl_to_img");
     const long long n = nvox * nc;
-    hipLaunchKernelGGL((k_cl_to_img<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
-                       (const cplx<R>*)T2, (const R*)p->d_U, (cplx<R>*)img, nvox, nc, (R)scale,
-                       (const cplx<R>*)add, (R)add_coef);
+    hipLaunchKernelGGL((k_cl_to_img<R, IO>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       p->stream, (const cplx<R>*)T2, (const R*)p->d_U, (cplx<IO>*)img, nvox, nc,
+                       (R)scale, (const cplx<R>*)add, (R)add_coef);
   }
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
 
-template <typename R>
+template <typename R, typename IO = R>
 static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1buf,
                          void* t2buf) {
-  if (has_bluestein(p)) return fft_forward_generic<R>(p, img, nc, grid, t1buf, t2buf);
+  if (has_bluestein(p)) return fft_forward_generic<R, IO>(p, img, nc, grid, t1buf, t2buf);
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -923,8 +928,8 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, v
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_fwd");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_fwd<R, PL>), dim3((unsigned)nb), dim3(xt), sh,    \
-                                    p->stream, (const cplx<R>*)img, (const R*)p->d_U,           \
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_fwd<R, PL, IO>), dim3((unsigned)nb), dim3(xt), sh, \
+                                    p->stream, (const cplx<IO>*)img, (const R*)p->d_U,          \
                                     (cplx<R>*)T1, g, stages_of(p->ax[0]),                       \
                                     (const cplx<R>*)p->ax[0].d_tw)
     N3D_PLAN_SWITCH(p->ax[0].plan_id, N3D_L);
@@ -965,10 +970,10 @@ static int fft_forward_t(nufft3d_plan* p, const void* img, int nc, void* grid, v
   return NUFFT3D_OK;
 }
 
-template <typename R>
+template <typename R, typename IO = R>
 static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
                          const void* add, double add_coef) {
-  if (has_bluestein(p)) return fft_adjoint_generic<R>(p, grid, nc, img, scale, add, add_coef);
+  if (has_bluestein(p)) return fft_adjoint_generic<R, IO>(p, grid, nc, img, scale, add, add_coef);
   const int N1 = p->N[0], N2 = p->N[1], N3 = p->N[2];
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const size_t esz = sizeof(cplx<R>);
@@ -1019,9 +1024,9 @@ static int fft_adjoint_t(nufft3d_plan* p, const void* grid, int nc, void* img, d
     const long long nb = (g.nlines + g.B - 1) / g.B;
     const size_t sh = (size_t)K1 * (g.B + 1) * esz;
     ProfScope ps(p, "k_fft_x_adj");
-#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_adj<R, PL>), dim3((unsigned)nb), dim3(xt), sh,    \
+#define N3D_L(PL) hipLaunchKernelGGL((k_fft_x_adj<R, PL, IO>), dim3((unsigned)nb), dim3(xt), sh, \
                                     p->stream, (const cplx<R>*)T1, (const R*)p->d_U,           \
-                                    (cplx<R>*)img, g, stages_of(p->ax[0]),                     \
+                                    (cplx<IO>*)img, g, stages_of(p->ax[0]),                    \
                                     (const cplx<R>*)p->ax[0].d_tw, (R)scale,                   \
                                     (const cplx<R>*)add, (R)add_coef)
     N3D_PLAN_SWITCH(p->ax[0].plan_id, N3D_L);
@@ -1097,13 +1102,21 @@ int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out) {
   return p->prec == 2 ? fft_plain_t<double>(p, in, nc, out) : fft_plain_t<float>(p, in, nc, out);
 }
 
-int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1, void* t2) {
+int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1, void* t2, int ep) {
+  if (ep == 1 && p->prec == 2) return fft_forward_t<double, float>(p, img, nc, grid, t1, t2);
   return p->prec == 2 ? fft_forward_t<double>(p, img, nc, grid, t1, t2)
                       : fft_forward_t<float>(p, img, nc, grid, t1, t2);
 }
 
 int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
-                const void* add, double add_coef) {
+                const void* add, double add_coef, int ep) {
+  if (ep == 1 && p->prec == 2) {
+    if (add) {
+      set_error("an fp32 image out of an fp64 transform takes no add term");
+      return NUFFT3D_ERR_INVALID;
+    }
+    return fft_adjoint_t<double, float>(p, grid, nc, img, scale, nullptr, 0.0);
+  }
   return p->prec == 2 ? fft_adjoint_t<double>(p, grid, nc, img, scale, add, add_coef)
                       : fft_adjoint_t<float>(p, grid, nc, img, scale, add, add_coef);
 }

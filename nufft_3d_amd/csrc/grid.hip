@@ -201,14 +201,14 @@ __device__ __forceinline__ void fma_coils(A* ar, A* ai, R w, const cplx<R>* val)
 // in z-y-x candidate order. The result goes to the sorted coil-interleaved
 // layout out[s*nc + c], or with USER straight to the caller's [M nc] layout
 // out[orig[s] + c*M] (no separate permutation pass).
-template <typename R, int CPL, int LPS, bool COMP, bool FULL, bool USER>
+template <typename R, int CPL, int LPS, bool COMP, bool FULL, bool USER, typename RO = R>
 __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ spos,
                                                 const unsigned short* __restrict__ fcode,
                                                 const int4* __restrict__ base,
                                                 const double* __restrict__ off,
                                                 const float4* __restrict__ off32,
                                                 const cplx<R>* __restrict__ grid,
-                                                cplx<R>* __restrict__ out,
+                                                cplx<RO>* __restrict__ out,
                                                 const R* __restrict__ wsamp, R scale,
                                                 const int* __restrict__ orig,
                                                 const double* __restrict__ kbtab,
@@ -267,22 +267,22 @@ __global__ void __launch_bounds__(256) k_interp(const long long* __restrict__ sp
   R sc = scale;
   if (wsamp) sc *= wsamp[s];
   if constexpr (USER) {
-    cplx<R>* op = out + orig[s] + (long long)g.c0 * g.M;
+    cplx<RO>* op = out + orig[s] + (long long)g.c0 * g.M;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const int k = coil_of<R, CPL, LPS, FULL>(q, c);
       if (k < g.cg) {
-        cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
+        cplx<RO> v; v.x = (RO)((R)ar[c] * sc); v.y = (RO)((R)ai[c] * sc);
         op[(long long)k * g.M] = v;
       }
     }
   } else {
-    cplx<R>* op = out + (long long)s * g.nc + g.c0;
+    cplx<RO>* op = out + (long long)s * g.nc + g.c0;
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
       const int k = coil_of<R, CPL, LPS, FULL>(q, c);
       if (k < g.cg) {
-        cplx<R> v; v.x = (R)ar[c] * sc; v.y = (R)ai[c] * sc;
+        cplx<RO> v; v.x = (RO)((R)ar[c] * sc); v.y = (RO)((R)ai[c] * sc);
         op[k] = v;
       }
     }
@@ -710,6 +710,106 @@ __global__ void __launch_bounds__(512) k_spread_tile(const long long* __restrict
   }
 }
 
+// Split-precision adjoint (fp32 samples, fp64 grid) with the samples staged
+// in LDS: one 512-lane work-group per 8x8x8 brick, one lane per grid point,
+// CG coils. Each hit's entry holds the slot of its sample in the brick's list
+// (ascending ids) and its fp32 weight from the plan's table; the brick's
+// sample rows pass through LDS in chunks of S, read once with 16-byte loads.
+// A point meets its hits chunk by chunk in list order and sums them in
+// double with the same products and order as k_spread, so the grids are
+// bitwise equal. Points with more than rmax hits are summed by the first nhb
+// work-groups (a multiple of 8, so the XCD of every later work-group is
+// unchanged) as k_spread_heavy does, overlapping the bricks.
+template <int CPL, int LPS>
+__global__ void __launch_bounds__(512) k_spread_tile_w(const long long* __restrict__ rowptr,
+                                                       const uint2* __restrict__ went,
+                                                       const long long* __restrict__ wbeg,
+                                                       const long long* __restrict__ wend,
+                                                       const int* __restrict__ wsamp_ids,
+                                                       const float2* __restrict__ ks,
+                                                       double2* __restrict__ grid,
+                                                       const int* __restrict__ bricks, int rmax,
+                                                       GridGeom g, const uint2* __restrict__ hitw,
+                                                       const HeavySeg* __restrict__ segs, int nseg,
+                                                       int nhb, double2* __restrict__ partial,
+                                                       KBParams kp) {
+  constexpr int CG = CPL * LPS;
+  constexpr int S = 512;
+  constexpr int V4 = CG / 2;        // 16-byte pieces of a staged row
+  constexpr int ROW4 = V4 + 1;      // row stride with 16 bytes of padding (LDS banks)
+  __shared__ float4 s_val[S * ROW4];
+  if ((int)blockIdx.x < nhb) {
+    spread_heavy_block<float, CPL, LPS, true, true>(blockIdx.x, segs, nseg, hitw, true, nullptr,
+                                                    nullptr, reinterpret_cast<const cplx<float>*>(ks),
+                                                    nullptr, reinterpret_cast<cplx<float>*>(partial),
+                                                    g, kp, nullptr);
+    return;
+  }
+  const long long bi = xcd_block_of(g.xchunk, blockIdx.x - nhb, gridDim.x - nhb);
+  const int t = bricks[bi];
+  const int l = threadIdx.x;
+  const int x = (t & 1023) * 8 + (l & 7), y = ((t >> 10) & 1023) * 8 + ((l >> 3) & 7),
+            z = (t >> 20) * 8 + (l >> 6);
+  const bool inside = x < g.K1 && y < g.K2 && z < g.K3;
+  const long long gpt = ((long long)z * g.K2 + y) * g.K1 + x;
+  long long e = 0, e1 = 0;
+  bool store = false;
+  if (inside) {
+    e = rowptr[gpt];
+    e1 = rowptr[gpt + 1];
+    store = e1 - e <= rmax;
+    if (!store) e1 = e;
+  }
+  double ar[CG], ai[CG];
+#pragma unroll
+  for (int c = 0; c < CG; ++c) { ar[c] = 0; ai[c] = 0; }
+  // entries are loaded four ahead; indices past the list are clamped to its
+  // last entry (or entry 0 for an empty list), so every load is in bounds
+  const long long ecl = e1 > e ? e1 - 1 : 0;
+  uint2 pf0 = went[e < ecl ? e : ecl], pf1 = went[e + 1 < ecl ? e + 1 : ecl];
+  uint2 pf2 = went[e + 2 < ecl ? e + 2 : ecl], pf3 = went[e + 3 < ecl ? e + 3 : ecl];
+  const long long base = wbeg[bi];
+  const int nsl = (int)(wend[bi] - base);
+  const float4* ks4 = reinterpret_cast<const float4*>(ks);
+  const long long rs4 = g.nc / 2;
+  const int co4 = g.c0 / 2;
+  for (int c0 = 0; c0 < nsl; c0 += S) {
+    const int cn = nsl - c0 < S ? nsl - c0 : S;
+    __syncthreads();   // the previous chunk is consumed
+    for (int i = l; i < cn * V4; i += blockDim.x) {
+      const int r = i / V4, k = i - r * V4;
+      const int sm = wsamp_ids[base + c0 + r];
+      s_val[r * ROW4 + k] = ks4[(long long)sm * rs4 + co4 + k];
+    }
+    __syncthreads();
+    const unsigned cend = (unsigned)(c0 + cn);
+    while (e < e1) {
+      const uint2 cur = pf0;
+      if (cur.x >= cend) break;
+      pf0 = pf1;
+      pf1 = pf2;
+      pf2 = pf3;
+      pf3 = went[e + 4 < ecl ? e + 4 : ecl];
+      ++e;
+      const float w = __uint_as_float(cur.y);
+      const float4* row = s_val + (int)(cur.x - (unsigned)c0) * ROW4;
+#pragma unroll
+      for (int k = 0; k < V4; ++k) {
+        const float4 v = row[k];
+        ar[2 * k] += (double)w * (double)v.x;
+        ai[2 * k] += (double)w * (double)v.y;
+        ar[2 * k + 1] += (double)w * (double)v.z;
+        ai[2 * k + 1] += (double)w * (double)v.w;
+      }
+    }
+  }
+  if (store) {
+    double2* gp = grid + gpt * g.nc + g.c0;
+#pragma unroll
+    for (int c = 0; c < CG; ++c) gp[c] = make_double2(ar[c], ai[c]);
+  }
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const uint2* __restrict__ hits, bool pw,
@@ -976,13 +1076,18 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
 
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
-                          const void* wsamp, double scale, bool user) {
+                          const void* wsamp, double scale, bool user, bool narrow) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
   const long long nthreads = p->nnz * (long long)LPS;
   const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
-  if (user)
+  if (narrow)   // float samples in the caller's layout out of a double grid
+    hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, true, float>), dim3(nb), dim3(256), 0,
+                       p->stream, p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off,
+                       p->d_off32, (const cplx<R>*)grid, (cplx<float>*)out, (const R*)wsamp,
+                       (R)scale, p->d_orig, kb_table_of(p), g, kp);
+  else if (user)
     hipLaunchKernelGGL((k_interp<R, CPL, LPS, COMP, FULL, true>), dim3(nb), dim3(256), 0, p->stream,
                        p->d_spos, p->d_fcode, p->d_base, (const double*)p->d_off, p->d_off32,
                        (const cplx<R>*)grid, (cplx<R>*)out, (const R*)wsamp, (R)scale, p->d_orig,
@@ -1012,7 +1117,8 @@ static long double kb_large_ld(long double q, long double alpha) {
 }
 
 // kb_tab_d's table: on each interval the polynomial through the values at
-// kKbCoef Chebyshev nodes (Vandermonde system in t, solved in long double).
+// kKbCoef Chebyshev nodes (Vandermonde system in t, solved in long double),
+// stored coefficient-major (coefficient j of interval i at j*kKbIntervals + i).
 int build_kb_table(nufft3d_plan* p) {
   if (p->d_kbtab) return NUFFT3D_OK;
   std::vector<double> tab((size_t)kKbIntervals * kKbCoef, 0.0);
@@ -1040,7 +1146,7 @@ int build_kb_table(nufft3d_plan* p) {
           for (int j = c; j <= kKbCoef; ++j) V[r][j] -= f * V[c][j];
         }
       }
-      for (int j = 0; j < kKbCoef; ++j) tab[(size_t)i * kKbCoef + j] = (double)(V[j][kKbCoef] / V[j][j]);
+      for (int j = 0; j < kKbCoef; ++j) tab[(size_t)j * kKbIntervals + i] = (double)(V[j][kKbCoef] / V[j][j]);
     }
   }
   N3D_HIP(hipMalloc(&p->d_kbtab, sizeof(double) * tab.size()));
@@ -1170,7 +1276,20 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   const uint2* hw = !(p->tune_flags & 128) ? p->d_hitw : nullptr;
   const uint2* hits = hw ? hw : p->d_hit;
   const bool pw = hw != nullptr;
-  if (kTileable && !wide && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
+  constexpr bool kTileW = sizeof(R) == 4 && COMP && FULL && (LPS * CPL == 8 || LPS * CPL == 4);
+  if (kTileW && wide && p->wl.d_went && hw && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
+      p->wl.rmax == ds.rmax) {
+    // the dense points' segments in the same launch (512 lanes: LPS per
+    // segment), unless tune flag 32 asks for their own launch
+    merged = ds.nhseg > 0 && !(p->tune_flags & 32);
+    const int nhb = merged ? (int)((((long long)ds.nhseg * LPS + 511) / 512 + 7) / 8 * 8) : 0;
+    ProfScope ps(p, "k_spread_tile_w");
+    hipLaunchKernelGGL((k_spread_tile_w<(kTileW ? CPL : 4), (kTileW ? LPS : 2)>),
+                       dim3((unsigned)(nhb + p->nbrick)), dim3(512), 0, p->stream, p->d_rowptr,
+                       p->wl.d_went, p->wl.d_beg, p->wl.d_end, p->wl.d_samp, (const float2*)samp,
+                       (double2*)grid, p->d_brick, ds.rmax, g, hw, ds.d_hseg, ds.nhseg, nhb,
+                       (double2*)partial, kp);
+  } else if (kTileable && !wide && p->bl.d_ent && (p->tune_flags & 8) && !wsamp && cg == CPL * LPS &&
       p->bl.rmax == ds.rmax) {
     ProfScope ps(p, "k_spread_tile");
     hipLaunchKernelGGL((k_spread_tile<(kTileable ? CPL * LPS : 8), COMP>), dim3((unsigned)p->nbrick),
@@ -1226,10 +1345,12 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
 
 // coil groups of up to coil_group (16 or 8); fp32 groups of more than 4
 // coils put 2 or 4 lanes on one sample / grid point (4 coils each), fp64
-// groups of more than 2 coils put 4 lanes on it (1, 2 or 4 coils each)
+// groups of more than 2 coils put 4 lanes on it (1, 2 or 4 coils each).
+// split: float samples with a double grid (spread: read, interp: written)
 template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
-                       const void* wsamp, double scale, bool user = false, bool wide = false) {
+                       const void* wsamp, double scale, bool user = false, bool split = false) {
+  const bool wide = split;
   const int gmax = p->coil_group;
   for (int c0 = 0; c0 < nc; c0 += gmax) {
     const int cg = nc - c0 < gmax ? nc - c0 : gmax;
@@ -1238,7 +1359,8 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
 #define N3D_GO2(CPL, LPS, FULL)                                                                  \
     do {                                                                                         \
       if constexpr (SPREAD) launch_spread<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, partial, wsamp, wide); \
-      else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale, user);  \
+      else launch_interp<R, CPL, LPS, COMP, FULL>(p, in, nc, c0, cg, out, wsamp, scale, user, \
+                                                  split && sizeof(R) == 8);                     \
     } while (0)
 #define N3D_GO(CPL, LPS)                                                                         \
     do {                                                                                         \
@@ -1246,7 +1368,7 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
       else N3D_GO2(CPL, LPS, false);                                                             \
     } while (0)
     if constexpr (!SPREAD) {
-      if (p->tile_interp && p->d_piece && p->npiece > 0) {
+      if (p->tile_interp && p->d_piece && p->npiece > 0 && !split) {
         launch_interp_tile<R, COMP>(p, in, nc, c0, cg, out, wsamp, scale, user);
         continue;
       }
@@ -1271,12 +1393,12 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
 
 template <typename R>
 static int interp_t(nufft3d_plan* p, const void* grid, int nc, void* out, const void* wsamp,
-                    double scale, bool user) {
+                    double scale, bool user, bool narrow = false) {
   if (p->nnz == 0) return NUFFT3D_OK;
   if (sizeof(R) == 4 && p->compensated)
     run_groups<R, true, false>(p, grid, nc, out, nullptr, wsamp, scale, user);
   else
-    run_groups<R, false, false>(p, grid, nc, out, nullptr, wsamp, scale, user);
+    run_groups<R, false, false>(p, grid, nc, out, nullptr, wsamp, scale, user, narrow);
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }
@@ -1286,7 +1408,14 @@ int permute_out(nufft3d_plan* p, const void* ks, int nc, void* k_user);
 // user_layout: the caller's [M nc] array (rows of out-of-bounds samples are
 // left to zero_oob); otherwise the sorted coil-interleaved layout
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
-           const void* wsamp, double scale, bool allow_ws) {
+           const void* wsamp, double scale, bool allow_ws, int ep) {
+  if (ep == 1 && p->prec == 2) {
+    if (!user_layout) {
+      set_error("float samples from an fp64 plan are written in the caller's layout only");
+      return NUFFT3D_ERR_INVALID;
+    }
+    return interp_t<double>(p, grid, nc, out, wsamp, scale, true, true);
+  }
   if (user_layout && (p->tune_flags & 16) && allow_ws) {
     // interpolate into the sorted layout, then scatter rows (separate pass)
     const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);

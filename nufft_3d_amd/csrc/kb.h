@@ -84,10 +84,12 @@ __device__ __forceinline__ double kb_tab_d(double dist2, const double* tab, cons
   int i = (int)x;
   i = i < kKbIntervals - 1 ? i : kKbIntervals - 1;
   const double t = 2.0 * (x - (double)i) - 1.0;
-  const double* c = tab + i * kKbCoef;
-  double g = c[kKbDeg];
+  // coefficient-major table: coefficient k of interval i at k*kKbIntervals + i,
+  // so lanes reading different intervals hit different LDS banks
+  const double* c = tab + i;
+  double g = c[kKbDeg * kKbIntervals];
 #pragma unroll
-  for (int k = kKbDeg - 1; k >= 0; --k) g = __builtin_fma(g, t, c[k]);
+  for (int k = kKbDeg - 1; k >= 0; --k) g = __builtin_fma(g, t, c[k * kKbIntervals]);
   return q > p.qb ? small : g;
 }
 

@@ -99,6 +99,22 @@ struct BrickLists {
   int max_len = 0;
 };
 
+// Lists of the LDS-staged split-precision adjoint (fp32 samples into an fp64
+// grid), built one z-slab of bricks at a time so that any hit count works:
+// brick b stages samples samp[beg[b] .. end[b]) (ascending ids), and every
+// hit of a point with at most rmax hits holds (slot in its brick's list,
+// fp32 weight bits) in the plan's hit order.
+struct WideLists {
+  int rmax = 0;
+  long long* d_beg = nullptr;   // [nbrick]
+  long long* d_end = nullptr;   // [nbrick]
+  int* d_samp = nullptr;        // [total]
+  uint2* d_went = nullptr;      // [nhits] (entries of dense points unused)
+  long long total = 0;
+  long long max_len = 0;
+  bool tried = false;
+};
+
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
@@ -192,6 +208,7 @@ struct nufft3d_plan {
   nufft3d::BrickLists bl;           // LDS-staged spread lists over the same bricks
   std::vector<int> h_brick_of;      // brick (x,y,z) -> traversal index, for building bl
   bool bl_tried = false;
+  nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
   int max_hits = 0;
 
   // LDS-tiled forward interpolation (3-D plans with a 7-point footprint,
@@ -254,8 +271,10 @@ int ensure_host(Workspace& w, size_t bytes);
 
 // grid.hip
 // allow_ws = false never takes the plan's workspaces (the direct caller-row write)
+// ep = 1 on an fp64-arithmetic plan: the samples are written as float
+// (caller layout only), each the double result rounded once
 int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout,
-           const void* wsamp, double scale, bool allow_ws = true);
+           const void* wsamp, double scale, bool allow_ws = true, int ep = 0);
 // ep = 1 on an fp64-arithmetic plan: float samples, the plan's float
 // weight table, double accumulation into a double grid (the split adjoint)
 int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layout,
@@ -270,10 +289,12 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
 int fft_setup_axis(nufft3d_plan* p, int axis);
 // deap, pad, FFT; t1/t2 = caller-given intermediates ([K1 N2 N3 nc], [K1 K2 N3 nc]
 // complex of the arithmetic precision) or NULL for the plan's workspaces
+// ep = 1 on an fp64-arithmetic plan: the image is float (read and widened,
+// or rounded once on the way out; no add term)
 int fft_forward(nufft3d_plan* p, const void* img, int nc, void* grid, void* t1 = nullptr,
-                void* t2 = nullptr);
+                void* t2 = nullptr, int ep = 0);
 int fft_adjoint(nufft3d_plan* p, const void* grid, int nc, void* img, double scale,
-                const void* img_add, double add_coef);                    // IFFT, crop, deap
+                const void* img_add, double add_coef, int ep = 0);        // IFFT, crop, deap
 int fft_plain(nufft3d_plan* p, const void* in, int nc, void* out);       // unshifted N-size FFT
 
 // solve.hip

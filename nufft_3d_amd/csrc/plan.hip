@@ -420,6 +420,62 @@ __global__ void k_bl_ent(const long long* __restrict__ rowptr, const uint2* __re
   }
 }
 
+// Split-adjoint lists, one z-slab of bricks (grid points [g0, g1), hits
+// [e0, ...)) at a time: keys (brick << 32) | sample of the light points'
+// hits, all ones for the dense points'
+__global__ void k_wl_keys(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
+                          long long g0, long long g1, long long e0, int K1, int K2,
+                          const int* __restrict__ brick_of, int bx, int by, int rmax,
+                          unsigned long long* __restrict__ keys) {
+  const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const unsigned long long br =
+      (unsigned long long)brick_of[((long long)(z >> 3) * by + (y >> 3)) * bx + (x >> 3)];
+  const bool dense = b - a > rmax;
+  for (long long e = a; e < b; ++e) keys[e - e0] = dense ? ~0ull : (br << 32) | hits[e].x;
+}
+
+// the slab's unique keys u[0, n) (ascending, light only) -> samp[off + i],
+// and each brick's range [beg, end) in samp
+__global__ void k_wl_lists(const unsigned long long* __restrict__ u, long long n, long long off,
+                           int* __restrict__ samp, long long* __restrict__ beg,
+                           long long* __restrict__ end) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long b = u[i] >> 32;
+  samp[off + i] = (int)(u[i] & 0xffffffffull);
+  if (i == 0 || (u[i - 1] >> 32) != b) beg[b] = off + i;
+  if (i == n - 1 || (u[i + 1] >> 32) != b) end[b] = off + i + 1;
+}
+
+// per light hit of the slab: (slot of its sample in the brick's list, weight bits)
+__global__ void k_wl_ent(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
+                         long long g0, long long g1, int K1, int K2,
+                         const int* __restrict__ brick_of, int bx, int by, int rmax,
+                         const int* __restrict__ samp, const long long* __restrict__ beg,
+                         const long long* __restrict__ end, uint2* __restrict__ went) {
+  const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  if (b - a > rmax) return;
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const int br = brick_of[((long long)(z >> 3) * by + (y >> 3)) * bx + (x >> 3)];
+  const long long base = beg[br], len = end[br] - base;
+  long long lo = 0;   // hits are in ascending sample order: search from the last slot
+  for (long long e = a; e < b; ++e) {
+    const uint2 h = hits[e];
+    long long l = lo, r = len;
+    while (l < r) {
+      const long long m = (l + r) >> 1;
+      if ((unsigned)samp[base + m] < h.x) l = m + 1; else r = m;
+    }
+    lo = l;
+    went[e] = make_uint2((unsigned)lo, h.y);
+  }
+}
+
 __global__ void k_off32(const double* __restrict__ off, long long nnz, float4* __restrict__ o32) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
@@ -685,6 +741,136 @@ static int ensure_brick_lists(nufft3d_plan* p) {
   p->bl_tried = true;
   const int bx = (p->K[0] + 7) / 8, by = (p->K[1] + 7) / 8;
   return build_brick_lists(p, p->h_brick_of, bx, by, p->dense[1].rmax);
+}
+
+// WideLists of a split-precision plan (points with at most rmax hits of the
+// plan's weight table): per z-slab of bricks, sort the (brick, sample) keys
+// of the slab's hits and keep the unique ones (pass 1 counts, pass 2 writes
+// the lists and each hit's slot). A slab of 2^31 hits or more leaves the
+// lists unbuilt (the gather spread is used).
+static int build_wide_lists(nufft3d_plan* p, const std::vector<int>& brick_of, int bx, int by,
+                            int rmax) {
+  WideLists& wl = p->wl;
+  const long long nh = p->nhits;
+  if (nh <= 0 || !p->d_hitw || p->K[2] == 1 || p->nbrick <= 0) return NUFFT3D_OK;
+  const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
+  const long long plane = (long long)K1 * K2;
+  const int nbz = (K3 + 7) / 8;
+  std::vector<long long> eb(nbz + 1);
+  for (int bz = 0; bz <= nbz; ++bz) {
+    const long long gz = std::min<long long>(8LL * bz, K3) * plane;
+    N3D_HIP(hipMemcpy(&eb[bz], p->d_rowptr + gz, sizeof(long long), hipMemcpyDeviceToHost));
+  }
+  long long smax = 0;
+  for (int bz = 0; bz < nbz; ++bz) smax = std::max(smax, eb[bz + 1] - eb[bz]);
+  if (smax >= 0x7fffffffLL) return NUFFT3D_OK;
+  int bbits = 1;
+  while ((1LL << bbits) <= p->nbrick) ++bbits;
+  int* d_bof = nullptr;
+  unsigned long long *keys = nullptr, *skeys = nullptr;
+  int* d_nsel = nullptr;
+  void* tmp = nullptr;
+  auto cleanup = [&]() {
+    if (d_bof) (void)hipFree(d_bof);
+    if (keys) (void)hipFree(keys);
+    if (skeys) (void)hipFree(skeys);
+    if (d_nsel) (void)hipFree(d_nsel);
+    if (tmp) (void)hipFree(tmp);
+  };
+  auto drop = [&]() {
+    for (void* q : {(void*)wl.d_beg, (void*)wl.d_end, (void*)wl.d_samp, (void*)wl.d_went})
+      if (q) (void)hipFree(q);
+    wl.d_beg = nullptr; wl.d_end = nullptr; wl.d_samp = nullptr; wl.d_went = nullptr;
+  };
+  auto fail = [&](const char* what) {
+    cleanup();
+    drop();
+    (void)hipGetLastError();
+    set_error(std::string("split-adjoint list construction: ") + what);
+    return NUFFT3D_ERR_NOMEM;
+  };
+  const size_t nk = (size_t)std::max<long long>(smax, 1);
+  if (hipMalloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
+      hipMalloc(&keys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      hipMalloc(&skeys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      hipMalloc(&d_nsel, sizeof(int)) != hipSuccess)
+    return fail("allocation failed");
+  N3D_HIP(hipMemcpy(d_bof, brick_of.data(), sizeof(int) * brick_of.size(), hipMemcpyHostToDevice));
+  size_t tb1 = 0, tb2 = 0;
+  N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nk, 0, 32 + bbits,
+                                            p->stream));
+  N3D_HIP(hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nk, p->stream));
+  if (hipMalloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
+  // one slab: its unique light keys left in keys[0, return value)
+  auto slab = [&](int bz, long long& nlight) -> int {
+    const long long g0 = 8LL * bz * plane, g1 = std::min<long long>(8LL * bz + 8, K3) * plane;
+    const int n = (int)(eb[bz + 1] - eb[bz]);
+    nlight = 0;
+    if (n == 0) return NUFFT3D_OK;
+    hipLaunchKernelGGL(k_wl_keys, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
+                       p->d_hitw, g0, g1, eb[bz], K1, K2, d_bof, bx, by, rmax, keys);
+    N3D_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, n, 0, 32 + bbits, p->stream));
+    N3D_HIP(hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, n, p->stream));
+    int nsel = 0;
+    N3D_HIP(hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    N3D_HIP(hipStreamSynchronize(p->stream));
+    unsigned long long last = 0;
+    if (nsel > 0) N3D_HIP(hipMemcpy(&last, keys + nsel - 1, sizeof(last), hipMemcpyDeviceToHost));
+    nlight = (nsel > 0 && last == ~0ull) ? nsel - 1 : nsel;   // the dense points' all-ones key
+    return NUFFT3D_OK;
+  };
+  long long total = 0;
+  std::vector<long long> nl(nbz);
+  for (int bz = 0; bz < nbz; ++bz) {
+    if (slab(bz, nl[bz])) return fail("sort failed");
+    total += nl[bz];
+  }
+  if (hipMalloc(&wl.d_beg, sizeof(long long) * p->nbrick) != hipSuccess ||
+      hipMalloc(&wl.d_end, sizeof(long long) * p->nbrick) != hipSuccess ||
+      hipMalloc(&wl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess ||
+      hipMalloc(&wl.d_went, sizeof(uint2) * nh) != hipSuccess)
+    return fail("allocation failed");
+  N3D_HIP(hipMemsetAsync(wl.d_beg, 0, sizeof(long long) * p->nbrick, p->stream));
+  N3D_HIP(hipMemsetAsync(wl.d_end, 0, sizeof(long long) * p->nbrick, p->stream));
+  N3D_HIP(hipMemsetAsync(wl.d_went, 0, sizeof(uint2) * nh, p->stream));
+  long long off = 0;
+  for (int bz = 0; bz < nbz; ++bz) {
+    long long n2 = 0;
+    if (slab(bz, n2) || n2 != nl[bz]) return fail("sort failed");
+    if (n2 > 0)
+      hipLaunchKernelGGL(k_wl_lists, dim3(nblocks(n2)), dim3(256), 0, p->stream,
+                         (const unsigned long long*)keys, n2, off, wl.d_samp, wl.d_beg, wl.d_end);
+    const long long g0 = 8LL * bz * plane, g1 = std::min<long long>(8LL * bz + 8, K3) * plane;
+    hipLaunchKernelGGL(k_wl_ent, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
+                       p->d_hitw, g0, g1, K1, K2, d_bof, bx, by, rmax, (const int*)wl.d_samp,
+                       (const long long*)wl.d_beg, (const long long*)wl.d_end, wl.d_went);
+    N3D_HIP(hipGetLastError());
+    off += n2;
+  }
+  std::vector<long long> hb(p->nbrick), he(p->nbrick);
+  N3D_HIP(hipMemcpyAsync(hb.data(), wl.d_beg, sizeof(long long) * p->nbrick, hipMemcpyDeviceToHost,
+                         p->stream));
+  N3D_HIP(hipMemcpyAsync(he.data(), wl.d_end, sizeof(long long) * p->nbrick, hipMemcpyDeviceToHost,
+                         p->stream));
+  N3D_HIP(hipStreamSynchronize(p->stream));
+  long long mx = 0;
+  for (long long b = 0; b < p->nbrick; ++b) mx = std::max(mx, he[b] - hb[b]);
+  cleanup();
+  wl.rmax = rmax;
+  wl.total = total;
+  wl.max_len = mx;
+  return NUFFT3D_OK;
+}
+
+// WideLists when asked for (tune flag 8 at plan creation): 8 B per hit and a
+// few seconds of sorting at 2.3G hits, for a kernel measured no faster than
+// the gather spread on radial trajectories (about 10% faster on uniform
+// density), so the autotuner times it only when the lists exist
+static int ensure_wide_lists(nufft3d_plan* p) {
+  if (p->wl.d_went || p->wl.tried || p->h_brick_of.empty() || !split_precision(p)) return NUFFT3D_OK;
+  p->wl.tried = true;
+  const int bx = (p->K[0] + 7) / 8, by = (p->K[1] + 7) / 8;
+  return build_wide_lists(p, p->h_brick_of, bx, by, p->dense[1].rmax);
 }
 
 // Pieces of the tiled interpolation: runs of equal sub-bin key in the sorted
@@ -984,6 +1170,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     if (p->tune_flags & 8) N3D_TRY(ensure_brick_lists(p));
   }
   N3D_TRY(build_hit_weights(p));
+  if (p->tune_flags & 8) N3D_TRY(ensure_wide_lists(p));
   N3D_TRY(build_kb_table(p));
 
   // deapodization (deap.m) and FFT twiddles
@@ -1026,7 +1213,8 @@ static void destroy_plan(nufft3d_plan* p) {
   void* ptrs[] = {p->d_orig, p->d_oob, p->d_inv, p->d_base, p->d_off, p->d_U, p->d_dens,
                   p->d_spos, p->d_fcode, p->d_rowptr, p->d_hit, p->d_hitw,
                   p->d_off32, p->d_piece, p->d_kbtab, p->dense[0].d_hseg, p->dense[0].d_hpt, p->dense[1].d_hseg,
-                  p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent};
+                  p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent,
+                  p->wl.d_beg, p->wl.d_end, p->wl.d_samp, p->wl.d_went};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (int a = 0; a < 3; ++a)
@@ -1335,6 +1523,9 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
     if (p->bl.d_ent)
       printf("\tspread bricks %lld: %lld staged samples (%.2f/sample), longest list %d\n",
              p->nbrick, p->bl.total, p->nnz ? (double)p->bl.total / p->nnz : 0.0, p->bl.max_len);
+    if (p->wl.d_went)
+      printf("\tsplit-adjoint bricks %lld: %lld staged samples (%.2f/sample), longest list %lld\n",
+             p->nbrick, p->wl.total, p->nnz ? (double)p->wl.total / p->nnz : 0.0, p->wl.max_len);
   }
   *out = p;
   return NUFFT3D_OK;
@@ -1366,6 +1557,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += 4 * p->nbrick;
   if (p->bl.d_ptr) b += 8 * (p->nbrick + 1) + 4 * std::max<int64_t>(1, p->bl.total);
   if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
+  if (p->wl.d_went) b += 16 * p->nbrick + 4 * std::max<int64_t>(1, p->wl.total) + 8 * p->nhits;
   for (int a = 0; a < 3; ++a)
     for (const FftAxis* x : {&p->ax[a], &p->axN[a]})
       if (x->d_tw) b += 2 * rsz * (x->bs ? x->L + 3 * x->L2 : x->L);   // roots (+ chirp, spectra)
@@ -1593,25 +1785,47 @@ static int adjoint_split(nufft3d_plan* p, const void* k, int nc, void* x) {
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
-  Workspace tmp_out, tmp_io;
+  Workspace tmp_out;
   Staged ki, xo;
   N3D_TRY(stage_in(p, k, nk * sizeof(float2), p->ws_img, ki));
-  int st = stage_out_c(p, x, nimg, tmp_out, xo);
+  int st = stage_out(p, x, nimg * sizeof(float2), tmp_out, xo);
   if (!st) st = ensure(p->ws_samp, (size_t)(p->nnz > 0 ? p->nnz : 1) * nc * sizeof(float2));
   if (!st) st = ensure(p->ws_grid, ngrid * sizeof(double2));
   if (!st) st = permute_in(p, ki.dev, nc, p->ws_samp.p, nullptr, 1);
   if (!st) st = spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr, 1);              // aNUFT.m:5
-  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, xo.dev, 1.0 / sqrt((double)p->nnz), nullptr, 0.0);
-  if (!st) st = finish_out_c(p, x, nimg, xo, tmp_io);
+  // the fp64 inverse transform writes the float image (rounded once)
+  if (!st) st = fft_adjoint(p, p->ws_grid.p, nc, xo.dev, 1.0 / sqrt((double)p->nnz), nullptr, 0.0, 1);
+  if (!st) st = finish_out(p, x, nimg * sizeof(float2), xo);
   release(tmp_out);
-  release(tmp_io);
   if (!st && ki.host) N3D_HIP(hipStreamSynchronize(p->stream));
+  return st;
+}
+
+// fNUFT of an fp32-array plan computing in fp64: the x pass reads and widens
+// the float image, the interpolation writes float samples (each double
+// result rounded once), with no conversion passes around them
+static int forward_split(nufft3d_plan* p, const void* x, int nc, void* k) {
+  const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
+  const size_t nk = (size_t)p->M * nc;
+  const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2] * nc;
+  Workspace tmp_out;
+  Staged xi, ko;
+  N3D_TRY(stage_in(p, x, nimg * sizeof(float2), p->ws_img, xi));
+  int st = stage_out(p, k, nk * sizeof(float2), tmp_out, ko);
+  if (!st) st = ensure(p->ws_grid, ngrid * sizeof(double2));
+  if (!st) st = fft_forward(p, xi.dev, nc, p->ws_grid.p, nullptr, nullptr, 1);          // fNUFT.m:5-6
+  if (!st) st = interp(p, p->ws_grid.p, nc, ko.dev, 1, nullptr, 1.0 / sqrt((double)p->nnz), true, 1);
+  if (!st) st = zero_oob(p, ko.dev, nc, 1);
+  if (!st) st = finish_out(p, k, nk * sizeof(float2), ko);
+  release(tmp_out);
+  if (!st && xi.host) N3D_HIP(hipStreamSynchronize(p->stream));
   return st;
 }
 
 int nufft3d_forward(nufft3d_plan* p, const void* x, int32_t nc, void* k) {
   N3D_PLAN_GUARD(p);
   if (!x || !k || nc <= 0) { set_error("invalid x/k/nc"); return NUFFT3D_ERR_INVALID; }
+  if (p->prec == 2 && p->io_prec == 1 && !(p->tune_flags & (16 | 512))) return forward_split(p, x, nc, k);
   const size_t esz = cp_esz(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2] * nc;
   const size_t nk = (size_t)p->M * nc;
@@ -2102,7 +2316,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     N3D_TRY(spread(p, ks.p, nc, p->ws_grid.p, 0, nullptr));
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
-  if (!st) st = ensure_brick_lists(p);
+  if (!st && !split) st = ensure_brick_lists(p);
   if (st) { release(xin); release(xout); release(ks); release(ks32); return st; }
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
   // spread (plans with brick lists)
@@ -2113,7 +2327,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
       for (int xcd : {0, 32, -1})
         for (int brick : {1, 0, 2}) {
           if (group == 8 && nc <= 8) continue;   // same launches as 16
-          if (brick == 2 && !p->bl.d_ent) continue;
+          if (brick == 2 && !(split ? (void*)p->wl.d_went : (void*)p->bl.d_ent)) continue;
           cands.push_back({fftb, group, xcd, brick, 0.f});
         }
   auto flags_for = [&](int brick, int keep) {

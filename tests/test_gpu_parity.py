@@ -571,3 +571,29 @@ def test_split_precision_adjoint_matches_fp64_arithmetic(radial, nc):
     x = _rand((24, 24, 24, nc), 97).astype(np.complex64)
     assert np.array_equal(split.fNUFT(x), full.fNUFT(x))
     assert np.array_equal(split.iNUFT(k, 4, 0.0), full.iNUFT(k, 4, 0.0))
+
+
+@pytest.mark.parametrize("radial,nc", [(1, 8), (1, 4), (0, 8), (1, 6)])
+def test_lds_tiled_split_adjoint_equals_gather_split(radial, nc):
+    # the split-precision adjoint (fp32 samples into an fp64 grid) with the
+    # samples staged in LDS per brick walks each point's hits in the same
+    # order, with the same float weights and double sums, as the gather
+    # kernel: bitwise the same image (coil groups of 8 run the staged
+    # kernel; 4 and 6 the gather); the k-space centre's dense points go to
+    # the dense-point kernels in both
+    N = 24
+    om = synth.koosh_ball(N, 300, 2 * N)
+    kw = dict(J=6, u=1.25, radial=radial, gpu=1)
+    k = _rand((om.shape[1], nc), 83).astype(np.complex64)
+    tiled = _plan_with_env({"NUFFT3D_TUNE": "10"}, om, N, **kw)
+    gather = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    assert tiled.compute_precision == 2
+    got = {}
+    ran = _kernels_run(tiled, lambda: got.setdefault("x", tiled.aNUFT(k)))
+    assert ("k_spread_tile_w" in ran) == (nc == 8), ran
+    assert "k_spread_hreduce" in ran
+    ran_g = _kernels_run(gather, lambda: got.setdefault("y", gather.aNUFT(k)))
+    assert "k_spread_tile_w" not in ran_g
+    assert np.array_equal(got["x"], got["y"])
+    orc = O.NufftOracle(om, N, J=6, u=1.25, radial=radial)
+    assert rel(got["x"], orc.aNUFT(k.astype(np.complex128))) < TOL32_OP

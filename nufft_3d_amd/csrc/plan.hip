@@ -476,6 +476,13 @@ __global__ void k_wl_ent(const long long* __restrict__ rowptr, const uint2* __re
   }
 }
 
+// first sample of each dense-point segment (its launch order key)
+__global__ void k_seg_first(const uint2* __restrict__ hits, const long long* __restrict__ starts,
+                            int n, unsigned* __restrict__ first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) first[i] = hits[starts[i]].x;
+}
+
 __global__ void k_off32(const double* __restrict__ off, long long nnz, float4* __restrict__ o32) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
@@ -619,7 +626,45 @@ static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long
     }
     pos += c;
   }
-  if (!hseg.empty()) {
+  if (!hseg.empty() && !(p->tune_flags & 1024)) {
+    // launch order by the segment's first sample: segments running together
+    // read nearby sample rows (the k-space centre's samples stay in L2
+    // instead of being fetched again by every point's segment of the same
+    // index); the partial slots, hence the sums, are unchanged
+    const int n = (int)hseg.size();
+    std::vector<long long> st(n);
+    for (int i = 0; i < n; ++i) st[i] = hseg[i].start;
+    long long* d_st = nullptr;
+    unsigned* d_first = nullptr;
+    if (hipMalloc(&d_st, sizeof(long long) * n) != hipSuccess ||
+        hipMalloc(&d_first, sizeof(unsigned) * n) != hipSuccess) {
+      if (d_st) (void)hipFree(d_st);
+      (void)hipGetLastError();
+      set_error("dense segment ordering: allocation failed");
+      return NUFFT3D_ERR_NOMEM;
+    }
+    std::vector<unsigned> first(n);
+    int st2 = hipMemcpy(d_st, st.data(), sizeof(long long) * n, hipMemcpyHostToDevice) != hipSuccess;
+    if (!st2) {
+      hipLaunchKernelGGL(k_seg_first, dim3((n + 255) / 256), dim3(256), 0, p->stream, p->d_hit,
+                         (const long long*)d_st, n, d_first);
+      st2 = hipStreamSynchronize(p->stream) != hipSuccess ||
+            hipMemcpy(first.data(), d_first, sizeof(unsigned) * n, hipMemcpyDeviceToHost) != hipSuccess;
+    }
+    (void)hipFree(d_st);
+    (void)hipFree(d_first);
+    if (st2) {
+      (void)hipGetLastError();
+      set_error("dense segment ordering failed");
+      return NUFFT3D_ERR_DEVICE;
+    }
+    std::vector<std::pair<unsigned long long, int>> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = {((unsigned long long)first[i] << 32) | (unsigned)i, i};
+    std::sort(ord.begin(), ord.end());
+    std::vector<HeavySeg> h2(n);
+    for (int i = 0; i < n; ++i) h2[i] = hseg[ord[i].second];
+    hseg.swap(h2);
+  } else if (!hseg.empty()) {
     int bits = 1;
     while ((1 << bits) < std::max((p->K[0] + 7) / 8, std::max((p->K[1] + 7) / 8, (p->K[2] + 7) / 8)))
       ++bits;

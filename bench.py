@@ -513,7 +513,8 @@ def run_pair(args, wname, full=True):
         rep = plan.autotune(nc, reps=3)
         tuned = dict(rep["candidates"][rep["best"]], tune_s=round(time.perf_counter() - t1, 3),
                      default_ms=rep["candidates"][0]["ms_per_pair"],
-                     z_threads=rep.get("z_threads"), z_threads_ms=rep.get("z_threads_ms"))
+                     z_threads=rep.get("z_threads"), z_threads_ms=rep.get("z_threads_ms"),
+                     x_threads=rep.get("x_threads"), x_threads_ms=rep.get("x_threads_ms"))
         log("autotune:", tuned)
     log("plan %.3f s  K=%s nnz=%d compute precision %d" % (plan_s, plan.K, plan.nnz,
                                                          plan.compute_precision))

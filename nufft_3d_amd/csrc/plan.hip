@@ -2432,6 +2432,25 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     zt_best = st ? keep_zt : (zt_ms[1] < zt_ms[0] ? 0 : 256);
     p->fft_zthreads = zt_best;
   }
+  // x-pass work-group size, timed likewise: 256 or 512 threads
+  float xt_ms[2] = {0.f, 0.f};
+  int xt_best = p->fft_xthreads;
+  if (!st && best >= 0) {
+    const int keep_xt = p->fft_xthreads;
+    for (int xi = 0; xi < 2 && !st; ++xi) {
+      p->fft_xthreads = xi ? 512 : 256;
+      st = pair();
+      if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      for (int r = 0; r < reps && !st; ++r) st = pair();
+      if (!st && hipEventRecord(e1, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      if (!st && hipEventSynchronize(e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      float ms = 0;
+      if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+      xt_ms[xi] = ms / reps;
+    }
+    xt_best = st ? keep_xt : (xt_ms[1] < xt_ms[0] ? 512 : 256);
+    p->fft_xthreads = xt_best;
+  }
   if (st == NUFFT3D_ERR_DEVICE) { (void)hipGetLastError(); set_error("device error while autotuning"); }
   if (!st && report) {
     std::string js = "{\"nc\": " + std::to_string(nc) + ", \"candidates\": [";
@@ -2444,9 +2463,10 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
                spread_kind[cands[i].brick], cands[i].ms);
       js += b;
     }
-    char zb[128];
-    snprintf(zb, sizeof zb, ", \"z_threads\": %d, \"z_threads_ms\": [%.5f, %.5f]", zt_best,
-             zt_ms[0], zt_ms[1]);
+    char zb[256];
+    snprintf(zb, sizeof zb, ", \"z_threads\": %d, \"z_threads_ms\": [%.5f, %.5f], "
+             "\"x_threads\": %d, \"x_threads_ms\": [%.5f, %.5f]", zt_best, zt_ms[0], zt_ms[1],
+             xt_best, xt_ms[0], xt_ms[1]);
     js += "], \"best\": " + std::to_string(best) + zb + "}";
     if ((int64_t)js.size() + 1 > len) { set_error("report buffer too small"); st = NUFFT3D_ERR_INVALID; }
     else std::memcpy(report, js.c_str(), js.size() + 1);

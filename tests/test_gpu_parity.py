@@ -280,6 +280,7 @@ def test_autotune_picks_a_measured_setting_and_keeps_results():
     assert {c["xcd_chunk"] for c in cands} == {0, 32, -1}
     assert {c["spread"] for c in cands} == {"gather", "gather-bricks", "lds-bricks"}
     assert rep["z_threads"] in (0, 256) and min(rep["z_threads_ms"]) > 0
+    assert rep["x_threads"] in (256, 512) and min(rep["x_threads_ms"]) > 0
     assert all(c["ms_per_pair"] > 0 for c in cands)
     assert cands[rep["best"]]["ms_per_pair"] == min(c["ms_per_pair"] for c in cands)
     after = p.aNUFT(p.fNUFT(x))

@@ -48,7 +48,9 @@ __device__ __forceinline__ long long xcd_block(int chunk) {
   return xcd_block_of(chunk, blockIdx.x, gridDim.x);
 }
 
+// a base in [0, K) plus an offset of at most 2K, wrapped into [0, K)
 __device__ __forceinline__ int wrapk(int g, int K) {
+  g = g < 0 ? g + K : (g >= K ? g - K : g);
   return g < 0 ? g + K : (g >= K ? g - K : g);
 }
 

@@ -302,7 +302,10 @@ __device__ __forceinline__ bool hit_test(double tx, double ty, double tz, const 
   return 4.0 * dx2 <= g.J2 && 4.0 * dy2 <= g.J2 && 4.0 * dz2 <= g.J2;
 }
 
-__device__ __forceinline__ int wrapg(int v, int K) { return v < 0 ? v + K : (v >= K ? v - K : v); }
+__device__ __forceinline__ int wrapg(int v, int K) {   // mod(v, K) (nufft_3d.m:178-180)
+  const int r = v % K;
+  return r < 0 ? r + K : r;
+}
 
 __global__ void k_hits_count(const int4* __restrict__ base, const double* __restrict__ off,
                              long long nnz, HitGeom g, long long* __restrict__ cnt_s,
@@ -991,10 +994,16 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   }
   const int Wj = (int)ceil(p->J / 2);                                          // nufft_3d.m:154
   p->W[0] = Wj; p->W[1] = Wj; p->W[2] = p->N[2] == 1 ? 0 : Wj;
+  // grids narrower than the footprint 2*ceil(J/2)+1 (N = 2 or 3 on an axis)
+  // are fine: wrapped candidates that land on one grid point are separate
+  // hits whose sum is the reference's summed duplicate entry
+  // (nufft_3d.m:178-180,203,208). The gridding kernels wrap an offset
+  // candidate with two corrections of one period, which covers ceil(J/2)
+  // up to 2K.
   for (int a = 0; a < 3; ++a)
-    if (p->K[a] > 1 && p->K[a] < 2 * p->W[a] + 1) {
-      set_error("oversampled size K=" + std::to_string(p->K[a]) +
-                " is smaller than the kernel footprint 2*ceil(J/2)+1");
+    if (p->K[a] > 1 && p->W[a] > 2 * p->K[a]) {
+      set_error("kernel width J=" + std::to_string(p->J) + " spans more than two periods of the " +
+                "oversampled size K=" + std::to_string(p->K[a]));
       (void)hipFree(om);
       return NUFFT3D_ERR_UNSUPPORTED;
     }

@@ -598,3 +598,37 @@ def test_lds_tiled_split_adjoint_equals_gather_split(radial, nc):
     assert np.array_equal(got["x"], got["y"])
     orc = O.NufftOracle(om, N, J=6, u=1.25, radial=radial)
     assert rel(got["x"], orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
+
+
+@pytest.mark.parametrize("N,u,J,radial,tol64", [((16, 16, 2), 1.25, 6, 1, TOL64),
+                                                ((8, 2, 2), 1.5, 4, 0, TOL64),
+                                                ((8, 2, 2), 1, 4, 0, 1e-9),
+                                                ((12, 12, 2), 1.5, 6, 1, TOL64),
+                                                ((10, 10, 2), 2, 4, 1, TOL64)])
+def test_grids_narrower_than_the_footprint_against_oracle(N, u, J, radial, tol64):
+    # K < 2*ceil(J/2)+1 on an axis: wrapped candidates landing on one grid
+    # point are summed, as the reference's sparse() sums duplicate entries
+    # (nufft_3d.m:178-180,203,208). u = 1 (K = N = 2 on two axes, every
+    # candidate wrapping twice) has a deapodization range of 4.4e6, which
+    # amplifies rounding at the field-of-view edge (an 8^3 grid at u = 1 with
+    # no narrow axis measures 2.6e-12 against the oracle; this case 6.5e-11),
+    # so that case is held to eps x range = 1e-9 (the reference warns that
+    # u = 1 does not work well, nufft_3d.m:82)
+    rng = np.random.default_rng(12)
+    om = rng.uniform(-min(N) / 2, min(N) / 2 - 0.5, size=(3, 1500))
+    om[:2] = rng.uniform(-N[0] / 2, N[0] / 2 - 0.5, size=(2, 1500))
+    orc = O.NufftOracle(om, N, J=J, u=u, radial=radial)
+    assert min(orc.K) < 2 * math.ceil(J / 2) + 1
+    g64 = nufft_3d(om, N, J=J, u=u, radial=radial, gpu=2)
+    g32 = nufft_3d(om, N, J=J, u=u, radial=radial, gpu=1)
+    assert tuple(g64.K) == tuple(orc.K)
+    x = _rand(orc.N + (2,), 61)
+    k = _rand((orc.M, 2), 62)
+    assert rel(g64.fNUFT(x), orc.fNUFT(x)) < tol64
+    assert rel(g64.aNUFT(k), orc.aNUFT(k)) < tol64
+    assert rel(g32.fNUFT(x), orc.fNUFT(x)) < TOL32_OP
+    assert rel(g32.aNUFT(k), orc.aNUFT(k)) < TOL32_OP
+    raw = orc.fNUFT(x) + 0.01 * _rand((orc.M, 2), 63)
+    want = orc.iNUFT(raw, 5, 0.05)
+    assert rel(g64.iNUFT(raw, 5, 0.05), want) < max(1e-10, 100 * tol64)
+    assert rel(g32.iNUFT(raw, 5, 0.05), want) < TOL32

@@ -431,3 +431,27 @@ def test_discrep_matches_a_dense_restatement():
         damp = damp * disc / np.linalg.norm(Acols @ x - data[:, 0])
     got = o.discrep(data, stddev=stddev, iters=8, maxit=4 * n)
     assert got == pytest.approx(damp, rel=2e-3)
+
+
+@pytest.mark.parametrize("N,u,J,radial", [((8, 8, 2), 1.25, 6, 1), ((8, 2, 2), 1, 4, 0), ((6, 6, 2), 1.5, 6, 1)])
+def test_H_on_grids_narrower_than_the_footprint(N, u, J, radial):
+    # K < 2*ceil(J/2)+1 on an axis: candidates wrap onto the same grid point
+    # and the reference's sparse() sums those entries (nufft_3d.m:178-180,
+    # 203,208); the brute force sums every lattice point within J mapped
+    # mod K, so a dropped or doubled duplicate fails
+    rng = np.random.default_rng(8)
+    om = rng.uniform(-min(N) / 2, min(N) / 2 - 0.5, size=(3, 40))
+    o = O.NufftOracle(om, N, J=J, u=u, radial=radial)
+    assert min(o.K) < 2 * math.ceil(J / 2) + 1
+    bf = _brute_H(o.om_s, o.ok, o.K, o.J, o.alpha, radial)
+    H = o.H.tocsr()
+    dup = 0
+    for m in range(om.shape[1]):
+        row = H.getrow(m)
+        got = dict(zip(row.indices.tolist(), row.data.tolist()))
+        want = bf.get(m, {})
+        assert set(got) == set(want), m
+        for c, v in want.items():
+            assert got[c] == pytest.approx(v, rel=4e-6, abs=1e-7)
+        dup += sum(1 for c in got if float(np.float32(got[c])) != got[c])
+    assert dup > 0          # some entries are sums of several float weights

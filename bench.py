@@ -294,6 +294,35 @@ def run_serving(args):
         alone = serve.serve_one_at_a_time(be, sub)
         log("one at a time:", json.dumps(alone))
     value = st["throughput_msamples_coil_s"]
+    # aggregate roofline of the served stream: algorithmic bytes of every
+    # request's fwd+adj pairs (an iNUFT of maxit iterations counts maxit+1,
+    # as its work does) over the makespan
+    alg = 0
+    for r in reqs:
+        K = 2 * int(math.ceil(r.N * 2.0 / 2))
+        alg += r.pairs() * algorithmic_bytes(r.N, K, r.M, r.nc)["pair"]
+    ach = alg / st["makespan_s"] / 1e9
+    roof = {"bound": "hbm", "kernel": "all kernels of the served stream (aggregate)",
+            "achieved": round(ach, 2), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(ach / 8000.0, 4), "traffic": None,
+            "algorithmic_bytes_total": alg}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the FP64 oracle's fwd+adj pair (CSR H and H', NumPy FFT, 1 thread)
+        # on the stream's smallest trajectory, one coil
+        from oracle.nufft_oracle import NufftOracle
+        key, om0, N0 = min(((r.plan_key, r.om, r.N) for r in reqs), key=lambda t: t[1].shape[1])
+        orc = NufftOracle(om0, N0)
+        xc = be.images[N0].cpu().numpy().T.astype(np.complex128)[..., None]
+        t0 = time.perf_counter()
+        kc = orc.fNUFT(xc)
+        orc.aNUFT(kc)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(om0.shape[1] / tc / 1e6, 4), "unit": "Msamples*coil/s", "cores": 1,
+               "kind": "port",
+               "sample": "FP64 oracle fNUFT+aNUFT of one coil on the stream's smallest trajectory "
+                         "(N=%d, %d samples): %.3f s (H assembly untimed)" % (N0, om0.shape[1], tc)}
+        log("cpu baseline: %s" % json.dumps(cpu))
     out = {
         "metric": "fwd+adj Msamples*coil/s at rel err <=1e-4 vs FP64; recon p50/p99 latency",
         "value": round(value, 3),
@@ -319,8 +348,8 @@ def run_serving(args):
         "one_at_a_time": alone,
         "served_vs_one_at_a_time": (round(value / alone["capacity_msamples_coil_s"], 3)
                                     if alone else None),
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -407,6 +436,29 @@ def run_lowrank(args):
         % (sake_it, sake_s, sake_s / sake_it, sinfo["rank"], serr))
     log("sake per-iteration kernel ms: %s" % json.dumps(
         dict(sorted(skern.items(), key=lambda kv: -kv[1])[:12])))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle's literal make_data_matrix loop (circshift copies, one
+        # [V nc]'[V nc] product per tap pair; pruno.m:146-179) for the first
+        # outer tap, scaled to all nk(nk+1)/2 pairs; its blocks checked
+        # against the FP64 plan's Gram
+        from oracle import pruno_oracle as P
+        taps_o, _ = P.kernel_taps(width, radial)
+        t0 = time.perf_counter()
+        AAo = P.make_data_matrix(d64, taps_o, jmax=1)
+        t_cpu = time.perf_counter() - t0
+        pairs_done = nk
+        t_full = t_cpu * (nk * (nk + 1) / 2) / pairs_done
+        A4o = AAo.reshape(nc, nk, nc, nk, order="F")[:, 0]
+        A4g = np.asarray(AA64).reshape(nc, nk, nc, nk, order="F")[:, 0]
+        cerr = float(np.linalg.norm(A4o - A4g) / np.linalg.norm(A4o))
+        cpu = {"value": round(flops / t_full / 1e12, 6), "unit": "TFLOP/s", "cores": 1,
+               "kind": "port",
+               "sample": "FP64 oracle make_data_matrix (literal circshift loop, NumPy, 1 thread) for "
+                         "outer tap 1 of %d: %d of %d tap-pair blocks in %.2f s, scaled to the full "
+                         "Gram: %.0f s; those blocks equal the FP64 plan's to %.1e"
+                         % (nk, pairs_done, nk * (nk + 1) // 2, t_cpu, t_full, cerr)}
+        log("cpu baseline: %s" % json.dumps(cpu))
     out = {
         "metric": "Hankel Gram TFLOP/s (split-BF16 MFMA) and PRUNO reconstruction time",
         "value": round(flops / g_ms / 1e9, 3),
@@ -430,7 +482,7 @@ def run_lowrank(args):
         "sake": {"iterations": sake_it, "seconds": round(sake_s, 3),
                  "s_per_iteration": round(sake_s / sake_it, 4), "rank": sinfo["rank"],
                  "noise": sinfo["noise"], "rel_err_vs_fp64": serr, "kernel_ms_per_iteration": skern},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

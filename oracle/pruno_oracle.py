@@ -50,18 +50,21 @@ def flip_index(n, center=1):
     return np.roll(np.arange(n, 0, -1), 2 * center - 1) - 1
 
 
-def make_data_matrix(data, taps, loraks=False, center=(1, 1, 1)):
+def make_data_matrix(data, taps, loraks=False, center=(1, 1, 1), jmax=None):
     """pruno.m:146-179 / sake.m:176-224: AA(:,j,:,k) = row'*col with
     circshift copies; with loraks the conjugate-reflection block
     BB(:,j,:,k) = row'*conj(col(flip.x,flip.y,flip.z,:)) (sake.m:210-214)
-    and the 2n x 2n matrix [AA BB; conj(BB) conj(AA)] (sake.m:221-224)."""
+    and the 2n x 2n matrix [AA BB; conj(BB) conj(AA)] (sake.m:221-224).
+    jmax: only the outer-loop taps j < jmax (a timed sample of the loop;
+    the blocks of the other taps stay zero)."""
     nx, ny, nz, nc = data.shape
     nk = len(taps)
+    jend = nk if jmax is None else min(nk, int(jmax))
     AA = np.zeros((nc, nk, nc, nk), dtype=np.complex128)
     BB = np.zeros((nc, nk, nc, nk), dtype=np.complex128) if loraks else None
     if loraks:
         fl = np.ix_(flip_index(nx, center[0]), flip_index(ny, center[1]), flip_index(nz, center[2]))
-    for j in range(nk):
+    for j in range(jend):
         row = np.roll(data, tuple(taps[j]), axis=(0, 1, 2)).reshape(-1, nc, order='F')
         for k in range(j, nk):
             colv = np.roll(data, tuple(taps[k]), axis=(0, 1, 2))

@@ -188,13 +188,19 @@ int nufft3d_release_workspaces(nufft3d_plan* plan);
 int nufft3d_set_stream(nufft3d_plan* plan, void* stream);
 
 /* Run iNUFT's MINRES iterations as captured HIP graphs (enable = 1, the
- * default) or launch by launch (0). No counterpart in the reference: for a
- * process whose other threads free device memory (or synchronise this
- * plan's stream) while it may be capturing; HIP then invalidates the
- * capture. The results are the same either way. (The library's own destroy
- * and release wait on the plan's completion event, not on a stream it does
- * not own.) */
+ * default) or launch by launch (0). No counterpart in the reference. The
+ * library's own device allocations and releases wait while any thread
+ * captures, so they do not invalidate a capture; a process whose other
+ * code frees device memory (or synchronises this plan's stream) meanwhile
+ * may still invalidate one, and that block then runs launch by launch. The
+ * results are the same either way. (The library's own destroy and release
+ * wait on the plan's completion event, not on a stream it does not own.) */
 int nufft3d_set_graphs(nufft3d_plan* plan, int32_t enable);
+
+/* MINRES graph blocks of this plan since its creation: captured (and
+ * instantiated) and failed captures (run launch by launch instead). No
+ * counterpart in the reference: measurement. */
+int nufft3d_graph_stats(const nufft3d_plan* plan, int64_t* captured, int64_t* failed);
 
 /* Options of pruno(obj,data,varargin) (pruno.m:26-41). */
 typedef struct nufft3d_pruno_opts {

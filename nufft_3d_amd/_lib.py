@@ -80,6 +80,7 @@ SIGNATURES = {
                                      C.POINTER(C.c_int32)]),
     "nufft3d_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nufft3d_set_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
+    "nufft3d_graph_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nufft3d_device_bytes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "nufft3d_compute_precision": (C.c_int32, [C.c_void_p]),
     "nufft3d_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_char_p, C.c_int64]),

@@ -623,7 +623,7 @@ static int setup_bluestein(nufft3d_plan* p, FftAxis& a, int L, Stages& st) {
   chirp_ld(L, -1, cr, ci, L);
   std::vector<unsigned char> hc(esz * L);
   for (int n = 0; n < L; ++n) put(hc, n, cr[n], ci[n]);
-  N3D_HIP(hipMalloc(&a.d_chirp, esz * L));
+  N3D_HIP(dev_malloc(&a.d_chirp, esz * L));
   N3D_HIP(hipMemcpy(a.d_chirp, hc.data(), esz * L, hipMemcpyHostToDevice));
   // b[m] = exp(-s i pi m^2 / L) on m in (-L, L), wrapped to length L2; its
   // L2-point DFT (e^{-2 pi i jk / L2}) over L2, by direct sums
@@ -653,7 +653,7 @@ static int setup_bluestein(nufft3d_plan* p, FftAxis& a, int L, Stages& st) {
       }
       put(hb, k, xr / L2, xi / L2);
     }
-    N3D_HIP(hipMalloc(&a.d_bhat[si], esz * L2));
+    N3D_HIP(dev_malloc(&a.d_bhat[si], esz * L2));
     N3D_HIP(hipMemcpy(a.d_bhat[si], hb.data(), esz * L2, hipMemcpyHostToDevice));
   }
   return NUFFT3D_OK;
@@ -698,7 +698,7 @@ int fft_setup_axis(nufft3d_plan* p, int axis) {
       reinterpret_cast<float2*>(host.data())[t] = v;
     }
   }
-  N3D_HIP(hipMalloc(&a.d_tw, esz * Ltw));
+  N3D_HIP(dev_malloc(&a.d_tw, esz * Ltw));
   N3D_HIP(hipMemcpy(a.d_tw, host.data(), esz * Ltw, hipMemcpyHostToDevice));
   return NUFFT3D_OK;
 }

@@ -518,12 +518,12 @@ static int gram_fft64(nufft3d_plan* p, const void* data, const void* flip, const
   double2 *F = nullptr, *G = nullptr, *P = nullptr, *Q = nullptr;
   struct Free {
     double2** q[4];
-    ~Free() { for (double2** a : q) if (*a) (void)hipFree(*a); }
+    ~Free() { for (double2** a : q) if (*a) (void)dev_free(*a); }
   } fr{{&F, &G, &P, &Q}};
   const int npair = nc * nc;
   const int chunk = (int)std::max<long long>(1, std::min<long long>(npair, (1LL << 31) / (long long)vb));
-  if (hipMalloc(&F, vb * nc) != hipSuccess || (flip && hipMalloc(&G, vb * nc) != hipSuccess) ||
-      hipMalloc(&P, vb * chunk) != hipSuccess || hipMalloc(&Q, vb * chunk) != hipSuccess) {
+  if (dev_malloc(&F, vb * nc) != hipSuccess || (flip && dev_malloc(&G, vb * nc) != hipSuccess) ||
+      dev_malloc(&P, vb * chunk) != hipSuccess || dev_malloc(&Q, vb * chunk) != hipSuccess) {
     (void)hipGetLastError();
     set_error("device allocation for the FFT Gram failed");
     return NUFFT3D_ERR_NOMEM;
@@ -563,12 +563,12 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
   int4* dtaps = nullptr;
   void* flip = nullptr;
   double2 *tmpA = nullptr, *tmpB = nullptr;
-  N3D_HIP(hipMalloc(&dtaps, sizeof(int4) * taps.size()));
+  N3D_HIP(dev_malloc(&dtaps, sizeof(int4) * taps.size()));
   N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
   if (loraks) {
-    N3D_HIP(hipMalloc(&flip, esz * g.V * g.nc));
-    N3D_HIP(hipMalloc(&tmpA, sizeof(double2) * nn));
-    N3D_HIP(hipMalloc(&tmpB, sizeof(double2) * nn));
+    N3D_HIP(dev_malloc(&flip, esz * g.V * g.nc));
+    N3D_HIP(dev_malloc(&tmpA, sizeof(double2) * nn));
+    N3D_HIP(dev_malloc(&tmpB, sizeof(double2) * nn));
     const int c1 = center ? center[0] - 1 : 0, c2 = center ? center[1] - 1 : 0,
               c3 = center ? center[2] - 1 : 0;
     ProfScope ps(p, "k_flip_conj");
@@ -629,10 +629,10 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
     const long long nq = (long long)nsx * 4 * g.nc * g.plane;
     unsigned short *Q = nullptr, *Q2 = nullptr;
     GramCol *dcols = nullptr, *dcols2 = nullptr;
-    N3D_HIP(hipMalloc(&Q, sizeof(unsigned short) * nq));
+    N3D_HIP(dev_malloc(&Q, sizeof(unsigned short) * nq));
     {
       const std::vector<GramCol> cols = make_cols(1);
-      N3D_HIP(hipMalloc(&dcols, sizeof(GramCol) * cols.size()));
+      N3D_HIP(dev_malloc(&dcols, sizeof(GramCol) * cols.size()));
       N3D_HIP(hipMemcpy(dcols, cols.data(), sizeof(GramCol) * cols.size(), hipMemcpyHostToDevice));
     }
     {
@@ -641,9 +641,9 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                          p->stream, (const float2*)data, Q, g, nsx);
     }
     if (loraks) {
-      N3D_HIP(hipMalloc(&Q2, sizeof(unsigned short) * nq));
+      N3D_HIP(dev_malloc(&Q2, sizeof(unsigned short) * nq));
       const std::vector<GramCol> cols2 = make_cols(-1);
-      N3D_HIP(hipMalloc(&dcols2, sizeof(GramCol) * cols2.size()));
+      N3D_HIP(dev_malloc(&dcols2, sizeof(GramCol) * cols2.size()));
       N3D_HIP(hipMemcpy(dcols2, cols2.data(), sizeof(GramCol) * cols2.size(), hipMemcpyHostToDevice));
       ProfScope ps(p, "k_gram_prep");
       hipLaunchKernelGGL(k_gram_prep, dim3(nblk((long long)nsx * g.nc * g.plane)), dim3(256), 0,
@@ -676,10 +676,10 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
       float* part = nullptr;
       double* red = nullptr;
       int rs = NUFFT3D_OK;
-      if (hipMalloc(&dtl, sizeof(int2) * ntiles) != hipSuccess ||
-          hipMalloc(&dtid, sizeof(int) * tid_of.size()) != hipSuccess ||
-          hipMalloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT) != hipSuccess ||
-          hipMalloc(&red, sizeof(double) * (size_t)ntiles * GT * GT) != hipSuccess) {
+      if (dev_malloc(&dtl, sizeof(int2) * ntiles) != hipSuccess ||
+          dev_malloc(&dtid, sizeof(int) * tid_of.size()) != hipSuccess ||
+          dev_malloc(&part, sizeof(float) * (size_t)nsplit * ntiles * GT * GT) != hipSuccess ||
+          dev_malloc(&red, sizeof(double) * (size_t)ntiles * GT * GT) != hipSuccess) {
         set_error("device allocation for the Gram tiles failed");
         rs = NUFFT3D_ERR_NOMEM;
       }
@@ -702,17 +702,17 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
                            p->stream, (const double*)red, (const int*)dtid, gg, sym ? 1 : 0, out);
         if (hipStreamSynchronize(p->stream) != hipSuccess) rs = NUFFT3D_ERR_DEVICE;
       }
-      if (dtl) (void)hipFree(dtl);
-      if (dtid) (void)hipFree(dtid);
-      if (part) (void)hipFree(part);
-      if (red) (void)hipFree(red);
+      if (dtl) (void)dev_free(dtl);
+      if (dtid) (void)dev_free(dtid);
+      if (part) (void)dev_free(part);
+      if (red) (void)dev_free(red);
       return rs;
     };
     st = gram_pair(Q, dcols, true, outA);
     if (!st && loraks) st = gram_pair(Q2, dcols2, false, tmpB);
-    (void)hipFree(Q); (void)hipFree(dcols);
-    if (Q2) (void)hipFree(Q2);
-    if (dcols2) (void)hipFree(dcols2);
+    (void)dev_free(Q); (void)dev_free(dcols);
+    if (Q2) (void)dev_free(Q2);
+    if (dcols2) (void)dev_free(dcols2);
   }
   if (!st && loraks) {
     ProfScope ps(p, "k_loraks_blocks");
@@ -721,10 +721,10 @@ int gram_device(nufft3d_plan* p, const void* data, const int dims[4], const std:
   }
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(dtaps);
-  if (flip) (void)hipFree(flip);
-  if (tmpA) (void)hipFree(tmpA);
-  if (tmpB) (void)hipFree(tmpB);
+  (void)dev_free(dtaps);
+  if (flip) (void)dev_free(flip);
+  if (tmpA) (void)dev_free(tmpA);
+  if (tmpB) (void)dev_free(tmpB);
   if (st == NUFFT3D_ERR_DEVICE) set_error("device error in the Gram kernel");
   return st;
 }
@@ -740,15 +740,15 @@ int heev_device(nufft3d_plan* p, double2* dA, int n, double* dW) {
   rocblas_set_stream(h, p->stream);
   double* E = nullptr;
   rocblas_int* info = nullptr;
-  N3D_HIP(hipMalloc(&E, sizeof(double) * (n > 0 ? n : 1)));
-  N3D_HIP(hipMalloc(&info, sizeof(rocblas_int)));
+  N3D_HIP(dev_malloc(&E, sizeof(double) * (n > 0 ? n : 1)));
+  N3D_HIP(dev_malloc(&info, sizeof(rocblas_int)));
   rocblas_status rs = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, n,
                                        (rocblas_double_complex*)dA, n, dW, E, info);
   rocblas_int hinfo = 0;
   (void)hipMemcpyAsync(&hinfo, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, p->stream);
   (void)hipStreamSynchronize(p->stream);
-  (void)hipFree(E);
-  (void)hipFree(info);
+  (void)dev_free(E);
+  (void)dev_free(info);
   rocblas_destroy_handle(h);
   if (rs != rocblas_status_success || hinfo != 0) {
     set_error("rocsolver_zheevd failed (status " + std::to_string((int)rs) + ", info " +
@@ -846,7 +846,7 @@ int null_operator(nufft3d_plan* p, const double2* dKt, int nk, int nc, int nv, i
   const int n = nk * nc;
   double2 *P = nullptr, *G = nullptr;
   int4* dtaps = nullptr;
-  N3D_HIP(hipMalloc(&P, sizeof(double2) * (size_t)n * n));
+  N3D_HIP(dev_malloc(&P, sizeof(double2) * (size_t)n * n));
   rocblas_handle h;
   if (rocblas_create_handle(&h) != rocblas_status_success) {
     set_error("rocblas_create_handle failed");
@@ -861,20 +861,20 @@ int null_operator(nufft3d_plan* p, const double2* dKt, int nk, int nc, int nv, i
                                     (rocblas_double_complex*)P, n);
   rocblas_destroy_handle(h);
   if (rs != rocblas_status_success) {
-    (void)hipFree(P);
+    (void)dev_free(P);
     set_error("rocblas_zgemm failed");
     return NUFFT3D_ERR_DEVICE;
   }
   const int D = 2 * w - 1;
   const long long nG = (long long)D * D * D * nc * nc;
-  N3D_HIP(hipMalloc(&G, sizeof(double2) * nG));
-  N3D_HIP(hipMalloc(&dtaps, sizeof(int4) * taps.size()));
+  N3D_HIP(dev_malloc(&G, sizeof(double2) * nG));
+  N3D_HIP(dev_malloc(&dtaps, sizeof(int4) * taps.size()));
   N3D_HIP(hipMemcpyAsync(dtaps, taps.data(), sizeof(int4) * taps.size(), hipMemcpyHostToDevice, p->stream));
   hipLaunchKernelGGL(k_null_G, dim3(nblk(nG)), dim3(256), 0, p->stream, (const double2*)P,
                      (const int4*)dtaps, nk, nc, w, G);
   const int st = lag_field(p, G, w, (long long)nc * nc, N, lambda * lambda, dB);
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(P); (void)hipFree(G); (void)hipFree(dtaps);
+  (void)dev_free(P); (void)dev_free(G); (void)dev_free(dtaps);
   return st;
 }
 
@@ -886,9 +886,9 @@ int lag_field(nufft3d_plan* p, const double2* G, int w, long long cc, const int 
   const int D = 2 * w - 1;
   const int N1 = N[0], N2 = N[1], N3 = N[2];
   double2 *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;
-  N3D_HIP(hipMalloc(&T1, sizeof(double2) * (size_t)D * D * N1 * cc));
-  N3D_HIP(hipMalloc(&T2, sizeof(double2) * (size_t)D * N2 * N1 * cc));
-  N3D_HIP(hipMalloc(&T3, sizeof(double2) * (size_t)N3 * N2 * N1 * cc));
+  N3D_HIP(dev_malloc(&T1, sizeof(double2) * (size_t)D * D * N1 * cc));
+  N3D_HIP(dev_malloc(&T2, sizeof(double2) * (size_t)D * N2 * N1 * cc));
+  N3D_HIP(dev_malloc(&T3, sizeof(double2) * (size_t)N3 * N2 * N1 * cc));
   // x: in [dz*dy][dx][cc] -> [dz*dy][x][cc]
   hipLaunchKernelGGL(k_null_axis, dim3(nblk((long long)D * D * N1 * cc)), dim3(256), 0, p->stream,
                      G, T1, D, w, N1, cc, (long long)D * D);
@@ -907,7 +907,7 @@ int lag_field(nufft3d_plan* p, const double2* G, int w, long long cc, const int 
                        (const double2*)T3, n, scale, (float2*)dB);
   N3D_HIP(hipGetLastError());
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(T1); (void)hipFree(T2); (void)hipFree(T3);
+  (void)dev_free(T1); (void)dev_free(T2); (void)dev_free(T3);
   return NUFFT3D_OK;
 }
 
@@ -957,7 +957,7 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
   const long long V = (long long)dims[0] * dims[1] * dims[2];
   // fully sampled check (pruno.m:77-81)
   unsigned long long* dcnt = nullptr;
-  N3D_HIP(hipMalloc(&dcnt, sizeof(unsigned long long)));
+  N3D_HIP(dev_malloc(&dcnt, sizeof(unsigned long long)));
   N3D_HIP(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), p->stream));
   if (p->prec == 2)
     hipLaunchKernelGGL((k_count_nonzero<double>), dim3(nblk(V * nc)), dim3(256), 0, p->stream,
@@ -968,7 +968,7 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
   unsigned long long cnt = 0;
   N3D_HIP(hipMemcpyAsync(&cnt, dcnt, sizeof(cnt), hipMemcpyDeviceToHost, p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(dcnt);
+  (void)dev_free(dcnt);
   const double density = (double)cnt / (double)(V * nc);
   if (density < 0.99) { set_error("data must be fully sampled"); return NUFFT3D_ERR_INVALID; }
 
@@ -979,8 +979,8 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
   const int n = nk * nc;
   double2* dAA = nullptr;
   double* dW = nullptr;
-  N3D_HIP(hipMalloc(&dAA, sizeof(double2) * (size_t)n * n));
-  N3D_HIP(hipMalloc(&dW, sizeof(double) * n));
+  N3D_HIP(dev_malloc(&dAA, sizeof(double2) * (size_t)n * n));
+  N3D_HIP(dev_malloc(&dW, sizeof(double) * n));
   int st = gram_device(p, kdata, dims, taps, dAA);
   if (!st) st = heev_device(p, dAA, n, dW);
   std::vector<double> wasc(n);
@@ -1011,9 +1011,9 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
     int* dcols = nullptr;
     double* dscale = nullptr;
     double2* dKt = nullptr;
-    N3D_HIP(hipMalloc(&dcols, sizeof(int) * cols.size()));
-    N3D_HIP(hipMalloc(&dscale, sizeof(double) * scale.size()));
-    N3D_HIP(hipMalloc(&dKt, sizeof(double2) * (size_t)n * std::max(nv, 1)));
+    N3D_HIP(dev_malloc(&dcols, sizeof(int) * cols.size()));
+    N3D_HIP(dev_malloc(&dscale, sizeof(double) * scale.size()));
+    N3D_HIP(dev_malloc(&dKt, sizeof(double2) * (size_t)n * std::max(nv, 1)));
     N3D_HIP(hipMemcpy(dcols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice));
     N3D_HIP(hipMemcpy(dscale, scale.data(), sizeof(double) * scale.size(), hipMemcpyHostToDevice));
     if (nv > 0)
@@ -1024,15 +1024,15 @@ int pruno_build(nufft3d_plan* p, const void* kdata, int nc, const nufft3d_pruno_
                              p->stream));
     if (dB) {
       const size_t esz = p->prec == 2 ? sizeof(double2) : sizeof(float2);
-      N3D_HIP(hipMalloc(dB, esz * (size_t)V * nc * nc));
+      N3D_HIP(dev_malloc(dB, esz * (size_t)V * nc * nc));
       if (nv > 0) st = null_operator(p, dKt, nk, nc, nv, w, taps, p->N, lambda, *dB);
       else N3D_HIP(hipMemset(*dB, 0, esz * (size_t)V * nc * nc));
     }
     N3D_HIP(hipStreamSynchronize(p->stream));
-    (void)hipFree(dcols); (void)hipFree(dscale); (void)hipFree(dKt);
+    (void)dev_free(dcols); (void)dev_free(dscale); (void)dev_free(dKt);
   }
-  (void)hipFree(dAA);
-  (void)hipFree(dW);
+  (void)dev_free(dAA);
+  (void)dev_free(dW);
   return st;
 }
 

@@ -1151,7 +1151,7 @@ int build_kb_table(nufft3d_plan* p) {
       for (int j = 0; j < kKbCoef; ++j) tab[(size_t)j * kKbIntervals + i] = (double)(V[j][kKbCoef] / V[j][j]);
     }
   }
-  N3D_HIP(hipMalloc(&p->d_kbtab, sizeof(double) * tab.size()));
+  N3D_HIP(dev_malloc(&p->d_kbtab, sizeof(double) * tab.size()));
   N3D_HIP(hipMemcpy(p->d_kbtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
   return NUFFT3D_OK;
 }
@@ -1245,7 +1245,7 @@ __global__ void k_hit_weights(const uint2* __restrict__ hits, long long nhits,
 int build_hit_weights(nufft3d_plan* p) {
   if (p->d_hitw || p->nhits <= 0) return NUFFT3D_OK;
   if (p->prec == 1 ? !p->d_off32 : !p->d_off) return NUFFT3D_OK;
-  N3D_HIP(hipMalloc(&p->d_hitw, sizeof(uint2) * p->nhits));
+  N3D_HIP(dev_malloc(&p->d_hitw, sizeof(uint2) * p->nhits));
   const GridGeom g = make_geom(p, 1, 0, 1);
   const dim3 grid((unsigned)((p->nhits + 255) / 256));
   if (p->prec == 1)

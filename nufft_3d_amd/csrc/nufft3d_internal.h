@@ -12,12 +12,43 @@
 
 #include <cstdint>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/nufft3d.h"
 
 namespace nufft3d {
+
+// Every device allocation and release of the library goes through these.
+// HIP synchronises the device in hipFree, which invalidates a stream capture
+// in progress on another thread, so allocations and releases wait while any
+// thread captures a graph (capture_scope holds the lock shared), and a
+// capture waits for them. A thread that is itself capturing allocates
+// without the lock (its own capture then fails and falls back).
+std::shared_mutex& alloc_capture_mutex();
+extern thread_local bool t_capturing;
+template <class T>
+inline hipError_t dev_malloc(T** ptr, size_t bytes) {
+  if (t_capturing) return hipMalloc(reinterpret_cast<void**>(ptr), bytes);
+  std::unique_lock<std::shared_mutex> lk(alloc_capture_mutex());
+  return hipMalloc(reinterpret_cast<void**>(ptr), bytes);
+}
+inline hipError_t dev_free(void* ptr) {
+  if (t_capturing) return hipFree(ptr);
+  std::unique_lock<std::shared_mutex> lk(alloc_capture_mutex());
+  return hipFree(ptr);
+}
+inline hipError_t dev_stream_destroy(hipStream_t st) {
+  if (t_capturing) return hipStreamDestroy(st);
+  std::unique_lock<std::shared_mutex> lk(alloc_capture_mutex());
+  return hipStreamDestroy(st);
+}
+struct capture_scope {
+  std::shared_lock<std::shared_mutex> lk;
+  capture_scope() : lk(alloc_capture_mutex()) { t_capturing = true; }
+  ~capture_scope() { t_capturing = false; }
+};
 
 void set_error(const std::string& msg);
 
@@ -234,6 +265,7 @@ struct nufft3d_plan {
   int use_graphs = 1;
   hipGraphExec_t mr_exec = nullptr;
   std::vector<uint64_t> mr_key;
+  long long graphs_captured = 0, graphs_failed = 0;   // MINRES block captures
   int capture_fail_at = 0;  // test hook: a MINRES block capture fails after this many
                             // iterations (NUFFT3D_TEST_CAPTURE_FAIL; 0 = never)
 

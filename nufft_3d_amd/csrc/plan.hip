@@ -22,6 +22,12 @@
 
 namespace nufft3d {
 
+std::shared_mutex& alloc_capture_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
+thread_local bool t_capturing = false;
+
 static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -34,12 +40,12 @@ int ensure(Workspace& w, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (w.bytes >= bytes) return NUFFT3D_OK;
   if (w.p) {
-    if (hipFree(w.p) != hipSuccess) { (void)hipGetLastError(); }
+    if (dev_free(w.p) != hipSuccess) { (void)hipGetLastError(); }
     w.p = nullptr;
     w.bytes = 0;
     bytes += bytes / 4;
   }
-  if (hipMalloc(&w.p, bytes) != hipSuccess) {
+  if (dev_malloc(&w.p, bytes) != hipSuccess) {
     (void)hipGetLastError();
     w.p = nullptr;
     set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
@@ -81,7 +87,7 @@ void prof_end(nufft3d_plan* p) {
 
 static void release(Workspace& w, bool host = false) {
   if (!w.p) return;
-  if (host) (void)hipHostFree(w.p); else (void)hipFree(w.p);
+  if (host) (void)hipHostFree(w.p); else (void)dev_free(w.p);
   w.p = nullptr;
   w.bytes = 0;
 }
@@ -549,7 +555,7 @@ template <typename R>
 static int density_t(nufft3d_plan* p) {
   const long long n = p->nnz;
   using C = cplx<R>;
-  N3D_HIP(hipMalloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
+  N3D_HIP(dev_malloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
   if (n == 0) return NUFFT3D_OK;
   const size_t ngrid = (size_t)p->K[0] * p->K[1] * p->K[2];
   N3D_TRY(ensure(p->ws_grid, ngrid * sizeof(C)));
@@ -575,15 +581,15 @@ static int density_t(nufft3d_plan* p) {
 template <typename R>
 static int user_density_t(nufft3d_plan* p, const double* dh, int64_t nd) {
   const long long n = p->nnz;
-  N3D_HIP(hipMalloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
+  N3D_HIP(dev_malloc(&p->d_dens, sizeof(R) * (n > 0 ? n : 1)));
   double* dd = nullptr;
-  N3D_HIP(hipMalloc(&dd, sizeof(double) * nd));
+  N3D_HIP(dev_malloc(&dd, sizeof(double) * nd));
   N3D_HIP(hipMemcpy(dd, dh, sizeof(double) * nd, hipMemcpyHostToDevice));
   if (n > 0)
     hipLaunchKernelGGL((k_gather_real<R>), dim3(nblocks(n)), dim3(256), 0, p->stream, dd,
                        p->d_orig, (const R*)nullptr, n, (R*)p->d_dens);
   N3D_HIP(hipStreamSynchronize(p->stream));
-  N3D_HIP(hipFree(dd));
+  N3D_HIP(dev_free(dd));
   // warning of nufft_3d.m:235-237: mean over all rows after zeroing out of bounds
   std::vector<double> tmp(dh, dh + nd);
   double s = 0;
@@ -639,9 +645,9 @@ static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long
     for (int i = 0; i < n; ++i) st[i] = hseg[i].start;
     long long* d_st = nullptr;
     unsigned* d_first = nullptr;
-    if (hipMalloc(&d_st, sizeof(long long) * n) != hipSuccess ||
-        hipMalloc(&d_first, sizeof(unsigned) * n) != hipSuccess) {
-      if (d_st) (void)hipFree(d_st);
+    if (dev_malloc(&d_st, sizeof(long long) * n) != hipSuccess ||
+        dev_malloc(&d_first, sizeof(unsigned) * n) != hipSuccess) {
+      if (d_st) (void)dev_free(d_st);
       (void)hipGetLastError();
       set_error("dense segment ordering: allocation failed");
       return NUFFT3D_ERR_NOMEM;
@@ -654,8 +660,8 @@ static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long
       st2 = hipStreamSynchronize(p->stream) != hipSuccess ||
             hipMemcpy(first.data(), d_first, sizeof(unsigned) * n, hipMemcpyDeviceToHost) != hipSuccess;
     }
-    (void)hipFree(d_st);
-    (void)hipFree(d_first);
+    (void)dev_free(d_st);
+    (void)dev_free(d_first);
     if (st2) {
       (void)hipGetLastError();
       set_error("dense segment ordering failed");
@@ -687,8 +693,8 @@ static int build_dense(nufft3d_plan* p, const std::vector<long long>& hcnt, long
   }
   ds.nhseg = (int)hseg.size();
   ds.nhpt = (int)hpt.size();
-  N3D_HIP(hipMalloc(&ds.d_hseg, sizeof(HeavySeg) * std::max<size_t>(1, hseg.size())));
-  N3D_HIP(hipMalloc(&ds.d_hpt, sizeof(HeavyPt) * std::max<size_t>(1, hpt.size())));
+  N3D_HIP(dev_malloc(&ds.d_hseg, sizeof(HeavySeg) * std::max<size_t>(1, hseg.size())));
+  N3D_HIP(dev_malloc(&ds.d_hpt, sizeof(HeavyPt) * std::max<size_t>(1, hpt.size())));
   if (!hseg.empty()) {
     N3D_HIP(hipMemcpy(ds.d_hseg, hseg.data(), sizeof(HeavySeg) * hseg.size(), hipMemcpyHostToDevice));
     N3D_HIP(hipMemcpy(ds.d_hpt, hpt.data(), sizeof(HeavyPt) * hpt.size(), hipMemcpyHostToDevice));
@@ -712,21 +718,21 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
   int* d_nsel = nullptr;
   void* tmp = nullptr;
   auto cleanup = [&]() {
-    if (d_bof) (void)hipFree(d_bof);
-    if (keys) (void)hipFree(keys);
-    if (skeys) (void)hipFree(skeys);
-    if (d_nsel) (void)hipFree(d_nsel);
-    if (tmp) (void)hipFree(tmp);
+    if (d_bof) (void)dev_free(d_bof);
+    if (keys) (void)dev_free(keys);
+    if (skeys) (void)dev_free(skeys);
+    if (d_nsel) (void)dev_free(d_nsel);
+    if (tmp) (void)dev_free(tmp);
   };
   auto fail = [&](const char* what) {
     cleanup();
     set_error(std::string("brick list construction: ") + what);
     return NUFFT3D_ERR_NOMEM;
   };
-  if (hipMalloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
-      hipMalloc(&keys, sizeof(unsigned long long) * nh) != hipSuccess ||
-      hipMalloc(&skeys, sizeof(unsigned long long) * nh) != hipSuccess ||
-      hipMalloc(&d_nsel, sizeof(int)) != hipSuccess)
+  if (dev_malloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
+      dev_malloc(&keys, sizeof(unsigned long long) * nh) != hipSuccess ||
+      dev_malloc(&skeys, sizeof(unsigned long long) * nh) != hipSuccess ||
+      dev_malloc(&d_nsel, sizeof(int)) != hipSuccess)
     return fail("allocation failed");
   N3D_HIP(hipMemcpy(d_bof, brick_of.data(), sizeof(int) * brick_of.size(), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_bl_keys, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
@@ -735,7 +741,7 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
   N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nh, 0, 32 + bbits,
                                             p->stream));
   N3D_HIP(hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nh, p->stream));
-  if (hipMalloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
+  if (dev_malloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
   // the dense points' all-ones keys sort last within the sorted bits too
   // (every brick index is below 2^bbits - 1)
   N3D_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, (int)nh, 0, 32 + bbits,
@@ -749,8 +755,8 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
     N3D_HIP(hipMemcpy(&last, keys + nsel - 1, sizeof(last), hipMemcpyDeviceToHost));
   const long long total = (nsel > 0 && (last >> 32) >= (unsigned long long)p->nbrick) ? nsel - 1 : nsel;
   std::vector<long long> hptr(p->nbrick + 1);
-  if (hipMalloc(&bl.d_ptr, sizeof(long long) * (p->nbrick + 1)) != hipSuccess ||
-      hipMalloc(&bl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess)
+  if (dev_malloc(&bl.d_ptr, sizeof(long long) * (p->nbrick + 1)) != hipSuccess ||
+      dev_malloc(&bl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess)
     return fail("allocation failed");
   hipLaunchKernelGGL(k_bl_ptr, dim3(nblocks(p->nbrick + 1)), dim3(256), 0, p->stream,
                      (const unsigned long long*)keys, total, p->nbrick, bl.d_ptr);
@@ -764,11 +770,11 @@ static int build_brick_lists(nufft3d_plan* p, const std::vector<int>& brick_of, 
   for (long long b = 0; b < p->nbrick; ++b) mx = std::max(mx, hptr[b + 1] - hptr[b]);
   if (mx >= (1LL << 17)) {   // slots would not fit the 17 bits of an entry
     cleanup();
-    (void)hipFree(bl.d_ptr); (void)hipFree(bl.d_samp);
+    (void)dev_free(bl.d_ptr); (void)dev_free(bl.d_samp);
     bl.d_ptr = nullptr; bl.d_samp = nullptr;
     return NUFFT3D_OK;
   }
-  if (hipMalloc(&bl.d_ent, sizeof(unsigned) * nh) != hipSuccess) return fail("allocation failed");
+  if (dev_malloc(&bl.d_ent, sizeof(unsigned) * nh) != hipSuccess) return fail("allocation failed");
   N3D_HIP(hipMemsetAsync(bl.d_ent, 0, sizeof(unsigned) * nh, p->stream));
   hipLaunchKernelGGL(k_bl_ent, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr,
                      p->d_hit, ngrid, p->K[0], p->K[1], d_bof, bx, by, rmax,
@@ -819,15 +825,15 @@ static int build_wide_lists(nufft3d_plan* p, const std::vector<int>& brick_of, i
   int* d_nsel = nullptr;
   void* tmp = nullptr;
   auto cleanup = [&]() {
-    if (d_bof) (void)hipFree(d_bof);
-    if (keys) (void)hipFree(keys);
-    if (skeys) (void)hipFree(skeys);
-    if (d_nsel) (void)hipFree(d_nsel);
-    if (tmp) (void)hipFree(tmp);
+    if (d_bof) (void)dev_free(d_bof);
+    if (keys) (void)dev_free(keys);
+    if (skeys) (void)dev_free(skeys);
+    if (d_nsel) (void)dev_free(d_nsel);
+    if (tmp) (void)dev_free(tmp);
   };
   auto drop = [&]() {
     for (void* q : {(void*)wl.d_beg, (void*)wl.d_end, (void*)wl.d_samp, (void*)wl.d_went})
-      if (q) (void)hipFree(q);
+      if (q) (void)dev_free(q);
     wl.d_beg = nullptr; wl.d_end = nullptr; wl.d_samp = nullptr; wl.d_went = nullptr;
   };
   auto fail = [&](const char* what) {
@@ -838,17 +844,17 @@ static int build_wide_lists(nufft3d_plan* p, const std::vector<int>& brick_of, i
     return NUFFT3D_ERR_NOMEM;
   };
   const size_t nk = (size_t)std::max<long long>(smax, 1);
-  if (hipMalloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
-      hipMalloc(&keys, sizeof(unsigned long long) * nk) != hipSuccess ||
-      hipMalloc(&skeys, sizeof(unsigned long long) * nk) != hipSuccess ||
-      hipMalloc(&d_nsel, sizeof(int)) != hipSuccess)
+  if (dev_malloc(&d_bof, sizeof(int) * brick_of.size()) != hipSuccess ||
+      dev_malloc(&keys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&skeys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&d_nsel, sizeof(int)) != hipSuccess)
     return fail("allocation failed");
   N3D_HIP(hipMemcpy(d_bof, brick_of.data(), sizeof(int) * brick_of.size(), hipMemcpyHostToDevice));
   size_t tb1 = 0, tb2 = 0;
   N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nk, 0, 32 + bbits,
                                             p->stream));
   N3D_HIP(hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nk, p->stream));
-  if (hipMalloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
+  if (dev_malloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess) return fail("allocation failed");
   // one slab: its unique light keys left in keys[0, return value)
   auto slab = [&](int bz, long long& nlight) -> int {
     const long long g0 = 8LL * bz * plane, g1 = std::min<long long>(8LL * bz + 8, K3) * plane;
@@ -873,10 +879,10 @@ static int build_wide_lists(nufft3d_plan* p, const std::vector<int>& brick_of, i
     if (slab(bz, nl[bz])) return fail("sort failed");
     total += nl[bz];
   }
-  if (hipMalloc(&wl.d_beg, sizeof(long long) * p->nbrick) != hipSuccess ||
-      hipMalloc(&wl.d_end, sizeof(long long) * p->nbrick) != hipSuccess ||
-      hipMalloc(&wl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess ||
-      hipMalloc(&wl.d_went, sizeof(uint2) * nh) != hipSuccess)
+  if (dev_malloc(&wl.d_beg, sizeof(long long) * p->nbrick) != hipSuccess ||
+      dev_malloc(&wl.d_end, sizeof(long long) * p->nbrick) != hipSuccess ||
+      dev_malloc(&wl.d_samp, sizeof(int) * std::max(1LL, total)) != hipSuccess ||
+      dev_malloc(&wl.d_went, sizeof(uint2) * nh) != hipSuccess)
     return fail("allocation failed");
   N3D_HIP(hipMemsetAsync(wl.d_beg, 0, sizeof(long long) * p->nbrick, p->stream));
   N3D_HIP(hipMemsetAsync(wl.d_end, 0, sizeof(long long) * p->nbrick, p->stream));
@@ -953,7 +959,7 @@ static int build_pieces(nufft3d_plan* p, const unsigned* skeys, const BinGeom& g
     s = e;
   }
   p->npiece = (long long)pc.size();
-  N3D_HIP(hipMalloc(&p->d_piece, sizeof(int4) * (pc.empty() ? 1 : pc.size())));
+  N3D_HIP(dev_malloc(&p->d_piece, sizeof(int4) * (pc.empty() ? 1 : pc.size())));
   if (!pc.empty())
     N3D_HIP(hipMemcpy(p->d_piece, pc.data(), sizeof(int4) * pc.size(), hipMemcpyHostToDevice));
   return NUFFT3D_OK;
@@ -962,7 +968,7 @@ static int build_pieces(nufft3d_plan* p, const unsigned* skeys, const BinGeom& g
 static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   const bool dev = is_device_ptr(om_in);
   double* om = nullptr;
-  N3D_HIP(hipMalloc(&om, sizeof(double) * 3 * (M > 0 ? M : 1)));
+  N3D_HIP(dev_malloc(&om, sizeof(double) * 3 * (M > 0 ? M : 1)));
   N3D_HIP(hipMemcpy(om, om_in, sizeof(double) * 3 * M,
                     dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   // all z == 0 -> 2-D (nufft_3d.m:104-106); decided on the host copy
@@ -989,7 +995,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   if (p->N[2] == 1) p->K[2] = 1;
   if ((double)p->K[0] * p->K[1] * p->K[2] >= 2147483647.0) {
     set_error("N or u too large");                                          // nufft_3d.m:111
-    (void)hipFree(om);
+    (void)dev_free(om);
     return NUFFT3D_ERR_INVALID;
   }
   const int Wj = (int)ceil(p->J / 2);                                          // nufft_3d.m:154
@@ -1004,7 +1010,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
     if (p->K[a] > 1 && p->W[a] > 2 * p->K[a]) {
       set_error("kernel width J=" + std::to_string(p->J) + " spans more than two periods of the " +
                 "oversampled size K=" + std::to_string(p->K[a]));
-      (void)hipFree(om);
+      (void)dev_free(om);
       return NUFFT3D_ERR_UNSUPPORTED;
     }
   p->M = M;
@@ -1060,11 +1066,11 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   unsigned *keys = nullptr, *skeys = nullptr;
   int *vals = nullptr, *svals = nullptr, *counts = nullptr;
   const long long Mm = M > 0 ? M : 1;
-  N3D_HIP(hipMalloc(&keys, sizeof(unsigned) * Mm));
-  N3D_HIP(hipMalloc(&skeys, sizeof(unsigned) * Mm));
-  N3D_HIP(hipMalloc(&vals, sizeof(int) * Mm));
-  N3D_HIP(hipMalloc(&svals, sizeof(int) * Mm));
-  N3D_HIP(hipMalloc(&counts, sizeof(int)));
+  N3D_HIP(dev_malloc(&keys, sizeof(unsigned) * Mm));
+  N3D_HIP(dev_malloc(&skeys, sizeof(unsigned) * Mm));
+  N3D_HIP(dev_malloc(&vals, sizeof(int) * Mm));
+  N3D_HIP(dev_malloc(&svals, sizeof(int) * Mm));
+  N3D_HIP(dev_malloc(&counts, sizeof(int)));
   N3D_HIP(hipMemsetAsync(counts, 0, sizeof(int), p->stream));
   if (M > 0)
     hipLaunchKernelGGL(k_keys, dim3(nblocks(M)), dim3(256), 0, p->stream, om, (long long)M, g,
@@ -1075,39 +1081,39 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, vals, svals, (int)M, 0,
                                              end_bit, p->stream));
   void* tmp = nullptr;
-  N3D_HIP(hipMalloc(&tmp, tb + 16));
+  N3D_HIP(dev_malloc(&tmp, tb + 16));
   if (M > 0)
     N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, vals, svals, (int)M, 0,
                                                end_bit, p->stream));
   int noob = 0;
   N3D_HIP(hipMemcpyAsync(&noob, counts, sizeof(int), hipMemcpyDeviceToHost, p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(tmp);
+  (void)dev_free(tmp);
   p->nnz = M - noob;
   if (p->verbose)
     printf("  %lld points (out of %lld) are within bounds.\n", (long long)p->nnz, (long long)M);
   const long long nnz = p->nnz;
-  N3D_HIP(hipMalloc(&p->d_orig, sizeof(int) * (nnz > 0 ? nnz : 1)));
-  N3D_HIP(hipMalloc(&p->d_oob, sizeof(int) * (noob > 0 ? noob : 1)));
-  N3D_HIP(hipMalloc(&p->d_base, sizeof(int4) * (nnz > 0 ? nnz : 1)));
-  N3D_HIP(hipMalloc(&p->d_off, 3 * sizeof(double) * (nnz > 0 ? nnz : 1)));
+  N3D_HIP(dev_malloc(&p->d_orig, sizeof(int) * (nnz > 0 ? nnz : 1)));
+  N3D_HIP(dev_malloc(&p->d_oob, sizeof(int) * (noob > 0 ? noob : 1)));
+  N3D_HIP(dev_malloc(&p->d_base, sizeof(int4) * (nnz > 0 ? nnz : 1)));
+  N3D_HIP(dev_malloc(&p->d_off, 3 * sizeof(double) * (nnz > 0 ? nnz : 1)));
   if (M > 0)
     hipLaunchKernelGGL(k_gather, dim3(nblocks(M)), dim3(256), 0, p->stream, om, skeys, svals,
                        (long long)M, nnz, g, p->d_orig, p->d_oob, p->d_base, (double*)p->d_off);
   N3D_HIP(hipGetLastError());
   // inverse of orig: caller row -> sorted position (-1 for out-of-bounds rows)
-  N3D_HIP(hipMalloc(&p->d_inv, sizeof(int) * (M > 0 ? M : 1)));
+  N3D_HIP(dev_malloc(&p->d_inv, sizeof(int) * (M > 0 ? M : 1)));
   N3D_HIP(hipMemsetAsync(p->d_inv, 0xff, sizeof(int) * (M > 0 ? M : 1), p->stream));
   if (nnz > 0)
     hipLaunchKernelGGL(k_invert, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
                        (const int*)p->d_orig, nnz, p->d_inv);
   N3D_HIP(hipGetLastError());
   if (p->tile_interp && nnz > 0) N3D_TRY(build_pieces(p, skeys, g));
-  (void)hipFree(keys); (void)hipFree(skeys); (void)hipFree(vals); (void)hipFree(svals);
-  (void)hipFree(om);
+  (void)dev_free(keys); (void)dev_free(skeys); (void)dev_free(vals); (void)dev_free(svals);
+  (void)dev_free(om);
 
   if (p->prec == 1) {
-    N3D_HIP(hipMalloc(&p->d_off32, sizeof(float4) * (nnz > 0 ? nnz : 1)));
+    N3D_HIP(dev_malloc(&p->d_off32, sizeof(float4) * (nnz > 0 ? nnz : 1)));
     if (nnz > 0)
       hipLaunchKernelGGL(k_off32, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
                          (const double*)p->d_off, nnz, p->d_off32);
@@ -1126,10 +1132,10 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   // count exceeds 2^31 for large J=6 plans
   long long* cnt_s = nullptr;
   unsigned long long* cnt_g = nullptr;
-  N3D_HIP(hipMalloc(&cnt_s, sizeof(long long) * (nnz + 1)));
-  N3D_HIP(hipMalloc(&cnt_g, sizeof(unsigned long long) * (ngrid + 1)));
-  N3D_HIP(hipMalloc(&p->d_spos, sizeof(long long) * (nnz + 1)));
-  N3D_HIP(hipMalloc(&p->d_rowptr, sizeof(long long) * (ngrid + 1)));
+  N3D_HIP(dev_malloc(&cnt_s, sizeof(long long) * (nnz + 1)));
+  N3D_HIP(dev_malloc(&cnt_g, sizeof(unsigned long long) * (ngrid + 1)));
+  N3D_HIP(dev_malloc(&p->d_spos, sizeof(long long) * (nnz + 1)));
+  N3D_HIP(dev_malloc(&p->d_rowptr, sizeof(long long) * (ngrid + 1)));
   N3D_HIP(hipMemsetAsync(cnt_s, 0, sizeof(long long) * (nnz + 1), p->stream));
   N3D_HIP(hipMemsetAsync(cnt_g, 0, sizeof(unsigned long long) * (ngrid + 1), p->stream));
   if (nnz > 0)
@@ -1141,7 +1147,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
                                            p->stream));
   N3D_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, (const long long*)cnt_g, p->d_rowptr,
                                            (long long)(ngrid + 1), p->stream));
-  N3D_HIP(hipMalloc(&tmp, std::max(tb, tb2) + 16));
+  N3D_HIP(dev_malloc(&tmp, std::max(tb, tb2) + 16));
   N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt_s, p->d_spos, (long long)(nnz + 1),
                                            p->stream));
   N3D_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, (const long long*)cnt_g, p->d_rowptr,
@@ -1153,17 +1159,17 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   N3D_HIP(hipMemcpyAsync(hcnt.data(), cnt_g, sizeof(long long) * ngrid, hipMemcpyDeviceToHost,
                          p->stream));
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(tmp);
+  (void)dev_free(tmp);
   p->nhits = nhits;
   const long long nh = nhits > 0 ? nhits : 1;
   unsigned *hk = nullptr, *hks = nullptr;
   unsigned long long *hv = nullptr, *hvs = nullptr;
-  N3D_HIP(hipMalloc(&p->d_fcode, sizeof(unsigned short) * nh));
-  N3D_HIP(hipMalloc(&p->d_hit, sizeof(uint2) * nh));
-  N3D_HIP(hipMalloc(&hk, sizeof(unsigned) * nh));
-  N3D_HIP(hipMalloc(&hks, sizeof(unsigned) * nh));
-  N3D_HIP(hipMalloc(&hv, sizeof(unsigned long long) * nh));
-  N3D_HIP(hipMalloc(&hvs, sizeof(unsigned long long) * nh));
+  N3D_HIP(dev_malloc(&p->d_fcode, sizeof(unsigned short) * nh));
+  N3D_HIP(dev_malloc(&p->d_hit, sizeof(uint2) * nh));
+  N3D_HIP(dev_malloc(&hk, sizeof(unsigned) * nh));
+  N3D_HIP(dev_malloc(&hks, sizeof(unsigned) * nh));
+  N3D_HIP(dev_malloc(&hv, sizeof(unsigned long long) * nh));
+  N3D_HIP(dev_malloc(&hvs, sizeof(unsigned long long) * nh));
   if (nnz > 0)
     hipLaunchKernelGGL(k_hits_fill, dim3(nblocks(nnz)), dim3(256), 0, p->stream, p->d_base,
                        (const double*)p->d_off, nnz, hg, (const long long*)p->d_spos, hk, hv,
@@ -1173,7 +1179,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   tb = 0;
   N3D_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, hk, hks, hv, hvs, nhits, 0, key_bits,
                                              p->stream));
-  N3D_HIP(hipMalloc(&tmp, tb + 16));
+  N3D_HIP(dev_malloc(&tmp, tb + 16));
   if (nhits > 0) {
     N3D_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, hk, hks, hv, hvs, nhits, 0, key_bits,
                                                p->stream));
@@ -1181,8 +1187,8 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
                        (const unsigned long long*)hvs, nhits, p->d_hit);
   }
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(tmp); (void)hipFree(hk); (void)hipFree(hks); (void)hipFree(hv); (void)hipFree(hvs);
-  (void)hipFree(cnt_s); (void)hipFree(cnt_g); (void)hipFree(counts);
+  (void)dev_free(tmp); (void)dev_free(hk); (void)dev_free(hks); (void)dev_free(hv); (void)dev_free(hvs);
+  (void)dev_free(cnt_s); (void)dev_free(cnt_g); (void)dev_free(counts);
 
   // dense grid points (the k-space centre of radial trajectories) are split
   // into segments of chunk_len hits whose partial sums add in segment order;
@@ -1216,7 +1222,7 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
       brick_of[((size_t)z * by + y) * bx + x] = (int)i;
     }
     p->nbrick = (long long)tab.size();
-    N3D_HIP(hipMalloc(&p->d_brick, sizeof(int) * tab.size()));
+    N3D_HIP(dev_malloc(&p->d_brick, sizeof(int) * tab.size()));
     N3D_HIP(hipMemcpy(p->d_brick, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
     p->h_brick_of.swap(brick_of);
     // the LDS-staged spread's lists are built when that spread is asked for
@@ -1231,12 +1237,12 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   deap_host(p);
   const size_t nimg = (size_t)p->N[0] * p->N[1] * p->N[2];
   if (p->prec == 2) {
-    N3D_HIP(hipMalloc(&p->d_U, sizeof(double) * nimg));
+    N3D_HIP(dev_malloc(&p->d_U, sizeof(double) * nimg));
     N3D_HIP(hipMemcpy(p->d_U, p->h_U.data(), sizeof(double) * nimg, hipMemcpyHostToDevice));
   } else {
     std::vector<float> u32(nimg);
     for (size_t i = 0; i < nimg; ++i) u32[i] = (float)p->h_U[i];
-    N3D_HIP(hipMalloc(&p->d_U, sizeof(float) * nimg));
+    N3D_HIP(dev_malloc(&p->d_U, sizeof(float) * nimg));
     N3D_HIP(hipMemcpy(p->d_U, u32.data(), sizeof(float) * nimg, hipMemcpyHostToDevice));
   }
   for (int a = 0; a < 3; ++a) N3D_TRY(fft_setup_axis(p, a));
@@ -1270,11 +1276,11 @@ static void destroy_plan(nufft3d_plan* p) {
                   p->dense[1].d_hpt, p->d_brick, p->bl.d_ptr, p->bl.d_samp, p->bl.d_ent,
                   p->wl.d_beg, p->wl.d_end, p->wl.d_samp, p->wl.d_went};
   for (void* q : ptrs)
-    if (q) (void)hipFree(q);
+    if (q) (void)dev_free(q);
   for (int a = 0; a < 3; ++a)
     for (FftAxis* x : {&p->ax[a], &p->axN[a]})
       for (void* q : {x->d_tw, x->d_chirp, x->d_bhat[0], x->d_bhat[1]})
-        if (q) (void)hipFree(q);
+        if (q) (void)dev_free(q);
   if (p->mr_exec) (void)hipGraphExecDestroy(p->mr_exec);
   for (auto& r : p->prof.recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto& e : p->prof.pool) (void)hipEventDestroy(e);
@@ -1282,7 +1288,7 @@ static void destroy_plan(nufft3d_plan* p) {
   release(p->ws_part); release(p->ws_img); release(p->ws_solve); release(p->ws_red);
   release(p->ws_b);
   release(p->ws_host, true);
-  if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+  if (p->own_stream && p->stream) (void)dev_stream_destroy(p->stream);
   delete p;
 }
 
@@ -1663,7 +1669,7 @@ int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
   if (hipSetDevice(p->device) != hipSuccess) { (void)hipGetLastError(); set_error("hipSetDevice failed"); return NUFFT3D_ERR_DEVICE; }
   if (p->own_stream && p->stream) {
     (void)hipStreamSynchronize(p->stream);
-    (void)hipStreamDestroy(p->stream);
+    (void)dev_stream_destroy(p->stream);
   } else if (p->done_ev) {
     // the plan's workspaces are shared by every stream it runs on: work on
     // the new stream starts after the last entry point on the old one ended
@@ -1671,6 +1677,13 @@ int nufft3d_set_stream(nufft3d_plan* p, void* stream) {
   }
   p->own_stream = false;
   p->stream = ns;
+  return NUFFT3D_OK;
+}
+
+int nufft3d_graph_stats(const nufft3d_plan* p, int64_t* captured, int64_t* failed) {
+  if (!p) { set_error("plan is NULL"); return NUFFT3D_ERR_INVALID; }
+  if (captured) *captured = p->graphs_captured;
+  if (failed) *failed = p->graphs_failed;
   return NUFFT3D_OK;
 }
 
@@ -2059,11 +2072,11 @@ static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, int maxit,
     for (double v : full)
       if (!std::isfinite(v) || v < 0) { set_error("W must be finite and nonnegative"); return NUFFT3D_ERR_INVALID; }
   double* dW = nullptr;
-  N3D_HIP(hipMalloc(&dW, sizeof(double) * M));
+  N3D_HIP(dev_malloc(&dW, sizeof(double) * M));
   N3D_HIP(hipMemcpyAsync(dW, full.data(), sizeof(double) * M, hipMemcpyHostToDevice, p->stream));
   const size_t esz = p->prec == 2 ? sizeof(double) : sizeof(float);
   void* wd = nullptr;
-  N3D_HIP(hipMalloc(&wd, esz * (p->nnz > 0 ? p->nnz : 1)));
+  N3D_HIP(dev_malloc(&wd, esz * (p->nnz > 0 ? p->nnz : 1)));
   if (p->nnz > 0) {
     if (p->prec == 2)
       hipLaunchKernelGGL((k_gather_real<double>), dim3(nblocks(p->nnz)), dim3(256), 0, p->stream,
@@ -2075,7 +2088,7 @@ static int make_weights(nufft3d_plan* p, const double* W, int64_t nW, int maxit,
                          (long long)p->nnz, (float*)wd);
   }
   N3D_HIP(hipStreamSynchronize(p->stream));
-  (void)hipFree(dW);
+  (void)dev_free(dW);
   *out = wd;
   return NUFFT3D_OK;
 }
@@ -2123,7 +2136,7 @@ int nufft3d_inverse(nufft3d_plan* p, const void* raw, int32_t nc, int32_t maxit,
   }
   release(tmp_out);
   release(tmp_io);
-  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (wsamp != p->d_dens) (void)dev_free(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
@@ -2173,7 +2186,7 @@ int nufft3d_inverse_phase(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
     st = NUFFT3D_ERR_DEVICE;
   }
   release(tmp_out); release(tmp_io); release(tmp_x); release(tmp_P); release(tmp_b);
-  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (wsamp != p->d_dens) (void)dev_free(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
@@ -2218,7 +2231,7 @@ int nufft3d_inverse_cs(nufft3d_plan* p, const void* raw, int32_t nc, int32_t max
     st = NUFFT3D_ERR_DEVICE;
   }
   release(tmp_out); release(tmp_io);
-  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (wsamp != p->d_dens) (void)dev_free(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
@@ -2643,9 +2656,9 @@ int nufft3d_inverse_pruno(nufft3d_plan* p, const void* raw, int32_t nc, int32_t 
     set_error("device error in iNUFT (pruno)");
     st = NUFFT3D_ERR_DEVICE;
   }
-  if (dB) (void)hipFree(dB);
+  if (dB) (void)dev_free(dB);
   release(tmp_out); release(tmp_b); release(tmp_x0); release(tmp_k); release(tmp_io);
-  if (wsamp != p->d_dens) (void)hipFree(wsamp);
+  if (wsamp != p->d_dens) (void)dev_free(wsamp);
   if (info) info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

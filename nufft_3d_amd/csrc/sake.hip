@@ -158,12 +158,12 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)dev_free(p);
   }
 };
 
 int dev_alloc(DevBuf& b, size_t bytes) {
-  if (hipMalloc(&b.p, bytes ? bytes : 16) != hipSuccess) {
+  if (dev_malloc(&b.p, bytes ? bytes : 16) != hipSuccess) {
     (void)hipGetLastError();
     b.p = nullptr;
     set_error("device allocation of " + std::to_string(bytes) + " bytes failed");

@@ -452,12 +452,16 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
         // the rotation state at the block's start: a capture that stops
         // part-way leaves the pointers rotated by fewer than a full period
         C* const rot[6] = {v, r1, r2, y, w, w2};
-        bool ok = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        bool ok;
         int st_cap = NUFFT3D_OK;
-        if (ok) {
-          for (int k = 0; k < kGraphBlock && st_cap == NUFFT3D_OK; ++k)
-            st_cap = (test_capture_fail_at == k + 1) ? NUFFT3D_ERR_DEVICE : iterate(itn + k);
-          ok = hipStreamEndCapture(p->stream, &gph) == hipSuccess && st_cap == NUFFT3D_OK && gph;
+        {
+          capture_scope cs;   // no device allocation or release anywhere meanwhile
+          ok = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+          if (ok) {
+            for (int k = 0; k < kGraphBlock && st_cap == NUFFT3D_OK; ++k)
+              st_cap = (test_capture_fail_at == k + 1) ? NUFFT3D_ERR_DEVICE : iterate(itn + k);
+            ok = hipStreamEndCapture(p->stream, &gph) == hipSuccess && st_cap == NUFFT3D_OK && gph;
+          }
         }
         if (!ok) {   // nothing captured ran: back to the block's start
           v = rot[0]; r1 = rot[1]; r2 = rot[2]; y = rot[3]; w = rot[4]; w2 = rot[5];
@@ -469,7 +473,9 @@ static int minres_core(nufft3d_plan* p, const void* b, long long n, int nc, int 
         if (ok) {
           p->mr_exec = ex;
           p->mr_key = key;
+          ++p->graphs_captured;
         } else {
+          ++p->graphs_failed;
           // nothing captured ran and the buffers are back at the block's
           // start: run this block eagerly from here on
           graphs = false;
@@ -1039,14 +1045,14 @@ static int fista_t(nufft3d_plan* p, const void* bv, int nc, int maxit, double da
   size_t sort_bytes = 0;
   struct Free {
     void** q[3];
-    ~Free() { for (void** a : q) if (*a) (void)hipFree(*a); }
+    ~Free() { for (void** a : q) if (*a) (void)dev_free(*a); }
   } fr{{(void**)&keys, (void**)&sorted, &sort_tmp}};
   if (nz > 0) {
-    N3D_HIP(hipMalloc(&keys, sizeof(double) * n));
-    N3D_HIP(hipMalloc(&sorted, sizeof(double) * n));
+    N3D_HIP(dev_malloc(&keys, sizeof(double) * n));
+    N3D_HIP(dev_malloc(&sorted, sizeof(double) * n));
     N3D_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, sorted, (int)n, 0, 64,
                                               p->stream));
-    N3D_HIP(hipMalloc(&sort_tmp, sort_bytes ? sort_bytes : 16));
+    N3D_HIP(dev_malloc(&sort_tmp, sort_bytes ? sort_bytes : 16));
   }
   if (info) {
     info->nrhs = 1;

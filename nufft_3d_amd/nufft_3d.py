@@ -143,6 +143,12 @@ class nufft_3d:
         capture). Results are the same."""
         _lib.check(self._lib.nufft3d_set_graphs(self._h, 1 if enable else 0))
 
+    def graph_stats(self):
+        """(captured, failed) MINRES graph blocks of this plan so far."""
+        a, b = C.c_int64(), C.c_int64()
+        _lib.check(self._lib.nufft3d_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     # ------------------------------------------------------------ helpers
     @property
     def cdtype(self):

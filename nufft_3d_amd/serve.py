@@ -109,10 +109,11 @@ def io_bytes(N, M, nc, kind):
 class GpuBackend:
     """Runs batches through the HIP library on torch-managed device memory."""
 
-    def __init__(self, sizes, device=0, max_coils=32, seed=11):
+    def __init__(self, sizes, device=0, max_coils=32, seed=11, graphs=True):
         import torch
         from . import synth
         self.torch = torch
+        self.graphs = graphs
         self.dev = torch.device("cuda", device)
         self.images, self.maps = {}, {}
         for N in sizes:
@@ -135,10 +136,13 @@ class GpuBackend:
     def make_plan(self, req):
         from .nufft_3d import nufft_3d
         plan = nufft_3d(req.om, req.N, gpu=1, device=self.dev.index or 0)
-        # the main thread frees evicted plans and trimmed workspaces while
-        # slots run; HIP invalidates a graph capture in progress on another
-        # thread when memory is freed, so MINRES runs launch by launch here
-        plan.set_graphs(False)
+        # MINRES blocks run as HIP graphs: the library holds its device
+        # allocations and releases (plan evictions, workspace trims and
+        # growth, plan builds on other slots) while any thread captures, so
+        # a capture is not invalidated by another thread's free; a capture
+        # that still fails falls back to eager launches (bitwise the same)
+        if not self.graphs:
+            plan.set_graphs(False)
         return plan
 
     def plan_bytes(self, plan):

@@ -632,3 +632,53 @@ def test_grids_narrower_than_the_footprint_against_oracle(N, u, J, radial, tol64
     want = orc.iNUFT(raw, 5, 0.05)
     assert rel(g64.iNUFT(raw, 5, 0.05), want) < max(1e-10, 100 * tol64)
     assert rel(g32.iNUFT(raw, 5, 0.05), want) < TOL32
+
+
+def test_graph_captures_survive_other_threads_allocating_and_freeing():
+    # one thread re-captures MINRES graph blocks on its own stream while
+    # another builds and destroys plans and trims workspaces (hipMalloc /
+    # hipFree); the library holds its allocations and releases while a
+    # capture is in progress, so no capture fails and every result equals
+    # the launch-by-launch solve bitwise
+    import threading
+    import torch
+    om = synth.koosh_ball(16, 80, 32)
+    p = nufft_3d(om, 16, gpu=1)
+    raw = _rand((om.shape[1], 2), 91).astype(np.complex64)
+    p.set_graphs(False)
+    want = p.iNUFT(raw, 14, 0.05)
+    stop = threading.Event()
+    churn = {"plans": 0}
+
+    def other():
+        om2 = synth.koosh_ball(12, 40, 24)
+        q = nufft_3d(om2, 12, gpu=1)
+        while not stop.is_set():
+            t = nufft_3d(om2, 12, gpu=1)
+            t.aNUFT(t.fNUFT(_rand((12, 12, 12, 1), 92).astype(np.complex64)))
+            t.close()
+            q.fNUFT(_rand((12, 12, 12, 3), 93).astype(np.complex64))
+            q.release_workspaces()
+            churn["plans"] += 1
+        q.close()
+
+    th = threading.Thread(target=other)
+    th.start()
+    got = []
+    s = torch.cuda.Stream()
+    try:
+        with torch.cuda.stream(s):
+            for _ in range(12):
+                p.set_graphs(True)            # a fresh capture on every call
+                got.append(p.iNUFT(raw, 14, 0.05))
+                p.set_graphs(False)
+            s.synchronize()
+    finally:
+        stop.set()
+        th.join()
+    captured, failed = p.graph_stats()
+    print("captured %d failed %d, other thread built %d plans" % (captured, failed, churn["plans"]),
+          flush=True)
+    assert churn["plans"] > 0 and captured >= 12 and failed == 0
+    for g in got:
+        assert np.array_equal(g, want)

@@ -589,7 +589,14 @@ __global__ void __launch_bounds__(256) k_spread(const long long* __restrict__ ro
     const int t = bricks[lin / 512];
     const int l = (int)(lin % 512);
     const int tx = t & 1023, ty = (t >> 10) & 1023, tz = t >> 20;
-    const int x = tx * 8 + (l & 7), y = ty * 8 + ((l >> 3) & 7), z = tz * 8 + (l >> 6);
+    // Z-order within the brick, so a wave's points form a compact 4x4x2
+    // block that shares more sample rows (256^3, J=6: 23.1 -> 22.7 ms; equal
+    // at 128^3, J=4); tune flag 2048: x fastest
+    int lx = (l & 1) | ((l >> 2) & 2) | ((l >> 4) & 4);
+    int ly = ((l >> 1) & 1) | ((l >> 3) & 2) | ((l >> 5) & 4);
+    int lz = ((l >> 2) & 1) | ((l >> 4) & 2) | ((l >> 6) & 4);
+    if (g.flags & 2048) { lx = l & 7; ly = (l >> 3) & 7; lz = l >> 6; }
+    const int x = tx * 8 + lx, y = ty * 8 + ly, z = tz * 8 + lz;
     if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
     gpt = ((long long)z * g.K2 + y) * g.K1 + x;
   }

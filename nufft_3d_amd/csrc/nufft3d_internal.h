@@ -193,7 +193,8 @@ struct nufft3d_plan {
                             // fp64 arithmetic behind fp32 arrays, not the kernel table,
                             // 512 = no split-precision adjoint (see split_precision),
                             // 1024 = dense-point segments in (segment index, brick)
-                            // order instead of by their first sample)
+                            // order instead of by their first sample, 2048 = the
+                            // spread's points x-fastest within a brick, not Z-order)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)

@@ -819,6 +819,127 @@ __global__ void __launch_bounds__(512) k_spread_tile_w(const long long* __restri
   }
 }
 
+// Split-precision adjoint from the block-staged lists (BlockLists): one
+// wavefront per piece (a run of chunks of one 4x4x4 block), one lane per
+// point of the block, 8 coils. A chunk's 64 sample rows (64 bytes of float
+// coils each) are staged in the wave's LDS slice, one row per lane, then
+// every lane walks its entries of the chunk, 4 steps per load pair, and sums
+// the exact products of fp32 weights and fp32 samples in double. A point's
+// sum runs over its chunks in order and within a chunk in list order, fixed
+// at plan time: deterministic (not bitwise the gather's order). The four
+// waves of a work-group run independent pieces, with no work-group barrier.
+constexpr int kBlkRow4 = 5;   // 16-byte units per staged row: 4 of data, 1 of padding (LDS banks)
+
+// LDS writes of a wavefront visible to its own later reads
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pieces,
+                                                    long long npiece, const int* __restrict__ blk,
+                                                    const long long* __restrict__ cbase,
+                                                    const int* __restrict__ samp,
+                                                    const long long* __restrict__ eoff,
+                                                    const unsigned* __restrict__ eslot,
+                                                    const float4* __restrict__ ew,
+                                                    const float2* __restrict__ ks,
+                                                    double2* __restrict__ grid,
+                                                    double2* __restrict__ partial, GridGeom g) {
+  __shared__ float4 s_row[4 * kBlkChunk * kBlkRow4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long pi = xcd_block(g.xchunk) * 4 + wv;
+  if (pi >= npiece) return;
+  const int4 pc = pieces[pi];
+  const int bc = blk[pc.x];
+  const int x = (bc & 1023) * 4 + (lane & 3), y = ((bc >> 10) & 1023) * 4 + ((lane >> 2) & 3),
+            z = (bc >> 20) * 4 + (lane >> 4);
+  double ar[8], ai[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
+  const long long cb = cbase[pc.x];
+  float4* srow = s_row + wv * (kBlkChunk * kBlkRow4);
+  const float4* ks4 = reinterpret_cast<const float4*>(ks);
+  const long long rs4 = g.nc / 2;   // 16-byte units per sample row
+  const int co4 = g.c0 / 2;
+  for (int c = pc.y; c < pc.z; ++c) {
+    const long long ch = cb + c;
+    const int sm = samp[ch * kBlkChunk + lane];
+    const float4* src = ks4 + (long long)sm * rs4 + co4;
+    const float4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+    wave_lds_sync();   // the previous chunk's rows are consumed
+    float4* dst = srow + lane * kBlkRow4;
+    dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
+    wave_lds_sync();
+    const long long q0 = eoff[ch], q1 = eoff[ch + 1];
+    // entry groups loaded one group ahead (index clamped into the chunk)
+    unsigned sl4 = 0xffffffffu;
+    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q0 < q1) { sl4 = eslot[q0 * 64 + lane]; w4 = ew[q0 * 64 + lane]; }
+    for (long long q = q0; q < q1; ++q) {
+      const long long qn = q + 1 < q1 ? q + 1 : q;
+      const unsigned nsl4 = eslot[qn * 64 + lane];
+      const float4 nw4 = ew[qn * 64 + lane];
+      const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned sl = (sl4 >> (8 * k)) & 255u;
+        if (sl != 255u) {
+          const float4* row = srow + sl * kBlkRow4;
+          const double w = (double)wk[k];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float4 v = row[j];
+            ar[2 * j] += w * (double)v.x;
+            ai[2 * j] += w * (double)v.y;
+            ar[2 * j + 1] += w * (double)v.z;
+            ai[2 * j + 1] += w * (double)v.w;
+          }
+        }
+      }
+      sl4 = nsl4;
+      w4 = nw4;
+    }
+  }
+  if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
+  double2* o = pc.w < 0 ? grid + (((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0
+                        : partial + ((long long)pc.w * 64 + lane) * 8;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = make_double2(ar[c], ai[c]);
+}
+
+// Blocks in several pieces: each point's partial sums added in piece order.
+// One work-group per block, one thread per (point, coil): 8 consecutive
+// threads read a point's 128-byte partial row.
+__global__ void __launch_bounds__(512) k_spread_blk_reduce(const int4* __restrict__ red,
+                                                           const int* __restrict__ blk,
+                                                           const double2* __restrict__ partial,
+                                                           double2* __restrict__ grid, GridGeom g) {
+  const int4 r = red[blockIdx.x];   // (block, first partial slot, pieces, 0)
+  const int lane = threadIdx.x >> 3, c = threadIdx.x & 7;
+  const int bc = blk[r.x];
+  const int x = (bc & 1023) * 4 + (lane & 3), y = ((bc >> 10) & 1023) * 4 + ((lane >> 2) & 3),
+            z = (bc >> 20) * 4 + (lane >> 4);
+  if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
+  const double2* pp = partial + ((long long)r.y * 64 + lane) * 8 + c;
+  double ax = 0, ay = 0;
+  int j = 0;
+  for (; j + 4 <= r.z; j += 4) {   // four pieces' loads in flight, added in piece order
+    const double2 v0 = pp[0], v1 = pp[512], v2 = pp[1024], v3 = pp[1536];
+    ax += v0.x; ay += v0.y;
+    ax += v1.x; ay += v1.y;
+    ax += v2.x; ay += v2.y;
+    ax += v3.x; ay += v3.y;
+    pp += 2048;
+  }
+  for (; j < r.z; ++j, pp += 512) {
+    const double2 v = pp[0];
+    ax += v.x; ay += v.y;
+  }
+  grid[(((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0 + c] = make_double2(ax, ay);
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const uint2* __restrict__ hits, bool pw,
@@ -1275,6 +1396,27 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   using G = typename std::conditional<sizeof(R) == 4, double2, cplx<R>>::type;
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
+  // the block-staged split adjoint (8-coil groups; its partials take the
+  // workspace spread() sized for them)
+  constexpr bool kBlk = sizeof(R) == 4 && COMP && FULL && LPS * CPL == 8;
+  if (kBlk && block_staged(p, cg, wsamp != nullptr) && wide) {
+    {
+      ProfScope ps(p, "k_spread_blk");
+      hipLaunchKernelGGL(k_spread_blk, dim3((unsigned)((p->bk.npiece + 3) / 4)), dim3(256), 0,
+                         p->stream, (const int4*)p->bk.d_piece, (long long)p->bk.npiece,
+                         (const int*)p->bk.d_blk, (const long long*)p->bk.d_cbase,
+                         (const int*)p->bk.d_samp, (const long long*)p->bk.d_eoff,
+                         (const unsigned*)p->bk.d_slot, (const float4*)p->bk.d_w,
+                         (const float2*)samp, (double2*)grid, (double2*)partial, g);
+    }
+    if (p->bk.nred > 0) {
+      ProfScope ps(p, "k_spread_blk_reduce");
+      hipLaunchKernelGGL(k_spread_blk_reduce, dim3((unsigned)p->bk.nred), dim3(512), 0, p->stream,
+                         (const int4*)p->bk.d_red, (const int*)p->bk.d_blk,
+                         (const double2*)partial, (double2*)grid, g);
+    }
+    return;
+  }
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
   // one lane per point (small fp32 groups, every fp64 group): short dense
   // segments; lane groups: long ones
@@ -1360,7 +1502,8 @@ template <typename R, bool COMP, bool SPREAD>
 static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void* partial,
                        const void* wsamp, double scale, bool user = false, bool split = false) {
   const bool wide = split;
-  const int gmax = p->coil_group;
+  // the block-staged split adjoint takes 8-coil groups whatever the group size
+  const int gmax = SPREAD && split && p->bk.d_piece && (p->tune_flags & 4096) ? 8 : p->coil_group;
   for (int c0 = 0; c0 < nc; c0 += gmax) {
     const int cg = nc - c0 < gmax ? nc - c0 : gmax;
     // full groups with an even coil stride and offset take aligned vector loads
@@ -1469,8 +1612,11 @@ int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layou
     const int cgmax = nc < p->coil_group ? nc : p->coil_group;
     void* partial = nullptr;
     const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
-    if (nseg > 0) {
-      N3D_TRY(ensure(p->ws_part, (size_t)nseg * cgmax * 2 * sizeof(double)));
+    size_t pbytes = (size_t)nseg * cgmax * 2 * sizeof(double);
+    if (p->bk.d_piece && (p->tune_flags & 4096))   // block-staged pieces: 64 points x 8 coils
+      pbytes = std::max(pbytes, (size_t)p->bk.npslot * 64 * 8 * sizeof(double2));
+    if (pbytes > 0) {
+      N3D_TRY(ensure(p->ws_part, pbytes));
       partial = p->ws_part.p;
     }
     run_groups<float, true, true>(p, samp, nc, grid, partial, wsamp, 1.0, false, true);

@@ -146,6 +146,38 @@ struct WideLists {
   bool tried = false;
 };
 
+// Block-staged adjoint (k_spread_blk): the grid in 4x4x4 blocks, one
+// wavefront per block and one lane per point. A block's samples (the
+// distinct samples that touch any of its points, ascending ids) are dealt
+// round-robin into chunks of kBlkChunk: sample i of the list goes to chunk
+// i % nch at slot i / nch, so each chunk is an even subsample of the block's
+// neighbourhood and a point meets about the same share of its hits in every
+// chunk (contiguous chunks leave most lanes idle: a run of sorted samples is
+// one small region). A chunk's sample rows pass through the wave's LDS; its
+// entries are groups of 4 steps x 64 lanes (4 slot bytes per lane in one
+// word, 4 fp32 weights per lane in one float4), lanes with fewer hits
+// padded with slot 255 and weight 0. Blocks with more entry rows than a
+// piece holds are split into pieces whose partial sums add in piece order.
+constexpr int kBlkChunk = 64;       // samples staged per chunk
+constexpr int kBlkPieceRows = 256;  // entry rows (steps) per piece, at least one chunk
+struct BlockLists {
+  long long nblk = 0;               // blocks, in launch order (Z-order of block coordinates)
+  int* d_blk = nullptr;             // [nblk] x | y << 10 | z << 20 (block coordinates)
+  long long* d_cbase = nullptr;     // [nblk + 1] first chunk of each block
+  long long nchunk = 0;
+  int* d_samp = nullptr;            // [nchunk * kBlkChunk] sample ids (padding: the chunk's first)
+  long long* d_eoff = nullptr;      // [nchunk + 1] first entry group of each chunk
+  unsigned* d_slot = nullptr;       // [groups * 64] 4 slots per lane (byte k: step 4j+k; 255 none)
+  float4* d_w = nullptr;            // [groups * 64] 4 weights per lane (0 for padding)
+  int4* d_piece = nullptr;          // [npiece] (block, first chunk, end chunk, partial slot or -1)
+  long long npiece = 0;
+  int4* d_red = nullptr;            // [nred] (block, first partial slot, pieces, 0)
+  long long nred = 0;
+  long long npslot = 0;
+  long long ngroups = 0;
+  bool tried = false;
+};
+
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
@@ -194,7 +226,9 @@ struct nufft3d_plan {
                             // 512 = no split-precision adjoint (see split_precision),
                             // 1024 = dense-point segments in (segment index, brick)
                             // order instead of by their first sample, 2048 = the
-                            // spread's points x-fastest within a brick, not Z-order)
+                            // spread's points x-fastest within a brick, not Z-order,
+                            // 4096 = the block-staged split adjoint where the plan
+                            // has its lists)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
@@ -243,6 +277,7 @@ struct nufft3d_plan {
   std::vector<int> h_brick_of;      // brick (x,y,z) -> traversal index, for building bl
   bool bl_tried = false;
   nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
+  nufft3d::BlockLists bk;           // block-staged lists of the split-precision adjoint
   int max_hits = 0;
 
   // LDS-tiled forward interpolation (3-D plans with a 7-point footprint,
@@ -291,6 +326,13 @@ namespace nufft3d {
 inline bool split_precision(const nufft3d_plan* p) {
   return p->prec == 2 && p->io_prec == 1 && !(p->tune_flags & (128 | 512));
 }
+
+// the block-staged split adjoint (k_spread_blk) runs for this launch: its
+// lists are built, tune flag 4096 asks for it, an 8-coil group, no data weights
+inline bool block_staged(const nufft3d_plan* p, int cg, bool wsamp) {
+  return p->bk.d_piece && (p->tune_flags & 4096) && cg == 8 && !wsamp && split_precision(p);
+}
+int ensure_block_lists(nufft3d_plan* p);   // builds them once if the plan qualifies
 
 int ensure(Workspace& w, size_t bytes);
 void prof_begin(nufft3d_plan* p, const char* name);

@@ -927,6 +927,371 @@ static int ensure_wide_lists(nufft3d_plan* p) {
   return build_wide_lists(p, p->h_brick_of, bx, by, p->dense[1].rmax);
 }
 
+// ---------------------------------------------------------------- block-staged adjoint lists
+// keys (block << 32 | sample) of every hit of grid points [g0, g1) (hits
+// [e0, ...)), blocks numbered in launch order by blk_of
+__global__ void k_bk_keys(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
+                          long long g0, long long g1, long long e0, int K1, int K2,
+                          const int* __restrict__ blk_of, int nbx, int nby,
+                          unsigned long long* __restrict__ keys) {
+  const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const unsigned long long bk =
+      (unsigned long long)blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
+  for (long long e = a; e < b; ++e) keys[e - e0] = (bk << 32) | hits[e].x;
+}
+
+// per hit of grid points [g0, g1): the slot of its sample in its block's
+// ascending list asc[beg[b], end[b])
+__global__ void k_bk_slot(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
+                          long long g0, long long g1, int K1, int K2,
+                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const int* __restrict__ asc, const long long* __restrict__ beg,
+                          const long long* __restrict__ end, unsigned* __restrict__ hslot) {
+  const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= g1) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  if (a == b) return;
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  const int bk = blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
+  const long long base = beg[bk], len = end[bk] - base;
+  long long lo = 0;   // hits are in ascending sample order: search from the last slot
+  for (long long e = a; e < b; ++e) {
+    const unsigned s = hits[e].x;
+    long long l = lo, r = len;
+    while (l < r) {
+      const long long m = (l + r) >> 1;
+      if ((unsigned)asc[base + m] < s) l = m + 1; else r = m;
+    }
+    lo = l;
+    hslot[e] = (unsigned)l;
+  }
+}
+
+// a block's list dealt into its chunks: list entry i -> chunk i % nch, slot
+// i / nch; the unused slots of a chunk hold its first sample (never read)
+__global__ void k_bk_samp(const int* __restrict__ asc, const long long* __restrict__ beg,
+                          const long long* __restrict__ end, const long long* __restrict__ cbase,
+                          int* __restrict__ samp) {
+  const long long b = blockIdx.x;
+  const long long base = beg[b], u = end[b] - base;
+  const long long c0 = cbase[b], nch = cbase[b + 1] - c0;
+  for (long long i = threadIdx.x; i < nch * kBlkChunk; i += blockDim.x) {
+    const long long c = i / kBlkChunk, o = i % kBlkChunk;
+    const long long li = o * nch + c;
+    samp[(c0 + c) * kBlkChunk + o] = asc[base + (li < u ? li : c)];
+  }
+}
+
+__device__ __forceinline__ void bk_point(long long gi, int K1, int K2, const int* blk_of, int nbx,
+                                         int nby, int& bk, int& lane) {
+  const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
+  bk = blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
+  lane = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+}
+
+// per point: the rank of each hit among the point's hits of the same chunk
+// (its step in that chunk); the point's thread owns its counters, so the
+// ranks follow the point's list order
+__global__ void k_bk_rank(const long long* __restrict__ rowptr, long long ngrid, int K1, int K2,
+                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const long long* __restrict__ cbase, const unsigned* __restrict__ hslot,
+                          unsigned short* __restrict__ cnt, unsigned short* __restrict__ hk) {
+  const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ngrid) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  if (a == b) return;
+  int bk, lane;
+  bk_point(gi, K1, K2, blk_of, nbx, nby, bk, lane);
+  const long long c0 = cbase[bk], nch = cbase[bk + 1] - c0;
+  for (long long e = a; e < b; ++e) {
+    const long long c = (long long)hslot[e] % nch;
+    unsigned short* q = cnt + (c0 + c) * 64 + lane;
+    const unsigned short k = *q;
+    *q = (unsigned short)(k + 1);
+    hk[e] = k;
+  }
+}
+
+// entry groups of each chunk: its largest per-lane step count over 4, rounded up
+__global__ void k_bk_groups(const unsigned short* __restrict__ cnt, long long nchunk,
+                            long long* __restrict__ groups) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunk) return;
+  int m = 0;
+  for (int l = 0; l < 64; ++l) m = max(m, (int)cnt[c * 64 + l]);
+  groups[c] = (m + 3) / 4;
+}
+
+__global__ void k_bk_fill(const long long* __restrict__ rowptr, long long ngrid, int K1, int K2,
+                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const long long* __restrict__ cbase, const long long* __restrict__ eoff,
+                          const unsigned* __restrict__ hslot, const unsigned short* __restrict__ hk,
+                          const uint2* __restrict__ hitw, unsigned char* __restrict__ slot,
+                          float* __restrict__ w) {
+  const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= ngrid) return;
+  const long long a = rowptr[gi], b = rowptr[gi + 1];
+  if (a == b) return;
+  int bk, lane;
+  bk_point(gi, K1, K2, blk_of, nbx, nby, bk, lane);
+  const long long c0 = cbase[bk], nch = cbase[bk + 1] - c0;
+  for (long long e = a; e < b; ++e) {
+    const long long sl = hslot[e];
+    const long long c = sl % nch, o = sl / nch;
+    const int k = hk[e];
+    const long long q = ((eoff[c0 + c] + k / 4) * 64 + lane) * 4 + (k & 3);
+    slot[q] = (unsigned char)o;
+    w[q] = __uint_as_float(hitw[e].y);
+  }
+}
+
+static void drop_block_lists(nufft3d_plan* p) {
+  BlockLists& bk = p->bk;
+  for (void* q : {(void*)bk.d_blk, (void*)bk.d_cbase, (void*)bk.d_samp, (void*)bk.d_eoff,
+                  (void*)bk.d_slot, (void*)bk.d_w, (void*)bk.d_piece, (void*)bk.d_red})
+    if (q) (void)dev_free(q);
+  const bool tried = bk.tried;
+  bk = BlockLists();
+  bk.tried = tried;
+}
+
+// BlockLists of a split-precision 3-D plan, from the plan's point-major hit
+// lists and weight table. The (block, sample) pairs are sorted one z-slab of
+// blocks at a time (any total hit count; a slab needs fewer than 2^31 hits).
+static int build_block_lists(nufft3d_plan* p) {
+  BlockLists& bk = p->bk;
+  const long long nh = p->nhits;
+  if (nh <= 0 || !p->d_hitw || p->K[2] == 1) return NUFFT3D_OK;
+  const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
+  const long long plane = (long long)K1 * K2, ngrid = plane * K3;
+  const int nbx = (K1 + 3) / 4, nby = (K2 + 3) / 4, nbz = (K3 + 3) / 4;
+  const long long nblk = (long long)nbx * nby * nbz;
+  if (nblk >= (1LL << 30)) return NUFFT3D_OK;
+  std::vector<long long> eb(nbz + 1);
+  for (int z = 0; z <= nbz; ++z) {
+    const long long gz = std::min<long long>(4LL * z, K3) * plane;
+    N3D_HIP(hipMemcpy(&eb[z], p->d_rowptr + gz, sizeof(long long), hipMemcpyDeviceToHost));
+  }
+  long long smax = 0;
+  for (int z = 0; z < nbz; ++z) smax = std::max(smax, eb[z + 1] - eb[z]);
+  if (smax >= 0x7fffffffLL) return NUFFT3D_OK;
+  // launch order: Z-order of the block coordinates
+  std::vector<int> blk(nblk), blk_of(nblk);
+  {
+    int bits = 1;
+    while ((1 << bits) < std::max(nbx, std::max(nby, nbz))) ++bits;
+    std::vector<std::pair<unsigned long long, int>> ord;
+    ord.reserve(nblk);
+    for (int z = 0; z < nbz; ++z)
+      for (int y = 0; y < nby; ++y)
+        for (int x = 0; x < nbx; ++x)
+          ord.push_back({(unsigned long long)morton3(x, y, z, bits), x | (y << 10) | (z << 20)});
+    std::sort(ord.begin(), ord.end());
+    for (long long i = 0; i < nblk; ++i) {
+      blk[i] = ord[i].second;
+      const int x = blk[i] & 1023, y = (blk[i] >> 10) & 1023, z = blk[i] >> 20;
+      blk_of[((long long)z * nby + y) * nbx + x] = (int)i;
+    }
+  }
+  int bbits = 1;
+  while ((1LL << bbits) <= nblk) ++bbits;
+  int* d_bof = nullptr;
+  unsigned long long *keys = nullptr, *skeys = nullptr;
+  int* d_nsel = nullptr;
+  void* tmp = nullptr;
+  int* asc = nullptr;
+  long long *beg = nullptr, *end = nullptr, *grp = nullptr;
+  unsigned* hslot = nullptr;
+  unsigned short *cnt = nullptr, *hk = nullptr;
+  auto cleanup = [&]() {
+    for (void* q : {(void*)d_bof, (void*)keys, (void*)skeys, (void*)d_nsel, tmp, (void*)asc,
+                    (void*)beg, (void*)end, (void*)grp, (void*)hslot, (void*)cnt, (void*)hk})
+      if (q) (void)dev_free(q);
+    d_bof = nullptr; keys = skeys = nullptr; d_nsel = nullptr; tmp = nullptr; asc = nullptr;
+    beg = end = grp = nullptr; hslot = nullptr; cnt = hk = nullptr;
+  };
+  // a plan that cannot hold the lists keeps the gather spread
+  auto fail = [&](const char* what) {
+    cleanup();
+    drop_block_lists(p);
+    (void)hipGetLastError();
+    if (p->verbose) printf("  block-staged adjoint lists not built: %s\n", what);
+    return NUFFT3D_OK;
+  };
+  const size_t nk = (size_t)std::max<long long>(smax, 1);
+  if (dev_malloc(&d_bof, sizeof(int) * nblk) != hipSuccess ||
+      dev_malloc(&keys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&skeys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&d_nsel, sizeof(int)) != hipSuccess ||
+      dev_malloc(&hslot, sizeof(unsigned) * nh) != hipSuccess ||
+      dev_malloc(&beg, sizeof(long long) * nblk) != hipSuccess ||
+      dev_malloc(&end, sizeof(long long) * nblk) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemcpy(d_bof, blk_of.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(beg, 0, sizeof(long long) * nblk, p->stream) != hipSuccess ||
+      hipMemsetAsync(end, 0, sizeof(long long) * nblk, p->stream) != hipSuccess)
+    return fail("copy failed");
+  size_t tb1 = 0, tb2 = 0;
+  if (hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nk, 0, 32 + bbits,
+                                        p->stream) != hipSuccess ||
+      hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nk, p->stream) != hipSuccess ||
+      dev_malloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess)
+    return fail("sort set-up failed");
+  // pass 1: per slab, the unique (block, sample) keys -> each block's list
+  // length; the lists themselves are kept (total = sum of the lengths)
+  std::vector<long long> nsel_slab(nbz, 0);
+  auto slab_keys = [&](int z, int& nsel) -> bool {
+    const long long g0 = 4LL * z * plane, g1 = std::min<long long>(4LL * z + 4, K3) * plane;
+    const int n = (int)(eb[z + 1] - eb[z]);
+    nsel = 0;
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_bk_keys, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
+                       p->d_hitw, g0, g1, eb[z], K1, K2, d_bof, nbx, nby, keys);
+    if (hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, n, 0, 32 + bbits, p->stream) != hipSuccess ||
+        hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, n, p->stream) != hipSuccess ||
+        hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+      return false;
+    return true;
+  };
+  long long total = 0;
+  for (int z = 0; z < nbz; ++z) {
+    int ns = 0;
+    if (!slab_keys(z, ns)) return fail("sort failed");
+    nsel_slab[z] = ns;
+    total += ns;
+  }
+  if (dev_malloc(&asc, sizeof(int) * std::max(1LL, total)) != hipSuccess)
+    return fail("allocation failed");
+  long long off = 0;
+  for (int z = 0; z < nbz; ++z) {
+    int ns = 0;
+    if (!slab_keys(z, ns) || ns != nsel_slab[z]) return fail("sort failed");
+    if (ns > 0)
+      hipLaunchKernelGGL(k_wl_lists, dim3(nblocks(ns)), dim3(256), 0, p->stream,
+                         (const unsigned long long*)keys, (long long)ns, off, asc, beg, end);
+    const long long g0 = 4LL * z * plane, g1 = std::min<long long>(4LL * z + 4, K3) * plane;
+    hipLaunchKernelGGL(k_bk_slot, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
+                       p->d_hitw, g0, g1, K1, K2, d_bof, nbx, nby, (const int*)asc,
+                       (const long long*)beg, (const long long*)end, hslot);
+    if (hipGetLastError() != hipSuccess) return fail("slot kernels failed");
+    off += ns;
+  }
+  (void)dev_free(keys); keys = nullptr;
+  (void)dev_free(skeys); skeys = nullptr;
+  (void)dev_free(tmp); tmp = nullptr;
+  // chunks per block (launch order), then the chunked lists
+  std::vector<long long> hb(nblk), he(nblk), cbase(nblk + 1, 0);
+  if (hipMemcpyAsync(hb.data(), beg, sizeof(long long) * nblk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+      hipMemcpyAsync(he.data(), end, sizeof(long long) * nblk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+      hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail("copy failed");
+  for (long long b = 0; b < nblk; ++b)
+    cbase[b + 1] = cbase[b] + (he[b] - hb[b] + kBlkChunk - 1) / kBlkChunk;
+  const long long nchunk = cbase[nblk];
+  if (dev_malloc(&bk.d_blk, sizeof(int) * nblk) != hipSuccess ||
+      dev_malloc(&bk.d_cbase, sizeof(long long) * (nblk + 1)) != hipSuccess ||
+      dev_malloc(&bk.d_samp, sizeof(int) * std::max(1LL, nchunk * kBlkChunk)) != hipSuccess ||
+      dev_malloc(&bk.d_eoff, sizeof(long long) * (nchunk + 1)) != hipSuccess ||
+      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * 64)) != hipSuccess ||
+      dev_malloc(&hk, sizeof(unsigned short) * nh) != hipSuccess ||
+      dev_malloc(&grp, sizeof(long long) * (nchunk + 1)) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemcpy(bk.d_blk, blk.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(bk.d_cbase, cbase.data(), sizeof(long long) * (nblk + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * 64), p->stream) != hipSuccess)
+    return fail("copy failed");
+  hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)nblk), dim3(256), 0, p->stream, (const int*)asc,
+                     (const long long*)beg, (const long long*)end, (const long long*)bk.d_cbase,
+                     bk.d_samp);
+  hipLaunchKernelGGL(k_bk_rank, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr, ngrid,
+                     K1, K2, d_bof, nbx, nby, (const long long*)bk.d_cbase, (const unsigned*)hslot,
+                     cnt, hk);
+  if (nchunk > 0)
+    hipLaunchKernelGGL(k_bk_groups, dim3(nblocks(nchunk)), dim3(256), 0, p->stream,
+                       (const unsigned short*)cnt, nchunk, grp);
+  if (hipGetLastError() != hipSuccess) return fail("list kernels failed");
+  std::vector<long long> hgrp(nchunk + 1, 0), eoff(nchunk + 1, 0);
+  if (nchunk > 0 &&
+      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nchunk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+       hipStreamSynchronize(p->stream) != hipSuccess))
+    return fail("copy failed");
+  for (long long c = 0; c < nchunk; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
+  const long long ngroups = eoff[nchunk];
+  if (dev_malloc(&bk.d_slot, sizeof(unsigned) * std::max(1LL, ngroups * 64)) != hipSuccess ||
+      dev_malloc(&bk.d_w, sizeof(float4) * std::max(1LL, ngroups * 64)) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemcpy(bk.d_eoff, eoff.data(), sizeof(long long) * (nchunk + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(bk.d_slot, 0xff, sizeof(unsigned) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess ||
+      hipMemsetAsync(bk.d_w, 0, sizeof(float4) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess)
+    return fail("copy failed");
+  hipLaunchKernelGGL(k_bk_fill, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr, ngrid,
+                     K1, K2, d_bof, nbx, nby, (const long long*)bk.d_cbase,
+                     (const long long*)bk.d_eoff, (const unsigned*)hslot, (const unsigned short*)hk,
+                     (const uint2*)p->d_hitw, (unsigned char*)bk.d_slot, (float*)bk.d_w);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail("fill failed");
+  // pieces: a block's chunks in runs of at most kBlkPieceRows entry rows
+  // (at least one chunk each); blocks in several pieces sum partials
+  std::vector<int4> pieces, red;
+  long long npslot = 0;
+  pieces.reserve(nblk + nblk / 4);
+  for (long long b = 0; b < nblk; ++b) {
+    const long long c0 = cbase[b], c1 = cbase[b + 1];
+    if (c0 == c1) {
+      pieces.push_back(make_int4((int)b, 0, 0, -1));
+      continue;
+    }
+    std::vector<std::pair<int, int>> runs;
+    long long c = c0;
+    while (c < c1) {
+      long long rows = 0, e = c;
+      while (e < c1 && (e == c || rows + 4 * hgrp[e] <= kBlkPieceRows)) rows += 4 * hgrp[e++];
+      runs.push_back({(int)(c - c0), (int)(e - c0)});
+      c = e;
+    }
+    if (runs.size() == 1) {
+      pieces.push_back(make_int4((int)b, runs[0].first, runs[0].second, -1));
+    } else {
+      red.push_back(make_int4((int)b, (int)npslot, (int)runs.size(), 0));
+      for (auto& r : runs) pieces.push_back(make_int4((int)b, r.first, r.second, (int)npslot++));
+    }
+  }
+  if (npslot >= 0x7fffffffLL || (long long)pieces.size() >= 0x7fffffffLL)
+    return fail("too many pieces");
+  if (dev_malloc(&bk.d_piece, sizeof(int4) * pieces.size()) != hipSuccess ||
+      dev_malloc(&bk.d_red, sizeof(int4) * std::max<size_t>(1, red.size())) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemcpy(bk.d_piece, pieces.data(), sizeof(int4) * pieces.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (!red.empty() &&
+       hipMemcpy(bk.d_red, red.data(), sizeof(int4) * red.size(), hipMemcpyHostToDevice) != hipSuccess))
+    return fail("copy failed");
+  cleanup();
+  bk.nblk = nblk;
+  bk.nchunk = nchunk;
+  bk.ngroups = ngroups;
+  bk.npiece = (long long)pieces.size();
+  bk.nred = (long long)red.size();
+  bk.npslot = npslot;
+  if (p->verbose)
+    printf("  block-staged adjoint: %lld blocks, %lld chunks, %lld entry groups (%.2f B/hit), "
+           "%lld pieces, %lld partial slots\n", nblk, nchunk, ngroups,
+           5.0 * 4 * 64 * ngroups / (double)nh, bk.npiece, npslot);
+  return NUFFT3D_OK;
+}
+
+// BlockLists when asked for (tune flag 4096 at plan creation) or timed by
+// the autotuner: split-precision 3-D plans with the weight table
+int ensure_block_lists(nufft3d_plan* p) {
+  if (p->bk.d_piece || p->bk.tried || !split_precision(p) || p->K[2] == 1 || !p->d_hitw)
+    return NUFFT3D_OK;
+  p->bk.tried = true;
+  return build_block_lists(p);
+}
+
 // Pieces of the tiled interpolation: runs of equal sub-bin key in the sorted
 // samples, cut into at most kPieceMax samples.
 constexpr int kPieceMax = 128;   // one sample per 4-lane group of a 512-thread work-group
@@ -1239,6 +1604,15 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   }
   N3D_TRY(build_hit_weights(p));
   if (p->tune_flags & 8) N3D_TRY(ensure_wide_lists(p));
+  if (p->tune_flags & 4096) N3D_TRY(ensure_block_lists(p));
+  // fp64-arithmetic plans read only the weight table in the spread (unless
+  // tune flag 128 evaluates the weights) and build no code-based lists, so
+  // their (sample, code) copy of the lists goes (8 B per hit)
+  if (p->prec == 2 && p->d_hitw && p->d_hit && !(p->tune_flags & 128)) {
+    N3D_HIP(hipStreamSynchronize(p->stream));
+    N3D_HIP(dev_free(p->d_hit));
+    p->d_hit = nullptr;
+  }
   N3D_TRY(build_kb_table(p));
 
   // deapodization (deap.m) and FFT twiddles
@@ -1285,6 +1659,7 @@ static void destroy_plan(nufft3d_plan* p) {
                   p->wl.d_beg, p->wl.d_end, p->wl.d_samp, p->wl.d_went};
   for (void* q : ptrs)
     if (q) (void)dev_free(q);
+  drop_block_lists(p);
   for (int a = 0; a < 3; ++a)
     for (FftAxis* x : {&p->ax[a], &p->axN[a]})
       for (void* q : {x->d_tw, x->d_chirp, x->d_bhat[0], x->d_bhat[1]})
@@ -1616,7 +1991,8 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += 4 * std::max<int64_t>(p->M, 1);                                // inv
   b += rsz * nimg + rsz * nnz;                                        // U, density
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
-  b += (2 + 8) * std::max<int64_t>(p->nhits, 1);                      // fcode, (sample, code) hits
+  b += 2 * std::max<int64_t>(p->nhits, 1);                            // fcode
+  if (p->d_hit) b += 8 * std::max<int64_t>(p->nhits, 1);              // (sample, code) hits
   if (p->d_hitw) b += 8 * p->nhits;                                   // (sample, weight) hits
   if (p->d_off32) b += 16 * nnz;
   if (p->d_piece) b += 16 * p->npiece;
@@ -1626,6 +2002,9 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   if (p->bl.d_ptr) b += 8 * (p->nbrick + 1) + 4 * std::max<int64_t>(1, p->bl.total);
   if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
   if (p->wl.d_went) b += 16 * p->nbrick + 4 * std::max<int64_t>(1, p->wl.total) + 8 * p->nhits;
+  if (p->bk.d_piece)
+    b += 12 * p->bk.nblk + 8 + (4 * kBlkChunk + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
+         16 * (p->bk.npiece + std::max<int64_t>(1, p->bk.nred));
   for (int a = 0; a < 3; ++a)
     for (const FftAxis* x : {&p->ax[a], &p->axN[a]})
       if (x->d_tw) b += 2 * rsz * (x->bs ? x->L + 3 * x->L2 : x->L);   // roots (+ chirp, spectra)
@@ -2392,22 +2771,25 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
   if (!st && !split) st = ensure_brick_lists(p);
+  if (!st && split) st = ensure_block_lists(p);
   if (st) { release(xin); release(xout); release(ks); release(ks32); return st; }
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
-  // spread (plans with brick lists)
+  // spread (plans with brick lists), 3 block-staged split adjoint (plans
+  // with block lists; 8-coil groups)
   struct Cand { int fftb, group, xcd, brick; float ms; };
   std::vector<Cand> cands;
   for (int fftb : {0, 8, 4})
     for (int group : {16, 8})
       for (int xcd : {0, 32, -1})
-        for (int brick : {1, 0, 2}) {
+        for (int brick : {1, 0, 2, 3}) {
           if (group == 8 && nc <= 8) continue;   // same launches as 16
           if (brick == 2 && !(split ? (void*)p->wl.d_went : (void*)p->bl.d_ent)) continue;
+          if (brick == 3 && !p->bk.d_piece) continue;   // (8-coil spread groups)
           cands.push_back({fftb, group, xcd, brick, 0.f});
         }
   auto flags_for = [&](int brick, int keep) {
-    keep &= ~(2 | 8);
-    return brick == 0 ? keep : brick == 1 ? (keep | 2) : (keep | 2 | 8);
+    keep &= ~(2 | 8 | 4096);
+    return brick == 0 ? keep : brick == 1 ? (keep | 2) : brick == 2 ? (keep | 2 | 8) : (keep | 2 | 4096);
   };
   const int keep_flags = p->tune_flags;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2441,6 +2823,11 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     p->tune_flags = flags_for(b.brick, keep_flags);
   } else {
     p->tune_flags = keep_flags;
+  }
+  // block lists the choice does not use (and the caller did not ask for) are freed
+  if (p->bk.d_piece && !(p->tune_flags & 4096)) {
+    (void)hipStreamSynchronize(p->stream);
+    drop_block_lists(p);
   }
   // z-pass work-group size, timed on the best setting: 256 threads in both
   // directions, and the automatic per-direction choice (0)
@@ -2485,7 +2872,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   if (!st && report) {
     std::string js = "{\"nc\": " + std::to_string(nc) + ", \"candidates\": [";
     for (size_t i = 0; i < cands.size(); ++i) {
-      static const char* spread_kind[3] = {"gather", "gather-bricks", "lds-bricks"};
+      static const char* spread_kind[4] = {"gather", "gather-bricks", "lds-bricks", "block-staged"};
       char b[200];
       snprintf(b, sizeof b, "%s{\"fft_lines\": %d, \"coil_group\": %d, \"xcd_chunk\": %d, "
                "\"brick_order\": %d, \"spread\": \"%s\", \"ms_per_pair\": %.5f}", i ? ", " : "",

@@ -1,0 +1,129 @@
+"""Block-staged split-precision adjoint (GPU).
+
+The aNUFT of a plan with fp32 arrays that computes in fp64 (u=1.25, J=6)
+sums float weight x float sample products in double into an fp64 grid
+(spmv_t, nufft_3d.m:274-282). With tune flag 4096 the grid is walked in
+4x4x4 blocks whose samples pass through LDS in round-robin chunks; a point
+adds the same exact products as the gather kernel in a different fixed
+order. Each image value is that double sum, transformed in fp64 and
+rounded once to float, so the two images differ only where the reordering
+moves a value across a float rounding boundary (a few elements by one
+float ulp: 1e-8 relative L2 bounds it, against the 6e-8 resolution of a
+float); the block kernel is bitwise repeatable. Against the FP64 oracle the
+fp32 gate per operator (5e-5) applies.
+"""
+import ctypes as C
+import json
+
+import numpy as np
+import pytest
+
+from nufft_3d_amd import nufft_3d, synth
+from oracle import nufft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL32_OP = 5e-5
+
+
+def rel(a, b):
+    a = np.asarray(a).ravel().astype(np.complex128)
+    b = np.asarray(b).ravel().astype(np.complex128)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _rand(shape, seed):
+    r = np.random.default_rng(seed)
+    return r.normal(size=shape) + 1j * r.normal(size=shape)
+
+
+def _plan_with_env(env, *a, **kw):
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return nufft_3d(*a, **kw)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _kernels_run(p, fn):
+    p._lib.nufft3d_profile_enable(p._h, 1)
+    fn()
+    buf = C.create_string_buffer(1 << 16)
+    p._lib.nufft3d_profile_read(p._h, buf, len(buf))
+    p._lib.nufft3d_profile_enable(p._h, 0)
+    return set(json.loads(buf.value.decode()))
+
+
+CASES = [
+    # N, trajectory, J, radial, coils
+    (24, "koosh", 1, 8),
+    (24, "koosh", 0, 8),
+    (24, "golden", 1, 16),
+    (26, "golden", 1, 8),     # K = 34: blocks cut by the grid edge
+    (16, "dense", 1, 8),      # 2,000 spokes on a 20^3 grid: blocks in many pieces
+    (24, "koosh", 1, 3),      # not an 8-coil group: the gather kernel
+]
+
+
+def _traj(N, kind):
+    if kind == "koosh":
+        return synth.koosh_ball(N, 300, 2 * N)
+    if kind == "golden":
+        return synth.golden_means(N, 300, 2 * N)
+    return synth.koosh_ball(N, 2000, 2 * N)
+
+
+@pytest.mark.parametrize("N,kind,radial,nc", CASES)
+def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc):
+    om = _traj(N, kind)
+    kw = dict(J=6, u=1.25, radial=radial, gpu=1)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096)}, om, N, **kw)
+    gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    assert blk.compute_precision == 2
+    k = _rand((om.shape[1], nc), 71).astype(np.complex64)
+    got = {}
+    ran = _kernels_run(blk, lambda: got.setdefault("b", blk.aNUFT(k)))
+    assert ("k_spread_blk" in ran) == (nc % 8 == 0), ran
+    if kind == "dense":
+        assert "k_spread_blk_reduce" in ran, ran
+    want = gat.aNUFT(k)
+    assert rel(got["b"], want) < 1e-8
+    assert np.array_equal(blk.aNUFT(k), got["b"])       # deterministic
+    orc = O.NufftOracle(om, N, J=6, u=1.25, radial=radial)
+    assert rel(got["b"], orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
+
+
+def test_block_staged_adjoint_on_a_grid_narrower_than_the_footprint():
+    # K3 = 4 < 7: a sample reaches some points through two wrapped
+    # candidates; each is its own entry of the same staged row
+    rng = np.random.default_rng(5)
+    N = (16, 16, 2)
+    om = rng.uniform(-8, 7.5, size=(3, 1500))
+    om[2] = rng.uniform(-1, 0.5, size=1500)
+    kw = dict(J=6, u=1.25, radial=1, gpu=1)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096)}, om, N, **kw)
+    gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    assert blk.compute_precision == 2 and min(blk.K) < 7
+    k = _rand((om.shape[1], 8), 72).astype(np.complex64)
+    ran = _kernels_run(blk, lambda: blk.aNUFT(k))
+    assert "k_spread_blk" in ran
+    assert rel(blk.aNUFT(k), gat.aNUFT(k)) < 1e-8
+    orc = O.NufftOracle(om, N, J=6, u=1.25, radial=1)
+    assert rel(blk.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
+
+
+def test_autotuner_times_the_block_staged_adjoint():
+    om = synth.golden_means(24, 300, 48)
+    g = nufft_3d(om, 24, J=6, u=1.25, gpu=1)
+    rep = g.autotune(nc=8, reps=1)
+    kinds = {c["spread"] for c in rep["candidates"]}
+    assert "block-staged" in kinds, kinds
+    k = _rand((om.shape[1], 8), 73).astype(np.complex64)
+    orc = O.NufftOracle(om, 24, J=6, u=1.25)
+    assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP

@@ -58,6 +58,7 @@ def algorithmic_bytes(N, K, M, nc):
         "k_spread": nc * (8 * M + 8 * k3) + 12 * M,
         "k_spread_tile": nc * (8 * M + 8 * k3) + 12 * M,
         "k_spread_blk": nc * (8 * M + 8 * k3) + 12 * M,
+        "k_interp_blk": nc * (8 * k3 + 8 * M) + 12 * M,
         "fft_fwd": nc * (8 * n3 + 8 * k3) + 4 * n3,
         "fft_adj": nc * (8 * k3 + 8 * n3) + 4 * n3,
         "pair": nc * (16 * n3 + 32 * k3 + 16 * M) + 24 * M + 8 * n3,
@@ -567,7 +568,8 @@ def run_pair(args, wname, full=True):
         tuned = dict(rep["candidates"][rep["best"]], tune_s=round(time.perf_counter() - t1, 3),
                      default_ms=rep["candidates"][0]["ms_per_pair"],
                      z_threads=rep.get("z_threads"), z_threads_ms=rep.get("z_threads_ms"),
-                     x_threads=rep.get("x_threads"), x_threads_ms=rep.get("x_threads_ms"))
+                     x_threads=rep.get("x_threads"), x_threads_ms=rep.get("x_threads_ms"),
+                     forward=rep.get("forward"), forward_ms=rep.get("forward_ms"))
         log("autotune:", tuned)
     log("plan %.3f s  K=%s nnz=%d compute precision %d" % (plan_s, plan.K, plan.nnz,
                                                          plan.compute_precision))

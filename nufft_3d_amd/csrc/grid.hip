@@ -940,6 +940,120 @@ __global__ void __launch_bounds__(512) k_spread_blk_reduce(const int4* __restric
   grid[(((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0 + c] = make_double2(ax, ay);
 }
 
+// Forward (fp64 arithmetic) from the block-staged lists (FwdLists): one
+// wavefront per group of 64 sorted samples, one lane per sample, 8 coils. A
+// chunk's 64 grid rows (128 bytes of fp64 coils each, and the point's
+// coordinates) are staged in the wave's LDS slice, one row per lane; each
+// lane walks its entries of the chunk and evaluates each hit's weight from
+// its own sample's offsets and the staged point's coordinates, with the same
+// function and arguments as k_interp (the same weights), and sums the
+// products in double. A sample's sum runs over its chunks in order, fixed at
+// plan time: deterministic (not bitwise the gather's z-y-x order). The four
+// waves of a work-group run independent groups after staging the kernel
+// table together.
+constexpr int kFwdRow = 9;   // 16-byte units per staged row: 8 of data, 1 of padding (LDS banks)
+
+template <typename RO, bool USER, bool TAB, bool STW>
+__global__ void __launch_bounds__(256) k_interp_blk(const long long* __restrict__ cbase,
+                                                    long long ngrp, const int* __restrict__ gpt,
+                                                    const long long* __restrict__ eoff,
+                                                    const unsigned* __restrict__ eslot,
+                                                    const float4* __restrict__ ew,
+                                                    const int4* __restrict__ base,
+                                                    const double* __restrict__ off,
+                                                    const double2* __restrict__ grid,
+                                                    cplx<RO>* __restrict__ out,
+                                                    const double* __restrict__ wsamp, double scale,
+                                                    const int* __restrict__ orig,
+                                                    const double* __restrict__ kbtab, GridGeom g,
+                                                    KBParams kp) {
+  __shared__ double2 s_row[4 * 64 * kFwdRow];
+  __shared__ int s_pc[4 * 64];
+  __shared__ double stab[TAB && !STW ? kKbIntervals * kKbCoef : 1];
+  const double* tab = nullptr;
+  if constexpr (TAB && !STW) tab = stage_kb_table(kbtab, stab);   // every thread (it synchronises)
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long j = xcd_block(g.xchunk) * 4 + wv;
+  if (j >= ngrp) return;
+  const long long s = j * 64 + lane;
+  const bool act = s < g.nnz;
+  const long long sv = act ? s : (long long)g.nnz - 1;   // a lane past the end has no entries
+  const int4 b = base[sv];
+  const double* f64 = off + 3 * sv;
+  const double fx = f64[0], fy = f64[1], fz = f64[2];
+  double ar[8], ai[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
+  double2* srow = s_row + wv * 64 * kFwdRow;
+  int* spc = s_pc + wv * 64;
+  const long long c0 = cbase[j], c1 = cbase[j + 1];
+  for (long long ch = c0; ch < c1; ++ch) {
+    const int gp = gpt[ch * 64 + lane];
+    const double4* src = reinterpret_cast<const double4*>(grid + (long long)gp * g.nc + g.c0);
+    const double4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+    const int px = gp % g.K1, py = (gp / g.K1) % g.K2, pz = gp / (g.K1 * g.K2);
+    wave_lds_sync();   // the previous chunk's rows are consumed
+    double2* dst = srow + lane * kFwdRow;
+    dst[0] = make_double2(r0.x, r0.y); dst[1] = make_double2(r0.z, r0.w);
+    dst[2] = make_double2(r1.x, r1.y); dst[3] = make_double2(r1.z, r1.w);
+    dst[4] = make_double2(r2.x, r2.y); dst[5] = make_double2(r2.z, r2.w);
+    dst[6] = make_double2(r3.x, r3.y); dst[7] = make_double2(r3.z, r3.w);
+    spc[lane] = px | (py << 10) | (pz << 20);
+    wave_lds_sync();
+    const long long q0 = eoff[ch], q1 = eoff[ch + 1];
+    unsigned sl4 = q0 < q1 ? eslot[q0 * 64 + lane] : 0xffffffffu;
+    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (STW) if (q0 < q1) w4 = ew[q0 * 64 + lane];
+    for (long long q = q0; q < q1; ++q) {
+      const long long qn = q + 1 < q1 ? q + 1 : q;
+      const unsigned nsl4 = eslot[qn * 64 + lane];
+      float4 nw4 = w4;
+      if constexpr (STW) nw4 = ew[qn * 64 + lane];
+      const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned sl = (sl4 >> (8 * k)) & 255u;
+        if (sl != 255u) {
+          double w;
+          if constexpr (STW) {
+            w = (double)wk[k];   // the plan's fp32 weight of this hit
+          } else {
+          const int pc = spc[sl];
+          // candidate offset of the staged point from the sample's base
+          // (K >= 2W + 2 per axis: one candidate reaches it)
+          int dx = (pc & 1023) - b.x, dy = ((pc >> 10) & 1023) - b.y, dz = (pc >> 20) - b.z;
+          dx = dx > g.W1 ? dx - g.K1 : (dx < -g.W1 ? dx + g.K1 : dx);
+          dy = dy > g.W2 ? dy - g.K2 : (dy < -g.W2 ? dy + g.K2 : dy);
+          dz = dz > g.W3 ? dz - g.K3 : (dz < -g.W3 ? dz + g.K3 : dz);
+          if constexpr (TAB) w = hit_weight_tab(dx, dy, dz, fx, fy, fz, tab, kp);
+          else w = hit_weight_d(dx, dy, dz, f64, kp);
+          }
+          const double2* row = srow + sl * kFwdRow;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const double2 v = row[c];
+            ar[c] += w * v.x;
+            ai[c] += w * v.y;
+          }
+        }
+      }
+      sl4 = nsl4;
+      w4 = nw4;
+    }
+  }
+  if (!act) return;
+  double sc = scale;
+  if (wsamp) sc *= wsamp[s];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    cplx<RO> v;
+    v.x = (RO)(ar[c] * sc);
+    v.y = (RO)(ai[c] * sc);
+    if constexpr (USER) out[orig[s] + (long long)(g.c0 + c) * g.M] = v;
+    else out[s * g.nc + g.c0 + c] = v;
+  }
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const uint2* __restrict__ hits, bool pw,
@@ -1209,6 +1323,31 @@ static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int
                           const void* wsamp, double scale, bool user, bool narrow) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
+  if constexpr (sizeof(R) == 8 && FULL && CPL * LPS == 8) {
+    if (fwd_block_staged(p, cg)) {   // the block-staged forward (8-coil groups)
+      ProfScope ps(p, "k_interp_blk");
+      const dim3 grid_((unsigned)((p->fk.ngrp + 3) / 4)), blk_(256);
+      const double* tb = kb_table_of(p);
+      const bool stw = p->fk.d_w && (p->tune_flags & 16384);
+#define N3D_IBLK(RO, USR, TB, SW)                                                                 \
+      hipLaunchKernelGGL((k_interp_blk<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,              \
+                         p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,      \
+                         p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid,     \
+                         (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp)
+#define N3D_IBLK2(RO, USR)                                                                        \
+      do {                                                                                        \
+        if (stw) N3D_IBLK(RO, USR, true, true);                                                   \
+        else if (tb) N3D_IBLK(RO, USR, true, false);                                              \
+        else N3D_IBLK(RO, USR, false, false);                                                     \
+      } while (0)
+      if (narrow) N3D_IBLK2(float, true);
+      else if (user) N3D_IBLK2(double, true);
+      else N3D_IBLK2(double, false);
+#undef N3D_IBLK2
+#undef N3D_IBLK
+      return;
+    }
+  }
   const long long nthreads = p->nnz * (long long)LPS;
   const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
@@ -1388,6 +1527,58 @@ int build_hit_weights(nufft3d_plan* p) {
   return NUFFT3D_OK;
 }
 
+// The block-staged forward's entries: per hit of sample s (sample-major,
+// z-y-x codes), its slot byte in its chunk and, with w, its fp32 weight from
+// the function the forward kernels evaluate (so stored and evaluated weights
+// are the same floats)
+template <bool TB>
+__global__ void k_fk_fill(const long long* __restrict__ spos, const unsigned short* __restrict__ fcode,
+                          long long nnz, const long long* __restrict__ cbase,
+                          const long long* __restrict__ eoff, const unsigned short* __restrict__ hslot,
+                          const unsigned short* __restrict__ hk, unsigned char* __restrict__ slot,
+                          float* __restrict__ w, const double* __restrict__ off,
+                          const double* __restrict__ tab, GridGeom g, KBParams kp) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const long long j = s / 64;
+  const int lane = (int)(s % 64);
+  const long long c0 = cbase[j], nch = cbase[j + 1] - c0;
+  const double* f = off + 3 * s;
+  for (long long e = spos[s]; e < spos[s + 1]; ++e) {
+    const long long sl = hslot[e];
+    const long long c = sl % nch;
+    const int k = hk[e];
+    const long long q = ((eoff[c0 + c] + k / 4) * 64 + lane) * 4 + (k & 3);
+    slot[q] = (unsigned char)(sl / nch);
+    if (w) {
+      int dx, dy, dz;
+      decode_hit(fcode[e], g, dx, dy, dz);
+      double v;
+      if constexpr (TB) v = hit_weight_tab(dx, dy, dz, f[0], f[1], f[2], tab, kp);
+      else v = hit_weight_d(dx, dy, dz, f, kp);
+      w[q] = (float)v;
+    }
+  }
+}
+
+int fk_fill(nufft3d_plan* p, const unsigned short* hslot, const unsigned short* hk) {
+  const long long nnz = p->nnz;
+  if (nnz <= 0) return NUFFT3D_OK;
+  const GridGeom g = make_geom(p, 1, 0, 1);
+  const double* tb = kb_table_of(p);
+  const dim3 grid((unsigned)((nnz + 255) / 256));
+  if (tb)
+    hipLaunchKernelGGL(k_fk_fill<true>, grid, dim3(256), 0, p->stream, p->d_spos, p->d_fcode, nnz,
+                       p->fk.d_cbase, p->fk.d_eoff, hslot, hk, (unsigned char*)p->fk.d_slot,
+                       (float*)p->fk.d_w, (const double*)p->d_off, tb, g, make_kb(p));
+  else
+    hipLaunchKernelGGL(k_fk_fill<false>, grid, dim3(256), 0, p->stream, p->d_spos, p->d_fcode, nnz,
+                       p->fk.d_cbase, p->fk.d_eoff, hslot, hk, (unsigned char*)p->fk.d_slot,
+                       (float*)p->fk.d_w, (const double*)p->d_off, tb, g, make_kb(p));
+  N3D_HIP(hipGetLastError());
+  return NUFFT3D_OK;
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
                           void* partial, const void* wsamp, bool wide) {
@@ -1503,7 +1694,11 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
                        const void* wsamp, double scale, bool user = false, bool split = false) {
   const bool wide = split;
   // the block-staged split adjoint takes 8-coil groups whatever the group size
-  const int gmax = SPREAD && split && p->bk.d_piece && (p->tune_flags & 4096) ? 8 : p->coil_group;
+  // (and the block-staged forward 8-coil groups of fp64 arithmetic)
+  const int gmax = (SPREAD && split && p->bk.d_piece && (p->tune_flags & 4096)) ||
+                           (!SPREAD && sizeof(R) == 8 && p->fk.d_slot && (p->tune_flags & 8192))
+                       ? 8
+                       : p->coil_group;
   for (int c0 = 0; c0 < nc; c0 += gmax) {
     const int cg = nc - c0 < gmax ? nc - c0 : gmax;
     // full groups with an even coil stride and offset take aligned vector loads

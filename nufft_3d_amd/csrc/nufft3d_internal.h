@@ -178,6 +178,26 @@ struct BlockLists {
   bool tried = false;
 };
 
+// Block-staged forward (k_interp_blk) of fp64-arithmetic 3-D plans: sorted
+// samples in groups of 64, one wavefront per group and one lane per sample.
+// A group's grid points (the distinct points its samples touch, ascending
+// index) are dealt round-robin into chunks of 64 as the adjoint's samples
+// are; a chunk's grid rows (8 coils of fp64) pass through the wave's LDS and
+// each lane walks its entries of the chunk (4 slot bytes per lane per word,
+// 255 = none). The weights are evaluated from the lane's own sample offsets
+// and the staged point's coordinates, as the gather kernel evaluates them.
+struct FwdLists {
+  long long ngrp = 0;               // sample groups (64 sorted samples each)
+  long long* d_cbase = nullptr;     // [ngrp + 1] first chunk of each group
+  long long nchunk = 0;
+  int* d_gpt = nullptr;             // [nchunk * 64] grid point indices (padding: the chunk's first)
+  long long* d_eoff = nullptr;      // [nchunk + 1] first entry group of each chunk
+  unsigned* d_slot = nullptr;       // [groups * 64] 4 slots per lane (byte k: step 4j+k; 255 none)
+  float4* d_w = nullptr;            // [groups * 64] the hits' fp32 weights (optional: tune flag 16384)
+  long long ngroups = 0;
+  bool tried = false;
+};
+
 struct Workspace {
   void* p = nullptr;
   size_t bytes = 0;
@@ -228,7 +248,9 @@ struct nufft3d_plan {
                             // order instead of by their first sample, 2048 = the
                             // spread's points x-fastest within a brick, not Z-order,
                             // 4096 = the block-staged split adjoint where the plan
-                            // has its lists)
+                            // has its lists, 8192 = the block-staged forward where
+                            // the plan has its lists, 16384 = that forward reads the
+                            // lists' stored weights instead of evaluating them)
   int xcd_chunk = 0;        // blocks per XCD run in the gridding block renumbering (0 = off)
   int coil_group = 16;      // coils per gridding launch (16 or 8)
   int fft_bundle = 0;       // lines per FFT work-group (0 = automatic)
@@ -278,6 +300,7 @@ struct nufft3d_plan {
   bool bl_tried = false;
   nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
   nufft3d::BlockLists bk;           // block-staged lists of the split-precision adjoint
+  nufft3d::FwdLists fk;             // block-staged lists of the fp64-arithmetic forward
   int max_hits = 0;
 
   // LDS-tiled forward interpolation (3-D plans with a 7-point footprint,
@@ -333,6 +356,13 @@ inline bool block_staged(const nufft3d_plan* p, int cg, bool wsamp) {
   return p->bk.d_piece && (p->tune_flags & 4096) && cg == 8 && !wsamp && split_precision(p);
 }
 int ensure_block_lists(nufft3d_plan* p);   // builds them once if the plan qualifies
+// the block-staged forward (k_interp_blk) runs for this launch: its lists
+// are built, tune flag 8192 asks for it, an 8-coil group
+inline bool fwd_block_staged(const nufft3d_plan* p, int cg) {
+  return p->fk.d_slot && (p->tune_flags & 8192) && cg == 8 && p->prec == 2;
+}
+int ensure_fwd_lists(nufft3d_plan* p, bool weights);
+int fk_fill(nufft3d_plan* p, const unsigned short* hslot, const unsigned short* hk);   // grid.hip
 
 int ensure(Workspace& w, size_t bytes);
 void prof_begin(nufft3d_plan* p, const char* name);

@@ -1292,6 +1292,249 @@ int ensure_block_lists(nufft3d_plan* p) {
   return build_block_lists(p);
 }
 
+// ---------------------------------------------------------------- block-staged forward lists
+// grid index of a forward hit (code: candidate offset biased by W, 5 bits per
+// axis) of a sample with wrapped base b, K >= 2W + 1 per axis
+__device__ __forceinline__ long long fk_point(int4 b, unsigned code, int K1, int K2, int K3, int W) {
+  const int dx = (int)(code & 31u) - W, dy = (int)((code >> 5) & 31u) - W, dz = (int)(code >> 10) - W;
+  return ((long long)wrapg(b.z + dz, K3) * K2 + wrapg(b.y + dy, K2)) * K1 + wrapg(b.x + dx, K1);
+}
+
+// keys (group << 32 | grid index) of the hits of samples [s0, s1) (hits from e0)
+__global__ void k_fk_keys(const long long* __restrict__ spos, const unsigned short* __restrict__ fcode,
+                          const int4* __restrict__ base, long long s0, long long s1, long long e0,
+                          int K1, int K2, int K3, int W, unsigned long long* __restrict__ keys) {
+  const long long s = s0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= s1) return;
+  const int4 b = base[s];
+  const unsigned long long grp = (unsigned long long)(s / 64) << 32;
+  for (long long e = spos[s]; e < spos[s + 1]; ++e)
+    keys[e - e0] = grp | (unsigned long long)fk_point(b, fcode[e], K1, K2, K3, W);
+}
+
+// per hit of samples [s0, s1): the slot of its grid point in its group's
+// ascending list lst[beg[j], end[j])
+__global__ void k_fk_slot(const long long* __restrict__ spos, const unsigned short* __restrict__ fcode,
+                          const int4* __restrict__ base, long long s0, long long s1, int K1, int K2,
+                          int K3, int W, const int* __restrict__ lst, const long long* __restrict__ beg,
+                          const long long* __restrict__ end, unsigned short* __restrict__ hslot) {
+  const long long s = s0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= s1) return;
+  const int4 b = base[s];
+  const long long j = s / 64;
+  const long long bs = beg[j], len = end[j] - bs;
+  for (long long e = spos[s]; e < spos[s + 1]; ++e) {
+    const int gi = (int)fk_point(b, fcode[e], K1, K2, K3, W);
+    long long l = 0, r = len;
+    while (l < r) {
+      const long long m = (l + r) >> 1;
+      if (lst[bs + m] < gi) l = m + 1; else r = m;
+    }
+    hslot[e] = (unsigned short)l;
+  }
+}
+
+// per sample: the rank of each hit among the sample's hits of the same chunk
+__global__ void k_fk_rank(const long long* __restrict__ spos, long long nnz,
+                          const long long* __restrict__ cbase, const unsigned short* __restrict__ hslot,
+                          unsigned short* __restrict__ cnt, unsigned short* __restrict__ hk) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const long long j = s / 64;
+  const int lane = (int)(s % 64);
+  const long long c0 = cbase[j], nch = cbase[j + 1] - c0;
+  for (long long e = spos[s]; e < spos[s + 1]; ++e) {
+    const long long c = hslot[e] % nch;
+    unsigned short* q = cnt + (c0 + c) * 64 + lane;
+    const unsigned short k = *q;
+    *q = (unsigned short)(k + 1);
+    hk[e] = k;
+  }
+}
+
+static void drop_fwd_lists(nufft3d_plan* p) {
+  FwdLists& fk = p->fk;
+  for (void* q : {(void*)fk.d_cbase, (void*)fk.d_gpt, (void*)fk.d_eoff, (void*)fk.d_slot,
+                  (void*)fk.d_w})
+    if (q) (void)dev_free(q);
+  const bool tried = fk.tried;
+  fk = FwdLists();
+  fk.tried = tried;
+}
+
+// FwdLists of an fp64-arithmetic 3-D plan with K >= 2*ceil(J/2) + 2 per axis
+// (a grid point is reached by at most one candidate of a sample), from the
+// sample-major hit codes. The (group, grid point) pairs are sorted a batch
+// of groups at a time.
+static int build_fwd_lists(nufft3d_plan* p, bool weights) {
+  FwdLists& fk = p->fk;
+  const long long nnz = p->nnz, nh = p->nhits;
+  if (nh <= 0 || p->prec != 2 || p->K[2] == 1 || !p->d_fcode) return NUFFT3D_OK;
+  const int W = p->W[0];
+  for (int a = 0; a < 3; ++a)
+    if (p->W[a] != W || p->K[a] < 2 * W + 2) return NUFFT3D_OK;
+  const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
+  const long long ngrp = (nnz + 63) / 64;
+  std::vector<long long> hspos(nnz + 1);
+  N3D_HIP(hipMemcpy(hspos.data(), p->d_spos, sizeof(long long) * (nnz + 1), hipMemcpyDeviceToHost));
+  // batches of whole groups with fewer than 2^28 hits each (one group alone if larger)
+  std::vector<long long> bs;   // first sample of each batch
+  for (long long s = 0; s < nnz;) {
+    bs.push_back(s);
+    long long e = s;
+    while (e < nnz) {
+      const long long n = std::min(e + 64, nnz);
+      if (e > s && hspos[n] - hspos[s] > (1LL << 28)) break;
+      e = n;
+    }
+    s = e;
+  }
+  bs.push_back(nnz);
+  const int nb = (int)bs.size() - 1;
+  long long bmax = 0;
+  for (int i = 0; i < nb; ++i) bmax = std::max(bmax, hspos[bs[i + 1]] - hspos[bs[i]]);
+  if (bmax >= 0x7fffffffLL) return NUFFT3D_OK;
+  int gbits = 1;
+  while ((1LL << gbits) <= ngrp) ++gbits;
+  unsigned long long *keys = nullptr, *skeys = nullptr;
+  int* d_nsel = nullptr;
+  void* tmp = nullptr;
+  int* lst = nullptr;
+  long long *beg = nullptr, *end = nullptr, *grp = nullptr;
+  unsigned short *hslot = nullptr, *cnt = nullptr, *hk = nullptr;
+  auto cleanup = [&]() {
+    for (void* q : {(void*)keys, (void*)skeys, (void*)d_nsel, tmp, (void*)lst, (void*)beg,
+                    (void*)end, (void*)grp, (void*)hslot, (void*)cnt, (void*)hk})
+      if (q) (void)dev_free(q);
+    keys = skeys = nullptr; d_nsel = nullptr; tmp = nullptr; lst = nullptr;
+    beg = end = grp = nullptr; hslot = cnt = hk = nullptr;
+  };
+  auto fail = [&](const char* what) {
+    cleanup();
+    drop_fwd_lists(p);
+    (void)hipGetLastError();
+    if (p->verbose) printf("  block-staged forward lists not built: %s\n", what);
+    return NUFFT3D_OK;
+  };
+  const size_t nk = (size_t)std::max<long long>(bmax, 1);
+  if (dev_malloc(&keys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&skeys, sizeof(unsigned long long) * nk) != hipSuccess ||
+      dev_malloc(&d_nsel, sizeof(int)) != hipSuccess ||
+      dev_malloc(&hslot, sizeof(unsigned short) * nh) != hipSuccess ||
+      dev_malloc(&beg, sizeof(long long) * ngrp) != hipSuccess ||
+      dev_malloc(&end, sizeof(long long) * ngrp) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemsetAsync(beg, 0, sizeof(long long) * ngrp, p->stream) != hipSuccess ||
+      hipMemsetAsync(end, 0, sizeof(long long) * ngrp, p->stream) != hipSuccess)
+    return fail("memset failed");
+  size_t tb1 = 0, tb2 = 0;
+  if (hipcub::DeviceRadixSort::SortKeys(nullptr, tb1, keys, skeys, (int)nk, 0, 32 + gbits,
+                                        p->stream) != hipSuccess ||
+      hipcub::DeviceSelect::Unique(nullptr, tb2, skeys, keys, d_nsel, (int)nk, p->stream) != hipSuccess ||
+      dev_malloc(&tmp, std::max(tb1, tb2) + 16) != hipSuccess)
+    return fail("sort set-up failed");
+  auto batch_keys = [&](int i, int& nsel) -> bool {
+    const long long s0 = bs[i], s1 = bs[i + 1];
+    const int n = (int)(hspos[s1] - hspos[s0]);
+    nsel = 0;
+    if (n == 0) return true;
+    hipLaunchKernelGGL(k_fk_keys, dim3(nblocks(s1 - s0)), dim3(256), 0, p->stream, p->d_spos,
+                       p->d_fcode, p->d_base, s0, s1, hspos[s0], K1, K2, K3, W, keys);
+    return hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, n, 0, 32 + gbits, p->stream) == hipSuccess &&
+           hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, n, p->stream) == hipSuccess &&
+           hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream) == hipSuccess &&
+           hipStreamSynchronize(p->stream) == hipSuccess;
+  };
+  std::vector<long long> nsel_b(nb, 0);
+  long long total = 0;
+  for (int i = 0; i < nb; ++i) {
+    int ns = 0;
+    if (!batch_keys(i, ns)) return fail("sort failed");
+    nsel_b[i] = ns;
+    total += ns;
+  }
+  if (dev_malloc(&lst, sizeof(int) * std::max(1LL, total)) != hipSuccess) return fail("allocation failed");
+  long long off = 0;
+  for (int i = 0; i < nb; ++i) {
+    int ns = 0;
+    if (!batch_keys(i, ns) || ns != nsel_b[i]) return fail("sort failed");
+    if (ns > 0)
+      hipLaunchKernelGGL(k_wl_lists, dim3(nblocks(ns)), dim3(256), 0, p->stream,
+                         (const unsigned long long*)keys, (long long)ns, off, lst, beg, end);
+    hipLaunchKernelGGL(k_fk_slot, dim3(nblocks(bs[i + 1] - bs[i])), dim3(256), 0, p->stream,
+                       p->d_spos, p->d_fcode, p->d_base, bs[i], bs[i + 1], K1, K2, K3, W,
+                       (const int*)lst, (const long long*)beg, (const long long*)end, hslot);
+    if (hipGetLastError() != hipSuccess) return fail("slot kernels failed");
+    off += ns;
+  }
+  (void)dev_free(keys); keys = nullptr;
+  (void)dev_free(skeys); skeys = nullptr;
+  (void)dev_free(tmp); tmp = nullptr;
+  std::vector<long long> hb(ngrp), he(ngrp), cbase(ngrp + 1, 0);
+  if (hipMemcpyAsync(hb.data(), beg, sizeof(long long) * ngrp, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+      hipMemcpyAsync(he.data(), end, sizeof(long long) * ngrp, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+      hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail("copy failed");
+  for (long long j = 0; j < ngrp; ++j) {
+    if (he[j] - hb[j] > 65535) return fail("a group's point list exceeds 16-bit slots");
+    cbase[j + 1] = cbase[j] + (he[j] - hb[j] + 63) / 64;
+  }
+  const long long nchunk = cbase[ngrp];
+  if (dev_malloc(&fk.d_cbase, sizeof(long long) * (ngrp + 1)) != hipSuccess ||
+      dev_malloc(&fk.d_gpt, sizeof(int) * std::max(1LL, nchunk * 64)) != hipSuccess ||
+      dev_malloc(&fk.d_eoff, sizeof(long long) * (nchunk + 1)) != hipSuccess ||
+      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * 64)) != hipSuccess ||
+      dev_malloc(&hk, sizeof(unsigned short) * nh) != hipSuccess ||
+      dev_malloc(&grp, sizeof(long long) * (nchunk + 1)) != hipSuccess)
+    return fail("allocation failed");
+  if (hipMemcpy(fk.d_cbase, cbase.data(), sizeof(long long) * (ngrp + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * 64), p->stream) != hipSuccess)
+    return fail("copy failed");
+  // chunked point lists (the same dealing as the adjoint's sample lists)
+  hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)ngrp), dim3(256), 0, p->stream, (const int*)lst,
+                     (const long long*)beg, (const long long*)end, (const long long*)fk.d_cbase,
+                     fk.d_gpt);
+  hipLaunchKernelGGL(k_fk_rank, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
+                     (const long long*)p->d_spos, nnz, (const long long*)fk.d_cbase,
+                     (const unsigned short*)hslot, cnt, hk);
+  if (nchunk > 0)
+    hipLaunchKernelGGL(k_bk_groups, dim3(nblocks(nchunk)), dim3(256), 0, p->stream,
+                       (const unsigned short*)cnt, nchunk, grp);
+  if (hipGetLastError() != hipSuccess) return fail("list kernels failed");
+  std::vector<long long> hgrp(nchunk + 1, 0), eoff(nchunk + 1, 0);
+  if (nchunk > 0 &&
+      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nchunk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+       hipStreamSynchronize(p->stream) != hipSuccess))
+    return fail("copy failed");
+  for (long long c = 0; c < nchunk; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
+  const long long ngroups = eoff[nchunk];
+  if (dev_malloc(&fk.d_slot, sizeof(unsigned) * std::max(1LL, ngroups * 64)) != hipSuccess ||
+      (weights && dev_malloc(&fk.d_w, sizeof(float4) * std::max(1LL, ngroups * 64)) != hipSuccess))
+    return fail("allocation failed");
+  fk.ngroups = ngroups;
+  if (hipMemcpy(fk.d_eoff, eoff.data(), sizeof(long long) * (nchunk + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(fk.d_slot, 0xff, sizeof(unsigned) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess ||
+      (weights &&
+       hipMemsetAsync(fk.d_w, 0, sizeof(float4) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess))
+    return fail("copy failed");
+  if (fk_fill(p, hslot, hk) != NUFFT3D_OK || hipStreamSynchronize(p->stream) != hipSuccess)
+    return fail("fill failed");
+  cleanup();
+  fk.ngrp = ngrp;
+  fk.nchunk = nchunk;
+  fk.ngroups = ngroups;
+  if (p->verbose)
+    printf("  block-staged forward: %lld sample groups, %lld chunks, %lld entry groups "
+           "(%.2f B/hit)\n", ngrp, nchunk, ngroups, 4.0 * 64 * ngroups / (double)nh);
+  return NUFFT3D_OK;
+}
+
+int ensure_fwd_lists(nufft3d_plan* p, bool weights) {
+  if (p->fk.d_slot || p->fk.tried || p->prec != 2 || p->K[2] == 1) return NUFFT3D_OK;
+  p->fk.tried = true;
+  return build_fwd_lists(p, weights);
+}
+
 // Pieces of the tiled interpolation: runs of equal sub-bin key in the sorted
 // samples, cut into at most kPieceMax samples.
 constexpr int kPieceMax = 128;   // one sample per 4-lane group of a 512-thread work-group
@@ -1605,6 +1848,8 @@ static int build_plan(nufft3d_plan* p, const double* om_in, int64_t M) {
   N3D_TRY(build_hit_weights(p));
   if (p->tune_flags & 8) N3D_TRY(ensure_wide_lists(p));
   if (p->tune_flags & 4096) N3D_TRY(ensure_block_lists(p));
+  N3D_TRY(build_kb_table(p));   // (the forward lists' stored weights evaluate it)
+  if (p->tune_flags & 8192) N3D_TRY(ensure_fwd_lists(p, (p->tune_flags & 16384) != 0));
   // fp64-arithmetic plans read only the weight table in the spread (unless
   // tune flag 128 evaluates the weights) and build no code-based lists, so
   // their (sample, code) copy of the lists goes (8 B per hit)
@@ -1660,6 +1905,7 @@ static void destroy_plan(nufft3d_plan* p) {
   for (void* q : ptrs)
     if (q) (void)dev_free(q);
   drop_block_lists(p);
+  drop_fwd_lists(p);
   for (int a = 0; a < 3; ++a)
     for (FftAxis* x : {&p->ax[a], &p->axN[a]})
       for (void* q : {x->d_tw, x->d_chirp, x->d_bhat[0], x->d_bhat[1]})
@@ -2005,6 +2251,9 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   if (p->bk.d_piece)
     b += 12 * p->bk.nblk + 8 + (4 * kBlkChunk + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
          16 * (p->bk.npiece + std::max<int64_t>(1, p->bk.nred));
+  if (p->fk.d_slot)
+    b += 8 * (p->fk.ngrp + 1) + (4 * 64 + 8) * p->fk.nchunk + 8 +
+         (p->fk.d_w ? 20 : 4) * 64 * p->fk.ngroups;
   for (int a = 0; a < 3; ++a)
     for (const FftAxis* x : {&p->ax[a], &p->axN[a]})
       if (x->d_tw) b += 2 * rsz * (x->bs ? x->L + 3 * x->L2 : x->L);   // roots (+ chirp, spectra)
@@ -2849,6 +3098,49 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     zt_best = st ? keep_zt : (zt_ms[1] < zt_ms[0] ? 0 : 256);
     p->fft_zthreads = zt_best;
   }
+  // forward kernel, timed likewise on the best setting: the gather, or the
+  // block-staged forward where its lists can be built (fp64 arithmetic, 3-D);
+  // lists the choice does not use (and the caller did not ask for) are freed
+  // (0 gather, 1 block-staged evaluating the weights, 2 block-staged reading
+  // the lists' stored weights)
+  float fw_ms[3] = {0.f, 0.f, 0.f};
+  int fw_best = (p->tune_flags & 8192) ? ((p->tune_flags & 16384) ? 2 : 1) : 0;
+  auto fw_flags = [](int keep, int fi) {
+    keep &= ~(8192 | 16384);
+    return fi == 0 ? keep : fi == 1 ? (keep | 8192) : (keep | 8192 | 16384);
+  };
+  if (!st && best >= 0 && p->prec == 2 && p->K[2] > 1) {
+    st = ensure_fwd_lists(p, true);
+    if (!st && p->fk.d_slot) {
+      const int keep = p->tune_flags;
+      for (int fi = 0; fi < 3 && !st; ++fi) {
+        if (fi == 2 && !p->fk.d_w) continue;
+        p->tune_flags = fw_flags(keep, fi);
+        st = pair();
+        if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+        for (int r = 0; r < reps && !st; ++r) st = pair();
+        if (!st && hipEventRecord(e1, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+        if (!st && hipEventSynchronize(e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+        float ms = 0;
+        if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
+        fw_ms[fi] = ms / reps;
+      }
+      if (!st) {
+        fw_best = 0;
+        for (int fi = 1; fi < 3; ++fi)
+          if (fw_ms[fi] > 0.f && fw_ms[fi] < fw_ms[fw_best]) fw_best = fi;
+      }
+      p->tune_flags = fw_flags(keep, fw_best);
+      if (fw_best == 0 && !(keep_flags & 8192)) {
+        (void)hipStreamSynchronize(p->stream);
+        drop_fwd_lists(p);
+      } else if (fw_best == 1 && p->fk.d_w && !(keep_flags & 16384)) {
+        (void)hipStreamSynchronize(p->stream);
+        (void)dev_free(p->fk.d_w);   // the evaluated form keeps no weights
+        p->fk.d_w = nullptr;
+      }
+    }
+  }
   // x-pass work-group size, timed likewise: 256 or 512 threads
   float xt_ms[2] = {0.f, 0.f};
   int xt_best = p->fft_xthreads;
@@ -2880,10 +3172,13 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
                spread_kind[cands[i].brick], cands[i].ms);
       js += b;
     }
-    char zb[256];
+    char zb[400];
     snprintf(zb, sizeof zb, ", \"z_threads\": %d, \"z_threads_ms\": [%.5f, %.5f], "
-             "\"x_threads\": %d, \"x_threads_ms\": [%.5f, %.5f]", zt_best, zt_ms[0], zt_ms[1],
-             xt_best, xt_ms[0], xt_ms[1]);
+             "\"x_threads\": %d, \"x_threads_ms\": [%.5f, %.5f], \"forward\": \"%s\", "
+             "\"forward_ms\": [%.5f, %.5f, %.5f]", zt_best, zt_ms[0], zt_ms[1],
+             xt_best, xt_ms[0], xt_ms[1],
+             fw_best == 0 ? "gather" : fw_best == 1 ? "block-staged" : "block-staged-stored",
+             fw_ms[0], fw_ms[1], fw_ms[2]);
     js += "], \"best\": " + std::to_string(best) + zb + "}";
     if ((int64_t)js.size() + 1 > len) { set_error("report buffer too small"); st = NUFFT3D_ERR_INVALID; }
     else std::memcpy(report, js.c_str(), js.size() + 1);

@@ -1,4 +1,5 @@
-"""Block-staged split-precision adjoint (GPU).
+"""Block-staged gridding kernels (GPU): the split-precision adjoint and the
+fp64-arithmetic forward.
 
 The aNUFT of a plan with fp32 arrays that computes in fp64 (u=1.25, J=6)
 sums float weight x float sample products in double into an fp64 grid
@@ -11,6 +12,14 @@ moves a value across a float rounding boundary (a few elements by one
 float ulp: 1e-8 relative L2 bounds it, against the 6e-8 resolution of a
 float); the block kernel is bitwise repeatable. Against the FP64 oracle the
 fp32 gate per operator (5e-5) applies.
+
+The forward of an fp64-arithmetic plan (spmv, nufft_3d.m:259-271) with tune
+flag 8192 runs over groups of 64 sorted samples whose grid rows pass through
+LDS in round-robin chunks; each hit's weight is evaluated as the gather
+kernel evaluates it, and the sum over a sample's hits runs in another fixed
+order: fp64 plans agree with the gather to double rounding (1e-13) and with
+the FP64 oracle at 1e-12; fp32-array plans as above (1e-8 against the
+gather, 5e-5 against the oracle).
 """
 import ctypes as C
 import json
@@ -124,6 +133,80 @@ def test_autotuner_times_the_block_staged_adjoint():
     rep = g.autotune(nc=8, reps=1)
     kinds = {c["spread"] for c in rep["candidates"]}
     assert "block-staged" in kinds, kinds
+    assert rep["forward"] in ("block-staged", "block-staged-stored", "gather")
+    assert len(rep["forward_ms"]) == 3
     k = _rand((om.shape[1], 8), 73).astype(np.complex64)
     orc = O.NufftOracle(om, 24, J=6, u=1.25)
     assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
+
+
+FWD_CASES = [
+    # N, trajectory, gpu, coils
+    (24, "golden", 1, 8),
+    (24, "koosh", 1, 16),
+    (26, "golden", 1, 8),
+    (16, "dense", 1, 8),
+    (24, "golden", 2, 8),     # fp64 plan: kernel.m's literal arithmetic, double outputs
+    (24, "koosh", 1, 3),      # not an 8-coil group: the gather kernel
+]
+
+
+@pytest.mark.parametrize("stored", [False, True])
+@pytest.mark.parametrize("N,kind,gpu,nc", FWD_CASES)
+def test_block_staged_forward_matches_gather_and_oracle(N, kind, gpu, nc, stored):
+    # stored: the lists hold each hit's fp32 weight, evaluated at plan time
+    # by the same function the kernels evaluate
+    om = _traj(N, kind)
+    kw = dict(J=6, u=1.25, gpu=gpu)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 8192 | (16384 if stored else 0))}, om, N, **kw)
+    gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    assert blk.compute_precision == 2
+    x = _rand((N, N, N, nc), 74)
+    if gpu == 1:
+        x = x.astype(np.complex64)
+    got = {}
+    ran = _kernels_run(blk, lambda: got.setdefault("b", blk.fNUFT(x)))
+    assert ("k_interp_blk" in ran) == (nc % 8 == 0), ran
+    want = gat.fNUFT(x)
+    assert rel(got["b"], want) < (1e-13 if gpu == 2 else 1e-8)
+    assert np.array_equal(blk.fNUFT(x), got["b"])       # deterministic
+    orc = O.NufftOracle(om, N, J=6, u=1.25)
+    assert rel(got["b"], orc.fNUFT(x.astype(np.complex128))) < (1e-12 if gpu == 2 else TOL32_OP)
+
+
+@pytest.mark.parametrize("gpu", [1, 2])
+def test_block_staged_forward_and_adjoint_in_the_inverse(gpu):
+    # iNUFT keeps its samples internal (fp64 rows, data weights): the forward
+    # writes the sorted layout with the density weights applied
+    N = 20
+    om = synth.golden_means(N, 250, 2 * N)
+    kw = dict(J=6, u=1.25, gpu=gpu)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096 | 8192 | 16384)}, om, N, **kw)
+    gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    orc = O.NufftOracle(om, N, J=6, u=1.25)
+    x = _rand((N, N, N, 8), 75)
+    raw = orc.fNUFT(x) + 0.01 * _rand((om.shape[1], 8), 76)
+    if gpu == 1:
+        raw = raw.astype(np.complex64)
+    got = {}
+    ran = _kernels_run(blk, lambda: got.setdefault("b", blk.iNUFT(raw, 6, 0.02)))
+    assert "k_interp_blk" in ran, ran
+    want = gat.iNUFT(raw, 6, 0.02)
+    assert rel(got["b"], want) < (1e-11 if gpu == 2 else 1e-7)
+    ref = orc.iNUFT(raw.astype(np.complex128), 6, 0.02)
+    assert rel(got["b"], ref) < (1e-10 if gpu == 2 else 1e-4)
+
+
+def test_block_staged_forward_not_built_on_a_narrow_grid():
+    # K3 = 4 < 2*ceil(J/2) + 2: a grid point can be reached by two candidates
+    # of one sample, so the forward keeps the gather kernel
+    rng = np.random.default_rng(6)
+    N = (16, 16, 2)
+    om = rng.uniform(-8, 7.5, size=(3, 1500))
+    om[2] = rng.uniform(-1, 0.5, size=1500)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 8192)}, om, N, J=6, u=1.25, gpu=1)
+    x = _rand(N + (8,), 77).astype(np.complex64)
+    ran = _kernels_run(blk, lambda: blk.fNUFT(x))
+    assert "k_interp_blk" not in ran and "k_interp" in ran, ran
+    orc = O.NufftOracle(om, N, J=6, u=1.25)
+    assert rel(blk.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP

@@ -819,16 +819,18 @@ __global__ void __launch_bounds__(512) k_spread_tile_w(const long long* __restri
   }
 }
 
-// Split-precision adjoint from the block-staged lists (BlockLists): one
-// wavefront per piece (a run of chunks of one 4x4x4 block), one lane per
-// point of the block, 8 coils. A chunk's 64 sample rows (64 bytes of float
-// coils each) are staged in the wave's LDS slice, one row per lane, then
-// every lane walks its entries of the chunk, 4 steps per load pair, and sums
-// the exact products of fp32 weights and fp32 samples in double. A point's
-// sum runs over its chunks in order and within a chunk in list order, fixed
-// at plan time: deterministic (not bitwise the gather's order). The four
-// waves of a work-group run independent pieces, with no work-group barrier.
-constexpr int kBlkRow4 = 5;   // 16-byte units per staged row: 4 of data, 1 of padding (LDS banks)
+// Adjoint of fp64-arithmetic plans from the block-staged lists (BlockLists):
+// one wavefront per piece (a run of chunks of one 4x4x4 block), one lane per
+// point of the block, 8 coils. A chunk's 64 sample rows are staged in the
+// wave's LDS slice, one row per lane: float coils (the split-precision
+// aNUFT, 64 bytes) or double coils times the sample's data weight (iNUFT's
+// internal samples, 128 bytes). Every lane then walks its entries of the
+// chunk, 4 steps per load pair, and sums weight x value in double (for float
+// rows the exact products of fp32 weights and fp32 samples). A point's sum
+// runs over its chunks in order and within a chunk in list order, fixed at
+// plan time: deterministic (not bitwise the gather's order). The four waves
+// of a work-group run independent pieces, with no work-group barrier.
+template <typename S> struct BlkRow { static constexpr int U = sizeof(S) == 8 ? 5 : 9; };  // 16-byte units (data + 1 of padding)
 
 // LDS writes of a wavefront visible to its own later reads
 __device__ __forceinline__ void wave_lds_sync() {
@@ -837,6 +839,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+template <typename S>
 __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pieces,
                                                     long long npiece, const int* __restrict__ blk,
                                                     const long long* __restrict__ cbase,
@@ -844,10 +847,13 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
                                                     const long long* __restrict__ eoff,
                                                     const unsigned* __restrict__ eslot,
                                                     const float4* __restrict__ ew,
-                                                    const float2* __restrict__ ks,
+                                                    const S* __restrict__ ks,
+                                                    const double* __restrict__ wsamp,
                                                     double2* __restrict__ grid,
                                                     double2* __restrict__ partial, GridGeom g) {
-  __shared__ float4 s_row[4 * kBlkChunk * kBlkRow4];
+  constexpr int RU = BlkRow<S>::U;
+  constexpr int DU = RU - 1;          // 16-byte units of data per row (4 float, 8 double)
+  __shared__ float4 s_row[4 * kBlkChunk * RU];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long pi = xcd_block(g.xchunk) * 4 + wv;
   if (pi >= npiece) return;
@@ -859,18 +865,33 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
 #pragma unroll
   for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
   const long long cb = cbase[pc.x];
-  float4* srow = s_row + wv * (kBlkChunk * kBlkRow4);
+  float4* srow = s_row + wv * (kBlkChunk * RU);
   const float4* ks4 = reinterpret_cast<const float4*>(ks);
-  const long long rs4 = g.nc / 2;   // 16-byte units per sample row
-  const int co4 = g.c0 / 2;
+  const long long rs4 = (long long)g.nc * sizeof(S) / 16;   // 16-byte units per sample row
+  const long long co4 = (long long)g.c0 * sizeof(S) / 16;
   for (int c = pc.y; c < pc.z; ++c) {
     const long long ch = cb + c;
     const int sm = samp[ch * kBlkChunk + lane];
     const float4* src = ks4 + (long long)sm * rs4 + co4;
-    const float4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+    double m = 1.0;   // iNUFT's data weight of the sample, applied as it is staged
+    if constexpr (sizeof(S) == 16) {
+      if (wsamp) m = wsamp[sm];
+    }
     wave_lds_sync();   // the previous chunk's rows are consumed
-    float4* dst = srow + lane * kBlkRow4;
-    dst[0] = r0; dst[1] = r1; dst[2] = r2; dst[3] = r3;
+    float4* dst = srow + lane * RU;
+#pragma unroll
+    for (int k = 0; k < DU; ++k) {
+      float4 v = src[k];
+      if constexpr (sizeof(S) == 16) {
+        if (wsamp) {
+          double2 d = __builtin_bit_cast(double2, v);
+          d.x *= m;
+          d.y *= m;
+          v = __builtin_bit_cast(float4, d);
+        }
+      }
+      dst[k] = v;
+    }
     wave_lds_sync();
     const long long q0 = eoff[ch], q1 = eoff[ch + 1];
     // entry groups loaded one group ahead (index clamped into the chunk)
@@ -886,15 +907,24 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
       for (int k = 0; k < 4; ++k) {
         const unsigned sl = (sl4 >> (8 * k)) & 255u;
         if (sl != 255u) {
-          const float4* row = srow + sl * kBlkRow4;
+          const float4* row = srow + sl * RU;
           const double w = (double)wk[k];
+          if constexpr (sizeof(S) == 8) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float4 v = row[j];
-            ar[2 * j] += w * (double)v.x;
-            ai[2 * j] += w * (double)v.y;
-            ar[2 * j + 1] += w * (double)v.z;
-            ai[2 * j + 1] += w * (double)v.w;
+            for (int j = 0; j < 4; ++j) {
+              const float4 v = row[j];
+              ar[2 * j] += w * (double)v.x;
+              ai[2 * j] += w * (double)v.y;
+              ar[2 * j + 1] += w * (double)v.z;
+              ai[2 * j + 1] += w * (double)v.w;
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const double2 v = __builtin_bit_cast(double2, row[j]);
+              ar[j] += w * v.x;
+              ai[j] += w * v.y;
+            }
           }
         }
       }
@@ -1587,18 +1617,20 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   using G = typename std::conditional<sizeof(R) == 4, double2, cplx<R>>::type;
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
-  // the block-staged split adjoint (8-coil groups; its partials take the
-  // workspace spread() sized for them)
-  constexpr bool kBlk = sizeof(R) == 4 && COMP && FULL && LPS * CPL == 8;
-  if (kBlk && block_staged(p, cg, wsamp != nullptr) && wide) {
+  // the block-staged adjoint (8-coil groups): float samples of the split
+  // aNUFT, or double samples (with data weights) of an fp64 plan or of iNUFT;
+  // its partials take the workspace spread() sized for them
+  constexpr bool kBlk = FULL && LPS * CPL == 8 && ((sizeof(R) == 4 && COMP) || sizeof(R) == 8);
+  if (kBlk && block_staged(p, cg) && (sizeof(R) == 8 || wide) && (sizeof(R) == 8 || !wsamp)) {
     {
       ProfScope ps(p, "k_spread_blk");
-      hipLaunchKernelGGL(k_spread_blk, dim3((unsigned)((p->bk.npiece + 3) / 4)), dim3(256), 0,
-                         p->stream, (const int4*)p->bk.d_piece, (long long)p->bk.npiece,
+      hipLaunchKernelGGL((k_spread_blk<cplx<R>>), dim3((unsigned)((p->bk.npiece + 3) / 4)), dim3(256),
+                         0, p->stream, (const int4*)p->bk.d_piece, (long long)p->bk.npiece,
                          (const int*)p->bk.d_blk, (const long long*)p->bk.d_cbase,
                          (const int*)p->bk.d_samp, (const long long*)p->bk.d_eoff,
                          (const unsigned*)p->bk.d_slot, (const float4*)p->bk.d_w,
-                         (const float2*)samp, (double2*)grid, (double2*)partial, g);
+                         (const cplx<R>*)samp, (const double*)wsamp, (double2*)grid,
+                         (double2*)partial, g);
     }
     if (p->bk.nred > 0) {
       ProfScope ps(p, "k_spread_blk_reduce");
@@ -1695,7 +1727,7 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
   const bool wide = split;
   // the block-staged split adjoint takes 8-coil groups whatever the group size
   // (and the block-staged forward 8-coil groups of fp64 arithmetic)
-  const int gmax = (SPREAD && split && p->bk.d_piece && (p->tune_flags & 4096)) ||
+  const int gmax = (SPREAD && p->bk.d_piece && (p->tune_flags & 4096) && p->prec == 2) ||
                            (!SPREAD && sizeof(R) == 8 && p->fk.d_slot && (p->tune_flags & 8192))
                        ? 8
                        : p->coil_group;
@@ -1781,8 +1813,11 @@ static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const
   const int cgmax = nc < p->coil_group ? nc : p->coil_group;
   void* partial = nullptr;
   const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
-  if (nseg > 0) {
-    N3D_TRY(ensure(p->ws_part, (size_t)nseg * cgmax * 2 * sizeof(double)));
+  size_t pbytes = (size_t)nseg * cgmax * 2 * sizeof(double);
+  if (sizeof(R) == 8 && p->bk.d_piece && (p->tune_flags & 4096))   // block-staged pieces
+    pbytes = std::max(pbytes, (size_t)p->bk.npslot * 64 * 8 * sizeof(double2));
+  if (pbytes > 0) {
+    N3D_TRY(ensure(p->ws_part, pbytes));
     partial = p->ws_part.p;
   }
   if (sizeof(R) == 4 && p->compensated) run_groups<R, true, true>(p, samp, nc, grid, partial, wsamp, 1.0);

@@ -350,10 +350,11 @@ inline bool split_precision(const nufft3d_plan* p) {
   return p->prec == 2 && p->io_prec == 1 && !(p->tune_flags & (128 | 512));
 }
 
-// the block-staged split adjoint (k_spread_blk) runs for this launch: its
-// lists are built, tune flag 4096 asks for it, an 8-coil group, no data weights
-inline bool block_staged(const nufft3d_plan* p, int cg, bool wsamp) {
-  return p->bk.d_piece && (p->tune_flags & 4096) && cg == 8 && !wsamp && split_precision(p);
+// the block-staged adjoint (k_spread_blk) of an fp64-arithmetic plan runs
+// for this launch: its lists are built, tune flag 4096 asks for it, an
+// 8-coil group
+inline bool block_staged(const nufft3d_plan* p, int cg) {
+  return p->bk.d_piece && (p->tune_flags & 4096) && cg == 8 && p->prec == 2;
 }
 int ensure_block_lists(nufft3d_plan* p);   // builds them once if the plan qualifies
 // the block-staged forward (k_interp_blk) runs for this launch: its lists

@@ -1284,9 +1284,9 @@ static int build_block_lists(nufft3d_plan* p) {
 }
 
 // BlockLists when asked for (tune flag 4096 at plan creation) or timed by
-// the autotuner: split-precision 3-D plans with the weight table
+// the autotuner: fp64-arithmetic 3-D plans with the weight table
 int ensure_block_lists(nufft3d_plan* p) {
-  if (p->bk.d_piece || p->bk.tried || !split_precision(p) || p->K[2] == 1 || !p->d_hitw)
+  if (p->bk.d_piece || p->bk.tried || p->prec != 2 || p->K[2] == 1 || !p->d_hitw)
     return NUFFT3D_OK;
   p->bk.tried = true;
   return build_block_lists(p);
@@ -3020,7 +3020,7 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
     return fft_adjoint(p, p->ws_grid.p, nc, xout.p, sc, nullptr, 0.0);
   };
   if (!st && !split) st = ensure_brick_lists(p);
-  if (!st && split) st = ensure_block_lists(p);
+  if (!st && p->prec == 2) st = ensure_block_lists(p);
   if (st) { release(xin); release(xout); release(ks); release(ks32); return st; }
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
   // spread (plans with brick lists), 3 block-staged split adjoint (plans

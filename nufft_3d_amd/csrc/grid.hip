@@ -839,7 +839,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename S>
+template <typename S, int C>
 __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pieces,
                                                     long long npiece, const int* __restrict__ blk,
                                                     const long long* __restrict__ cbase,
@@ -853,7 +853,7 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
                                                     double2* __restrict__ partial, GridGeom g) {
   constexpr int RU = BlkRow<S>::U;
   constexpr int DU = RU - 1;          // 16-byte units of data per row (4 float, 8 double)
-  __shared__ float4 s_row[4 * kBlkChunk * RU];
+  __shared__ float4 s_row[4 * C * RU];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long pi = xcd_block(g.xchunk) * 4 + wv;
   if (pi >= npiece) return;
@@ -865,32 +865,35 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
 #pragma unroll
   for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
   const long long cb = cbase[pc.x];
-  float4* srow = s_row + wv * (kBlkChunk * RU);
+  float4* srow = s_row + wv * (C * RU);
   const float4* ks4 = reinterpret_cast<const float4*>(ks);
   const long long rs4 = (long long)g.nc * sizeof(S) / 16;   // 16-byte units per sample row
   const long long co4 = (long long)g.c0 * sizeof(S) / 16;
   for (int c = pc.y; c < pc.z; ++c) {
     const long long ch = cb + c;
-    const int sm = samp[ch * kBlkChunk + lane];
-    const float4* src = ks4 + (long long)sm * rs4 + co4;
-    double m = 1.0;   // iNUFT's data weight of the sample, applied as it is staged
-    if constexpr (sizeof(S) == 16) {
-      if (wsamp) m = wsamp[sm];
-    }
     wave_lds_sync();   // the previous chunk's rows are consumed
-    float4* dst = srow + lane * RU;
 #pragma unroll
-    for (int k = 0; k < DU; ++k) {
-      float4 v = src[k];
+    for (int h = 0; h < C / 64; ++h) {   // rows lane, lane + 64 of the chunk
+      const int sm = samp[ch * C + h * 64 + lane];
+      const float4* src = ks4 + (long long)sm * rs4 + co4;
+      double m = 1.0;   // iNUFT's data weight of the sample, applied as it is staged
       if constexpr (sizeof(S) == 16) {
-        if (wsamp) {
-          double2 d = __builtin_bit_cast(double2, v);
-          d.x *= m;
-          d.y *= m;
-          v = __builtin_bit_cast(float4, d);
-        }
+        if (wsamp) m = wsamp[sm];
       }
-      dst[k] = v;
+      float4* dst = srow + (h * 64 + lane) * RU;
+#pragma unroll
+      for (int k = 0; k < DU; ++k) {
+        float4 v = src[k];
+        if constexpr (sizeof(S) == 16) {
+          if (wsamp) {
+            double2 d = __builtin_bit_cast(double2, v);
+            d.x *= m;
+            d.y *= m;
+            v = __builtin_bit_cast(float4, d);
+          }
+        }
+        dst[k] = v;
+      }
     }
     wave_lds_sync();
     const long long q0 = eoff[ch], q1 = eoff[ch + 1];
@@ -1624,13 +1627,17 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   if (kBlk && block_staged(p, cg) && (sizeof(R) == 8 || wide) && (sizeof(R) == 8 || !wsamp)) {
     {
       ProfScope ps(p, "k_spread_blk");
-      hipLaunchKernelGGL((k_spread_blk<cplx<R>>), dim3((unsigned)((p->bk.npiece + 3) / 4)), dim3(256),
-                         0, p->stream, (const int4*)p->bk.d_piece, (long long)p->bk.npiece,
-                         (const int*)p->bk.d_blk, (const long long*)p->bk.d_cbase,
-                         (const int*)p->bk.d_samp, (const long long*)p->bk.d_eoff,
-                         (const unsigned*)p->bk.d_slot, (const float4*)p->bk.d_w,
-                         (const cplx<R>*)samp, (const double*)wsamp, (double2*)grid,
-                         (double2*)partial, g);
+#define N3D_SBLK(CC)                                                                              \
+      hipLaunchKernelGGL((k_spread_blk<cplx<R>, CC>), dim3((unsigned)((p->bk.npiece + 3) / 4)),   \
+                         dim3(256), 0, p->stream, (const int4*)p->bk.d_piece,                     \
+                         (long long)p->bk.npiece, (const int*)p->bk.d_blk,                        \
+                         (const long long*)p->bk.d_cbase, (const int*)p->bk.d_samp,               \
+                         (const long long*)p->bk.d_eoff, (const unsigned*)p->bk.d_slot,           \
+                         (const float4*)p->bk.d_w, (const cplx<R>*)samp, (const double*)wsamp,    \
+                         (double2*)grid, (double2*)partial, g)
+      if (p->bk.C == 128) N3D_SBLK(128);
+      else N3D_SBLK(64);
+#undef N3D_SBLK
     }
     if (p->bk.nred > 0) {
       ProfScope ps(p, "k_spread_blk_reduce");

@@ -149,7 +149,7 @@ struct WideLists {
 // Block-staged adjoint (k_spread_blk): the grid in 4x4x4 blocks, one
 // wavefront per block and one lane per point. A block's samples (the
 // distinct samples that touch any of its points, ascending ids) are dealt
-// round-robin into chunks of kBlkChunk: sample i of the list goes to chunk
+// round-robin into chunks of C samples: sample i of the list goes to chunk
 // i % nch at slot i / nch, so each chunk is an even subsample of the block's
 // neighbourhood and a point meets about the same share of its hits in every
 // chunk (contiguous chunks leave most lanes idle: a run of sorted samples is
@@ -158,14 +158,14 @@ struct WideLists {
 // word, 4 fp32 weights per lane in one float4), lanes with fewer hits
 // padded with slot 255 and weight 0. Blocks with more entry rows than a
 // piece holds are split into pieces whose partial sums add in piece order.
-constexpr int kBlkChunk = 64;       // samples staged per chunk
 constexpr int kBlkPieceRows = 256;  // entry rows (steps) per piece, at least one chunk
 struct BlockLists {
+  int C = 64;                       // samples staged per chunk (64 or 128)
   long long nblk = 0;               // blocks, in launch order (Z-order of block coordinates)
   int* d_blk = nullptr;             // [nblk] x | y << 10 | z << 20 (block coordinates)
   long long* d_cbase = nullptr;     // [nblk + 1] first chunk of each block
   long long nchunk = 0;
-  int* d_samp = nullptr;            // [nchunk * kBlkChunk] sample ids (padding: the chunk's first)
+  int* d_samp = nullptr;            // [nchunk * C] sample ids (padding: the chunk's first)
   long long* d_eoff = nullptr;      // [nchunk + 1] first entry group of each chunk
   unsigned* d_slot = nullptr;       // [groups * 64] 4 slots per lane (byte k: step 4j+k; 255 none)
   float4* d_w = nullptr;            // [groups * 64] 4 weights per lane (0 for padding)
@@ -300,6 +300,7 @@ struct nufft3d_plan {
   bool bl_tried = false;
   nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
   nufft3d::BlockLists bk;           // block-staged lists of the split-precision adjoint
+  int blk_chunk = 64;               // samples per chunk of the block-staged adjoint's lists
   nufft3d::FwdLists fk;             // block-staged lists of the fp64-arithmetic forward
   int max_hits = 0;
 

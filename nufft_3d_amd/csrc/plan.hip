@@ -974,14 +974,14 @@ __global__ void k_bk_slot(const long long* __restrict__ rowptr, const uint2* __r
 // i / nch; the unused slots of a chunk hold its first sample (never read)
 __global__ void k_bk_samp(const int* __restrict__ asc, const long long* __restrict__ beg,
                           const long long* __restrict__ end, const long long* __restrict__ cbase,
-                          int* __restrict__ samp) {
+                          int C, int* __restrict__ samp) {
   const long long b = blockIdx.x;
   const long long base = beg[b], u = end[b] - base;
   const long long c0 = cbase[b], nch = cbase[b + 1] - c0;
-  for (long long i = threadIdx.x; i < nch * kBlkChunk; i += blockDim.x) {
-    const long long c = i / kBlkChunk, o = i % kBlkChunk;
+  for (long long i = threadIdx.x; i < nch * C; i += blockDim.x) {
+    const long long c = i / C, o = i % C;
     const long long li = o * nch + c;
-    samp[(c0 + c) * kBlkChunk + o] = asc[base + (li < u ? li : c)];
+    samp[(c0 + c) * C + o] = asc[base + (li < u ? li : c)];
   }
 }
 
@@ -1063,6 +1063,7 @@ static void drop_block_lists(nufft3d_plan* p) {
 // blocks at a time (any total hit count; a slab needs fewer than 2^31 hits).
 static int build_block_lists(nufft3d_plan* p) {
   BlockLists& bk = p->bk;
+  const int C = p->blk_chunk == 128 ? 128 : 64;   // samples staged per chunk
   const long long nh = p->nhits;
   if (nh <= 0 || !p->d_hitw || p->K[2] == 1) return NUFFT3D_OK;
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
@@ -1190,11 +1191,11 @@ static int build_block_lists(nufft3d_plan* p) {
       hipStreamSynchronize(p->stream) != hipSuccess)
     return fail("copy failed");
   for (long long b = 0; b < nblk; ++b)
-    cbase[b + 1] = cbase[b] + (he[b] - hb[b] + kBlkChunk - 1) / kBlkChunk;
+    cbase[b + 1] = cbase[b] + (he[b] - hb[b] + C - 1) / C;
   const long long nchunk = cbase[nblk];
   if (dev_malloc(&bk.d_blk, sizeof(int) * nblk) != hipSuccess ||
       dev_malloc(&bk.d_cbase, sizeof(long long) * (nblk + 1)) != hipSuccess ||
-      dev_malloc(&bk.d_samp, sizeof(int) * std::max(1LL, nchunk * kBlkChunk)) != hipSuccess ||
+      dev_malloc(&bk.d_samp, sizeof(int) * std::max(1LL, nchunk * C)) != hipSuccess ||
       dev_malloc(&bk.d_eoff, sizeof(long long) * (nchunk + 1)) != hipSuccess ||
       dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * 64)) != hipSuccess ||
       dev_malloc(&hk, sizeof(unsigned short) * nh) != hipSuccess ||
@@ -1205,7 +1206,7 @@ static int build_block_lists(nufft3d_plan* p) {
       hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * 64), p->stream) != hipSuccess)
     return fail("copy failed");
   hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)nblk), dim3(256), 0, p->stream, (const int*)asc,
-                     (const long long*)beg, (const long long*)end, (const long long*)bk.d_cbase,
+                     (const long long*)beg, (const long long*)end, (const long long*)bk.d_cbase, C,
                      bk.d_samp);
   hipLaunchKernelGGL(k_bk_rank, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr, ngrid,
                      K1, K2, d_bof, nbx, nby, (const long long*)bk.d_cbase, (const unsigned*)hslot,
@@ -1270,6 +1271,7 @@ static int build_block_lists(nufft3d_plan* p) {
        hipMemcpy(bk.d_red, red.data(), sizeof(int4) * red.size(), hipMemcpyHostToDevice) != hipSuccess))
     return fail("copy failed");
   cleanup();
+  bk.C = C;
   bk.nblk = nblk;
   bk.nchunk = nchunk;
   bk.ngroups = ngroups;
@@ -1492,7 +1494,7 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
     return fail("copy failed");
   // chunked point lists (the same dealing as the adjoint's sample lists)
   hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)ngrp), dim3(256), 0, p->stream, (const int*)lst,
-                     (const long long*)beg, (const long long*)end, (const long long*)fk.d_cbase,
+                     (const long long*)beg, (const long long*)end, (const long long*)fk.d_cbase, 64,
                      fk.d_gpt);
   hipLaunchKernelGGL(k_fk_rank, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
                      (const long long*)p->d_spos, nnz, (const long long*)fk.d_cbase,
@@ -2149,6 +2151,7 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_TEST_CAPTURE_FAIL")) p->capture_fail_at = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_TILE_CG")) p->tile_cg = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_TILE_LAYERS")) p->tile_layers = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_BLK_CHUNK")) p->blk_chunk = atoi(tv);
   if (o.tile > 0) {
     p->T[0] = p->T[1] = p->T[2] = o.tile;
     p->tile_user = true;
@@ -2249,7 +2252,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
   if (p->wl.d_went) b += 16 * p->nbrick + 4 * std::max<int64_t>(1, p->wl.total) + 8 * p->nhits;
   if (p->bk.d_piece)
-    b += 12 * p->bk.nblk + 8 + (4 * kBlkChunk + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
+    b += 12 * p->bk.nblk + 8 + (4 * p->bk.C + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
          16 * (p->bk.npiece + std::max<int64_t>(1, p->bk.nred));
   if (p->fk.d_slot)
     b += 8 * (p->fk.ngrp + 1) + (4 * 64 + 8) * p->fk.nchunk + 8 +

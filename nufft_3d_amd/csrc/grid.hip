@@ -869,6 +869,8 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
   const float4* ks4 = reinterpret_cast<const float4*>(ks);
   const long long rs4 = (long long)g.nc * sizeof(S) / 16;   // 16-byte units per sample row
   const long long co4 = (long long)g.c0 * sizeof(S) / 16;
+  // a whole 8-coil group at a 16-byte aligned row offset takes 16-byte loads
+  const bool full = g.cg == 8 && (sizeof(S) == 16 || ((g.nc & 1) == 0 && (g.c0 & 1) == 0));
   for (int c = pc.y; c < pc.z; ++c) {
     const long long ch = cb + c;
     wave_lds_sync();   // the previous chunk's rows are consumed
@@ -876,6 +878,7 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
     for (int h = 0; h < C / 64; ++h) {   // rows lane, lane + 64 of the chunk
       const int sm = samp[ch * C + h * 64 + lane];
       const float4* src = ks4 + (long long)sm * rs4 + co4;
+      const S* srcc = ks + (long long)sm * g.nc + g.c0;   // coil by coil (partial groups)
       double m = 1.0;   // iNUFT's data weight of the sample, applied as it is staged
       if constexpr (sizeof(S) == 16) {
         if (wsamp) m = wsamp[sm];
@@ -883,7 +886,16 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
       float4* dst = srow + (h * 64 + lane) * RU;
 #pragma unroll
       for (int k = 0; k < DU; ++k) {
-        float4 v = src[k];
+        float4 v;
+        if (full) {
+          v = src[k];
+        } else if constexpr (sizeof(S) == 8) {   // coils 2k, 2k+1 (zero past the group)
+          const float2 a = 2 * k < g.cg ? srcc[2 * k] : make_float2(0.f, 0.f);
+          const float2 b = 2 * k + 1 < g.cg ? srcc[2 * k + 1] : make_float2(0.f, 0.f);
+          v = make_float4(a.x, a.y, b.x, b.y);
+        } else {                                 // coil k
+          v = k < g.cg ? __builtin_bit_cast(float4, srcc[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         if constexpr (sizeof(S) == 16) {
           if (wsamp) {
             double2 d = __builtin_bit_cast(double2, v);
@@ -936,10 +948,16 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
     }
   }
   if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
-  double2* o = pc.w < 0 ? grid + (((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0
-                        : partial + ((long long)pc.w * 64 + lane) * 8;
+  if (pc.w < 0) {
+    double2* o = grid + (((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) o[c] = make_double2(ar[c], ai[c]);
+    for (int c = 0; c < 8; ++c)
+      if (c < g.cg) o[c] = make_double2(ar[c], ai[c]);
+  } else {
+    double2* o = partial + ((long long)pc.w * 64 + lane) * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o[c] = make_double2(ar[c], ai[c]);
+  }
 }
 
 // Blocks in several pieces: each point's partial sums added in piece order.
@@ -951,6 +969,7 @@ __global__ void __launch_bounds__(512) k_spread_blk_reduce(const int4* __restric
                                                            double2* __restrict__ grid, GridGeom g) {
   const int4 r = red[blockIdx.x];   // (block, first partial slot, pieces, 0)
   const int lane = threadIdx.x >> 3, c = threadIdx.x & 7;
+  if (c >= g.cg) return;
   const int bc = blk[r.x];
   const int x = (bc & 1023) * 4 + (lane & 3), y = ((bc >> 10) & 1023) * 4 + ((lane >> 2) & 3),
             z = (bc >> 20) * 4 + (lane >> 4);
@@ -1022,15 +1041,21 @@ __global__ void __launch_bounds__(256) k_interp_blk(const long long* __restrict_
   const long long c0 = cbase[j], c1 = cbase[j + 1];
   for (long long ch = c0; ch < c1; ++ch) {
     const int gp = gpt[ch * 64 + lane];
-    const double4* src = reinterpret_cast<const double4*>(grid + (long long)gp * g.nc + g.c0);
-    const double4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+    const double2* srcc = grid + (long long)gp * g.nc + g.c0;
     const int px = gp % g.K1, py = (gp / g.K1) % g.K2, pz = gp / (g.K1 * g.K2);
     wave_lds_sync();   // the previous chunk's rows are consumed
     double2* dst = srow + lane * kFwdRow;
-    dst[0] = make_double2(r0.x, r0.y); dst[1] = make_double2(r0.z, r0.w);
-    dst[2] = make_double2(r1.x, r1.y); dst[3] = make_double2(r1.z, r1.w);
-    dst[4] = make_double2(r2.x, r2.y); dst[5] = make_double2(r2.z, r2.w);
-    dst[6] = make_double2(r3.x, r3.y); dst[7] = make_double2(r3.z, r3.w);
+    if (g.cg == 8) {
+      const double4* src = reinterpret_cast<const double4*>(srcc);
+      const double4 r0 = src[0], r1 = src[1], r2 = src[2], r3 = src[3];
+      dst[0] = make_double2(r0.x, r0.y); dst[1] = make_double2(r0.z, r0.w);
+      dst[2] = make_double2(r1.x, r1.y); dst[3] = make_double2(r1.z, r1.w);
+      dst[4] = make_double2(r2.x, r2.y); dst[5] = make_double2(r2.z, r2.w);
+      dst[6] = make_double2(r3.x, r3.y); dst[7] = make_double2(r3.z, r3.w);
+    } else {   // a partial group: its coils, zero past them
+#pragma unroll
+      for (int k = 0; k < 8; ++k) dst[k] = k < g.cg ? srcc[k] : make_double2(0.0, 0.0);
+    }
     spc[lane] = px | (py << 10) | (pz << 20);
     wave_lds_sync();
     const long long q0 = eoff[ch], q1 = eoff[ch + 1];
@@ -1079,6 +1104,7 @@ __global__ void __launch_bounds__(256) k_interp_blk(const long long* __restrict_
   if (wsamp) sc *= wsamp[s];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
+    if (c >= g.cg) break;
     cplx<RO> v;
     v.x = (RO)(ar[c] * sc);
     v.y = (RO)(ai[c] * sc);
@@ -1351,36 +1377,38 @@ static GridGeom make_geom(const nufft3d_plan* p, int nc, int c0, int cg) {
   return g;
 }
 
+// the block-staged forward of one coil group of at most 8 coils
+static void launch_interp_blk(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
+                              const void* wsamp, double scale, bool user, bool narrow) {
+  GridGeom g = make_geom(p, nc, c0, cg);
+  KBParams kp = make_kb(p);
+  ProfScope ps(p, "k_interp_blk");
+  const dim3 grid_((unsigned)((p->fk.ngrp + 3) / 4)), blk_(256);
+  const double* tb = kb_table_of(p);
+  const bool stw = p->fk.d_w && (p->tune_flags & 16384);
+#define N3D_IBLK(RO, USR, TB, SW)                                                                 \
+  hipLaunchKernelGGL((k_interp_blk<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,              \
+                     p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,      \
+                     p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid,     \
+                     (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp)
+#define N3D_IBLK2(RO, USR)                                                                        \
+  do {                                                                                        \
+    if (stw) N3D_IBLK(RO, USR, true, true);                                                   \
+    else if (tb) N3D_IBLK(RO, USR, true, false);                                              \
+    else N3D_IBLK(RO, USR, false, false);                                                     \
+  } while (0)
+  if (narrow) N3D_IBLK2(float, true);
+  else if (user) N3D_IBLK2(double, true);
+  else N3D_IBLK2(double, false);
+#undef N3D_IBLK2
+#undef N3D_IBLK
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_interp(nufft3d_plan* p, const void* grid, int nc, int c0, int cg, void* out,
                           const void* wsamp, double scale, bool user, bool narrow) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
-  if constexpr (sizeof(R) == 8 && FULL && CPL * LPS == 8) {
-    if (fwd_block_staged(p, cg)) {   // the block-staged forward (8-coil groups)
-      ProfScope ps(p, "k_interp_blk");
-      const dim3 grid_((unsigned)((p->fk.ngrp + 3) / 4)), blk_(256);
-      const double* tb = kb_table_of(p);
-      const bool stw = p->fk.d_w && (p->tune_flags & 16384);
-#define N3D_IBLK(RO, USR, TB, SW)                                                                 \
-      hipLaunchKernelGGL((k_interp_blk<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,              \
-                         p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,      \
-                         p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid,     \
-                         (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp)
-#define N3D_IBLK2(RO, USR)                                                                        \
-      do {                                                                                        \
-        if (stw) N3D_IBLK(RO, USR, true, true);                                                   \
-        else if (tb) N3D_IBLK(RO, USR, true, false);                                              \
-        else N3D_IBLK(RO, USR, false, false);                                                     \
-      } while (0)
-      if (narrow) N3D_IBLK2(float, true);
-      else if (user) N3D_IBLK2(double, true);
-      else N3D_IBLK2(double, false);
-#undef N3D_IBLK2
-#undef N3D_IBLK
-      return;
-    }
-  }
   const long long nthreads = p->nnz * (long long)LPS;
   const int nb = (int)((nthreads + 255) / 256);
   ProfScope ps(p, "k_interp");
@@ -1612,6 +1640,35 @@ int fk_fill(nufft3d_plan* p, const unsigned short* hslot, const unsigned short* 
   return NUFFT3D_OK;
 }
 
+// the block-staged adjoint of one coil group of at most 8 coils: float
+// samples of the split aNUFT, or double samples (with data weights) of an
+// fp64 plan or of iNUFT; its partials take the workspace spread() sized
+template <typename R>
+static void launch_spread_blk(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
+                              void* partial, const void* wsamp) {
+  GridGeom g = make_geom(p, nc, c0, cg);
+  {
+    ProfScope ps(p, "k_spread_blk");
+#define N3D_SBLK(CC)                                                                              \
+    hipLaunchKernelGGL((k_spread_blk<cplx<R>, CC>), dim3((unsigned)((p->bk.npiece + 3) / 4)),   \
+                       dim3(256), 0, p->stream, (const int4*)p->bk.d_piece,                     \
+                       (long long)p->bk.npiece, (const int*)p->bk.d_blk,                        \
+                       (const long long*)p->bk.d_cbase, (const int*)p->bk.d_samp,               \
+                       (const long long*)p->bk.d_eoff, (const unsigned*)p->bk.d_slot,           \
+                       (const float4*)p->bk.d_w, (const cplx<R>*)samp, (const double*)wsamp,    \
+                       (double2*)grid, (double2*)partial, g)
+    if (p->bk.C == 128) N3D_SBLK(128);
+    else N3D_SBLK(64);
+#undef N3D_SBLK
+  }
+  if (p->bk.nred > 0) {
+    ProfScope ps(p, "k_spread_blk_reduce");
+    hipLaunchKernelGGL(k_spread_blk_reduce, dim3((unsigned)p->bk.nred), dim3(512), 0, p->stream,
+                       (const int4*)p->bk.d_red, (const int*)p->bk.d_blk,
+                       (const double2*)partial, (double2*)grid, g);
+  }
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
                           void* partial, const void* wsamp, bool wide) {
@@ -1620,33 +1677,6 @@ static void launch_spread(nufft3d_plan* p, const void* samp, int nc, int c0, int
   using G = typename std::conditional<sizeof(R) == 4, double2, cplx<R>>::type;
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
-  // the block-staged adjoint (8-coil groups): float samples of the split
-  // aNUFT, or double samples (with data weights) of an fp64 plan or of iNUFT;
-  // its partials take the workspace spread() sized for them
-  constexpr bool kBlk = FULL && LPS * CPL == 8 && ((sizeof(R) == 4 && COMP) || sizeof(R) == 8);
-  if (kBlk && block_staged(p, cg) && (sizeof(R) == 8 || wide) && (sizeof(R) == 8 || !wsamp)) {
-    {
-      ProfScope ps(p, "k_spread_blk");
-#define N3D_SBLK(CC)                                                                              \
-      hipLaunchKernelGGL((k_spread_blk<cplx<R>, CC>), dim3((unsigned)((p->bk.npiece + 3) / 4)),   \
-                         dim3(256), 0, p->stream, (const int4*)p->bk.d_piece,                     \
-                         (long long)p->bk.npiece, (const int*)p->bk.d_blk,                        \
-                         (const long long*)p->bk.d_cbase, (const int*)p->bk.d_samp,               \
-                         (const long long*)p->bk.d_eoff, (const unsigned*)p->bk.d_slot,           \
-                         (const float4*)p->bk.d_w, (const cplx<R>*)samp, (const double*)wsamp,    \
-                         (double2*)grid, (double2*)partial, g)
-      if (p->bk.C == 128) N3D_SBLK(128);
-      else N3D_SBLK(64);
-#undef N3D_SBLK
-    }
-    if (p->bk.nred > 0) {
-      ProfScope ps(p, "k_spread_blk_reduce");
-      hipLaunchKernelGGL(k_spread_blk_reduce, dim3((unsigned)p->bk.nred), dim3(512), 0, p->stream,
-                         (const int4*)p->bk.d_red, (const int*)p->bk.d_blk,
-                         (const double2*)partial, (double2*)grid, g);
-    }
-    return;
-  }
   const long long ngrid = (long long)p->K[0] * p->K[1] * p->K[2];
   // one lane per point (small fp32 groups, every fp64 group): short dense
   // segments; lane groups: long ones
@@ -1753,7 +1783,16 @@ static void run_groups(nufft3d_plan* p, const void* in, int nc, void* out, void*
       if (cg == (CPL) * (LPS) && ((CPL) == 1 || even)) N3D_GO2(CPL, LPS, true);                  \
       else N3D_GO2(CPL, LPS, false);                                                             \
     } while (0)
-    if constexpr (!SPREAD) {
+    if constexpr (SPREAD) {
+      if (block_staged(p, cg) && (sizeof(R) == 8 || (wide && COMP && !wsamp))) {
+        launch_spread_blk<R>(p, in, nc, c0, cg, out, partial, wsamp);
+        continue;
+      }
+    } else {
+      if (sizeof(R) == 8 && fwd_block_staged(p, cg)) {
+        launch_interp_blk(p, in, nc, c0, cg, out, wsamp, scale, user, split);
+        continue;
+      }
       if (p->tile_interp && p->d_piece && p->npiece > 0 && !split) {
         launch_interp_tile<R, COMP>(p, in, nc, c0, cg, out, wsamp, scale, user);
         continue;
@@ -1781,6 +1820,10 @@ template <typename R>
 static int interp_t(nufft3d_plan* p, const void* grid, int nc, void* out, const void* wsamp,
                     double scale, bool user, bool narrow = false) {
   if (p->nnz == 0) return NUFFT3D_OK;
+  if (!p->d_fcode && p->nhits > 0 && !(sizeof(R) == 8 && p->fk.d_slot && (p->tune_flags & 8192))) {
+    set_error("the plan's gather lists were released for the block-staged kernels");
+    return NUFFT3D_ERR_INVALID;
+  }
   if (sizeof(R) == 4 && p->compensated)
     run_groups<R, true, false>(p, grid, nc, out, nullptr, wsamp, scale, user);
   else
@@ -1817,6 +1860,11 @@ int interp(nufft3d_plan* p, const void* grid, int nc, void* out, int user_layout
 // samples in the internal layout ks[s*nc + c] (sorted order)
 template <typename R>
 static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const void* wsamp) {
+  if (!p->d_hitw && !p->d_hit && p->nhits > 0 &&
+      !(sizeof(R) == 8 && p->bk.d_piece && (p->tune_flags & 4096))) {
+    set_error("the plan's gather lists were released for the block-staged kernels");
+    return NUFFT3D_ERR_INVALID;
+  }
   const int cgmax = nc < p->coil_group ? nc : p->coil_group;
   void* partial = nullptr;
   const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
@@ -1842,7 +1890,8 @@ int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layou
       samp = p->ws_samp.p;
       wsamp = nullptr;
     }
-    if (!p->d_hitw || (p->tune_flags & 128)) {   // the float samples carry no offsets
+    const bool blk = p->bk.d_piece && (p->tune_flags & 4096);
+    if (!blk && (!p->d_hitw || (p->tune_flags & 128))) {   // the float samples carry no offsets
       set_error("the split-precision adjoint needs the plan's weight table");
       return NUFFT3D_ERR_INVALID;
     }

@@ -352,16 +352,16 @@ inline bool split_precision(const nufft3d_plan* p) {
 }
 
 // the block-staged adjoint (k_spread_blk) of an fp64-arithmetic plan runs
-// for this launch: its lists are built, tune flag 4096 asks for it, an
-// 8-coil group
+// for this launch: its lists are built, tune flag 4096 asks for it, a coil
+// group of at most 8
 inline bool block_staged(const nufft3d_plan* p, int cg) {
-  return p->bk.d_piece && (p->tune_flags & 4096) && cg == 8 && p->prec == 2;
+  return p->bk.d_piece && (p->tune_flags & 4096) && cg >= 1 && cg <= 8 && p->prec == 2;
 }
 int ensure_block_lists(nufft3d_plan* p);   // builds them once if the plan qualifies
 // the block-staged forward (k_interp_blk) runs for this launch: its lists
-// are built, tune flag 8192 asks for it, an 8-coil group
+// are built, tune flag 8192 asks for it, a coil group of at most 8
 inline bool fwd_block_staged(const nufft3d_plan* p, int cg) {
-  return p->fk.d_slot && (p->tune_flags & 8192) && cg == 8 && p->prec == 2;
+  return p->fk.d_slot && (p->tune_flags & 8192) && cg >= 1 && cg <= 8 && p->prec == 2;
 }
 int ensure_fwd_lists(nufft3d_plan* p, bool weights);
 int fk_fill(nufft3d_plan* p, const unsigned short* hslot, const unsigned short* hk);   // grid.hip

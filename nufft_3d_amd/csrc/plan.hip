@@ -2240,7 +2240,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   b += 4 * std::max<int64_t>(p->M, 1);                                // inv
   b += rsz * nimg + rsz * nnz;                                        // U, density
   b += 8 * (nnz + 1) + 8 * (ngrid + 1);                               // spos, rowptr
-  b += 2 * std::max<int64_t>(p->nhits, 1);                            // fcode
+  if (p->d_fcode) b += 2 * std::max<int64_t>(p->nhits, 1);           // fcode
   if (p->d_hit) b += 8 * std::max<int64_t>(p->nhits, 1);              // (sample, code) hits
   if (p->d_hitw) b += 8 * p->nhits;                                   // (sample, weight) hits
   if (p->d_off32) b += 16 * nnz;
@@ -3025,6 +3025,10 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
   if (!st && !split) st = ensure_brick_lists(p);
   if (!st && p->prec == 2) st = ensure_block_lists(p);
   if (st) { release(xin); release(xout); release(ks); release(ks32); return st; }
+  // NUFFT3D_TUNE_BLOCK_ONLY=1 restricts the choice to the block-staged kernels
+  // where the plan has their lists
+  const char* ob = getenv("NUFFT3D_TUNE_BLOCK_ONLY");
+  const bool only_blk = ob && atoi(ob) != 0;
   // brick: 0 linear gather spread, 1 brick-order gather spread, 2 LDS-staged
   // spread (plans with brick lists), 3 block-staged split adjoint (plans
   // with block lists; 8-coil groups)
@@ -3037,6 +3041,8 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
           if (group == 8 && nc <= 8) continue;   // same launches as 16
           if (brick == 2 && !(split ? (void*)p->wl.d_went : (void*)p->bl.d_ent)) continue;
           if (brick == 3 && !p->bk.d_piece) continue;   // (8-coil spread groups)
+          if (brick != 3 && !p->d_hitw) continue;       // gather lists released
+          if (brick != 3 && only_blk && p->bk.d_piece) continue;
           cands.push_back({fftb, group, xcd, brick, 0.f});
         }
   auto flags_for = [&](int brick, int keep) {
@@ -3118,6 +3124,8 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
       const int keep = p->tune_flags;
       for (int fi = 0; fi < 3 && !st; ++fi) {
         if (fi == 2 && !p->fk.d_w) continue;
+        if (fi == 0 && !p->d_fcode) continue;   // gather lists released
+        if (fi == 0 && only_blk) continue;
         p->tune_flags = fw_flags(keep, fi);
         st = pair();
         if (!st && hipEventRecord(e0, p->stream) != hipSuccess) st = NUFFT3D_ERR_DEVICE;
@@ -3129,9 +3137,10 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
         fw_ms[fi] = ms / reps;
       }
       if (!st) {
-        fw_best = 0;
-        for (int fi = 1; fi < 3; ++fi)
-          if (fw_ms[fi] > 0.f && fw_ms[fi] < fw_ms[fw_best]) fw_best = fi;
+        fw_best = -1;
+        for (int fi = 0; fi < 3; ++fi)
+          if (fw_ms[fi] > 0.f && (fw_best < 0 || fw_ms[fi] < fw_ms[fw_best])) fw_best = fi;
+        if (fw_best < 0) fw_best = 1;
       }
       p->tune_flags = fw_flags(keep, fw_best);
       if (fw_best == 0 && !(keep_flags & 8192)) {
@@ -3143,6 +3152,18 @@ int nufft3d_autotune(nufft3d_plan* p, int32_t nc, int32_t reps, char* report, in
         p->fk.d_w = nullptr;
       }
     }
+  }
+  // with both block-staged kernels chosen, every gridding launch of the plan
+  // runs them (any coil count, data weights, either precision of samples):
+  // the gather kernels' per-hit lists (8 B weight table + 2 B codes per hit)
+  // go, unless a kernel that reads them was asked for (LDS-staged brick
+  // spreads, evaluated spread weights, the tiled interpolation). A later
+  // autotune then times the block-staged kernels only.
+  if (!st && p->bk.d_piece && p->fk.d_slot && (p->tune_flags & 4096) && (p->tune_flags & 8192) &&
+      !(p->tune_flags & (8 | 128)) && !p->tile_interp && (p->d_hitw || p->d_fcode)) {
+    (void)hipStreamSynchronize(p->stream);
+    for (void** q : {(void**)&p->d_hitw, (void**)&p->d_fcode, (void**)&p->d_hit})
+      if (*q) { (void)dev_free(*q); *q = nullptr; }
   }
   // x-pass work-group size, timed likewise: 256 or 512 threads
   float xt_ms[2] = {0.f, 0.f};

@@ -76,7 +76,8 @@ CASES = [
     (24, "golden", 1, 16),
     (26, "golden", 1, 8),     # K = 34: blocks cut by the grid edge
     (16, "dense", 1, 8),      # 2,000 spokes on a 20^3 grid: blocks in many pieces
-    (24, "koosh", 1, 3),      # not an 8-coil group: the gather kernel
+    (24, "koosh", 1, 3),      # a partial coil group (coils loaded one by one, the rest zero)
+    (24, "golden", 1, 11),    # an 8-coil group and a partial one
 ]
 
 
@@ -98,7 +99,7 @@ def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc):
     k = _rand((om.shape[1], nc), 71).astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.aNUFT(k)))
-    assert ("k_spread_blk" in ran) == (nc % 8 == 0), ran
+    assert "k_spread_blk" in ran and "k_spread" not in ran, ran
     if kind == "dense":
         assert "k_spread_blk_reduce" in ran, ran
     want = gat.aNUFT(k)
@@ -169,7 +170,8 @@ FWD_CASES = [
     (26, "golden", 1, 8),
     (16, "dense", 1, 8),
     (24, "golden", 2, 8),     # fp64 plan: kernel.m's literal arithmetic, double outputs
-    (24, "koosh", 1, 3),      # not an 8-coil group: the gather kernel
+    (24, "koosh", 1, 3),      # a partial coil group
+    (24, "koosh", 2, 5),      # a partial coil group of an fp64 plan
 ]
 
 
@@ -188,7 +190,7 @@ def test_block_staged_forward_matches_gather_and_oracle(N, kind, gpu, nc, stored
         x = x.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.fNUFT(x)))
-    assert ("k_interp_blk" in ran) == (nc % 8 == 0), ran
+    assert "k_interp_blk" in ran and "k_interp" not in ran, ran
     want = gat.fNUFT(x)
     assert rel(got["b"], want) < (1e-12 if gpu == 2 else 1e-8)
     assert np.array_equal(blk.fNUFT(x), got["b"])       # deterministic
@@ -232,3 +234,35 @@ def test_block_staged_forward_not_built_on_a_narrow_grid():
     assert "k_interp_blk" not in ran and "k_interp" in ran, ran
     orc = O.NufftOracle(om, N, J=6, u=1.25)
     assert rel(blk.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP
+
+
+def test_gather_lists_released_after_the_autotuner_picks_block_kernels():
+    # with both block-staged kernels chosen, the plan frees the gather
+    # kernels' per-hit lists (fp32 weight table, codes); every later launch
+    # (any coil count: 8-coil groups and a partial group, data weights,
+    # float or double samples) runs the block-staged kernels
+    import os
+    N = 24
+    om = synth.golden_means(N, 600, 2 * N)
+    os.environ["NUFFT3D_TUNE_BLOCK_ONLY"] = "1"
+    try:
+        g = nufft_3d(om, N, J=6, u=1.25, gpu=1)
+        rep = g.autotune(nc=8, reps=1)
+    finally:
+        os.environ.pop("NUFFT3D_TUNE_BLOCK_ONLY", None)
+    assert rep["candidates"][rep["best"]]["spread"] == "block-staged", rep
+    assert rep["forward"].startswith("block-staged"), rep
+    # the same lists built at creation keep the gather lists beside them
+    kept = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096 | 8192 | 16384)}, om, N, J=6, u=1.25, gpu=1)
+    assert g.device_bytes()[0] < kept.device_bytes()[0], (g.device_bytes(), kept.device_bytes())
+    orc = O.NufftOracle(om, N, J=6, u=1.25)
+    for nc in (3, 8, 11):
+        x = _rand((N, N, N, nc), 80 + nc).astype(np.complex64)
+        k = _rand((om.shape[1], nc), 90 + nc).astype(np.complex64)
+        ran = _kernels_run(g, lambda: (g.fNUFT(x), g.aNUFT(k)))
+        assert "k_interp_blk" in ran and "k_spread_blk" in ran, ran
+        assert "k_interp" not in ran and "k_spread" not in ran, ran
+        assert rel(g.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP
+        assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
+    raw = orc.fNUFT(_rand((N, N, N, 3), 99)).astype(np.complex64)
+    assert rel(g.iNUFT(raw, 5, 0.05), orc.iNUFT(raw.astype(np.complex128), 5, 0.05)) < 1e-4

@@ -571,8 +571,9 @@ def run_pair(args, wname, full=True):
                      x_threads=rep.get("x_threads"), x_threads_ms=rep.get("x_threads_ms"),
                      forward=rep.get("forward"), forward_ms=rep.get("forward_ms"))
         log("autotune:", tuned)
-    log("plan %.3f s  K=%s nnz=%d compute precision %d" % (plan_s, plan.K, plan.nnz,
-                                                         plan.compute_precision))
+    plan_bytes = plan.device_bytes()[0]   # after the autotuner kept or released lists
+    log("plan %.3f s  K=%s nnz=%d compute precision %d, %.2f GB on the device"
+        % (plan_s, plan.K, plan.nnz, plan.compute_precision, plan_bytes / 1e9))
     K = plan.K[0]
 
     if full and args.copy_probe > 0:
@@ -742,6 +743,7 @@ def run_pair(args, wname, full=True):
         "rel_err_vs_fp64": rel,
         "accuracy_gate_1e-4": ok,
         "plan_s": round(plan_s, 4),
+        "plan_gb": round(plan_bytes / 1e9, 3),
         "autotune": tuned,
         "roofline": roof,
         "kernels": kstats,

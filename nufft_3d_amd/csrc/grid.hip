@@ -884,27 +884,33 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
         if (wsamp) m = wsamp[sm];
       }
       float4* dst = srow + (h * 64 + lane) * RU;
+      float4 v[DU];
+      if (full) {   // all loads issued before any is used
 #pragma unroll
-      for (int k = 0; k < DU; ++k) {
-        float4 v;
-        if (full) {
-          v = src[k];
-        } else if constexpr (sizeof(S) == 8) {   // coils 2k, 2k+1 (zero past the group)
-          const float2 a = 2 * k < g.cg ? srcc[2 * k] : make_float2(0.f, 0.f);
-          const float2 b = 2 * k + 1 < g.cg ? srcc[2 * k + 1] : make_float2(0.f, 0.f);
-          v = make_float4(a.x, a.y, b.x, b.y);
-        } else {                                 // coil k
-          v = k < g.cg ? __builtin_bit_cast(float4, srcc[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        if constexpr (sizeof(S) == 16) {
-          if (wsamp) {
-            double2 d = __builtin_bit_cast(double2, v);
-            d.x *= m;
-            d.y *= m;
-            v = __builtin_bit_cast(float4, d);
+        for (int k = 0; k < DU; ++k) v[k] = src[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < DU; ++k) {
+          if constexpr (sizeof(S) == 8) {   // coils 2k, 2k+1 (zero past the group)
+            const float2 a = 2 * k < g.cg ? srcc[2 * k] : make_float2(0.f, 0.f);
+            const float2 b = 2 * k + 1 < g.cg ? srcc[2 * k + 1] : make_float2(0.f, 0.f);
+            v[k] = make_float4(a.x, a.y, b.x, b.y);
+          } else {                          // coil k
+            v[k] = k < g.cg ? __builtin_bit_cast(float4, srcc[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
           }
         }
-        dst[k] = v;
+      }
+#pragma unroll
+      for (int k = 0; k < DU; ++k) {
+        if constexpr (sizeof(S) == 16) {
+          if (wsamp) {
+            double2 d = __builtin_bit_cast(double2, v[k]);
+            d.x *= m;
+            d.y *= m;
+            v[k] = __builtin_bit_cast(float4, d);
+          }
+        }
+        dst[k] = v[k];
       }
     }
     wave_lds_sync();

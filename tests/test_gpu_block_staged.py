@@ -266,3 +266,29 @@ def test_gather_lists_released_after_the_autotuner_picks_block_kernels():
         assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
     raw = orc.fNUFT(_rand((N, N, N, 3), 99)).astype(np.complex64)
     assert rel(g.iNUFT(raw, 5, 0.05), orc.iNUFT(raw.astype(np.complex128), 5, 0.05)) < 1e-4
+
+
+@pytest.mark.parametrize("N,J,u,radial,gpu", [((24, 20, 16), 6, 1.25, 1, 1),
+                                              ((20, 20, 20), 6, 1.25, 0, 1),
+                                              ((20, 16, 24), 5, 1.25, 0, 2),
+                                              ((16, 16, 16), 7, 1.5, 1, 2)])
+def test_block_staged_kernels_on_other_shapes_and_kernels(N, J, u, radial, gpu):
+    # non-cubic grids, the separable kernel, odd J and a wider footprint (J=7:
+    # 9 candidates per axis), both kernels, fp32 arrays (u = 1.25: fp64
+    # arithmetic behind them) and fp64 plans (any u)
+    rng = np.random.default_rng(J + 3 * radial)
+    om = np.stack([rng.uniform(-n / 2, n / 2 - 0.5, size=4000) for n in N])
+    kw = dict(J=J, u=u, radial=radial, gpu=gpu)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096 | 8192 | 16384)}, om, N, **kw)
+    assert blk.compute_precision == 2
+    x = _rand(tuple(N) + (8,), 101)
+    k = _rand((om.shape[1], 8), 102)
+    if gpu == 1:
+        x, k = x.astype(np.complex64), k.astype(np.complex64)
+    got = {}
+    ran = _kernels_run(blk, lambda: (got.setdefault("f", blk.fNUFT(x)), got.setdefault("a", blk.aNUFT(k))))
+    assert "k_interp_blk" in ran and "k_spread_blk" in ran, ran
+    orc = O.NufftOracle(om, N, J=J, u=u, radial=radial)
+    tol = 1e-12 if gpu == 2 else TOL32_OP
+    assert rel(got["f"], orc.fNUFT(x.astype(np.complex128))) < tol
+    assert rel(got["a"], orc.aNUFT(k.astype(np.complex128))) < tol

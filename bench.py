@@ -59,6 +59,7 @@ def algorithmic_bytes(N, K, M, nc):
         "k_spread_tile": nc * (8 * M + 8 * k3) + 12 * M,
         "k_spread_blk": nc * (8 * M + 8 * k3) + 12 * M,
         "k_interp_blk": nc * (8 * k3 + 8 * M) + 12 * M,
+        "k_interp_blkw": nc * (8 * k3 + 8 * M) + 12 * M,
         "fft_fwd": nc * (8 * n3 + 8 * k3) + 4 * n3,
         "fft_adj": nc * (8 * k3 + 8 * n3) + 4 * n3,
         "pair": nc * (16 * n3 + 32 * k3 + 16 * M) + 24 * M + 8 * n3,

@@ -1119,6 +1119,133 @@ __global__ void __launch_bounds__(256) k_interp_blk(const long long* __restrict_
   }
 }
 
+// The same forward from lists built with groups of 256 sorted samples: one
+// work-group per group, one thread per sample. The group's chunks of 128
+// grid rows are staged in LDS by the whole work-group (two threads per row,
+// four coils each) and each wavefront walks its own entries of the chunk; a
+// larger group shares each staged row among more samples. The next chunk's
+// rows are loaded into registers while the current chunk is consumed.
+constexpr int kFwdRowsW = 128;
+
+template <typename RO, bool USER, bool TAB, bool STW>
+__global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict__ cbase,
+                                                     long long ngrp, const int* __restrict__ gpt,
+                                                     const long long* __restrict__ eoff,
+                                                     const unsigned* __restrict__ eslot,
+                                                     const float4* __restrict__ ew,
+                                                     const int4* __restrict__ base,
+                                                     const double* __restrict__ off,
+                                                     const double2* __restrict__ grid,
+                                                     cplx<RO>* __restrict__ out,
+                                                     const double* __restrict__ wsamp, double scale,
+                                                     const int* __restrict__ orig,
+                                                     const double* __restrict__ kbtab, GridGeom g,
+                                                     KBParams kp) {
+  __shared__ double2 s_row[kFwdRowsW * kFwdRow];
+  __shared__ int s_pc[kFwdRowsW];
+  __shared__ double stab[TAB && !STW ? kKbIntervals * kKbCoef : 1];
+  const double* tab = nullptr;
+  if constexpr (TAB && !STW) tab = stage_kb_table(kbtab, stab);   // every thread (it synchronises)
+  const long long j = xcd_block(g.xchunk);
+  if (j >= ngrp) return;   // the whole work-group
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const long long s = j * 256 + t;
+  const bool act = s < g.nnz;
+  const long long sv = act ? s : (long long)g.nnz - 1;   // a thread past the end has no entries
+  const int4 b = base[sv];
+  const double* f64 = off + 3 * sv;
+  const double fx = f64[0], fy = f64[1], fz = f64[2];
+  double ar[8], ai[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
+  // staging role: row r of the chunk, coils 4h .. 4h + 3
+  const int r = t >> 1, h = t & 1;
+  double4 v0 = make_double4(0.0, 0.0, 0.0, 0.0), v1 = v0;
+  int pcv = 0;
+  auto fetch = [&](long long ch) {
+    const int gp = gpt[ch * kFwdRowsW + r];
+    const double2* src = grid + (long long)gp * g.nc + g.c0 + 4 * h;
+    if (g.cg == 8) {
+      v0 = reinterpret_cast<const double4*>(src)[0];
+      v1 = reinterpret_cast<const double4*>(src)[1];
+    } else {   // a partial group: its coils, zero past them
+      const int n = g.cg - 4 * h;
+      const double2 z = make_double2(0.0, 0.0);
+      const double2 a0 = n > 0 ? src[0] : z, a1 = n > 1 ? src[1] : z;
+      const double2 a2 = n > 2 ? src[2] : z, a3 = n > 3 ? src[3] : z;
+      v0 = make_double4(a0.x, a0.y, a1.x, a1.y);
+      v1 = make_double4(a2.x, a2.y, a3.x, a3.y);
+    }
+    const int px = gp % g.K1, py = (gp / g.K1) % g.K2, pz = gp / (g.K1 * g.K2);
+    pcv = px | (py << 10) | (pz << 20);
+  };
+  const long long c0 = cbase[j], c1 = cbase[j + 1];
+  if (c0 < c1) fetch(c0);
+  for (long long ch = c0; ch < c1; ++ch) {
+    __syncthreads();   // the previous chunk's rows are consumed
+    double2* dst = s_row + r * kFwdRow + 4 * h;
+    dst[0] = make_double2(v0.x, v0.y);
+    dst[1] = make_double2(v0.z, v0.w);
+    dst[2] = make_double2(v1.x, v1.y);
+    dst[3] = make_double2(v1.z, v1.w);
+    if (h == 0) s_pc[r] = pcv;
+    __syncthreads();
+    const long long q0 = eoff[ch * 4 + wv], q1 = eoff[ch * 4 + wv + 1];
+    unsigned sl4 = q0 < q1 ? eslot[q0 * 64 + lane] : 0xffffffffu;
+    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (STW) if (q0 < q1) w4 = ew[q0 * 64 + lane];
+    if (ch + 1 < c1) fetch(ch + 1);
+    for (long long q = q0; q < q1; ++q) {
+      const long long qn = q + 1 < q1 ? q + 1 : q;
+      const unsigned nsl4 = eslot[qn * 64 + lane];
+      float4 nw4 = w4;
+      if constexpr (STW) nw4 = ew[qn * 64 + lane];
+      const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned sl = (sl4 >> (8 * k)) & 255u;
+        if (sl != 255u) {
+          double w;
+          if constexpr (STW) {
+            w = (double)wk[k];   // the plan's fp32 weight of this hit
+          } else {
+          const int pc = s_pc[sl];
+          // candidate offset of the staged point from the sample's base
+          // (K >= 2W + 2 per axis: one candidate reaches it)
+          int dx = (pc & 1023) - b.x, dy = ((pc >> 10) & 1023) - b.y, dz = (pc >> 20) - b.z;
+          dx = dx > g.W1 ? dx - g.K1 : (dx < -g.W1 ? dx + g.K1 : dx);
+          dy = dy > g.W2 ? dy - g.K2 : (dy < -g.W2 ? dy + g.K2 : dy);
+          dz = dz > g.W3 ? dz - g.K3 : (dz < -g.W3 ? dz + g.K3 : dz);
+          if constexpr (TAB) w = hit_weight_tab(dx, dy, dz, fx, fy, fz, tab, kp);
+          else w = hit_weight_d(dx, dy, dz, f64, kp);
+          }
+          const double2* row = s_row + sl * kFwdRow;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            const double2 v = row[c];
+            ar[c] += w * v.x;
+            ai[c] += w * v.y;
+          }
+        }
+      }
+      sl4 = nsl4;
+      w4 = nw4;
+    }
+  }
+  if (!act) return;
+  double sc = scale;
+  if (wsamp) sc *= wsamp[s];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (c >= g.cg) break;
+    cplx<RO> v;
+    v.x = (RO)(ar[c] * sc);
+    v.y = (RO)(ai[c] * sc);
+    if constexpr (USER) out[orig[s] + (long long)(g.c0 + c) * g.M] = v;
+    else out[s * g.nc + g.c0 + c] = v;
+  }
+}
+
 template <typename R, int CPL, int LPS, bool COMP, bool FULL>
 __global__ void __launch_bounds__(256) k_spread_heavy(const HeavySeg* __restrict__ segs, int nseg,
                                                       const uint2* __restrict__ hits, bool pw,
@@ -1388,15 +1515,24 @@ static void launch_interp_blk(nufft3d_plan* p, const void* grid, int nc, int c0,
                               const void* wsamp, double scale, bool user, bool narrow) {
   GridGeom g = make_geom(p, nc, c0, cg);
   KBParams kp = make_kb(p);
-  ProfScope ps(p, "k_interp_blk");
-  const dim3 grid_((unsigned)((p->fk.ngrp + 3) / 4)), blk_(256);
+  const bool wide = p->fk.G == 256;
+  ProfScope ps(p, wide ? "k_interp_blkw" : "k_interp_blk");
+  const dim3 grid_((unsigned)(wide ? p->fk.ngrp : (p->fk.ngrp + 3) / 4)), blk_(256);
   const double* tb = kb_table_of(p);
   const bool stw = p->fk.d_w && (p->tune_flags & 16384);
 #define N3D_IBLK(RO, USR, TB, SW)                                                                 \
-  hipLaunchKernelGGL((k_interp_blk<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,              \
-                     p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,      \
-                     p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid,     \
-                     (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp)
+  do {                                                                                        \
+    if (wide)                                                                                 \
+      hipLaunchKernelGGL((k_interp_blkw<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,         \
+                         p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,  \
+                         p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid, \
+                         (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp);  \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_interp_blk<RO, USR, TB, SW>), grid_, blk_, 0, p->stream,          \
+                         p->fk.d_cbase, p->fk.ngrp, p->fk.d_gpt, p->fk.d_eoff, p->fk.d_slot,  \
+                         p->fk.d_w, p->d_base, (const double*)p->d_off, (const double2*)grid, \
+                         (cplx<RO>*)out, (const double*)wsamp, scale, p->d_orig, tb, g, kp);  \
+  } while (0)
 #define N3D_IBLK2(RO, USR)                                                                        \
   do {                                                                                        \
     if (stw) N3D_IBLK(RO, USR, true, true);                                                   \
@@ -1604,18 +1740,18 @@ __global__ void k_fk_fill(const long long* __restrict__ spos, const unsigned sho
                           const long long* __restrict__ eoff, const unsigned short* __restrict__ hslot,
                           const unsigned short* __restrict__ hk, unsigned char* __restrict__ slot,
                           float* __restrict__ w, const double* __restrict__ off,
-                          const double* __restrict__ tab, GridGeom g, KBParams kp) {
+                          const double* __restrict__ tab, int G, GridGeom g, KBParams kp) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
-  const long long j = s / 64;
-  const int lane = (int)(s % 64);
+  const long long j = s / G;
+  const int t = (int)(s % G), nw = G / 64, wv = t / 64, lane = t % 64;
   const long long c0 = cbase[j], nch = cbase[j + 1] - c0;
   const double* f = off + 3 * s;
   for (long long e = spos[s]; e < spos[s + 1]; ++e) {
     const long long sl = hslot[e];
     const long long c = sl % nch;
     const int k = hk[e];
-    const long long q = ((eoff[c0 + c] + k / 4) * 64 + lane) * 4 + (k & 3);
+    const long long q = ((eoff[(c0 + c) * nw + wv] + k / 4) * 64 + lane) * 4 + (k & 3);
     slot[q] = (unsigned char)(sl / nch);
     if (w) {
       int dx, dy, dz;
@@ -1637,11 +1773,11 @@ int fk_fill(nufft3d_plan* p, const unsigned short* hslot, const unsigned short* 
   if (tb)
     hipLaunchKernelGGL(k_fk_fill<true>, grid, dim3(256), 0, p->stream, p->d_spos, p->d_fcode, nnz,
                        p->fk.d_cbase, p->fk.d_eoff, hslot, hk, (unsigned char*)p->fk.d_slot,
-                       (float*)p->fk.d_w, (const double*)p->d_off, tb, g, make_kb(p));
+                       (float*)p->fk.d_w, (const double*)p->d_off, tb, p->fk.G, g, make_kb(p));
   else
     hipLaunchKernelGGL(k_fk_fill<false>, grid, dim3(256), 0, p->stream, p->d_spos, p->d_fcode, nnz,
                        p->fk.d_cbase, p->fk.d_eoff, hslot, hk, (unsigned char*)p->fk.d_slot,
-                       (float*)p->fk.d_w, (const double*)p->d_off, tb, g, make_kb(p));
+                       (float*)p->fk.d_w, (const double*)p->d_off, tb, p->fk.G, g, make_kb(p));
   N3D_HIP(hipGetLastError());
   return NUFFT3D_OK;
 }

@@ -187,11 +187,14 @@ struct BlockLists {
 // 255 = none). The weights are evaluated from the lane's own sample offsets
 // and the staged point's coordinates, as the gather kernel evaluates them.
 struct FwdLists {
-  long long ngrp = 0;               // sample groups (64 sorted samples each)
+  int G = 64;                       // sorted samples per group (64: one wavefront; 256: four
+                                    // wavefronts sharing each staged chunk)
+  int C = 64;                       // grid rows per chunk (64; 128 for groups of 256)
+  long long ngrp = 0;               // sample groups
   long long* d_cbase = nullptr;     // [ngrp + 1] first chunk of each group
   long long nchunk = 0;
   int* d_gpt = nullptr;             // [nchunk * 64] grid point indices (padding: the chunk's first)
-  long long* d_eoff = nullptr;      // [nchunk + 1] first entry group of each chunk
+  long long* d_eoff = nullptr;      // [nchunk * G/64 + 1] first entry group of each (chunk, wavefront)
   unsigned* d_slot = nullptr;       // [groups * 64] 4 slots per lane (byte k: step 4j+k; 255 none)
   float4* d_w = nullptr;            // [groups * 64] the hits' fp32 weights (optional: tune flag 16384)
   long long ngroups = 0;
@@ -301,6 +304,8 @@ struct nufft3d_plan {
   nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
   nufft3d::BlockLists bk;           // block-staged lists of the split-precision adjoint
   int blk_chunk = 64;               // samples per chunk of the block-staged adjoint's lists
+  int fwd_group = 256;              // samples per group of the block-staged forward's lists
+                                    // (64 where a group's union could pass 16-bit slots)
   nufft3d::FwdLists fk;             // block-staged lists of the fp64-arithmetic forward
   int max_hits = 0;
 

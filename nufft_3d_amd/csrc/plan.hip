@@ -1305,11 +1305,11 @@ __device__ __forceinline__ long long fk_point(int4 b, unsigned code, int K1, int
 // keys (group << 32 | grid index) of the hits of samples [s0, s1) (hits from e0)
 __global__ void k_fk_keys(const long long* __restrict__ spos, const unsigned short* __restrict__ fcode,
                           const int4* __restrict__ base, long long s0, long long s1, long long e0,
-                          int K1, int K2, int K3, int W, unsigned long long* __restrict__ keys) {
+                          int K1, int K2, int K3, int W, int G, unsigned long long* __restrict__ keys) {
   const long long s = s0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= s1) return;
   const int4 b = base[s];
-  const unsigned long long grp = (unsigned long long)(s / 64) << 32;
+  const unsigned long long grp = (unsigned long long)(s / G) << 32;
   for (long long e = spos[s]; e < spos[s + 1]; ++e)
     keys[e - e0] = grp | (unsigned long long)fk_point(b, fcode[e], K1, K2, K3, W);
 }
@@ -1318,12 +1318,13 @@ __global__ void k_fk_keys(const long long* __restrict__ spos, const unsigned sho
 // ascending list lst[beg[j], end[j])
 __global__ void k_fk_slot(const long long* __restrict__ spos, const unsigned short* __restrict__ fcode,
                           const int4* __restrict__ base, long long s0, long long s1, int K1, int K2,
-                          int K3, int W, const int* __restrict__ lst, const long long* __restrict__ beg,
-                          const long long* __restrict__ end, unsigned short* __restrict__ hslot) {
+                          int K3, int W, int G, const int* __restrict__ lst,
+                          const long long* __restrict__ beg, const long long* __restrict__ end,
+                          unsigned short* __restrict__ hslot) {
   const long long s = s0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= s1) return;
   const int4 b = base[s];
-  const long long j = s / 64;
+  const long long j = s / G;
   const long long bs = beg[j], len = end[j] - bs;
   for (long long e = spos[s]; e < spos[s + 1]; ++e) {
     const int gi = (int)fk_point(b, fcode[e], K1, K2, K3, W);
@@ -1337,21 +1338,35 @@ __global__ void k_fk_slot(const long long* __restrict__ spos, const unsigned sho
 }
 
 // per sample: the rank of each hit among the sample's hits of the same chunk
-__global__ void k_fk_rank(const long long* __restrict__ spos, long long nnz,
+__global__ void k_fk_rank(const long long* __restrict__ spos, long long nnz, int G,
                           const long long* __restrict__ cbase, const unsigned short* __restrict__ hslot,
                           unsigned short* __restrict__ cnt, unsigned short* __restrict__ hk) {
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
-  const long long j = s / 64;
-  const int lane = (int)(s % 64);
+  const long long j = s / G;
+  const int t = (int)(s % G);
   const long long c0 = cbase[j], nch = cbase[j + 1] - c0;
   for (long long e = spos[s]; e < spos[s + 1]; ++e) {
     const long long c = hslot[e] % nch;
-    unsigned short* q = cnt + (c0 + c) * 64 + lane;
+    unsigned short* q = cnt + (c0 + c) * G + t;
     const unsigned short k = *q;
     *q = (unsigned short)(k + 1);
     hk[e] = k;
   }
+}
+
+// entry groups of each (chunk, wavefront) of a group of G samples: the
+// wavefront's largest per-lane step count over 4, rounded up
+__global__ void k_fk_groups(const unsigned short* __restrict__ cnt, long long nchunk, int G,
+                            long long* __restrict__ groups) {
+  const int nw = G / 64;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nchunk * nw) return;
+  const long long c = i / nw;
+  const int w = (int)(i % nw);
+  int m = 0;
+  for (int l = 0; l < 64; ++l) m = max(m, (int)cnt[c * G + w * 64 + l]);
+  groups[i] = (m + 3) / 4;
 }
 
 static void drop_fwd_lists(nufft3d_plan* p) {
@@ -1376,7 +1391,11 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
   for (int a = 0; a < 3; ++a)
     if (p->W[a] != W || p->K[a] < 2 * W + 2) return NUFFT3D_OK;
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
-  const long long ngrp = (nnz + 63) / 64;
+  // samples per group (one wavefront per 64) and grid rows per chunk; a
+  // group's union of points is indexed by 16-bit slots
+  const int G = p->fwd_group == 256 && 256LL * W * W * W < 65535 ? 256 : 64;
+  const int C = G == 256 ? 128 : 64, NW = G / 64;
+  const long long ngrp = (nnz + G - 1) / G;
   std::vector<long long> hspos(nnz + 1);
   N3D_HIP(hipMemcpy(hspos.data(), p->d_spos, sizeof(long long) * (nnz + 1), hipMemcpyDeviceToHost));
   // batches of whole groups with fewer than 2^28 hits each (one group alone if larger)
@@ -1385,7 +1404,7 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
     bs.push_back(s);
     long long e = s;
     while (e < nnz) {
-      const long long n = std::min(e + 64, nnz);
+      const long long n = std::min(e + G, nnz);
       if (e > s && hspos[n] - hspos[s] > (1LL << 28)) break;
       e = n;
     }
@@ -1441,7 +1460,7 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
     nsel = 0;
     if (n == 0) return true;
     hipLaunchKernelGGL(k_fk_keys, dim3(nblocks(s1 - s0)), dim3(256), 0, p->stream, p->d_spos,
-                       p->d_fcode, p->d_base, s0, s1, hspos[s0], K1, K2, K3, W, keys);
+                       p->d_fcode, p->d_base, s0, s1, hspos[s0], K1, K2, K3, W, G, keys);
     return hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, n, 0, 32 + gbits, p->stream) == hipSuccess &&
            hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, n, p->stream) == hipSuccess &&
            hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream) == hipSuccess &&
@@ -1464,7 +1483,7 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
       hipLaunchKernelGGL(k_wl_lists, dim3(nblocks(ns)), dim3(256), 0, p->stream,
                          (const unsigned long long*)keys, (long long)ns, off, lst, beg, end);
     hipLaunchKernelGGL(k_fk_slot, dim3(nblocks(bs[i + 1] - bs[i])), dim3(256), 0, p->stream,
-                       p->d_spos, p->d_fcode, p->d_base, bs[i], bs[i + 1], K1, K2, K3, W,
+                       p->d_spos, p->d_fcode, p->d_base, bs[i], bs[i + 1], K1, K2, K3, W, G,
                        (const int*)lst, (const long long*)beg, (const long long*)end, hslot);
     if (hipGetLastError() != hipSuccess) return fail("slot kernels failed");
     off += ns;
@@ -1479,42 +1498,45 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
     return fail("copy failed");
   for (long long j = 0; j < ngrp; ++j) {
     if (he[j] - hb[j] > 65535) return fail("a group's point list exceeds 16-bit slots");
-    cbase[j + 1] = cbase[j] + (he[j] - hb[j] + 63) / 64;
+    cbase[j + 1] = cbase[j] + (he[j] - hb[j] + C - 1) / C;
   }
   const long long nchunk = cbase[ngrp];
+  const long long nent = nchunk * NW;   // (chunk, wavefront) entry ranges
   if (dev_malloc(&fk.d_cbase, sizeof(long long) * (ngrp + 1)) != hipSuccess ||
-      dev_malloc(&fk.d_gpt, sizeof(int) * std::max(1LL, nchunk * 64)) != hipSuccess ||
-      dev_malloc(&fk.d_eoff, sizeof(long long) * (nchunk + 1)) != hipSuccess ||
-      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * 64)) != hipSuccess ||
+      dev_malloc(&fk.d_gpt, sizeof(int) * std::max(1LL, nchunk * C)) != hipSuccess ||
+      dev_malloc(&fk.d_eoff, sizeof(long long) * (nent + 1)) != hipSuccess ||
+      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * G)) != hipSuccess ||
       dev_malloc(&hk, sizeof(unsigned short) * nh) != hipSuccess ||
-      dev_malloc(&grp, sizeof(long long) * (nchunk + 1)) != hipSuccess)
+      dev_malloc(&grp, sizeof(long long) * (nent + 1)) != hipSuccess)
     return fail("allocation failed");
   if (hipMemcpy(fk.d_cbase, cbase.data(), sizeof(long long) * (ngrp + 1), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * 64), p->stream) != hipSuccess)
+      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * G), p->stream) != hipSuccess)
     return fail("copy failed");
   // chunked point lists (the same dealing as the adjoint's sample lists)
   hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)ngrp), dim3(256), 0, p->stream, (const int*)lst,
-                     (const long long*)beg, (const long long*)end, (const long long*)fk.d_cbase, 64,
+                     (const long long*)beg, (const long long*)end, (const long long*)fk.d_cbase, C,
                      fk.d_gpt);
   hipLaunchKernelGGL(k_fk_rank, dim3(nblocks(nnz)), dim3(256), 0, p->stream,
-                     (const long long*)p->d_spos, nnz, (const long long*)fk.d_cbase,
+                     (const long long*)p->d_spos, nnz, G, (const long long*)fk.d_cbase,
                      (const unsigned short*)hslot, cnt, hk);
-  if (nchunk > 0)
-    hipLaunchKernelGGL(k_bk_groups, dim3(nblocks(nchunk)), dim3(256), 0, p->stream,
-                       (const unsigned short*)cnt, nchunk, grp);
+  if (nent > 0)
+    hipLaunchKernelGGL(k_fk_groups, dim3(nblocks(nent)), dim3(256), 0, p->stream,
+                       (const unsigned short*)cnt, nchunk, G, grp);
   if (hipGetLastError() != hipSuccess) return fail("list kernels failed");
-  std::vector<long long> hgrp(nchunk + 1, 0), eoff(nchunk + 1, 0);
-  if (nchunk > 0 &&
-      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nchunk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+  std::vector<long long> hgrp(nent + 1, 0), eoff(nent + 1, 0);
+  if (nent > 0 &&
+      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nent, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
        hipStreamSynchronize(p->stream) != hipSuccess))
     return fail("copy failed");
-  for (long long c = 0; c < nchunk; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
-  const long long ngroups = eoff[nchunk];
+  for (long long c = 0; c < nent; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
+  const long long ngroups = eoff[nent];
   if (dev_malloc(&fk.d_slot, sizeof(unsigned) * std::max(1LL, ngroups * 64)) != hipSuccess ||
       (weights && dev_malloc(&fk.d_w, sizeof(float4) * std::max(1LL, ngroups * 64)) != hipSuccess))
     return fail("allocation failed");
   fk.ngroups = ngroups;
-  if (hipMemcpy(fk.d_eoff, eoff.data(), sizeof(long long) * (nchunk + 1), hipMemcpyHostToDevice) != hipSuccess ||
+  fk.G = G;
+  fk.C = C;
+  if (hipMemcpy(fk.d_eoff, eoff.data(), sizeof(long long) * (nent + 1), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemsetAsync(fk.d_slot, 0xff, sizeof(unsigned) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess ||
       (weights &&
        hipMemsetAsync(fk.d_w, 0, sizeof(float4) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess))
@@ -1526,8 +1548,8 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
   fk.nchunk = nchunk;
   fk.ngroups = ngroups;
   if (p->verbose)
-    printf("  block-staged forward: %lld sample groups, %lld chunks, %lld entry groups "
-           "(%.2f B/hit)\n", ngrp, nchunk, ngroups, 4.0 * 64 * ngroups / (double)nh);
+    printf("  block-staged forward: %lld groups of %d samples, %lld chunks of %d points, %lld entry "
+           "groups (%.2f B/hit)\n", ngrp, G, nchunk, C, ngroups, 4.0 * 64 * ngroups / (double)nh);
   return NUFFT3D_OK;
 }
 
@@ -2152,6 +2174,7 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_TILE_CG")) p->tile_cg = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_TILE_LAYERS")) p->tile_layers = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_BLK_CHUNK")) p->blk_chunk = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_FWD_GROUP")) p->fwd_group = atoi(tv);
   if (o.tile > 0) {
     p->T[0] = p->T[1] = p->T[2] = o.tile;
     p->tile_user = true;
@@ -2255,7 +2278,7 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
     b += 12 * p->bk.nblk + 8 + (4 * p->bk.C + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
          16 * (p->bk.npiece + std::max<int64_t>(1, p->bk.nred));
   if (p->fk.d_slot)
-    b += 8 * (p->fk.ngrp + 1) + (4 * 64 + 8) * p->fk.nchunk + 8 +
+    b += 8 * (p->fk.ngrp + 1) + (4 * p->fk.C + 8 * p->fk.G / 64) * p->fk.nchunk + 8 +
          (p->fk.d_w ? 20 : 4) * 64 * p->fk.ngroups;
   for (int a = 0; a < 3; ++a)
     for (const FftAxis* x : {&p->ax[a], &p->axN[a]})

@@ -163,6 +163,10 @@ def test_autotuner_times_the_block_staged_adjoint():
     assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
 
 
+# the block-staged forward kernels: groups of 64 samples (one wavefront) or of
+# 256 (a work-group sharing each staged chunk)
+FWD_BLK = {"k_interp_blk", "k_interp_blkw"}
+
 FWD_CASES = [
     # N, trajectory, gpu, coils
     (24, "golden", 1, 8),
@@ -175,14 +179,16 @@ FWD_CASES = [
 ]
 
 
+@pytest.mark.parametrize("group", [64, 256])
 @pytest.mark.parametrize("stored", [False, True])
 @pytest.mark.parametrize("N,kind,gpu,nc", FWD_CASES)
-def test_block_staged_forward_matches_gather_and_oracle(N, kind, gpu, nc, stored):
+def test_block_staged_forward_matches_gather_and_oracle(N, kind, gpu, nc, stored, group):
     # stored: the lists hold each hit's fp32 weight, evaluated at plan time
-    # by the same function the kernels evaluate
+    # by the same function the kernels evaluate; group: samples per group
     om = _traj(N, kind)
     kw = dict(J=6, u=1.25, gpu=gpu)
-    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 8192 | (16384 if stored else 0))}, om, N, **kw)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 8192 | (16384 if stored else 0)),
+                          "NUFFT3D_FWD_GROUP": str(group)}, om, N, **kw)
     gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
     assert blk.compute_precision == 2
     x = _rand((N, N, N, nc), 74)
@@ -190,7 +196,8 @@ def test_block_staged_forward_matches_gather_and_oracle(N, kind, gpu, nc, stored
         x = x.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.fNUFT(x)))
-    assert "k_interp_blk" in ran and "k_interp" not in ran, ran
+    assert FWD_BLK & ran == {"k_interp_blkw" if group == 256 else "k_interp_blk"}, ran
+    assert "k_interp" not in ran, ran
     want = gat.fNUFT(x)
     assert rel(got["b"], want) < (1e-12 if gpu == 2 else 1e-8)
     assert np.array_equal(blk.fNUFT(x), got["b"])       # deterministic
@@ -214,7 +221,7 @@ def test_block_staged_forward_and_adjoint_in_the_inverse(gpu):
         raw = raw.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.iNUFT(raw, 6, 0.02)))
-    assert "k_interp_blk" in ran and "k_spread_blk" in ran, ran
+    assert FWD_BLK & ran and "k_spread_blk" in ran, ran
     want = gat.iNUFT(raw, 6, 0.02)
     assert rel(got["b"], want) < (1e-11 if gpu == 2 else 1e-7)
     ref = orc.iNUFT(raw.astype(np.complex128), 6, 0.02)
@@ -231,7 +238,7 @@ def test_block_staged_forward_not_built_on_a_narrow_grid():
     blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 8192)}, om, N, J=6, u=1.25, gpu=1)
     x = _rand(N + (8,), 77).astype(np.complex64)
     ran = _kernels_run(blk, lambda: blk.fNUFT(x))
-    assert "k_interp_blk" not in ran and "k_interp" in ran, ran
+    assert not FWD_BLK & ran and "k_interp" in ran, ran
     orc = O.NufftOracle(om, N, J=6, u=1.25)
     assert rel(blk.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP
 
@@ -260,7 +267,7 @@ def test_gather_lists_released_after_the_autotuner_picks_block_kernels():
         x = _rand((N, N, N, nc), 80 + nc).astype(np.complex64)
         k = _rand((om.shape[1], nc), 90 + nc).astype(np.complex64)
         ran = _kernels_run(g, lambda: (g.fNUFT(x), g.aNUFT(k)))
-        assert "k_interp_blk" in ran and "k_spread_blk" in ran, ran
+        assert FWD_BLK & ran and "k_spread_blk" in ran, ran
         assert "k_interp" not in ran and "k_spread" not in ran, ran
         assert rel(g.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP
         assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
@@ -287,7 +294,7 @@ def test_block_staged_kernels_on_other_shapes_and_kernels(N, J, u, radial, gpu):
         x, k = x.astype(np.complex64), k.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: (got.setdefault("f", blk.fNUFT(x)), got.setdefault("a", blk.aNUFT(k))))
-    assert "k_interp_blk" in ran and "k_spread_blk" in ran, ran
+    assert FWD_BLK & ran and "k_spread_blk" in ran, ran
     orc = O.NufftOracle(om, N, J=J, u=u, radial=radial)
     tol = 1e-12 if gpu == 2 else TOL32_OP
     assert rel(got["f"], orc.fNUFT(x.astype(np.complex128))) < tol

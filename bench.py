@@ -58,6 +58,7 @@ def algorithmic_bytes(N, K, M, nc):
         "k_spread": nc * (8 * M + 8 * k3) + 12 * M,
         "k_spread_tile": nc * (8 * M + 8 * k3) + 12 * M,
         "k_spread_blk": nc * (8 * M + 8 * k3) + 12 * M,
+        "k_spread_blkw": nc * (8 * M + 8 * k3) + 12 * M,
         "k_interp_blk": nc * (8 * k3 + 8 * M) + 12 * M,
         "k_interp_blkw": nc * (8 * k3 + 8 * M) + 12 * M,
         "fft_fwd": nc * (8 * n3 + 8 * k3) + 4 * n3,

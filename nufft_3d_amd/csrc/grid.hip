@@ -966,32 +966,175 @@ __global__ void __launch_bounds__(256) k_spread_blk(const int4* __restrict__ pie
   }
 }
 
+// The same adjoint from the wide lists: one work-group per piece of an
+// 8x8x4 block, four wavefronts of 4x4x4 points (wavefront ((y>>2)&1,
+// (x>>2)&1), lane (x&3, y&3, z&3)). A chunk's kBlkWideRows sample rows are
+// staged in LDS by the whole work-group (one row per thread) and each
+// wavefront walks its own entries of the chunk; the next chunk's row is
+// loaded into registers while the current chunk is consumed.
+template <typename S>
+__global__ void __launch_bounds__(256) k_spread_blkw(const int4* __restrict__ pieces,
+                                                     long long npiece, const int* __restrict__ blk,
+                                                     const long long* __restrict__ cbase,
+                                                     const int* __restrict__ samp,
+                                                     const long long* __restrict__ eoff,
+                                                     const unsigned* __restrict__ eslot,
+                                                     const float4* __restrict__ ew,
+                                                     const S* __restrict__ ks,
+                                                     const double* __restrict__ wsamp,
+                                                     double2* __restrict__ grid,
+                                                     double2* __restrict__ partial, GridGeom g) {
+  constexpr int C = kBlkWideRows;
+  constexpr int RU = BlkRow<S>::U;
+  constexpr int DU = RU - 1;          // 16-byte units of data per row (4 float, 8 double)
+  __shared__ float4 s_row[C * RU];
+  const long long pi = xcd_block(g.xchunk);
+  if (pi >= npiece) return;   // the whole work-group
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const int4 pc = pieces[pi];
+  const int bc = blk[pc.x];
+  const int x = (bc & 1023) * 8 + (wv & 1) * 4 + (lane & 3);
+  const int y = ((bc >> 10) & 1023) * 8 + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  const int z = (bc >> 20) * 4 + (lane >> 4);
+  double ar[8], ai[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
+  const long long cb = cbase[pc.x];
+  const float4* ks4 = reinterpret_cast<const float4*>(ks);
+  const long long rs4 = (long long)g.nc * sizeof(S) / 16;   // 16-byte units per sample row
+  const long long co4 = (long long)g.c0 * sizeof(S) / 16;
+  // a whole 8-coil group at a 16-byte aligned row offset takes 16-byte loads
+  const bool full = g.cg == 8 && (sizeof(S) == 16 || ((g.nc & 1) == 0 && (g.c0 & 1) == 0));
+  float4 v[DU];
+  // row t of chunk ch into v (threads past the chunk's rows load nothing)
+  auto fetch = [&](long long ch) {
+    if (t >= C) return;
+    const int sm = samp[ch * C + t];
+    const float4* src = ks4 + (long long)sm * rs4 + co4;
+    const S* srcc = ks + (long long)sm * g.nc + g.c0;   // coil by coil (partial groups)
+    if (full) {   // all loads issued before any is used
+#pragma unroll
+      for (int k = 0; k < DU; ++k) v[k] = src[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < DU; ++k) {
+        if constexpr (sizeof(S) == 8) {   // coils 2k, 2k+1 (zero past the group)
+          const float2 a = 2 * k < g.cg ? srcc[2 * k] : make_float2(0.f, 0.f);
+          const float2 b = 2 * k + 1 < g.cg ? srcc[2 * k + 1] : make_float2(0.f, 0.f);
+          v[k] = make_float4(a.x, a.y, b.x, b.y);
+        } else {                          // coil k
+          v[k] = k < g.cg ? __builtin_bit_cast(float4, srcc[k]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+    }
+    if constexpr (sizeof(S) == 16) {   // iNUFT's data weight of the sample, applied as it is staged
+      if (wsamp) {
+        const double m = wsamp[sm];
+#pragma unroll
+        for (int k = 0; k < DU; ++k) {
+          double2 d = __builtin_bit_cast(double2, v[k]);
+          d.x *= m;
+          d.y *= m;
+          v[k] = __builtin_bit_cast(float4, d);
+        }
+      }
+    }
+  };
+  if (pc.y < pc.z) fetch(cb + pc.y);
+  for (int c = pc.y; c < pc.z; ++c) {
+    const long long ch = cb + c;
+    __syncthreads();   // the previous chunk's rows are consumed
+    if (t < C) {
+      float4* dst = s_row + t * RU;
+#pragma unroll
+      for (int k = 0; k < DU; ++k) dst[k] = v[k];
+    }
+    __syncthreads();
+    const long long q0 = eoff[ch * 4 + wv], q1 = eoff[ch * 4 + wv + 1];
+    // entry groups loaded one group ahead (index clamped into the range)
+    unsigned sl4 = 0xffffffffu;
+    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q0 < q1) { sl4 = eslot[q0 * 64 + lane]; w4 = ew[q0 * 64 + lane]; }
+    if (c + 1 < pc.z) fetch(ch + 1);
+    for (long long q = q0; q < q1; ++q) {
+      const long long qn = q + 1 < q1 ? q + 1 : q;
+      const unsigned nsl4 = eslot[qn * 64 + lane];
+      const float4 nw4 = ew[qn * 64 + lane];
+      const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned sl = (sl4 >> (8 * k)) & 255u;
+        if (sl != 255u) {
+          const float4* row = s_row + sl * RU;
+          const double w = (double)wk[k];
+          if constexpr (sizeof(S) == 8) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float4 r = row[j];
+              ar[2 * j] += w * (double)r.x;
+              ai[2 * j] += w * (double)r.y;
+              ar[2 * j + 1] += w * (double)r.z;
+              ai[2 * j + 1] += w * (double)r.w;
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const double2 r = __builtin_bit_cast(double2, row[j]);
+              ar[j] += w * r.x;
+              ai[j] += w * r.y;
+            }
+          }
+        }
+      }
+      sl4 = nsl4;
+      w4 = nw4;
+    }
+  }
+  if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
+  if (pc.w < 0) {
+    double2* o = grid + (((long long)z * g.K2 + y) * g.K1 + x) * g.nc + g.c0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      if (c < g.cg) o[c] = make_double2(ar[c], ai[c]);
+  } else {
+    double2* o = partial + ((long long)pc.w * 256 + t) * 8;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) o[c] = make_double2(ar[c], ai[c]);
+  }
+}
+
 // Blocks in several pieces: each point's partial sums added in piece order.
 // One work-group per block, one thread per (point, coil): 8 consecutive
 // threads read a point's 128-byte partial row.
+// T points per block (64: 4x4x4 blocks; 256: wide 8x8x4 blocks, a point's
+// thread as in k_spread_blkw): work-group (red entry, wavefront) covers the
+// 64 points of one wavefront.
 __global__ void __launch_bounds__(512) k_spread_blk_reduce(const int4* __restrict__ red,
-                                                           const int* __restrict__ blk,
+                                                           const int* __restrict__ blk, int T,
                                                            const double2* __restrict__ partial,
                                                            double2* __restrict__ grid, GridGeom g) {
   const int4 r = red[blockIdx.x];   // (block, first partial slot, pieces, 0)
-  const int lane = threadIdx.x >> 3, c = threadIdx.x & 7;
+  const int wv = blockIdx.y, lane = threadIdx.x >> 3, c = threadIdx.x & 7;
   if (c >= g.cg) return;
   const int bc = blk[r.x];
-  const int x = (bc & 1023) * 4 + (lane & 3), y = ((bc >> 10) & 1023) * 4 + ((lane >> 2) & 3),
-            z = (bc >> 20) * 4 + (lane >> 4);
+  const int bs = T == 256 ? 8 : 4;
+  const int x = (bc & 1023) * bs + (wv & 1) * 4 + (lane & 3);
+  const int y = ((bc >> 10) & 1023) * bs + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  const int z = (bc >> 20) * 4 + (lane >> 4);
   if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
-  const double2* pp = partial + ((long long)r.y * 64 + lane) * 8 + c;
+  const long long ps = (long long)T * 8;   // one piece's partial rows
+  const double2* pp = partial + ((long long)r.y * T + wv * 64 + lane) * 8 + c;
   double ax = 0, ay = 0;
   int j = 0;
   for (; j + 4 <= r.z; j += 4) {   // four pieces' loads in flight, added in piece order
-    const double2 v0 = pp[0], v1 = pp[512], v2 = pp[1024], v3 = pp[1536];
+    const double2 v0 = pp[0], v1 = pp[ps], v2 = pp[2 * ps], v3 = pp[3 * ps];
     ax += v0.x; ay += v0.y;
     ax += v1.x; ay += v1.y;
     ax += v2.x; ay += v2.y;
     ax += v3.x; ay += v3.y;
-    pp += 2048;
+    pp += 4 * ps;
   }
-  for (; j < r.z; ++j, pp += 512) {
+  for (; j < r.z; ++j, pp += ps) {
     const double2 v = pp[0];
     ax += v.x; ay += v.y;
   }
@@ -1789,7 +1932,16 @@ template <typename R>
 static void launch_spread_blk(nufft3d_plan* p, const void* samp, int nc, int c0, int cg, void* grid,
                               void* partial, const void* wsamp) {
   GridGeom g = make_geom(p, nc, c0, cg);
-  {
+  if (p->bk.T == 256) {
+    ProfScope ps(p, "k_spread_blkw");
+    hipLaunchKernelGGL((k_spread_blkw<cplx<R>>), dim3((unsigned)p->bk.npiece), dim3(256), 0,
+                       p->stream, (const int4*)p->bk.d_piece, (long long)p->bk.npiece,
+                       (const int*)p->bk.d_blk, (const long long*)p->bk.d_cbase,
+                       (const int*)p->bk.d_samp, (const long long*)p->bk.d_eoff,
+                       (const unsigned*)p->bk.d_slot, (const float4*)p->bk.d_w,
+                       (const cplx<R>*)samp, (const double*)wsamp, (double2*)grid,
+                       (double2*)partial, g);
+  } else {
     ProfScope ps(p, "k_spread_blk");
 #define N3D_SBLK(CC)                                                                              \
     hipLaunchKernelGGL((k_spread_blk<cplx<R>, CC>), dim3((unsigned)((p->bk.npiece + 3) / 4)),   \
@@ -1805,8 +1957,9 @@ static void launch_spread_blk(nufft3d_plan* p, const void* samp, int nc, int c0,
   }
   if (p->bk.nred > 0) {
     ProfScope ps(p, "k_spread_blk_reduce");
-    hipLaunchKernelGGL(k_spread_blk_reduce, dim3((unsigned)p->bk.nred), dim3(512), 0, p->stream,
-                       (const int4*)p->bk.d_red, (const int*)p->bk.d_blk,
+    hipLaunchKernelGGL(k_spread_blk_reduce, dim3((unsigned)p->bk.nred, (unsigned)(p->bk.T / 64)),
+                       dim3(512), 0, p->stream,
+                       (const int4*)p->bk.d_red, (const int*)p->bk.d_blk, p->bk.T,
                        (const double2*)partial, (double2*)grid, g);
   }
 }
@@ -2012,7 +2165,7 @@ static int spread_t(nufft3d_plan* p, const void* samp, int nc, void* grid, const
   const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
   size_t pbytes = (size_t)nseg * cgmax * 2 * sizeof(double);
   if (sizeof(R) == 8 && p->bk.d_piece && (p->tune_flags & 4096))   // block-staged pieces
-    pbytes = std::max(pbytes, (size_t)p->bk.npslot * 64 * 8 * sizeof(double2));
+    pbytes = std::max(pbytes, (size_t)p->bk.npslot * p->bk.T * 8 * sizeof(double2));
   if (pbytes > 0) {
     N3D_TRY(ensure(p->ws_part, pbytes));
     partial = p->ws_part.p;
@@ -2042,7 +2195,7 @@ int spread(nufft3d_plan* p, const void* samp, int nc, void* grid, int user_layou
     const int nseg = std::max(p->dense[0].nhseg, p->dense[1].nhseg);
     size_t pbytes = (size_t)nseg * cgmax * 2 * sizeof(double);
     if (p->bk.d_piece && (p->tune_flags & 4096))   // block-staged pieces: 64 points x 8 coils
-      pbytes = std::max(pbytes, (size_t)p->bk.npslot * 64 * 8 * sizeof(double2));
+      pbytes = std::max(pbytes, (size_t)p->bk.npslot * p->bk.T * 8 * sizeof(double2));
     if (pbytes > 0) {
       N3D_TRY(ensure(p->ws_part, pbytes));
       partial = p->ws_part.p;

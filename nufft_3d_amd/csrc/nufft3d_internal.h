@@ -158,15 +158,20 @@ struct WideLists {
 // word, 4 fp32 weights per lane in one float4), lanes with fewer hits
 // padded with slot 255 and weight 0. Blocks with more entry rows than a
 // piece holds are split into pieces whose partial sums add in piece order.
+// Wide lists (k_spread_blkw): 8x8x4 blocks of four 4x4x4 wavefronts that
+// share each staged chunk of 255 samples (slot 255 stays "none"); entries
+// per (chunk, wavefront), a chunk's rows counted as its largest wavefront's.
 constexpr int kBlkPieceRows = 256;  // entry rows (steps) per piece, at least one chunk
+constexpr int kBlkWideRows = 255;   // samples per chunk of the wide lists
 struct BlockLists {
-  int C = 64;                       // samples staged per chunk (64 or 128)
+  int C = 64;                       // samples staged per chunk (64 or 128; kBlkWideRows wide)
+  int T = 64;                       // points per block: 64 (4x4x4) or 256 (8x8x4, wide)
   long long nblk = 0;               // blocks, in launch order (Z-order of block coordinates)
   int* d_blk = nullptr;             // [nblk] x | y << 10 | z << 20 (block coordinates)
   long long* d_cbase = nullptr;     // [nblk + 1] first chunk of each block
   long long nchunk = 0;
   int* d_samp = nullptr;            // [nchunk * C] sample ids (padding: the chunk's first)
-  long long* d_eoff = nullptr;      // [nchunk + 1] first entry group of each chunk
+  long long* d_eoff = nullptr;      // [nchunk * T/64 + 1] first entry group of each (chunk, wavefront)
   unsigned* d_slot = nullptr;       // [groups * 64] 4 slots per lane (byte k: step 4j+k; 255 none)
   float4* d_w = nullptr;            // [groups * 64] 4 weights per lane (0 for padding)
   int4* d_piece = nullptr;          // [npiece] (block, first chunk, end chunk, partial slot or -1)
@@ -304,6 +309,8 @@ struct nufft3d_plan {
   nufft3d::WideLists wl;            // LDS-staged lists of the split-precision adjoint
   nufft3d::BlockLists bk;           // block-staged lists of the split-precision adjoint
   int blk_chunk = 64;               // samples per chunk of the block-staged adjoint's lists
+  int blk_wide = 1;                 // block-staged adjoint over 8x8x4 blocks (0: 4x4x4)
+  int blk_piece = 0;                // entry rows per piece of the block-staged adjoint (0: automatic)
   int fwd_group = 256;              // samples per group of the block-staged forward's lists
                                     // (64 where a group's union could pass 16-bit slots)
   nufft3d::FwdLists fk;             // block-staged lists of the fp64-arithmetic forward

@@ -932,14 +932,14 @@ static int ensure_wide_lists(nufft3d_plan* p) {
 // [e0, ...)), blocks numbered in launch order by blk_of
 __global__ void k_bk_keys(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
                           long long g0, long long g1, long long e0, int K1, int K2,
-                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const int* __restrict__ blk_of, int nbx, int nby, int sxy,
                           unsigned long long* __restrict__ keys) {
   const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= g1) return;
   const long long a = rowptr[gi], b = rowptr[gi + 1];
   const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
   const unsigned long long bk =
-      (unsigned long long)blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
+      (unsigned long long)blk_of[((long long)(z >> 2) * nby + (y >> sxy)) * nbx + (x >> sxy)];
   for (long long e = a; e < b; ++e) keys[e - e0] = (bk << 32) | hits[e].x;
 }
 
@@ -947,7 +947,7 @@ __global__ void k_bk_keys(const long long* __restrict__ rowptr, const uint2* __r
 // ascending list asc[beg[b], end[b])
 __global__ void k_bk_slot(const long long* __restrict__ rowptr, const uint2* __restrict__ hits,
                           long long g0, long long g1, int K1, int K2,
-                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const int* __restrict__ blk_of, int nbx, int nby, int sxy,
                           const int* __restrict__ asc, const long long* __restrict__ beg,
                           const long long* __restrict__ end, unsigned* __restrict__ hslot) {
   const long long gi = g0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -955,7 +955,7 @@ __global__ void k_bk_slot(const long long* __restrict__ rowptr, const uint2* __r
   const long long a = rowptr[gi], b = rowptr[gi + 1];
   if (a == b) return;
   const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
-  const int bk = blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
+  const int bk = blk_of[((long long)(z >> 2) * nby + (y >> sxy)) * nbx + (x >> sxy)];
   const long long base = beg[bk], len = end[bk] - base;
   long long lo = 0;   // hits are in ascending sample order: search from the last slot
   for (long long e = a; e < b; ++e) {
@@ -985,48 +985,56 @@ __global__ void k_bk_samp(const int* __restrict__ asc, const long long* __restri
   }
 }
 
+// a point's block and its thread in the block's work: lane (x&3, y&3, z&3)
+// of wavefront ((y>>2)&1, (x>>2)&1) (wide 8x8x4 blocks; one wavefront of
+// 4x4x4 points otherwise)
 __device__ __forceinline__ void bk_point(long long gi, int K1, int K2, const int* blk_of, int nbx,
-                                         int nby, int& bk, int& lane) {
+                                         int nby, int sxy, int& bk, int& t) {
   const int x = (int)(gi % K1), y = (int)((gi / K1) % K2), z = (int)(gi / ((long long)K1 * K2));
-  bk = blk_of[((long long)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)];
-  lane = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+  bk = blk_of[((long long)(z >> 2) * nby + (y >> sxy)) * nbx + (x >> sxy)];
+  t = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+  if (sxy == 3) t |= (((x >> 2) & 1) | (((y >> 2) & 1) << 1)) << 6;
 }
 
 // per point: the rank of each hit among the point's hits of the same chunk
 // (its step in that chunk); the point's thread owns its counters, so the
 // ranks follow the point's list order
 __global__ void k_bk_rank(const long long* __restrict__ rowptr, long long ngrid, int K1, int K2,
-                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const int* __restrict__ blk_of, int nbx, int nby, int sxy, int T,
                           const long long* __restrict__ cbase, const unsigned* __restrict__ hslot,
                           unsigned short* __restrict__ cnt, unsigned short* __restrict__ hk) {
   const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= ngrid) return;
   const long long a = rowptr[gi], b = rowptr[gi + 1];
   if (a == b) return;
-  int bk, lane;
-  bk_point(gi, K1, K2, blk_of, nbx, nby, bk, lane);
+  int bk, t;
+  bk_point(gi, K1, K2, blk_of, nbx, nby, sxy, bk, t);
   const long long c0 = cbase[bk], nch = cbase[bk + 1] - c0;
   for (long long e = a; e < b; ++e) {
     const long long c = (long long)hslot[e] % nch;
-    unsigned short* q = cnt + (c0 + c) * 64 + lane;
+    unsigned short* q = cnt + (c0 + c) * T + t;
     const unsigned short k = *q;
     *q = (unsigned short)(k + 1);
     hk[e] = k;
   }
 }
 
-// entry groups of each chunk: its largest per-lane step count over 4, rounded up
-__global__ void k_bk_groups(const unsigned short* __restrict__ cnt, long long nchunk,
+// entry groups of each (chunk, wavefront) of blocks of T points: the
+// wavefront's largest per-lane step count over 4, rounded up
+__global__ void k_bk_groups(const unsigned short* __restrict__ cnt, long long nchunk, int T,
                             long long* __restrict__ groups) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nchunk) return;
+  const int nw = T / 64;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nchunk * nw) return;
+  const long long c = i / nw;
+  const int w = (int)(i % nw);
   int m = 0;
-  for (int l = 0; l < 64; ++l) m = max(m, (int)cnt[c * 64 + l]);
-  groups[c] = (m + 3) / 4;
+  for (int l = 0; l < 64; ++l) m = max(m, (int)cnt[c * T + w * 64 + l]);
+  groups[i] = (m + 3) / 4;
 }
 
 __global__ void k_bk_fill(const long long* __restrict__ rowptr, long long ngrid, int K1, int K2,
-                          const int* __restrict__ blk_of, int nbx, int nby,
+                          const int* __restrict__ blk_of, int nbx, int nby, int sxy,
                           const long long* __restrict__ cbase, const long long* __restrict__ eoff,
                           const unsigned* __restrict__ hslot, const unsigned short* __restrict__ hk,
                           const uint2* __restrict__ hitw, unsigned char* __restrict__ slot,
@@ -1035,14 +1043,15 @@ __global__ void k_bk_fill(const long long* __restrict__ rowptr, long long ngrid,
   if (gi >= ngrid) return;
   const long long a = rowptr[gi], b = rowptr[gi + 1];
   if (a == b) return;
-  int bk, lane;
-  bk_point(gi, K1, K2, blk_of, nbx, nby, bk, lane);
+  int bk, t;
+  bk_point(gi, K1, K2, blk_of, nbx, nby, sxy, bk, t);
+  const int nw = sxy == 3 ? 4 : 1, wv = t >> 6, lane = t & 63;
   const long long c0 = cbase[bk], nch = cbase[bk + 1] - c0;
   for (long long e = a; e < b; ++e) {
     const long long sl = hslot[e];
     const long long c = sl % nch, o = sl / nch;
     const int k = hk[e];
-    const long long q = ((eoff[c0 + c] + k / 4) * 64 + lane) * 4 + (k & 3);
+    const long long q = ((eoff[(c0 + c) * nw + wv] + k / 4) * 64 + lane) * 4 + (k & 3);
     slot[q] = (unsigned char)o;
     w[q] = __uint_as_float(hitw[e].y);
   }
@@ -1063,12 +1072,16 @@ static void drop_block_lists(nufft3d_plan* p) {
 // blocks at a time (any total hit count; a slab needs fewer than 2^31 hits).
 static int build_block_lists(nufft3d_plan* p) {
   BlockLists& bk = p->bk;
-  const int C = p->blk_chunk == 128 ? 128 : 64;   // samples staged per chunk
+  // wide: 8x8x4 blocks, four wavefronts, chunks of kBlkWideRows samples
+  const bool wide = p->blk_wide != 0;
+  const int C = wide ? kBlkWideRows : p->blk_chunk == 128 ? 128 : 64;   // samples staged per chunk
+  const int T = wide ? 256 : 64, NW = T / 64, sxy = wide ? 3 : 2;
   const long long nh = p->nhits;
   if (nh <= 0 || !p->d_hitw || p->K[2] == 1) return NUFFT3D_OK;
   const int K1 = p->K[0], K2 = p->K[1], K3 = p->K[2];
   const long long plane = (long long)K1 * K2, ngrid = plane * K3;
-  const int nbx = (K1 + 3) / 4, nby = (K2 + 3) / 4, nbz = (K3 + 3) / 4;
+  const int bxy = 1 << sxy;
+  const int nbx = (K1 + bxy - 1) / bxy, nby = (K2 + bxy - 1) / bxy, nbz = (K3 + 3) / 4;
   const long long nblk = (long long)nbx * nby * nbz;
   if (nblk >= (1LL << 30)) return NUFFT3D_OK;
   std::vector<long long> eb(nbz + 1);
@@ -1150,7 +1163,7 @@ static int build_block_lists(nufft3d_plan* p) {
     nsel = 0;
     if (n == 0) return true;
     hipLaunchKernelGGL(k_bk_keys, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
-                       p->d_hitw, g0, g1, eb[z], K1, K2, d_bof, nbx, nby, keys);
+                       p->d_hitw, g0, g1, eb[z], K1, K2, d_bof, nbx, nby, sxy, keys);
     if (hipcub::DeviceRadixSort::SortKeys(tmp, tb1, keys, skeys, n, 0, 32 + bbits, p->stream) != hipSuccess ||
         hipcub::DeviceSelect::Unique(tmp, tb2, skeys, keys, d_nsel, n, p->stream) != hipSuccess ||
         hipMemcpyAsync(&nsel, d_nsel, sizeof(int), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
@@ -1176,7 +1189,7 @@ static int build_block_lists(nufft3d_plan* p) {
                          (const unsigned long long*)keys, (long long)ns, off, asc, beg, end);
     const long long g0 = 4LL * z * plane, g1 = std::min<long long>(4LL * z + 4, K3) * plane;
     hipLaunchKernelGGL(k_bk_slot, dim3(nblocks(g1 - g0)), dim3(256), 0, p->stream, p->d_rowptr,
-                       p->d_hitw, g0, g1, K1, K2, d_bof, nbx, nby, (const int*)asc,
+                       p->d_hitw, g0, g1, K1, K2, d_bof, nbx, nby, sxy, (const int*)asc,
                        (const long long*)beg, (const long long*)end, hslot);
     if (hipGetLastError() != hipSuccess) return fail("slot kernels failed");
     off += ns;
@@ -1193,50 +1206,57 @@ static int build_block_lists(nufft3d_plan* p) {
   for (long long b = 0; b < nblk; ++b)
     cbase[b + 1] = cbase[b] + (he[b] - hb[b] + C - 1) / C;
   const long long nchunk = cbase[nblk];
+  const long long nent = nchunk * NW;   // (chunk, wavefront) entry ranges
   if (dev_malloc(&bk.d_blk, sizeof(int) * nblk) != hipSuccess ||
       dev_malloc(&bk.d_cbase, sizeof(long long) * (nblk + 1)) != hipSuccess ||
       dev_malloc(&bk.d_samp, sizeof(int) * std::max(1LL, nchunk * C)) != hipSuccess ||
-      dev_malloc(&bk.d_eoff, sizeof(long long) * (nchunk + 1)) != hipSuccess ||
-      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * 64)) != hipSuccess ||
+      dev_malloc(&bk.d_eoff, sizeof(long long) * (nent + 1)) != hipSuccess ||
+      dev_malloc(&cnt, sizeof(unsigned short) * std::max(1LL, nchunk * T)) != hipSuccess ||
       dev_malloc(&hk, sizeof(unsigned short) * nh) != hipSuccess ||
-      dev_malloc(&grp, sizeof(long long) * (nchunk + 1)) != hipSuccess)
+      dev_malloc(&grp, sizeof(long long) * (nent + 1)) != hipSuccess)
     return fail("allocation failed");
   if (hipMemcpy(bk.d_blk, blk.data(), sizeof(int) * nblk, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(bk.d_cbase, cbase.data(), sizeof(long long) * (nblk + 1), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * 64), p->stream) != hipSuccess)
+      hipMemsetAsync(cnt, 0, sizeof(unsigned short) * std::max(1LL, nchunk * T), p->stream) != hipSuccess)
     return fail("copy failed");
   hipLaunchKernelGGL(k_bk_samp, dim3((unsigned)nblk), dim3(256), 0, p->stream, (const int*)asc,
                      (const long long*)beg, (const long long*)end, (const long long*)bk.d_cbase, C,
                      bk.d_samp);
   hipLaunchKernelGGL(k_bk_rank, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr, ngrid,
-                     K1, K2, d_bof, nbx, nby, (const long long*)bk.d_cbase, (const unsigned*)hslot,
-                     cnt, hk);
-  if (nchunk > 0)
-    hipLaunchKernelGGL(k_bk_groups, dim3(nblocks(nchunk)), dim3(256), 0, p->stream,
-                       (const unsigned short*)cnt, nchunk, grp);
+                     K1, K2, d_bof, nbx, nby, sxy, T, (const long long*)bk.d_cbase,
+                     (const unsigned*)hslot, cnt, hk);
+  if (nent > 0)
+    hipLaunchKernelGGL(k_bk_groups, dim3(nblocks(nent)), dim3(256), 0, p->stream,
+                       (const unsigned short*)cnt, nchunk, T, grp);
   if (hipGetLastError() != hipSuccess) return fail("list kernels failed");
-  std::vector<long long> hgrp(nchunk + 1, 0), eoff(nchunk + 1, 0);
-  if (nchunk > 0 &&
-      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nchunk, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+  std::vector<long long> hgrp(nent + 1, 0), eoff(nent + 1, 0);
+  if (nent > 0 &&
+      (hipMemcpyAsync(hgrp.data(), grp, sizeof(long long) * nent, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
        hipStreamSynchronize(p->stream) != hipSuccess))
     return fail("copy failed");
-  for (long long c = 0; c < nchunk; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
-  const long long ngroups = eoff[nchunk];
+  for (long long c = 0; c < nent; ++c) eoff[c + 1] = eoff[c] + hgrp[c];
+  const long long ngroups = eoff[nent];
   if (dev_malloc(&bk.d_slot, sizeof(unsigned) * std::max(1LL, ngroups * 64)) != hipSuccess ||
       dev_malloc(&bk.d_w, sizeof(float4) * std::max(1LL, ngroups * 64)) != hipSuccess)
     return fail("allocation failed");
-  if (hipMemcpy(bk.d_eoff, eoff.data(), sizeof(long long) * (nchunk + 1), hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(bk.d_eoff, eoff.data(), sizeof(long long) * (nent + 1), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemsetAsync(bk.d_slot, 0xff, sizeof(unsigned) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess ||
       hipMemsetAsync(bk.d_w, 0, sizeof(float4) * std::max(1LL, ngroups * 64), p->stream) != hipSuccess)
     return fail("copy failed");
   hipLaunchKernelGGL(k_bk_fill, dim3(nblocks(ngrid)), dim3(256), 0, p->stream, p->d_rowptr, ngrid,
-                     K1, K2, d_bof, nbx, nby, (const long long*)bk.d_cbase,
+                     K1, K2, d_bof, nbx, nby, sxy, (const long long*)bk.d_cbase,
                      (const long long*)bk.d_eoff, (const unsigned*)hslot, (const unsigned short*)hk,
                      (const uint2*)p->d_hitw, (unsigned char*)bk.d_slot, (float*)bk.d_w);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
     return fail("fill failed");
   // pieces: a block's chunks in runs of at most kBlkPieceRows entry rows
-  // (at least one chunk each); blocks in several pieces sum partials
+  // (at least one chunk each; a chunk's rows are its largest wavefront's);
+  // blocks in several pieces sum partials
+  std::vector<long long> crow(nchunk, 0);
+  for (long long c = 0; c < nchunk; ++c)
+    for (int w = 0; w < NW; ++w) crow[c] = std::max(crow[c], 4 * hgrp[c * NW + w]);
+  // (wide blocks: four times the rows, so fewer partials to write and add)
+  const long long prow = p->blk_piece > 0 ? p->blk_piece : wide ? 4 * kBlkPieceRows : kBlkPieceRows;
   std::vector<int4> pieces, red;
   long long npslot = 0;
   pieces.reserve(nblk + nblk / 4);
@@ -1250,7 +1270,7 @@ static int build_block_lists(nufft3d_plan* p) {
     long long c = c0;
     while (c < c1) {
       long long rows = 0, e = c;
-      while (e < c1 && (e == c || rows + 4 * hgrp[e] <= kBlkPieceRows)) rows += 4 * hgrp[e++];
+      while (e < c1 && (e == c || rows + crow[e] <= prow)) rows += crow[e++];
       runs.push_back({(int)(c - c0), (int)(e - c0)});
       c = e;
     }
@@ -1272,6 +1292,7 @@ static int build_block_lists(nufft3d_plan* p) {
     return fail("copy failed");
   cleanup();
   bk.C = C;
+  bk.T = T;
   bk.nblk = nblk;
   bk.nchunk = nchunk;
   bk.ngroups = ngroups;
@@ -1279,8 +1300,8 @@ static int build_block_lists(nufft3d_plan* p) {
   bk.nred = (long long)red.size();
   bk.npslot = npslot;
   if (p->verbose)
-    printf("  block-staged adjoint: %lld blocks, %lld chunks, %lld entry groups (%.2f B/hit), "
-           "%lld pieces, %lld partial slots\n", nblk, nchunk, ngroups,
+    printf("  block-staged adjoint: %lld blocks of %d points, %lld chunks of %d samples, %lld entry "
+           "groups (%.2f B/hit), %lld pieces, %lld partial slots\n", nblk, T, nchunk, C, ngroups,
            5.0 * 4 * 64 * ngroups / (double)nh, bk.npiece, npslot);
   return NUFFT3D_OK;
 }
@@ -2174,6 +2195,8 @@ int nufft3d_plan_create(const double* om, int64_t M, const int32_t N[3],
   if (const char* tv = getenv("NUFFT3D_TILE_CG")) p->tile_cg = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_TILE_LAYERS")) p->tile_layers = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_BLK_CHUNK")) p->blk_chunk = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_BLK_WIDE")) p->blk_wide = atoi(tv);
+  if (const char* tv = getenv("NUFFT3D_BLK_PIECE")) p->blk_piece = atoi(tv);
   if (const char* tv = getenv("NUFFT3D_FWD_GROUP")) p->fwd_group = atoi(tv);
   if (o.tile > 0) {
     p->T[0] = p->T[1] = p->T[2] = o.tile;
@@ -2275,7 +2298,8 @@ int nufft3d_device_bytes(const nufft3d_plan* p, int64_t* plan_bytes, int64_t* wo
   if (p->bl.d_ent) b += 4 * std::max<int64_t>(p->nhits, 1);
   if (p->wl.d_went) b += 16 * p->nbrick + 4 * std::max<int64_t>(1, p->wl.total) + 8 * p->nhits;
   if (p->bk.d_piece)
-    b += 12 * p->bk.nblk + 8 + (4 * p->bk.C + 8) * p->bk.nchunk + 8 + 20 * 64 * p->bk.ngroups +
+    b += 12 * p->bk.nblk + 8 + (4 * p->bk.C + 8 * p->bk.T / 64) * p->bk.nchunk + 8 +
+         20 * 64 * p->bk.ngroups +
          16 * (p->bk.npiece + std::max<int64_t>(1, p->bk.nred));
   if (p->fk.d_slot)
     b += 8 * (p->fk.ngrp + 1) + (4 * p->fk.C + 8 * p->fk.G / 64) * p->fk.nchunk + 8 +

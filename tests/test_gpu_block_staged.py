@@ -69,6 +69,15 @@ def _kernels_run(p, fn):
     return set(json.loads(buf.value.decode()))
 
 
+# the block-staged adjoint kernels: 4x4x4 blocks (one wavefront) or wide
+# 8x8x4 blocks (four wavefronts sharing each staged chunk)
+ADJ_BLK = {"k_spread_blk", "k_spread_blkw"}
+
+
+def _adj_kernel(wide):
+    return "k_spread_blkw" if wide else "k_spread_blk"
+
+
 CASES = [
     # N, trajectory, J, radial, coils
     (24, "koosh", 1, 8),
@@ -89,17 +98,18 @@ def _traj(N, kind):
     return synth.koosh_ball(N, 2000, 2 * N)
 
 
+@pytest.mark.parametrize("wide", [0, 1])
 @pytest.mark.parametrize("N,kind,radial,nc", CASES)
-def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc):
+def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc, wide):
     om = _traj(N, kind)
     kw = dict(J=6, u=1.25, radial=radial, gpu=1)
-    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096)}, om, N, **kw)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096), "NUFFT3D_BLK_WIDE": str(wide)}, om, N, **kw)
     gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
     assert blk.compute_precision == 2
     k = _rand((om.shape[1], nc), 71).astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.aNUFT(k)))
-    assert "k_spread_blk" in ran and "k_spread" not in ran, ran
+    assert ADJ_BLK & ran == {_adj_kernel(wide)} and "k_spread" not in ran, ran
     if kind == "dense":
         assert "k_spread_blk_reduce" in ran, ran
     want = gat.aNUFT(k)
@@ -109,8 +119,9 @@ def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc):
     assert rel(got["b"], orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
 
 
+@pytest.mark.parametrize("wide", [0, 1])
 @pytest.mark.parametrize("kind,nc", [("golden", 8), ("dense", 16)])
-def test_block_staged_adjoint_of_an_fp64_plan(kind, nc):
+def test_block_staged_adjoint_of_an_fp64_plan(kind, nc, wide):
     # double samples into the double grid (fp64 plans, and iNUFT's internal
     # samples on any fp64-arithmetic plan): the same products summed in
     # another fixed order. The deapodization (its range is ~1e7 at u=1.25,
@@ -119,12 +130,12 @@ def test_block_staged_adjoint_of_an_fp64_plan(kind, nc):
     N = 16 if kind == "dense" else 24
     om = _traj(N, kind)
     kw = dict(J=6, u=1.25, gpu=2)
-    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096)}, om, N, **kw)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096), "NUFFT3D_BLK_WIDE": str(wide)}, om, N, **kw)
     gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
     k = _rand((om.shape[1], nc), 78)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.aNUFT(k)))
-    assert "k_spread_blk" in ran, ran
+    assert ADJ_BLK & ran == {_adj_kernel(wide)}, ran
     assert rel(got["b"], gat.aNUFT(k)) < 1e-12
     assert np.array_equal(blk.aNUFT(k), got["b"])
     orc = O.NufftOracle(om, N, J=6, u=1.25)
@@ -144,7 +155,7 @@ def test_block_staged_adjoint_on_a_grid_narrower_than_the_footprint():
     assert blk.compute_precision == 2 and min(blk.K) < 7
     k = _rand((om.shape[1], 8), 72).astype(np.complex64)
     ran = _kernels_run(blk, lambda: blk.aNUFT(k))
-    assert "k_spread_blk" in ran
+    assert ADJ_BLK & ran
     assert rel(blk.aNUFT(k), gat.aNUFT(k)) < 1e-8
     orc = O.NufftOracle(om, N, J=6, u=1.25, radial=1)
     assert rel(blk.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
@@ -221,7 +232,7 @@ def test_block_staged_forward_and_adjoint_in_the_inverse(gpu):
         raw = raw.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: got.setdefault("b", blk.iNUFT(raw, 6, 0.02)))
-    assert FWD_BLK & ran and "k_spread_blk" in ran, ran
+    assert FWD_BLK & ran and ADJ_BLK & ran, ran
     want = gat.iNUFT(raw, 6, 0.02)
     assert rel(got["b"], want) < (1e-11 if gpu == 2 else 1e-7)
     ref = orc.iNUFT(raw.astype(np.complex128), 6, 0.02)
@@ -267,7 +278,7 @@ def test_gather_lists_released_after_the_autotuner_picks_block_kernels():
         x = _rand((N, N, N, nc), 80 + nc).astype(np.complex64)
         k = _rand((om.shape[1], nc), 90 + nc).astype(np.complex64)
         ran = _kernels_run(g, lambda: (g.fNUFT(x), g.aNUFT(k)))
-        assert FWD_BLK & ran and "k_spread_blk" in ran, ran
+        assert FWD_BLK & ran and ADJ_BLK & ran, ran
         assert "k_interp" not in ran and "k_spread" not in ran, ran
         assert rel(g.fNUFT(x), orc.fNUFT(x.astype(np.complex128))) < TOL32_OP
         assert rel(g.aNUFT(k), orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
@@ -294,7 +305,7 @@ def test_block_staged_kernels_on_other_shapes_and_kernels(N, J, u, radial, gpu):
         x, k = x.astype(np.complex64), k.astype(np.complex64)
     got = {}
     ran = _kernels_run(blk, lambda: (got.setdefault("f", blk.fNUFT(x)), got.setdefault("a", blk.aNUFT(k))))
-    assert FWD_BLK & ran and "k_spread_blk" in ran, ran
+    assert FWD_BLK & ran and ADJ_BLK & ran, ran
     orc = O.NufftOracle(om, N, J=J, u=u, radial=radial)
     tol = 1e-12 if gpu == 2 else TOL32_OP
     assert rel(got["f"], orc.fNUFT(x.astype(np.complex128))) < tol

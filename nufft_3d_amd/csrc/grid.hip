@@ -832,6 +832,13 @@ __global__ void __launch_bounds__(512) k_spread_tile_w(const long long* __restri
 // of a work-group run independent pieces, with no work-group barrier.
 template <typename S> struct BlkRow { static constexpr int U = sizeof(S) == 8 ? 5 : 9; };  // 16-byte units (data + 1 of padding)
 
+// entry groups the wide block-staged kernels keep in flight ahead of use:
+// at the 256^3 precision workload the forward measured 6.18 / 6.08 / 6.11 /
+// 6.35 ms with 1 / 2 / 3 / 5 groups, and the adjoint 6.70 ms with 1 and
+// 6.98-7.17 ms with more (the registers cost it a wave per SIMD)
+constexpr int kEntryAheadAdj = 1;
+constexpr int kEntryAheadFwd = 2;
+
 // LDS writes of a wavefront visible to its own later reads
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1051,15 +1058,25 @@ __global__ void __launch_bounds__(256) k_spread_blkw(const int4* __restrict__ pi
     }
     __syncthreads();
     const long long q0 = eoff[ch * 4 + wv], q1 = eoff[ch * 4 + wv + 1];
-    // entry groups loaded one group ahead (index clamped into the range)
-    unsigned sl4 = 0xffffffffu;
-    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (q0 < q1) { sl4 = eslot[q0 * 64 + lane]; w4 = ew[q0 * 64 + lane]; }
+    // entry groups loaded kEntryAheadAdj groups ahead (indices clamped into the range)
+    unsigned slq[kEntryAheadAdj];
+    float4 wq[kEntryAheadAdj];
+#pragma unroll
+    for (int d = 0; d < kEntryAheadAdj; ++d) {
+      const long long qd = q0 + d < q1 ? q0 + d : q1 - 1;
+      slq[d] = 0xffffffffu;
+      wq[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (q0 < q1) { slq[d] = eslot[qd * 64 + lane]; wq[d] = ew[qd * 64 + lane]; }
+    }
     if (c + 1 < pc.z) fetch(ch + 1);
     for (long long q = q0; q < q1; ++q) {
-      const long long qn = q + 1 < q1 ? q + 1 : q;
-      const unsigned nsl4 = eslot[qn * 64 + lane];
-      const float4 nw4 = ew[qn * 64 + lane];
+      const unsigned sl4 = slq[0];
+      const float4 w4 = wq[0];
+#pragma unroll
+      for (int d = 0; d + 1 < kEntryAheadAdj; ++d) { slq[d] = slq[d + 1]; wq[d] = wq[d + 1]; }
+      const long long qn = q + kEntryAheadAdj < q1 ? q + kEntryAheadAdj : q1 - 1;
+      slq[kEntryAheadAdj - 1] = eslot[qn * 64 + lane];
+      wq[kEntryAheadAdj - 1] = ew[qn * 64 + lane];
       const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -1086,8 +1103,6 @@ __global__ void __launch_bounds__(256) k_spread_blkw(const int4* __restrict__ pi
           }
         }
       }
-      sl4 = nsl4;
-      w4 = nw4;
     }
   }
   if (x >= g.K1 || y >= g.K2 || z >= g.K3) return;
@@ -1334,15 +1349,25 @@ __global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict
     if (h == 0) s_pc[r] = pcv;
     __syncthreads();
     const long long q0 = eoff[ch * 4 + wv], q1 = eoff[ch * 4 + wv + 1];
-    unsigned sl4 = q0 < q1 ? eslot[q0 * 64 + lane] : 0xffffffffu;
-    float4 w4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (STW) if (q0 < q1) w4 = ew[q0 * 64 + lane];
+    // entry groups loaded kEntryAheadFwd groups ahead (indices clamped into the range)
+    unsigned slq[kEntryAheadFwd];
+    float4 wq[kEntryAheadFwd];
+#pragma unroll
+    for (int d = 0; d < kEntryAheadFwd; ++d) {
+      const long long qd = q0 + d < q1 ? q0 + d : q1 - 1;
+      slq[d] = q0 < q1 ? eslot[qd * 64 + lane] : 0xffffffffu;
+      wq[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if constexpr (STW) if (q0 < q1) wq[d] = ew[qd * 64 + lane];
+    }
     if (ch + 1 < c1) fetch(ch + 1);
     for (long long q = q0; q < q1; ++q) {
-      const long long qn = q + 1 < q1 ? q + 1 : q;
-      const unsigned nsl4 = eslot[qn * 64 + lane];
-      float4 nw4 = w4;
-      if constexpr (STW) nw4 = ew[qn * 64 + lane];
+      const unsigned sl4 = slq[0];
+      const float4 w4 = wq[0];
+#pragma unroll
+      for (int d = 0; d + 1 < kEntryAheadFwd; ++d) { slq[d] = slq[d + 1]; wq[d] = wq[d + 1]; }
+      const long long qn = q + kEntryAheadFwd < q1 ? q + kEntryAheadFwd : q1 - 1;
+      slq[kEntryAheadFwd - 1] = eslot[qn * 64 + lane];
+      if constexpr (STW) wq[kEntryAheadFwd - 1] = ew[qn * 64 + lane];
       const float wk[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -1371,8 +1396,6 @@ __global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict
           }
         }
       }
-      sl4 = nsl4;
-      w4 = nw4;
     }
   }
   if (!act) return;

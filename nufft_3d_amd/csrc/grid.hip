@@ -1546,12 +1546,17 @@ __global__ void __launch_bounds__(256) k_permute_in_tile(const int* __restrict__
 // plane, then each in-bounds row's nc values are written as one contiguous
 // sample row at its sorted position inv[row] (out-of-bounds rows have -1).
 // Every sorted position is written once; values and weights as k_permute_in.
+constexpr int kPermRows = 256;   // caller rows per k_permute_in_rows work-group (= its threads)
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_permute_in_rows(const int* __restrict__ inv, long long M,
                                                          int nc, const cplx<R>* __restrict__ ku,
                                                          const R* __restrict__ wsamp,
                                                          cplx<R>* __restrict__ ks) {
-  constexpr int TS = 64, CC = 32;
+  // 256 caller rows per work-group, 8 coils per pass: each thread's loads of
+  // a pass are issued together, and a row's 8 coils leave as one 64-byte
+  // (float) or 128-byte (double) write
+  constexpr int TS = kPermRows, CC = 8;
   __shared__ cplx<R> t[TS][CC + 1];
   __shared__ int si[TS];
   __shared__ R sw[TS];
@@ -1564,19 +1569,27 @@ __global__ void __launch_bounds__(256) k_permute_in_rows(const int* __restrict__
   }
   for (int c0 = 0; c0 < nc; c0 += CC) {
     const int cc = nc - c0 < CC ? nc - c0 : CC;
-    __syncthreads();   // si/sw written; the previous chunk's tile is consumed
-    for (int i = threadIdx.x; i < TS * cc; i += blockDim.x) {
-      const int c = i / TS, j = i % TS;
-      if (j < nr) t[j][c] = ku[r0 + j + (long long)(c0 + c) * M];
+    __syncthreads();   // si/sw written; the previous pass's tile is consumed
+    cplx<R> v[CC];
+#pragma unroll
+    for (int c = 0; c < CC; ++c) {   // row threadIdx.x of coil c0 + c
+      const int j = threadIdx.x;
+      if (c < cc && j < nr) v[c] = ku[r0 + j + (long long)(c0 + c) * M];
     }
+#pragma unroll
+    for (int c = 0; c < CC; ++c)
+      if (c < cc && (int)threadIdx.x < nr) t[threadIdx.x][c] = v[c];
     __syncthreads();
-    for (int i = threadIdx.x; i < nr * cc; i += blockDim.x) {
-      const int j = i / cc, c = i % cc;
+#pragma unroll
+    for (int k = 0; k < CC; ++k) {   // rows 32k .. 32k + 31, one coil per thread
+      const int i = threadIdx.x + k * TS;
+      const int j = i / CC, c = i % CC;
+      if (j >= nr || c >= cc) continue;
       const int s = si[j];
       if (s < 0) continue;
-      cplx<R> v = t[j][c];
-      if (wsamp) { v.x *= sw[j]; v.y *= sw[j]; }
-      ks[(long long)s * nc + c0 + c] = v;
+      cplx<R> w = t[j][c];
+      if (wsamp) { w.x *= sw[j]; w.y *= sw[j]; }
+      ks[(long long)s * nc + c0 + c] = w;
     }
   }
 }
@@ -2288,7 +2301,7 @@ int permute_in(nufft3d_plan* p, const void* k_user, int nc, void* ks, const void
   ProfScope ps(p, "k_permute_in");
   const int pr = ep ? ep : p->prec;
   if (nc >= 4 && p->d_inv && !(p->tune_flags & 64)) {   // caller rows read in order
-    const unsigned nb = (unsigned)((p->M + 63) / 64);
+    const unsigned nb = (unsigned)((p->M + kPermRows - 1) / kPermRows);
     if (pr == 2)
       hipLaunchKernelGGL((k_permute_in_rows<double>), dim3(nb), dim3(256), 0, p->stream, p->d_inv,
                          (long long)p->M, nc, (const double2*)k_user, (const double*)wsamp,

@@ -320,7 +320,9 @@ int apply_normal(nufft3d_plan* p, const void* x, int nc, double damp, const void
   N3D_TRY(fft_forward(p, x, nc, p->ws_grid.p));
   N3D_TRY(interp(p, p->ws_grid.p, nc, p->ws_samp.p, 0, wsamp, sc));
   N3D_TRY(spread(p, p->ws_samp.p, nc, p->ws_grid.p, 0, nullptr));
-  N3D_TRY(fft_adjoint(p, p->ws_grid.p, nc, y, sc, x, damp * damp));
+  // the damping term damp^2 x (none when damp = 0: the x pass then skips
+  // reading x)
+  N3D_TRY(fft_adjoint(p, p->ws_grid.p, nc, y, sc, damp != 0.0 ? x : nullptr, damp * damp));
   return NUFFT3D_OK;
 }
 

@@ -119,6 +119,28 @@ def test_block_staged_adjoint_matches_gather_and_oracle(N, kind, radial, nc, wid
     assert rel(got["b"], orc.aNUFT(k.astype(np.complex128))) < TOL32_OP
 
 
+@pytest.mark.parametrize("piece", [16, 64, 4096])
+def test_block_staged_adjoint_piece_sizes(piece):
+    # pieces of 16 / 64 / 4096 entry rows split the dense trajectory's blocks
+    # into many, some or few partials; each split is its own fixed order of
+    # the same products (deterministic), all within the float rounding of
+    # the gather's image
+    N = 16
+    om = _traj(N, "dense")
+    kw = dict(J=6, u=1.25, gpu=1)
+    blk = _plan_with_env({"NUFFT3D_TUNE": str(2 | 4096), "NUFFT3D_BLK_PIECE": str(piece)},
+                         om, N, **kw)
+    gat = _plan_with_env({"NUFFT3D_TUNE": "2"}, om, N, **kw)
+    k = _rand((om.shape[1], 8), 79).astype(np.complex64)
+    got = {}
+    ran = _kernels_run(blk, lambda: got.setdefault("b", blk.aNUFT(k)))
+    assert "k_spread_blkw" in ran, ran
+    if piece < 4096:
+        assert "k_spread_blk_reduce" in ran, ran
+    assert rel(got["b"], gat.aNUFT(k)) < 1e-8
+    assert np.array_equal(blk.aNUFT(k), got["b"])
+
+
 @pytest.mark.parametrize("wide", [0, 1])
 @pytest.mark.parametrize("kind,nc", [("golden", 8), ("dense", 16)])
 def test_block_staged_adjoint_of_an_fp64_plan(kind, nc, wide):

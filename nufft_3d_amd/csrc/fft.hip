@@ -341,8 +341,14 @@ __global__ void __launch_bounds__(512) k_fft_x_fwd(const cplx<IO>* __restrict__ 
   const long long l0 = (long long)blockIdx.x * B;
   // a bundle of one image row's coils (B == nc): row blockIdx.x, coil = line
   const bool one = B == g.nc;
-  for (int e = threadIdx.x; e < g.K1 * B; e += blockDim.x) { a[e].x = 0; a[e].y = 0; }
-  __syncthreads();
+  // zeros at the padding positions only ([N1 - N1/2, K1 - N1/2) of each line)
+  {
+    const int p0 = g.N1 - g.N1 / 2;
+    for (int e = threadIdx.x; e < (g.K1 - g.N1) * B; e += blockDim.x) {
+      cplx<R> z; z.x = 0; z.y = 0;
+      a[(p0 + e / B) * B + e % B] = z;
+    }
+  }
   for (int e = threadIdx.x; e < g.N1 * B; e += blockDim.x) {
     const int li = e / g.N1, x = e % g.N1;
     const long long l = l0 + li;

@@ -1278,12 +1278,14 @@ __global__ void __launch_bounds__(256) k_interp_blk(const long long* __restrict_
 }
 
 // The same forward from lists built with groups of 256 sorted samples: one
-// work-group per group, one thread per sample. The group's chunks of 128
-// grid rows are staged in LDS by the whole work-group (two threads per row,
-// four coils each) and each wavefront walks its own entries of the chunk; a
-// larger group shares each staged row among more samples. The next chunk's
+// work-group per group, one thread per sample. The group's chunks of 255
+// grid rows (slot 255 stays "none") are staged in LDS by the whole
+// work-group (one row per thread) and each wavefront walks its own entries
+// of the chunk; a larger group shares each staged row among more samples,
+// and a larger chunk evens out the lanes' step counts (6.25 -> 6.00 ms at
+// the 256^3 precision workload against chunks of 128). The next chunk's
 // rows are loaded into registers while the current chunk is consumed.
-constexpr int kFwdRowsW = 128;
+constexpr int kFwdRowsW = 255;
 
 template <typename RO, bool USER, bool TAB, bool STW>
 __global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict__ cbase,
@@ -1316,23 +1318,28 @@ __global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict
   double ar[8], ai[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) { ar[c] = 0; ai[c] = 0; }
-  // staging role: row r of the chunk, coils 4h .. 4h + 3
-  const int r = t >> 1, h = t & 1;
-  double4 v0 = make_double4(0.0, 0.0, 0.0, 0.0), v1 = v0;
+  // staging role: row t of the chunk, all 8 coils (threads past the rows idle)
+  const int r = t;
+  double4 v0 = make_double4(0.0, 0.0, 0.0, 0.0), v1 = v0, v2 = v0, v3 = v0;
   int pcv = 0;
   auto fetch = [&](long long ch) {
+    if (r >= kFwdRowsW) return;
     const int gp = gpt[ch * kFwdRowsW + r];
-    const double2* src = grid + (long long)gp * g.nc + g.c0 + 4 * h;
+    const double2* src = grid + (long long)gp * g.nc + g.c0;
     if (g.cg == 8) {
       v0 = reinterpret_cast<const double4*>(src)[0];
       v1 = reinterpret_cast<const double4*>(src)[1];
+      v2 = reinterpret_cast<const double4*>(src)[2];
+      v3 = reinterpret_cast<const double4*>(src)[3];
     } else {   // a partial group: its coils, zero past them
-      const int n = g.cg - 4 * h;
       const double2 z = make_double2(0.0, 0.0);
-      const double2 a0 = n > 0 ? src[0] : z, a1 = n > 1 ? src[1] : z;
-      const double2 a2 = n > 2 ? src[2] : z, a3 = n > 3 ? src[3] : z;
-      v0 = make_double4(a0.x, a0.y, a1.x, a1.y);
-      v1 = make_double4(a2.x, a2.y, a3.x, a3.y);
+      double2 a[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] = k < g.cg ? src[k] : z;
+      v0 = make_double4(a[0].x, a[0].y, a[1].x, a[1].y);
+      v1 = make_double4(a[2].x, a[2].y, a[3].x, a[3].y);
+      v2 = make_double4(a[4].x, a[4].y, a[5].x, a[5].y);
+      v3 = make_double4(a[6].x, a[6].y, a[7].x, a[7].y);
     }
     const int px = gp % g.K1, py = (gp / g.K1) % g.K2, pz = gp / (g.K1 * g.K2);
     pcv = px | (py << 10) | (pz << 20);
@@ -1341,12 +1348,18 @@ __global__ void __launch_bounds__(256) k_interp_blkw(const long long* __restrict
   if (c0 < c1) fetch(c0);
   for (long long ch = c0; ch < c1; ++ch) {
     __syncthreads();   // the previous chunk's rows are consumed
-    double2* dst = s_row + r * kFwdRow + 4 * h;
-    dst[0] = make_double2(v0.x, v0.y);
-    dst[1] = make_double2(v0.z, v0.w);
-    dst[2] = make_double2(v1.x, v1.y);
-    dst[3] = make_double2(v1.z, v1.w);
-    if (h == 0) s_pc[r] = pcv;
+    if (r < kFwdRowsW) {
+      double2* dst = s_row + r * kFwdRow;
+      dst[0] = make_double2(v0.x, v0.y);
+      dst[1] = make_double2(v0.z, v0.w);
+      dst[2] = make_double2(v1.x, v1.y);
+      dst[3] = make_double2(v1.z, v1.w);
+      dst[4] = make_double2(v2.x, v2.y);
+      dst[5] = make_double2(v2.z, v2.w);
+      dst[6] = make_double2(v3.x, v3.y);
+      dst[7] = make_double2(v3.z, v3.w);
+      s_pc[r] = pcv;
+    }
     __syncthreads();
     const long long q0 = eoff[ch * 4 + wv], q1 = eoff[ch * 4 + wv + 1];
     // entry groups loaded kEntryAheadFwd groups ahead (indices clamped into the range)

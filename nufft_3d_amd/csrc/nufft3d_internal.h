@@ -183,18 +183,20 @@ struct BlockLists {
   bool tried = false;
 };
 
-// Block-staged forward (k_interp_blk) of fp64-arithmetic 3-D plans: sorted
-// samples in groups of 64, one wavefront per group and one lane per sample.
-// A group's grid points (the distinct points its samples touch, ascending
-// index) are dealt round-robin into chunks of 64 as the adjoint's samples
-// are; a chunk's grid rows (8 coils of fp64) pass through the wave's LDS and
-// each lane walks its entries of the chunk (4 slot bytes per lane per word,
-// 255 = none). The weights are evaluated from the lane's own sample offsets
-// and the staged point's coordinates, as the gather kernel evaluates them.
+// Block-staged forward (k_interp_blkw / k_interp_blk) of fp64-arithmetic 3-D
+// plans: sorted samples in groups of 256 (four wavefronts; 64, one
+// wavefront, where a group's union could pass 16-bit slots), one lane per
+// sample. A group's grid points (the distinct points its samples touch,
+// ascending index) are dealt round-robin into chunks of 255 (64) as the
+// adjoint's samples are; a chunk's grid rows (8 coils of fp64) pass through
+// LDS and each lane walks its entries of the chunk (4 slot bytes per lane per
+// word, 255 = none), per (chunk, wavefront). The weights are evaluated from
+// the lane's own sample offsets and the staged point's coordinates, as the
+// gather kernel evaluates them, or read as stored at plan time.
 struct FwdLists {
   int G = 64;                       // sorted samples per group (64: one wavefront; 256: four
                                     // wavefronts sharing each staged chunk)
-  int C = 64;                       // grid rows per chunk (64; 128 for groups of 256)
+  int C = 64;                       // grid rows per chunk (64; 255 for groups of 256)
   long long ngrp = 0;               // sample groups
   long long* d_cbase = nullptr;     // [ngrp + 1] first chunk of each group
   long long nchunk = 0;

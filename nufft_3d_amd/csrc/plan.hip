@@ -1415,7 +1415,7 @@ static int build_fwd_lists(nufft3d_plan* p, bool weights) {
   // samples per group (one wavefront per 64) and grid rows per chunk; a
   // group's union of points is indexed by 16-bit slots
   const int G = p->fwd_group == 256 && 256LL * W * W * W < 65535 ? 256 : 64;
-  const int C = G == 256 ? 128 : 64, NW = G / 64;
+  const int C = G == 256 ? 255 : 64, NW = G / 64;
   const long long ngrp = (nnz + G - 1) / G;
   std::vector<long long> hspos(nnz + 1);
   N3D_HIP(hipMemcpy(hspos.data(), p->d_spos, sizeof(long long) * (nnz + 1), hipMemcpyDeviceToHost));
